@@ -1,0 +1,63 @@
+"""Build libddm.so (HIP kernels for gfx950 + host engine) in-tree.
+
+    python -m hackathon_amd.build          # or hackathon_amd.build.build()
+
+Device code: hipcc --offload-arch=gfx950 -O3 with packed-FP32 instructions
+disabled.  Host engine: g++ with -ffp-contract=off (the V8-hypot bin rule
+needs every multiply and add rounded separately).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libddm.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-Wno-unused-command-line-argument"]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+              "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
+
+KERNEL_SOURCES = ["ddm_kernels.hip"]
+HOST_SOURCES = ["ddm_engine.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout)
+        raise RuntimeError("build step failed: " + os.path.basename(cmd[-1]))
+    return r.stdout
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source and link hackathon_amd/libddm.so; returns its path."""
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in KERNEL_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        _run([HIPCC, *DEVICE_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs,
+          "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

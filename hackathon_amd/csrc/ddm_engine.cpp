@@ -1,0 +1,651 @@
+// libddm engine: host side of the C ABI in include/ddm.h.
+//
+// Owns the device buffers of one geometry, the host FP64 tables (window,
+// twiddles, bins, temporal metadata) and one HIP stream.  It is the MI355X
+// counterpart of DdmEngine (motility-ddm src/ddm-engine.ts:62-426) and of the
+// host precompute it pulls from src/ddm_ref.js (makeBins :84-170, temporalMeta
+// :538-560, gpuTwiddles :566-574) and src/fft.js (blackmanHarris :126-133).
+#include "../../include/ddm.h"
+#include "ddm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr double kPi = 3.14159265358979323846;
+
+int ilog2(long n) {
+  int l = 0;
+  while ((1L << l) < n) l++;
+  return l;
+}
+bool is_pow2(long n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+// ------------------------------------------------------------ host tables
+
+// 4-term periodic Blackman-Harris in FP64 (src/fft.js:126-133).
+std::vector<double> blackman_harris(int n) {
+  std::vector<double> w(n);
+  for (int i = 0; i < n; i++) {
+    double x = 2 * kPi * i / n;
+    w[i] = 0.35875 - 0.48829 * std::cos(x) + 0.14128 * std::cos(2 * x) - 0.01168 * std::cos(3 * x);
+  }
+  return w;
+}
+
+// exp(-2 pi i m / n), m < n, FP64 rounded once to f32 (src/ddm_ref.js:566-574
+// keeps a half table for radix 2; the radix-8 passes index up to n-1).
+std::vector<float> twiddles(int n) {
+  std::vector<float> t(2 * n);
+  for (int m = 0; m < n; m++) {
+    double a = -2 * kPi * m / n;
+    t[2 * m] = (float)std::cos(a);
+    t[2 * m + 1] = (float)std::sin(a);
+  }
+  return t;
+}
+
+// V8's two-argument Math.hypot: scale by the larger magnitude, Kahan-sum the
+// squared ratios in argument order, sqrt times the scale.  The reference bins
+// by it (src/ddm_ref.js:93,122,135); it is not correctly rounded, so a
+// different hypot moves Pythagorean bin-edge modes.  This translation unit is
+// compiled with -ffp-contract=off so no step is fused.
+double v8_hypot(double a, double b) {
+  double aa = std::fabs(a), ab = std::fabs(b);
+  if (std::isinf(aa) || std::isinf(ab)) return INFINITY;
+  if (std::isnan(aa) || std::isnan(ab)) return NAN;
+  double mx = std::max(aa, ab);
+  if (mx == 0) return 0;
+  double sum = 0, comp = 0;
+  const double v[2] = {aa, ab};
+  for (double x : v) {
+    double r = x / mx;
+    double summand = r * r - comp;
+    double prelim = sum + summand;
+    comp = (prelim - sum) - summand;
+    sum = prelim;
+  }
+  return std::sqrt(sum) * mx;
+}
+
+// Math.round: nearest integer, ties toward +infinity.
+double js_round(double x) {
+  double r = std::floor(x);
+  return (x - r >= 0.5) ? r + 1 : r;
+}
+
+struct Bins {
+  int NB = 0, K = 0;
+  std::vector<uint32_t> offsets;   // NB+1
+  std::vector<int32_t> modes4;     // 4 per binned mode: mode, kx, ky, 0
+  std::vector<double> kcenter;     // NB
+  std::vector<uint32_t> nmodes;    // NB
+  std::vector<double> knodes;      // NB*K
+};
+
+// Radial bins linear in |k| on [kmin, kmax): half-plane modes, skipping the
+// kx = 0, ky <= 0 half axis (conjugates of ky > 0) and the Nyquist lines;
+// per-bin mean |k|; K equal-weight quantile nodes of the bin's |k| distribution
+// convolved with the window's power leakage (src/ddm_ref.js:40-55, 84-170).
+Bins make_bins(int N, int NB, double kmin, double kmax, int hypot_mode, int K) {
+  auto hyp = [hypot_mode](double a, double b) { return hypot_mode ? std::hypot(a, b) : v8_hypot(a, b); };
+  const int NH = N / 2 + 1;
+  const double width = (kmax - kmin) / NB;
+  std::vector<std::vector<int>> lists(NB);
+  for (int kyi = 0; kyi < N; kyi++) {
+    const int ky = kyi < N / 2 ? kyi : kyi - N;
+    for (int kx = 0; kx < NH; kx++) {
+      if (kx == 0 && ky <= 0) continue;
+      if (kx == N / 2 || ky == -N / 2) continue;
+      double b = std::floor((hyp(kx, ky) - kmin) / width);
+      if (b >= 0 && b < NB) lists[(int)b].push_back(kyi * NH + kx);
+    }
+  }
+  Bins out;
+  out.NB = NB;
+  out.K = K;
+  out.offsets.resize(NB + 1);
+  out.kcenter.resize(NB);
+  out.nmodes.resize(NB);
+  out.knodes.assign((size_t)NB * K, 0.0);
+  uint32_t total = 0;
+  for (int b = 0; b < NB; b++) {
+    out.offsets[b] = total;
+    total += (uint32_t)lists[b].size();
+  }
+  out.offsets[NB] = total;
+  out.modes4.resize(4 * (size_t)total);
+  // window power leakage at offsets -4..4 step 0.5
+  std::vector<double> w = blackman_harris(N), ld, lp;
+  for (double x = -4; x <= 4 + 1e-9; x += 0.5) {
+    double re = 0, im = 0;
+    for (int i = 0; i < N; i++) {
+      double a = (-2 * kPi * x * i) / N;
+      re += w[i] * std::cos(a);
+      im += w[i] * std::sin(a);
+    }
+    ld.push_back(x);
+    lp.push_back(re * re + im * im);
+  }
+  double lsum = 0;
+  for (double v : lp) lsum += v;
+  for (double& v : lp) v /= lsum;
+  const double h0 = std::max(0.0, kmin - 6), hres = 0.01;
+  const long hlen = (long)std::ceil((kmax + 6 - h0) / hres) + 1;
+  std::vector<double> hist(hlen), hk(hlen);
+  for (int b = 0; b < NB; b++) {
+    const auto& lst = lists[b];
+    double sk = 0;
+    for (size_t i = 0; i < lst.size(); i++) {
+      int m = lst[i], kx = m % NH, kyi = m / NH, ky = kyi < N / 2 ? kyi : kyi - N;
+      size_t o = 4 * (out.offsets[b] + i);
+      out.modes4[o] = m;
+      out.modes4[o + 1] = kx;
+      out.modes4[o + 2] = ky;
+      out.modes4[o + 3] = 0;
+      sk += hyp(kx, ky);
+    }
+    out.nmodes[b] = (uint32_t)lst.size();
+    out.kcenter[b] = lst.empty() ? kmin + (b + 0.5) * width : sk / (double)lst.size();
+    if (K <= 0) continue;
+    std::fill(hist.begin(), hist.end(), 0.0);
+    std::fill(hk.begin(), hk.end(), 0.0);
+    for (int m : lst) {
+      int kx = m % NH, kyi = m / NH, ky = kyi < N / 2 ? kyi : kyi - N;
+      for (size_t i = 0; i < ld.size(); i++) {
+        double ax = kx + ld[i];
+        for (size_t j = 0; j < ld.size(); j++) {
+          double k = hyp(ax, ky + ld[j]);
+          double wt = lp[i] * lp[j];
+          double c = std::min((double)(hlen - 1), std::max(0.0, js_round((k - h0) / hres)));
+          hist[(long)c] += wt;
+          hk[(long)c] += wt * k;
+        }
+      }
+    }
+    double wsum = 0;
+    for (double v : hist) wsum += v;
+    if (!(wsum > 0)) {
+      for (int n = 0; n < K; n++) out.knodes[(size_t)b * K + n] = out.kcenter[b];
+      continue;
+    }
+    double acc = 0, sw = 0, swk = 0;
+    int n = 0;
+    for (long c = 0; c < hlen && n < K; c++) {
+      double rem = hist[c];
+      double kc = hist[c] > 0 ? hk[c] / hist[c] : 0;
+      while (rem > 0 && n < K) {
+        double take = std::min(rem, ((double)(n + 1) / K) * wsum - acc);
+        sw += take;
+        swk += take * kc;
+        acc += take;
+        rem -= take;
+        if (acc >= ((double)(n + 1) / K) * wsum * (1 - 1e-12)) {
+          out.knodes[(size_t)b * K + n] = sw > 0 ? swk / sw : kc;
+          n++;
+          sw = 0;
+          swk = 0;
+        }
+      }
+    }
+    for (; n < K; n++) out.knodes[(size_t)b * K + n] = kmax;
+  }
+  return out;
+}
+
+}  // namespace
+
+struct ddm_engine {
+  ddm_geometry g{};
+  int NH = 0, TPAD = 0, logn = 0, logt = 0;
+  long modes = 0;
+  int chunk = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Bins bins;
+  int nitems = 0;
+  std::vector<int> bin_items_h;
+  // device buffers
+  float2* hist = nullptr;
+  float2* scratch = nullptr;
+  int2* rowinfo = nullptr;
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  float* win = nullptr;
+  float2* tw_n = nullptr;
+  float2* tw_t = nullptr;
+  float2* wx = nullptr;
+  int4* items = nullptr;
+  int* bin_modes = nullptr;
+  int* bin_items = nullptr;
+  uint32_t* nmodes = nullptr;
+  uint32_t* order = nullptr;
+  float* mask = nullptr;
+  float* npairs = nullptr;
+  float* S_part = nullptr;
+  float* p_part = nullptr;
+  float* P_part = nullptr;
+  float* radD = nullptr;
+  float* radP = nullptr;
+  bool temporal_busy = false;
+  // pinned staging of the per-analysis metadata (order, 0/1 mask, N(tau));
+  // meta_ev marks when the last upload from it finished
+  uint32_t* meta_order = nullptr;
+  float* meta_mask = nullptr;
+  float* meta_np = nullptr;
+  hipEvent_t meta_ev = nullptr;
+  // profiling
+  bool prof = false;
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+int fail(ddm_engine* e, int code, const std::string& msg) {
+  if (e)
+    e->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+int hip_fail(ddm_engine* e, hipError_t he, const char* what) {
+  return fail(e, DDM_EHIP, std::string(what) + ": " + hipGetErrorString(he));
+}
+
+#define HIPCHK(e, call, what)                          \
+  do {                                                 \
+    hipError_t _he = (call);                           \
+    if (_he != hipSuccess) return hip_fail(e, _he, what); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+hipEvent_t take_event(ddm_engine* e) {
+  if (!e->pool.empty()) {
+    hipEvent_t ev = e->pool.back();
+    e->pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev = nullptr;
+  if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+  return ev;
+}
+
+// Wrap a launch with events when profiling is on.
+template <typename F>
+hipError_t timed(ddm_engine* e, int kind, F&& launch) {
+  if (!e->prof) return launch();
+  hipEvent_t a = take_event(e), b = take_event(e);
+  if (!a || !b) return launch();
+  hipEventRecord(a, e->stream);
+  hipError_t r = launch();
+  hipEventRecord(b, e->stream);
+  e->recs.push_back({kind, a, b});
+  return r;
+}
+
+void free_all(ddm_engine* e) {
+  void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->wx,
+                  e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
+                  e->p_part, e->P_part, e->radD, e->radP};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
+  for (void* p : hptrs)
+    if (p) hipHostFree(p);
+  if (e->meta_ev) hipEventDestroy(e->meta_ev);
+  for (auto& r : e->recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto ev : e->pool) hipEventDestroy(ev);
+  if (e->stream) hipStreamDestroy(e->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ddm_last_error(const ddm_engine* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
+  if (!gp || !out) return fail(nullptr, DDM_EINVAL, "ddm_create: null argument");
+  *out = nullptr;
+  const ddm_geometry g = *gp;
+  if (!is_pow2(g.N) || g.N < 16 || g.N > 2048)
+    return fail(nullptr, DDM_EINVAL, "ddm_create: N must be a power of two in [16, 2048], got " + std::to_string(g.N));
+  if (g.T < 2 || g.T > 4096) return fail(nullptr, DDM_EINVAL, "ddm_create: T must be in [2, 4096]");
+  if (g.L < 1 || g.L > g.T) return fail(nullptr, DDM_EINVAL, "ddm_create: L must be in [1, T]");
+  if (g.NB < 1) return fail(nullptr, DDM_EINVAL, "ddm_create: NB must be >= 1");
+  if (!(g.kmax > g.kmin) || !(g.kmin >= 0)) return fail(nullptr, DDM_EINVAL, "ddm_create: need 0 <= kmin < kmax");
+  if (g.knodes < 0 || g.knodes > 64) return fail(nullptr, DDM_EINVAL, "ddm_create: knodes must be in [0, 64]");
+  ddm_engine* e = new (std::nothrow) ddm_engine();
+  if (!e) return fail(nullptr, DDM_ENOMEM, "ddm_create: out of host memory");
+  e->g = g;
+  e->NH = g.N / 2 + 1;
+  e->modes = (long)g.N * e->NH;
+  e->logn = ilog2(g.N);
+  e->TPAD = std::max(256, 1 << ilog2(2L * g.T));
+  e->logt = ilog2(e->TPAD);
+  auto bail = [&](int code) {
+    g_create_error = e->err;
+    free_all(e);
+    delete e;
+    return code;
+  };
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) return (hip_fail(e, he, "hipStreamCreate"), bail(DDM_EHIP));
+
+  e->bins = make_bins(g.N, g.NB, g.kmin, g.kmax, g.hypot_mode, g.knodes);
+  const long binned = e->bins.offsets[g.NB];
+
+  // Temporal work items: (bin, first mode, count); up to M modes of one bin.
+  const long target_items = 2048;
+  int M = (int)std::min<long>(64, std::max<long>(1, (binned + target_items - 1) / target_items));
+  std::vector<int> items_h, bin_modes_h(std::max<long>(binned, 1));
+  e->bin_items_h.assign(g.NB + 1, 0);
+  for (int b = 0; b < g.NB; b++) {
+    e->bin_items_h[b] = (int)(items_h.size() / 4);
+    for (uint32_t i = e->bins.offsets[b]; i < e->bins.offsets[b + 1]; i += M) {
+      int cnt = (int)std::min<uint32_t>(M, e->bins.offsets[b + 1] - i);
+      items_h.insert(items_h.end(), {b, (int)i, cnt, 0});
+    }
+  }
+  e->nitems = (int)(items_h.size() / 4);
+  e->bin_items_h[g.NB] = e->nitems;
+  for (long i = 0; i < binned; i++) bin_modes_h[i] = e->bins.modes4[4 * i];
+
+  // Spatial chunk: frames whose row spectra (NH*N*8 B each) fit ~32 MB.
+  const int F = ddm::cols_frames_per_wg(e->logn);
+  long per_frame = (long)e->NH * g.N * 8;
+  int chunk = (int)std::max<long>(F, ((32L << 20) / per_frame) / F * F);
+  chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
+  e->chunk = chunk;
+  e->stage_bytes = (size_t)chunk * g.N * g.N * 4;
+
+  const size_t nitems = std::max(e->nitems, 1);
+  if ((he = dalloc(&e->hist, (size_t)e->modes * g.T)) != hipSuccess ||
+      (he = dalloc(&e->scratch, (size_t)chunk * e->NH * g.N)) != hipSuccess ||
+      (he = dalloc(&e->rowinfo, (size_t)chunk * g.N)) != hipSuccess ||
+      (he = hipMalloc(&e->stage, e->stage_bytes)) != hipSuccess || (he = dalloc(&e->win, g.N)) != hipSuccess ||
+      (he = dalloc(&e->tw_n, g.N)) != hipSuccess || (he = dalloc(&e->tw_t, e->TPAD)) != hipSuccess ||
+      (he = dalloc(&e->wx, 4)) != hipSuccess || (he = dalloc(&e->items, nitems)) != hipSuccess ||
+      (he = dalloc(&e->bin_modes, bin_modes_h.size())) != hipSuccess ||
+      (he = dalloc(&e->bin_items, g.NB + 1)) != hipSuccess || (he = dalloc(&e->nmodes, g.NB)) != hipSuccess ||
+      (he = dalloc(&e->order, g.T)) != hipSuccess || (he = dalloc(&e->mask, g.T)) != hipSuccess ||
+      (he = dalloc(&e->npairs, g.L)) != hipSuccess || (he = dalloc(&e->S_part, nitems * e->TPAD)) != hipSuccess ||
+      (he = dalloc(&e->p_part, nitems * g.T)) != hipSuccess || (he = dalloc(&e->P_part, nitems)) != hipSuccess ||
+      (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess) {
+    hip_fail(e, he, "ddm_create: device allocation");
+    return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
+  }
+  if ((he = hipHostMalloc((void**)&e->meta_order, g.T * 4)) != hipSuccess ||
+      (he = hipHostMalloc((void**)&e->meta_mask, g.T * 4)) != hipSuccess ||
+      (he = hipHostMalloc((void**)&e->meta_np, g.L * 4)) != hipSuccess ||
+      (he = hipEventCreateWithFlags(&e->meta_ev, hipEventDisableTiming)) != hipSuccess) {
+    hip_fail(e, he, "ddm_create: pinned host allocation");
+    return bail(DDM_EHIP);
+  }
+
+  std::vector<double> w64 = blackman_harris(g.N);
+  std::vector<float> w32(g.N);
+  for (int i = 0; i < g.N; i++) w32[i] = (float)w64[i];
+  // DFT of the f32 window at kx = 0..3, in FP64: the only columns the row-mean
+  // correction touches (the 4-term window's DFT vanishes for 4 <= kx <= N-4).
+  std::vector<float> wx_h(8);
+  for (int kx = 0; kx < 4; kx++) {
+    double re = 0, im = 0;
+    for (int x = 0; x < g.N; x++) {
+      double a = -2 * kPi * ((long)kx * x % g.N) / g.N;
+      re += (double)w32[x] * std::cos(a);
+      im += (double)w32[x] * std::sin(a);
+    }
+    wx_h[2 * kx] = (float)re;
+    wx_h[2 * kx + 1] = (float)im;
+  }
+  std::vector<float> twn = twiddles(g.N), twt = twiddles(e->TPAD);
+  std::vector<int4> items4(nitems, int4{0, 0, 0, 0});
+  std::memcpy(items4.data(), items_h.data(), items_h.size() * sizeof(int));
+  if ((he = hipMemcpy(e->win, w32.data(), g.N * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->tw_n, twn.data(), twn.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->tw_t, twt.data(), twt.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->wx, wx_h.data(), 32, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->items, items4.data(), items4.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->bin_modes, bin_modes_h.data(), bin_modes_h.size() * 4, hipMemcpyHostToDevice)) !=
+          hipSuccess ||
+      (he = hipMemcpy(e->bin_items, e->bin_items_h.data(), (g.NB + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->nmodes, e->bins.nmodes.data(), g.NB * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemset(e->hist, 0, (size_t)e->modes * g.T * 8)) != hipSuccess ||
+      (he = hipMemset(e->radD, 0, (size_t)g.NB * g.L * 4)) != hipSuccess ||
+      (he = hipMemset(e->radP, 0, (size_t)g.NB * 4)) != hipSuccess) {
+    hip_fail(e, he, "ddm_create: table upload");
+    return bail(DDM_EHIP);
+  }
+  *out = e;
+  return DDM_OK;
+}
+
+void ddm_destroy(ddm_engine* e) {
+  if (!e) return;
+  hipStreamSynchronize(e->stream);
+  free_all(e);
+  delete e;
+}
+
+int ddm_get_sizes(const ddm_engine* e, ddm_sizes* s) {
+  if (!e || !s) return DDM_EINVAL;
+  s->N = e->g.N;
+  s->NH = e->NH;
+  s->T = e->g.T;
+  s->TPAD = e->TPAD;
+  s->L = e->g.L;
+  s->NB = e->g.NB;
+  s->knodes = e->g.knodes;
+  s->modes = e->modes;
+  s->binned_modes = e->bins.offsets[e->g.NB];
+  s->hist_bytes = (int64_t)e->modes * e->g.T * 8;
+  s->work_items = e->nitems;
+  s->chunk_frames = e->chunk;
+  return DDM_OK;
+}
+
+int ddm_get_bins(const ddm_engine* e, uint32_t* off, int32_t* modes4, double* kc, uint32_t* nm, double* kn) {
+  if (!e) return DDM_EINVAL;
+  const Bins& b = e->bins;
+  if (off) std::memcpy(off, b.offsets.data(), b.offsets.size() * 4);
+  if (modes4) std::memcpy(modes4, b.modes4.data(), b.modes4.size() * 4);
+  if (kc) std::memcpy(kc, b.kcenter.data(), b.kcenter.size() * 8);
+  if (nm) std::memcpy(nm, b.nmodes.data(), b.nmodes.size() * 4);
+  if (kn) std::memcpy(kn, b.knodes.data(), b.knodes.size() * 8);
+  return DDM_OK;
+}
+
+static int check_ingest(ddm_engine* e, const void* frames, int32_t dtype, int32_t nframes, int32_t first_slot) {
+  if (!e) return DDM_EINVAL;
+  if (!frames) return fail(e, DDM_EINVAL, "ingest: null frame pointer");
+  if (dtype < DDM_U8 || dtype > DDM_F32) return fail(e, DDM_EINVAL, "ingest: unknown dtype");
+  if (nframes < 0) return fail(e, DDM_EINVAL, "ingest: negative frame count");
+  if (first_slot < 0) return fail(e, DDM_EINVAL, "ingest: negative slot");
+  if (nframes > e->g.T) return fail(e, DDM_EINVAL, "ingest: at most T frames per call (one ring turn)");
+  return DDM_OK;
+}
+
+static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframes, int slot) {
+  const float scale = dtype == DDM_U8 ? 1.0f / 255.0f : dtype == DDM_U16 ? 1.0f / 65535.0f : 1.0f;
+  hipError_t he = timed(e, DDM_K_ROWS, [&] {
+    return ddm::launch_rows(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
+                            e->stream);
+  });
+  if (he != hipSuccess) return hip_fail(e, he, "spatial rows launch");
+  he = timed(e, DDM_K_COLS, [&] {
+    return ddm::launch_cols(e->logn, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
+                            e->hist, e->g.T, slot % e->g.T, e->stream);
+  });
+  if (he != hipSuccess) return hip_fail(e, he, "spatial cols launch");
+  return DDM_OK;
+}
+
+int ddm_ingest(ddm_engine* e, const void* frames, int32_t dtype, int32_t nframes, int32_t first_slot) {
+  int rc = check_ingest(e, frames, dtype, nframes, first_slot);
+  if (rc) return rc;
+  const size_t px = (size_t)e->g.N * e->g.N;
+  const size_t bpp = dtype == DDM_U8 ? 1 : dtype == DDM_U16 ? 2 : 4;
+  for (int f0 = 0; f0 < nframes; f0 += e->chunk) {
+    int nf = std::min(e->chunk, nframes - f0);
+    HIPCHK(e, hipMemcpyAsync(e->stage, (const char*)frames + f0 * px * bpp, nf * px * bpp, hipMemcpyHostToDevice,
+                             e->stream),
+           "ingest: host to device copy");
+    if ((rc = run_spatial(e, e->stage, dtype, nf, first_slot + f0))) return rc;
+  }
+  // the caller may reuse its (pageable) buffer as soon as this returns
+  HIPCHK(e, hipStreamSynchronize(e->stream), "ingest: stream");
+  return DDM_OK;
+}
+
+int ddm_ingest_device(ddm_engine* e, const void* dframes, int32_t dtype, int32_t nframes, int32_t first_slot) {
+  int rc = check_ingest(e, dframes, dtype, nframes, first_slot);
+  if (rc) return rc;
+  const size_t px = (size_t)e->g.N * e->g.N;
+  const size_t bpp = dtype == DDM_U8 ? 1 : dtype == DDM_U16 ? 2 : 4;
+  for (int f0 = 0; f0 < nframes; f0 += e->chunk) {
+    int nf = std::min(e->chunk, nframes - f0);
+    if ((rc = run_spatial(e, (const char*)dframes + f0 * px * bpp, dtype, nf, first_slot + f0))) return rc;
+  }
+  return DDM_OK;
+}
+
+int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask) {
+  if (!e) return DDM_EINVAL;
+  if (!order || !mask) return fail(e, DDM_EINVAL, "computeTemporal: order/mask must have length T");
+  if (e->temporal_busy) return fail(e, DDM_ESTATE, "computeTemporal is already running");
+  e->temporal_busy = true;
+  const int T = e->g.T, L = e->g.L;
+  for (int j = 0; j < T; j++) {
+    if (order[j] >= (uint32_t)T) {
+      e->temporal_busy = false;
+      return fail(e, DDM_EINVAL, "computeTemporal: order[" + std::to_string(j) + "] is not a slot < T");
+    }
+  }
+  // the pinned staging buffers may still feed the previous upload
+  hipError_t he = hipEventSynchronize(e->meta_ev);
+  if (he != hipSuccess) {
+    e->temporal_busy = false;
+    return hip_fail(e, he, "computeTemporal: metadata staging");
+  }
+  bool identity = true;
+  float* m01 = e->meta_mask;
+  for (int j = 0; j < T; j++) {
+    identity = identity && order[j] == (uint32_t)j;
+    e->meta_order[j] = order[j];
+    m01[j] = mask[j] != 0.f ? 1.f : 0.f;
+  }
+  // exact valid-pair counts N(tau) (temporalMeta, src/ddm_ref.js:552-556)
+  float* np = e->meta_np;
+  for (int tau = 0; tau < L; tau++) {
+    int c = 0;
+    for (int t = 0; t + tau < T; t++) c += (m01[t] != 0.f && m01[t + tau] != 0.f);
+    np[tau] = (float)c;
+  }
+  const float n0 = np[0];
+  int rc = DDM_OK;
+  if ((he = hipMemcpyAsync(e->order, e->meta_order, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (he = hipMemcpyAsync(e->mask, m01, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (he = hipMemcpyAsync(e->npairs, np, L * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (he = hipEventRecord(e->meta_ev, e->stream)) != hipSuccess) {
+    rc = hip_fail(e, he, "computeTemporal: metadata upload");
+  } else if (e->nitems > 0 &&
+             (he = timed(e, DDM_K_TEMPORAL, [&] {
+                return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
+                                            identity ? 1 : 0, n0 > 0.5f ? 1.0f / n0 : 0.0f, e->tw_t, e->S_part,
+                                            e->p_part, e->P_part, e->stream);
+              })) != hipSuccess) {
+    rc = hip_fail(e, he, "temporal launch");
+  } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
+                return ddm::launch_finalize(e->logt, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
+                                            e->nmodes, e->mask, e->npairs, T, L, n0, e->tw_t, e->radD, e->radP,
+                                            e->stream);
+              })) != hipSuccess) {
+    rc = hip_fail(e, he, "radial finalize launch");
+  }
+  e->temporal_busy = false;
+  return rc;
+}
+
+int ddm_read_radial(ddm_engine* e, float* D, float* P, float* drift) {
+  if (!e) return DDM_EINVAL;
+  HIPCHK(e, hipStreamSynchronize(e->stream), "readRadial: stream");
+  if (D) HIPCHK(e, hipMemcpy(D, e->radD, (size_t)e->g.NB * e->g.L * 4, hipMemcpyDeviceToHost), "readRadial: D");
+  if (P) HIPCHK(e, hipMemcpy(P, e->radP, (size_t)e->g.NB * 4, hipMemcpyDeviceToHost), "readRadial: P");
+  if (drift) std::memset(drift, 0, (size_t)e->g.NB * e->g.L * 8 * sizeof(float));
+  return DDM_OK;
+}
+
+int ddm_read_history(ddm_engine* e, float* out) {
+  if (!e) return DDM_EINVAL;
+  if (!out) return fail(e, DDM_EINVAL, "readHistory: null output");
+  HIPCHK(e, hipStreamSynchronize(e->stream), "readHistory: stream");
+  HIPCHK(e, hipMemcpy(out, e->hist, (size_t)e->modes * e->g.T * 8, hipMemcpyDeviceToHost), "readHistory: copy");
+  return DDM_OK;
+}
+
+int ddm_idle(ddm_engine* e) {
+  if (!e) return DDM_EINVAL;
+  HIPCHK(e, hipStreamSynchronize(e->stream), "idle");
+  return DDM_OK;
+}
+
+int ddm_history_device(ddm_engine* e, void** dptr) {
+  if (!e || !dptr) return DDM_EINVAL;
+  *dptr = e->hist;
+  return DDM_OK;
+}
+
+int ddm_set_profiling(ddm_engine* e, int32_t enable) {
+  if (!e) return DDM_EINVAL;
+  HIPCHK(e, hipStreamSynchronize(e->stream), "set_profiling: stream");
+  for (auto& r : e->recs) {
+    e->pool.push_back(r.a);
+    e->pool.push_back(r.b);
+  }
+  e->recs.clear();
+  e->prof = enable != 0;
+  return DDM_OK;
+}
+
+int ddm_kernel_times(ddm_engine* e, double* ms, int32_t* count, int32_t n) {
+  if (!e || !ms || !count) return DDM_EINVAL;
+  HIPCHK(e, hipStreamSynchronize(e->stream), "kernel_times: stream");
+  for (int k = 0; k < n; k++) {
+    ms[k] = 0;
+    count[k] = 0;
+  }
+  for (auto& r : e->recs) {
+    float t = 0;
+    HIPCHK(e, hipEventElapsedTime(&t, r.a, r.b), "kernel_times: elapsed");
+    if (r.kind < n) {
+      ms[r.kind] += t;
+      count[r.kind] += 1;
+    }
+  }
+  return DDM_OK;
+}
+
+}  // extern "C"

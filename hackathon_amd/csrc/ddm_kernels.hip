@@ -1,0 +1,592 @@
+// HIP kernels of the DDM hot path for gfx950.
+//
+//  k_rows      frame stack -> row spectra.  Decode u8/u16/f32, exact per-row
+//              mean removal, separable Blackman-Harris window, R2C FFT along x
+//              (two real rows packed into one complex FFT), store kx-major so
+//              the column pass reads contiguous columns.
+//  k_cols      row spectra -> history.  Mean correction on the kx <= 3 columns
+//              (turns the per-row means into the frame mean, see below), FFT
+//              along y, x 1/N, corner turn into hist[(m*T + slot)] with F frames
+//              per workgroup so every store run is F*8 contiguous bytes.
+//  k_temporal  one work item = up to M modes of ONE radial bin.  Per mode:
+//              gather (order, mask), masked-mean removal, p = m|y|^2, zero pad
+//              to TPAD, forward FFT, |Y|^2; p and |Y|^2 are summed over the
+//              item's modes (compensated) and written as a per-item partial.
+//  k_finalize  one workgroup per bin: sum the bin's partials, inverse FFT of the
+//              summed |Y|^2 (= sum of the modes' Re G by linearity), direct
+//              masked sums A, B of the summed p, D = (A + B - 2 Re G)/N(tau)/n.
+//
+// Reference: src/ddm.wgsl (reduce_mean/fft_rows/fft_cols :115-192, temporal
+// :202-259, radial :265-322) and its FP64 twin src/ddm_ref.js:229-288,360-412.
+#include "ddm_internal.h"
+#include "ddm_fft.h"
+
+namespace ddm {
+
+constexpr int NT = 256;
+
+// Real rows per spatial-row workgroup = 2 * rows_pairs_per_wg: about 32 KB of
+// complex LDS, at most N rows.
+__host__ __device__ constexpr int rows_pairs_per_wg(int logn) {
+  return logn >= 11 ? 4 : ((4096 >> logn) < (1 << (logn - 1)) ? ((4096 >> logn) < 64 ? (4096 >> logn) : 64)
+                                                                : (1 << (logn - 1)));
+}
+
+// ----------------------------------------------------------------- helpers
+
+template <typename T> struct vec8b;
+template <> struct vec8b<uint8_t> { static constexpr int n = 8; using type = uint2; };
+template <> struct vec8b<uint16_t> { static constexpr int n = 4; using type = uint2; };
+template <> struct vec8b<float> { static constexpr int n = 2; using type = uint2; };
+
+template <typename T> __device__ __forceinline__ void unpack8(uint2 v, T* out);
+template <> __device__ __forceinline__ void unpack8<uint8_t>(uint2 v, uint8_t* o) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    o[i] = (v.x >> (8 * i)) & 0xff;
+    o[4 + i] = (v.y >> (8 * i)) & 0xff;
+  }
+}
+template <> __device__ __forceinline__ void unpack8<uint16_t>(uint2 v, uint16_t* o) {
+  o[0] = v.x & 0xffff;
+  o[1] = v.x >> 16;
+  o[2] = v.y & 0xffff;
+  o[3] = v.y >> 16;
+}
+template <> __device__ __forceinline__ void unpack8<float>(uint2 v, float* o) {
+  o[0] = __uint_as_float(v.x);
+  o[1] = __uint_as_float(v.y);
+}
+
+// Sum over aligned segments of SEG lanes (SEG a power of two <= 64).
+template <int SEG, typename T> __device__ __forceinline__ T seg_sum(T v) {
+#pragma unroll
+  for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ----------------------------------------------------------------- k_rows
+//
+// Workgroup = RW = 2B consecutive rows of one frame.  Integer pixels: the row
+// sum S_y is exact, and the value fed to the FFT is (I*N - S_y) * (s/N), i.e.
+// the row mean is removed exactly.  f32 pixels: the row mean is removed as an
+// unevaluated (hi, lo) pair, like reduce_mean does per frame (src/ddm.wgsl:129-150).
+// rowinfo[y] keeps (S_y, 0) or (hi_y, lo_y) for the column pass.
+template <int LOGN, int B, typename Tin>
+__global__ __launch_bounds__(NT) void k_rows(const Tin* __restrict__ frames, int nframes, float scale,
+                                             const float* __restrict__ win, const float2* __restrict__ tw_g,
+                                             float2* __restrict__ scratch, int2* __restrict__ rowinfo) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1, RW = 2 * B;
+  constexpr int VEC = vec8b<Tin>::n;              // pixels per 8-byte load
+  constexpr int NV = RW * N / VEC;                // vector loads per workgroup
+  constexpr int LPT = (NV + NT - 1) / NT;         // loads per thread
+  constexpr int SEG = (N / VEC) < 64 ? (N / VEC) : 64;  // lanes per row within a wave
+  constexpr bool IS_INT = !std::is_same<Tin, float>::value;
+  extern __shared__ float2 lds[];
+  float2* buf = lds;                               // B * N
+  float2* tw = lds + B * N;                        // N
+  constexpr int NSEG = (N / VEC) / SEG;           // segment partials per row
+  float* rowsum_f = reinterpret_cast<float*>(tw + N);      // RW (f32 path: hi sums)
+  float* rowsum_lo = rowsum_f + RW;                        // RW (f32 path: residual sums)
+  int* rowsum_i = reinterpret_cast<int*>(rowsum_lo + RW);  // RW (integer path)
+  float* part_f = reinterpret_cast<float*>(rowsum_i + RW); // RW * NSEG
+  int* part_i = reinterpret_cast<int*>(part_f + RW * NSEG);  // RW * NSEG
+  const int tid = threadIdx.x;
+  const int groups = N / RW;
+  const int f = blockIdx.x / groups;
+  const int y0 = (blockIdx.x % groups) * RW;
+  if (f >= nframes) return;
+
+  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  const uint2* src = reinterpret_cast<const uint2*>(frames + ((long)f * N + y0) * N);
+  uint2 raw[LPT];
+#pragma unroll
+  for (int i = 0; i < LPT; i++) {
+    int v = tid + i * NT;
+    raw[i] = (NV % NT == 0 || v < NV) ? src[v] : make_uint2(0, 0);
+  }
+  // Row sums in a fixed order (deterministic): lane segments -> per-segment
+  // partials in LDS -> one thread per row adds its partials.
+#pragma unroll
+  for (int i = 0; i < LPT; i++) {
+    int v = tid + i * NT;
+    Tin px[VEC];
+    unpack8<Tin>(raw[i], px);
+    if constexpr (IS_INT) {
+      int s = 0;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) s += px[q];
+      s = seg_sum<SEG>(s);
+      if ((NV % NT == 0 || v < NV) && (tid % SEG) == 0) part_i[(v / (N / VEC)) * NSEG + (v % (N / VEC)) / SEG] = s;
+    } else {
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) s += px[q];
+      s = seg_sum<SEG>(s);
+      if ((NV % NT == 0 || v < NV) && (tid % SEG) == 0) part_f[(v / (N / VEC)) * NSEG + (v % (N / VEC)) / SEG] = s;
+    }
+  }
+  __syncthreads();
+  if (tid < RW) {
+    if constexpr (IS_INT) {
+      int s = 0;
+      for (int q = 0; q < NSEG; q++) s += part_i[tid * NSEG + q];
+      rowsum_i[tid] = s;
+    } else {
+      float s = 0.f;
+      for (int q = 0; q < NSEG; q++) s += part_f[tid * NSEG + q];
+      rowsum_f[tid] = s;
+    }
+  }
+  __syncthreads();
+  if constexpr (!IS_INT) {
+    // second pass: residual about the first-pass mean (hi, lo pair)
+#pragma unroll
+    for (int i = 0; i < LPT; i++) {
+      int v = tid + i * NT;
+      float px[VEC];
+      unpack8<float>(raw[i], px);
+      int r = (v * VEC) / N;
+      float hi = (NV % NT == 0 || v < NV) ? rowsum_f[r] * (1.0f / N) : 0.f;
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) s += px[q] - hi;
+      s = seg_sum<SEG>(s);
+      if ((NV % NT == 0 || v < NV) && (tid % SEG) == 0) part_f[r * NSEG + (v % (N / VEC)) / SEG] = s;
+    }
+    __syncthreads();
+    if (tid < RW) {
+      float s = 0.f;
+      for (int q = 0; q < NSEG; q++) s += part_f[tid * NSEG + q];
+      rowsum_lo[tid] = s;
+    }
+    __syncthreads();
+  }
+  // decode, subtract the row mean, window, pack rows (2p, 2p+1) as (re, im)
+  float* bufs = reinterpret_cast<float*>(buf);
+#pragma unroll
+  for (int i = 0; i < LPT; i++) {
+    int v = tid + i * NT;
+    if (NV % NT == 0 || v < NV) {
+      Tin px[VEC];
+      unpack8<Tin>(raw[i], px);
+      int e0 = v * VEC;
+      int r = e0 / N;
+      int x0 = e0 % N;
+      float wy = win[y0 + r];
+      float* dst = bufs + 2 * ((r >> 1) * N) + (r & 1);
+#pragma unroll
+      for (int q = 0; q < VEC; q++) {
+        float d;
+        if constexpr (IS_INT) {
+          d = (float)((int)px[q] * N - rowsum_i[r]) * (scale / N);
+        } else {
+          float hi = rowsum_f[r] * (1.0f / N);
+          float lo = rowsum_lo[r] * (1.0f / N);
+          d = ((px[q] - hi) - lo) * scale;
+        }
+        dst[2 * (x0 + q)] = d * win[x0 + q] * wy;
+      }
+    }
+  }
+  if constexpr (IS_INT) {
+    if (tid < RW) rowinfo[(long)f * N + y0 + tid] = make_int2(rowsum_i[tid], 0);
+  } else {
+    if (tid < RW)
+      rowinfo[(long)f * N + y0 + tid] =
+          make_int2(__float_as_int(rowsum_f[tid] * (1.0f / N)), __float_as_int(rowsum_lo[tid] * (1.0f / N)));
+  }
+  __syncthreads();
+  lds_fft<LOGN, B, NT, false>(buf, tw, tid);
+  // untangle the packed real transforms, store kx-major: scratch[(f*NH + kx)*N + y]
+  constexpr int NOUT = NH * RW;
+  for (int e = tid; e < NOUT; e += NT) {
+    int r = e % RW;
+    int kx = e / RW;
+    int p = r >> 1;
+    float2 z = buf[p * N + kx];
+    float2 zc = cconj(buf[p * N + ((N - kx) & (N - 1))]);
+    float2 o;
+    if (r & 1) {
+      float2 u = csub(z, zc);
+      o = make_float2(0.5f * u.y, -0.5f * u.x);
+    } else {
+      o = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    }
+    scratch[((long)f * NH + kx) * N + y0 + r] = o;
+  }
+}
+
+// ----------------------------------------------------------------- k_cols
+//
+// Mean correction.  The rows were centred on their own means c_y; the
+// reference centres the whole frame on its mean mu (src/fft.js:152-160).  The
+// difference (c_y - mu) is constant along each row, so after the row FFT it
+// only touches the columns where the window's DFT Wx(kx) is nonzero, kx <= 3
+// for the 4-term window: row spectrum += (c_y - mu) * s * Wx(kx) * w(y).
+// Integer frames: c_y - mu = (N*S_y - sum_y S_y) / N^2, exact in int64.
+template <int LOGN, int F>
+__global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch, const int2* __restrict__ rowinfo,
+                                             int nframes, int is_int, float scale, const float* __restrict__ win,
+                                             const float2* __restrict__ tw_g, const float2* __restrict__ wx,
+                                             float2* __restrict__ hist, int T, int first_slot) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
+  extern __shared__ float2 lds[];
+  float2* buf = lds;                  // F * N
+  float2* tw = lds + F * N;           // N
+  float* mu = reinterpret_cast<float*>(tw + N);   // F (hi) + F (lo)
+  long long* tot = reinterpret_cast<long long*>(mu + 2 * F);  // F
+  const int tid = threadIdx.x;
+  const int kx = blockIdx.x;
+  const int f0 = blockIdx.y * F;
+  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  for (int e = tid; e < F * N; e += NT) {
+    int f = e / N, y = e % N;
+    buf[e] = (f0 + f < nframes) ? scratch[((long)(f0 + f) * NH + kx) * N + y] : make_float2(0.f, 0.f);
+  }
+  if (kx <= 3) {
+    if (tid < F && f0 + tid < nframes) {
+      const int2* ri = rowinfo + (long)(f0 + tid) * N;
+      if (is_int) {
+        long long s = 0;
+        for (int y = 0; y < N; y++) s += ri[y].x;
+        tot[tid] = s;
+      } else {
+        float sh = 0.f, ch = 0.f, sl = 0.f, cl = 0.f;
+        for (int y = 0; y < N; y++) {
+          comp_add(sh, ch, __int_as_float(ri[y].x));
+          comp_add(sl, cl, __int_as_float(ri[y].y));
+        }
+        mu[tid] = (sh + ch) * (1.0f / N);
+        mu[F + tid] = (sl + cl) * (1.0f / N);
+      }
+    }
+    __syncthreads();
+    float2 w = wx[kx];
+    for (int e = tid; e < F * N; e += NT) {
+      int f = e / N, y = e % N;
+      if (f0 + f >= nframes) continue;
+      int2 ri = rowinfo[(long)(f0 + f) * N + y];
+      float d;
+      if (is_int) {
+        long long num = (long long)N * ri.x - tot[f];
+        d = (float)num * (scale / ((float)N * (float)N));
+      } else {
+        d = ((__int_as_float(ri.x) - mu[f]) + (__int_as_float(ri.y) - mu[F + f])) * scale;
+      }
+      float c = d * win[y];
+      buf[e] = cadd(buf[e], make_float2(c * w.x, c * w.y));
+    }
+  }
+  __syncthreads();
+  lds_fft<LOGN, F, NT, false>(buf, tw, tid);
+  const float s = 1.0f / N;
+  for (int e = tid; e < F * N; e += NT) {
+    int f = e % F, ky = e / F;
+    if (f0 + f >= nframes) continue;
+    int slot = (first_slot + f0 + f) % T;
+    float2 v = buf[f * N + ky];
+    hist[((long)ky * NH + kx) * T + slot] = make_float2(v.x * s, v.y * s);
+  }
+}
+
+// ----------------------------------------------------------------- k_temporal
+
+// Block-wide sum of a complex value (NT threads), result to all threads.
+__device__ __forceinline__ float2 block_sum2(float2 v, float* red, int tid) {
+  v.x = seg_sum<64>(v.x);
+  v.y = seg_sum<64>(v.y);
+  const int w = tid >> 6;
+  if ((tid & 63) == 0) {
+    red[2 * w] = v.x;
+    red[2 * w + 1] = v.y;
+  }
+  __syncthreads();
+  float2 r = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < NT / 64; i++) {
+    r.x += red[2 * i];
+    r.y += red[2 * i + 1];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist, int T, const int4* __restrict__ items,
+                                                 const int* __restrict__ bin_modes, const uint32_t* __restrict__ order,
+                                                 const float* __restrict__ mask, int identity, float inv_n0,
+                                                 const float2* __restrict__ tw_g, float* __restrict__ S_part,
+                                                 float* __restrict__ p_part, float* __restrict__ P_part) {
+  constexpr int n = 1 << LOGT;
+  constexpr int KJ = n / NT;           // frequency bins per thread (n >= 256 enforced by the host: n >= 2T, T >= 2,
+                                       // and the host pads n to at least NT)
+  constexpr int TJ = (n / 2 + NT - 1) / NT;  // time samples per thread (T <= n/2)
+  extern __shared__ float2 lds[];
+  float2* buf = lds;                   // n
+  float2* tw = lds + n;                // n
+  float* red = reinterpret_cast<float*>(tw + n);  // 2 * NT/64
+  const int tid = threadIdx.x;
+  const int4 it = items[blockIdx.x];
+  const int start = it.y, count = it.z;
+  for (int i = tid; i < n; i += NT) tw[i] = tw_g[i];
+
+  float mk[TJ];
+  uint32_t od[TJ];
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    int j = tid + i * NT;
+    mk[i] = (j < T && mask[j] != 0.f) ? 1.f : 0.f;
+    od[i] = (j < T) ? order[j] : 0u;
+  }
+  float ap[TJ], cp[TJ];
+  float aS[KJ], cS[KJ];
+#pragma unroll
+  for (int i = 0; i < TJ; i++) ap[i] = cp[i] = 0.f;
+#pragma unroll
+  for (int i = 0; i < KJ; i++) aS[i] = cS[i] = 0.f;
+  float aP = 0.f, cP = 0.f;
+  __syncthreads();
+
+  for (int mi = 0; mi < count; mi++) {
+    const long mode = bin_modes[start + mi];
+    const float2* x = hist + mode * T;
+    float2 y[TJ];
+#pragma unroll
+    for (int i = 0; i < TJ; i++) {
+      int j = tid + i * NT;
+      y[i] = (j < T) ? x[j] : make_float2(0.f, 0.f);
+    }
+    if (!identity) {
+#pragma unroll
+      for (int i = 0; i < TJ; i++) {
+        int j = tid + i * NT;
+        if (j < T) buf[j] = y[i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < TJ; i++) {
+        int j = tid + i * NT;
+        y[i] = (j < T) ? buf[od[i]] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+    }
+    float2 sy = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < TJ; i++) {
+      y[i] = make_float2(y[i].x * mk[i], y[i].y * mk[i]);
+      sy = cadd(sy, y[i]);
+      comp_add(aP, cP, y[i].x * y[i].x + y[i].y * y[i].y);
+    }
+    float2 mean = block_sum2(sy, red, tid);
+    mean.x *= inv_n0;
+    mean.y *= inv_n0;
+#pragma unroll
+    for (int i = 0; i < TJ; i++) {
+      int j = tid + i * NT;
+      float2 v = make_float2((y[i].x - mean.x) * mk[i], (y[i].y - mean.y) * mk[i]);
+      comp_add(ap[i], cp[i], v.x * v.x + v.y * v.y);
+      if (j < n / 2) buf[j] = v;
+    }
+    for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
+    __syncthreads();
+    lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
+#pragma unroll
+    for (int i = 0; i < KJ; i++) {
+      float2 v = buf[tid + i * NT];
+      comp_add(aS[i], cS[i], v.x * v.x + v.y * v.y);
+    }
+    __syncthreads();
+  }
+  float* Sp = S_part + (long)blockIdx.x * n;
+#pragma unroll
+  for (int i = 0; i < KJ; i++) Sp[tid + i * NT] = aS[i] + cS[i];
+  float* pp = p_part + (long)blockIdx.x * T;
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    int j = tid + i * NT;
+    if (j < T) pp[j] = ap[i] + cp[i];
+  }
+  float2 tp = block_sum2(make_float2(aP + cP, 0.f), red, tid);
+  if (tid == 0) P_part[blockIdx.x] = tp.x;
+}
+
+// ----------------------------------------------------------------- k_finalize
+
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_finalize(const float* __restrict__ S_part, const float* __restrict__ p_part,
+                                                 const float* __restrict__ P_part, const int* __restrict__ bin_items,
+                                                 const uint32_t* __restrict__ nmodes, const float* __restrict__ mask01,
+                                                 const float* __restrict__ npairs, int T, int L, float n0,
+                                                 const float2* __restrict__ tw_g, float* __restrict__ radD,
+                                                 float* __restrict__ radP) {
+  constexpr int n = 1 << LOGT;
+  extern __shared__ float2 lds[];
+  float2* buf = lds;                                  // n
+  float2* tw = lds + n;                               // n
+  float* p = reinterpret_cast<float*>(tw + n);        // n/2 (T <= n/2)
+  float* red = p + n / 2;                             // 2*NT/64
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  const int i0 = bin_items[b], i1 = bin_items[b + 1];
+  for (int i = tid; i < n; i += NT) {
+    tw[i] = tw_g[i];
+    float s = 0.f, c = 0.f;
+    for (int it = i0; it < i1; it++) comp_add(s, c, S_part[(long)it * n + i]);
+    buf[i] = make_float2(s + c, 0.f);
+  }
+  for (int t = tid; t < T; t += NT) {
+    float s = 0.f, c = 0.f;
+    for (int it = i0; it < i1; it++) comp_add(s, c, p_part[(long)it * T + t]);
+    p[t] = s + c;
+  }
+  float ps = 0.f, pc = 0.f;
+  for (int it = i0 + tid; it < i1; it += NT) comp_add(ps, pc, P_part[it]);
+  float2 Ptot = block_sum2(make_float2(ps + pc, 0.f), red, tid);  // has barriers
+  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
+  const uint32_t nm = nmodes[b];
+  const float invn = 1.0f / n;
+  for (int tau = tid; tau < L; tau += NT) {
+    float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
+    for (int t = 0; t + tau < T; t++) {
+      comp_add(sa, ca, mask01[t] * p[t + tau]);
+      comp_add(sb, cb, p[t] * mask01[t + tau]);
+    }
+    float np = npairs[tau];
+    float g = buf[tau].x * invn;
+    float d = 0.f;
+    if (np > 0.5f && nm > 0) d = (((sa + ca) + (sb + cb)) - 2.0f * g) / np / (float)nm;
+    radD[(long)b * L + tau] = d;
+  }
+  if (tid == 0) radP[b] = (nm > 0 && n0 > 0.5f) ? Ptot.x / n0 / (float)nm : 0.f;
+}
+
+// ----------------------------------------------------------------- launchers
+
+template <typename Tin, int LOGN>
+static hipError_t launch_rows_t(const void* frames, int nframes, float scale, const float* win, const float2* tw,
+                                float2* scratch, int2* rowinfo, hipStream_t st) {
+  constexpr int N = 1 << LOGN;
+  constexpr int B = rows_pairs_per_wg(LOGN);
+  constexpr int RW = 2 * B;
+  static_assert(RW <= N, "rows per workgroup exceed N");
+  constexpr int VEC = vec8b<Tin>::n;
+  constexpr int SEG = (N / VEC) < 64 ? (N / VEC) : 64;
+  constexpr int NSEG = (N / VEC) / SEG;
+  size_t lds = sizeof(float2) * (B * N + N) + sizeof(float) * 2 * RW + sizeof(int) * RW +
+               (sizeof(float) + sizeof(int)) * RW * NSEG;
+  auto kern = k_rows<LOGN, B, Tin>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(nframes * (N / RW));
+  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, (const Tin*)frames, nframes, scale, win, tw, scratch, rowinfo);
+  return hipGetLastError();
+}
+
+template <typename Tin>
+static hipError_t launch_rows_dt(int logn, const void* frames, int nframes, float scale, const float* win,
+                                 const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
+  switch (logn) {
+#define CASE(L) \
+  case L: return launch_rows_t<Tin, L>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
+    CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rows(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
+                       const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
+  if (dtype == 0) return launch_rows_dt<uint8_t>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  if (dtype == 1) return launch_rows_dt<uint16_t>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  return launch_rows_dt<float>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+}
+
+int cols_frames_per_wg(int logn) { return logn >= 11 ? 4 : (logn >= 10 ? 8 : 16); }
+
+template <int LOGN>
+static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+                                const float* win, const float2* tw, const float2* wx, float2* hist, int T,
+                                int first_slot, hipStream_t st) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
+  constexpr int F = (LOGN >= 11) ? 4 : (LOGN >= 10 ? 8 : 16);
+  size_t lds = sizeof(float2) * (F * N + N) + sizeof(float) * 2 * F + sizeof(long long) * F;
+  auto kern = k_cols<LOGN, F>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(NH, (nframes + F - 1) / F);
+  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T,
+                     first_slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+                       const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
+                       hipStream_t st) {
+  switch (logn) {
+#define CASE(L) \
+  case L: return launch_cols_t<L>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+    CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int LOGT>
+static hipError_t launch_temporal_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+                                    const uint32_t* order, const float* mask, int identity, float inv_n0,
+                                    const float2* tw, float* S_part, float* p_part, float* P_part, hipStream_t st) {
+  constexpr int n = 1 << LOGT;
+  size_t lds = sizeof(float2) * 2 * n + sizeof(float) * 2 * (NT / 64);
+  auto kern = k_temporal<LOGT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
+                     tw, S_part, p_part, P_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+                           const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
+                           float* S_part, float* p_part, float* P_part, hipStream_t st) {
+  switch (logt) {
+#define CASE(L)                                                                                                    \
+  case L:                                                                                                          \
+    return launch_temporal_t<L>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
+                                P_part, st);
+    CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int LOGT>
+static hipError_t launch_finalize_t(const float* S_part, const float* p_part, const float* P_part, const int* bin_items,
+                                    int NB, const uint32_t* nmodes, const float* mask, const float* npairs, int T,
+                                    int L, float n0, const float2* tw, float* radD, float* radP, hipStream_t st) {
+  constexpr int n = 1 << LOGT;
+  size_t lds = sizeof(float2) * 2 * n + sizeof(float) * (n / 2) + sizeof(float) * 2 * (NT / 64);
+  auto kern = k_finalize<LOGT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, S_part, p_part, P_part, bin_items, nmodes, mask, npairs, T, L,
+                     n0, tw, radD, radP);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, const float* P_part,
+                           const int* bin_items, int NB, const uint32_t* nmodes, const float* mask,
+                           const float* npairs, int T, int L, float n0, const float2* tw, float* radD, float* radP,
+                           hipStream_t st) {
+  switch (logt) {
+#define CASE(L)                                                                                                       \
+  case L:                                                                                                             \
+    return launch_finalize_t<L>(S_part, p_part, P_part, bin_items, NB, nmodes, mask, npairs, T, L, n0, tw, radD, radP, \
+                                st);
+    CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ddm

@@ -1,0 +1,142 @@
+/*
+ * ddm.h -- C ABI of the MI355X differential dynamic microscopy engine (libddm.so).
+ *
+ * This is the drop-in boundary for the motility-ddm hot path: a frame stack goes
+ * in; the half-plane spectral history, the bin-averaged structure function
+ * D(q, tau) and static power P(q) come out.  It replaces the WebGPU engine class
+ * `DdmEngine` (motility-ddm src/ddm-engine.ts:62-426) and the host-side tables it
+ * takes from src/ddm_ref.js.  The N-API addon (hackathon_amd/napi/ddm_napi.c)
+ * binds each entry point 1:1 for the TypeScript facade; Python binds it with
+ * ctypes (hackathon_amd/engine.py).
+ *
+ * Conventions
+ *   - Every call returns an int status: DDM_OK (0), DDM_DROPPED (1, never
+ *     returned by this engine: ingest is stream-ordered and does not drop), or
+ *     a negative error whose text ddm_last_error() returns.  The reference
+ *     throws `Error(message)` in the same situations (src/ddm-engine.ts:126-163,
+ *     :313, :332; src/ddm_ref.js:540).
+ *   - No C++ exception crosses the ABI.
+ *   - Host buffers are owned by the caller; device memory is owned by the
+ *     engine, except the device frame stack passed to ddm_ingest_device.
+ *   - One engine = one HIP stream.  An engine is not thread-safe (as the
+ *     reference engine, single JS thread).
+ *   - Layouts are the reference's: mode m = ky*NH + kx with NH = N/2+1,
+ *     history hist[(m*T + slot)*2 + {re,im}] (src/ddm_ref.js:175-205),
+ *     radial D[b*L + tau] and P[b] (src/ddm_ref.js:336-348).
+ */
+#ifndef DDM_H
+#define DDM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ddm_engine ddm_engine;
+
+enum {
+  DDM_OK = 0,
+  DDM_DROPPED = 1,
+  DDM_EINVAL = -1,   /* bad argument or geometry */
+  DDM_EHIP = -2,     /* HIP runtime error (message has the HIP error string) */
+  DDM_ESTATE = -3,   /* call not valid in the engine's state (e.g. re-entry) */
+  DDM_ENOMEM = -4    /* device allocation failed */
+};
+
+/* Pixel types of a frame stack.  Integer frames are scaled to [0, 1]:
+ * u8 by 1/255 (what the reference's luma path does, test/sim.mjs:188-193),
+ * u16 by 1/65535; f32 frames are taken as luma already (ingestLuma). */
+enum { DDM_U8 = 0, DDM_U16 = 1, DDM_F32 = 2 };
+
+/* Geometry; the reference fixes it (DDM, src/ddm_ref.js:18-31: N=256, T=256,
+ * L=128, NB=32, kmin=4, kmax=100, KNODES=4).  Here every field is a runtime
+ * parameter.  N: power of two in [16, 2048].  T: ring slots, 2..4096 (the
+ * temporal FFT length is the next power of two >= 2T).  L: lags 1..T.
+ * NB >= 1 bins linear in |k| on [kmin, kmax).  hypot_mode 0 bins by V8's
+ * Math.hypot (bit-exact with the reference), 1 by the correctly rounded
+ * hypot.  knodes: effective-|k| quadrature nodes per bin (reference: 4). */
+typedef struct ddm_geometry {
+  int32_t N, T, L, NB;
+  double kmin, kmax;
+  int32_t hypot_mode;
+  int32_t knodes;
+} ddm_geometry;
+
+/* Derived sizes of an engine (bytes where named so). */
+typedef struct ddm_sizes {
+  int32_t N, NH, T, TPAD, L, NB, knodes;
+  int64_t modes;          /* N * NH */
+  int64_t binned_modes;   /* modes that fall in a bin */
+  int64_t hist_bytes;     /* modes * T * 8 */
+  int32_t work_items;     /* temporal work items (bin, mode range) */
+  int32_t chunk_frames;   /* frames per spatial chunk */
+} ddm_sizes;
+
+/* DdmEngine.create (src/ddm-engine.ts:125-149): allocate device buffers for the
+ * geometry, build the bin tables (makeBins, src/ddm_ref.js:84-170) and the
+ * twiddle / window tables on the host in FP64 (src/ddm-engine.ts:193-197). */
+int ddm_create(const ddm_geometry *g, ddm_engine **out);
+
+/* DdmEngine.destroy (src/ddm-engine.ts:421-425).  NULL is ignored. */
+void ddm_destroy(ddm_engine *e);
+
+/* Message of the last failed call on `e` (or of the last failed ddm_create
+ * when e is NULL).  Valid until the next call on the same engine. */
+const char *ddm_last_error(const ddm_engine *e);
+
+int ddm_get_sizes(const ddm_engine *e, ddm_sizes *s);
+
+/* Bin tables of makeBins (src/ddm_ref.js:84-170): binOffsets[NB+1],
+ * binModes[4*binned_modes] = (mode, kx, ky, 0), kCenter[NB], nModes[NB],
+ * kNodes[NB*knodes].  Any pointer may be NULL. */
+int ddm_get_bins(const ddm_engine *e, uint32_t *binOffsets, int32_t *binModes, double *kCenter, uint32_t *nModes,
+                 double *kNodes);
+
+/* Frame-stack ingest from HOST memory: nframes frames of N*N pixels (row-major,
+ * y*N + x) of `dtype`, transformed (mean removal, separable Blackman-Harris
+ * window, R2C 2D FFT, x 1/N) into ring slots (first_slot + i) mod T.
+ * Batched form of ingestLuma / History.ingest (src/ddm-engine.ts:308-324,
+ * src/ddm_ref.js:196-205); ingestLuma is nframes = 1, dtype = DDM_F32. */
+int ddm_ingest(ddm_engine *e, const void *frames, int32_t dtype, int32_t nframes, int32_t first_slot);
+
+/* Same, with the frame stack already resident in device memory (HBM).  The
+ * call is asynchronous on the engine's stream; `dframes` must stay valid until
+ * the next ddm_idle / read. */
+int ddm_ingest_device(ddm_engine *e, const void *dframes, int32_t dtype, int32_t nframes, int32_t first_slot);
+
+/* computeTemporal (src/ddm-engine.ts:331-365): masked Wiener-Khinchin
+ * structure function of every binned mode and its radial reduction.
+ * order[j] = ring slot holding time index j, mask[j] != 0 if valid (length T).
+ * Asynchronous; read results with ddm_read_radial. */
+int ddm_compute_temporal(ddm_engine *e, const uint32_t *order, const float *mask);
+
+/* readRadial (src/ddm-engine.ts:391-395, radialFromGpu src/ddm_ref.js:584-590):
+ * D[NB*L], P[NB]; `drift` (NB*L*8, may be NULL) is zero-filled: the phase-drift
+ * sums are not computed by this engine. */
+int ddm_read_radial(ddm_engine *e, float *D, float *P, float *drift);
+
+/* readHistory (src/ddm-engine.ts:397-400): hist[(m*T + slot)*2 + {re,im}],
+ * modes*T*2 floats. */
+int ddm_read_history(ddm_engine *e, float *out);
+
+/* idle (src/ddm-engine.ts:417-419): wait for all work on the engine's stream. */
+int ddm_idle(ddm_engine *e);
+
+/* Device pointer of the spectral history (modes*T complex f32), for callers
+ * that keep data on the GPU. */
+int ddm_history_device(ddm_engine *e, void **dptr);
+
+/* Per-kernel timing with HIP events on the engine's stream.  enable != 0 starts
+ * recording (and clears previous records).  ddm_kernel_times waits for the
+ * stream and returns, for each kernel kind k (0 spatial rows, 1 spatial cols /
+ * corner turn, 2 temporal, 3 radial finalize), the summed milliseconds and the
+ * launch count. */
+enum { DDM_K_ROWS = 0, DDM_K_COLS = 1, DDM_K_TEMPORAL = 2, DDM_K_FINALIZE = 3, DDM_K_COUNT = 4 };
+int ddm_set_profiling(ddm_engine *e, int32_t enable);
+int ddm_kernel_times(ddm_engine *e, double *ms, int32_t *count, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

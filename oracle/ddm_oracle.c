@@ -1,0 +1,784 @@
+/*
+ * ddm_oracle.c -- FP64 CPU restatement of the motility-ddm hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP engine is held
+ * to; it is never linked into, loaded by, or called from the product library
+ * (hackathon_amd/libddm.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.
+ *
+ * Every function names the reference lines it restates (paths relative to
+ * motility-ddm/ in mehrinsoha77/Hackathon).  Generalisations beyond the
+ * reference (any power-of-two N, non power-of-two T via padding to the next
+ * power of two >= 2T, any L <= T) are marked as such; they reduce to the
+ * reference exactly at the reference geometry.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle.py):
+ *   orc_fft, orc_dft            direct DFT to 1e-12 (src test t_fft.mjs:23-40), numpy.fft
+ *   orc_blackman_harris         w(0)=6e-5, w(N/2)=1, symmetry (t_fft.mjs:71-77)
+ *   orc_spectrum2d_half         direct 2D DFT (t_fft.mjs:42-69), numpy.fft.rfft2
+ *   orc_temporal_mode(_direct)  WK vs direct sums with masks/shuffle, phase rotation,
+ *                               D(0)=0 (t_fft.mjs:79-137)
+ *   orc_v8_hypot                node's Math.hypot on 2.1M integer pairs (fixture)
+ *   orc_make_bins               brute-force integer enumeration (indices, counts);
+ *                               kCenter/kNodes against vectors written by importing
+ *                               src/ddm_ref.js (tests/golden/make_golden.mjs)
+ *   special functions           reference KAT values (t_kernels.mjs:39-51), scipy
+ *   isf3d / isf2d               Simpson quadrature over the Schulz pdf (t_kernels.mjs:60-72)
+ *   orc_isf2d_fast              |table - exact| < 2e-5 (t_kernels.mjs:85-102); exact table
+ *                               values: parity unpinned (motility_fit.js does not parse
+ *                               on this Node, so it cannot be imported)
+ *   orc_nnls3                   KKT conditions (t_kernels.mjs:107-134)
+ *   orc_lm_fit                  noise-free recovery, closed-form Brownian data
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off; contraction would change
+ * V8-hypot bin edges).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------ window */
+
+/* 4-term periodic Blackman-Harris; src/fft.js:126-133 */
+void orc_blackman_harris(int n, double *w) {
+  for (int i = 0; i < n; i++) {
+    double x = (2 * ORC_PI * i) / n;
+    w[i] = 0.35875 - 0.48829 * cos(x) + 0.14128 * cos(2 * x) - 0.01168 * cos(3 * x);
+  }
+}
+
+/* ------------------------------------------------------------------ FFTs */
+
+static int is_pow2(long n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+/* Radix-2 Stockham autosort, natural order, unnormalised both ways;
+ * src/fft.js:59-93 (twiddles :18-29).  Interleaved re/im, length 2n. */
+int orc_fft(int n, double *data, int inverse) {
+  if (!is_pow2(n)) return -1;
+  if (n == 1) return 0;
+  int half = n >> 1;
+  double sgn = inverse ? -1.0 : 1.0;
+  double *tw = (double *)malloc(sizeof(double) * n);
+  double *a = (double *)malloc(sizeof(double) * 2 * n);
+  double *b = (double *)malloc(sizeof(double) * 2 * n);
+  for (int m = 0; m < half; m++) {
+    double ang = (-2 * ORC_PI * m) / n;
+    tw[2 * m] = cos(ang);
+    tw[2 * m + 1] = sin(ang);
+  }
+  memcpy(a, data, sizeof(double) * 2 * n);
+  double *src = a, *dst = b;
+  for (int Ns = 1; Ns < n; Ns <<= 1) {
+    int step = half / Ns;
+    for (int j = 0; j < half; j++) {
+      int k = j % Ns;
+      double wr = tw[2 * k * step];
+      double wi = sgn * tw[2 * k * step + 1];
+      int i0 = 2 * j, i1 = 2 * (j + half);
+      double xr = src[i1] * wr - src[i1 + 1] * wi;
+      double xi = src[i1] * wi + src[i1 + 1] * wr;
+      int o = 2 * ((j - k) * 2 + k);
+      int o2 = o + 2 * Ns;
+      dst[o] = src[i0] + xr;
+      dst[o + 1] = src[i0 + 1] + xi;
+      dst[o2] = src[i0] - xr;
+      dst[o2 + 1] = src[i0 + 1] - xi;
+    }
+    double *t = src;
+    src = dst;
+    dst = t;
+  }
+  memcpy(data, src, sizeof(double) * 2 * n);
+  free(tw);
+  free(a);
+  free(b);
+  return 0;
+}
+
+/* Direct O(n^2) DFT; src/fft.js:101-119.  Works for any n (used for the
+ * radix-3 geometry N=384, which the reference's fft() rejects). */
+void orc_dft(int n, const double *in, double *out, int inverse) {
+  double sgn = inverse ? 1.0 : -1.0;
+  for (int k = 0; k < n; k++) {
+    double sr = 0, si = 0;
+    for (int t = 0; t < n; t++) {
+      double ang = (sgn * 2 * ORC_PI * (double)(((long)k * t) % n)) / n;
+      double c = cos(ang), s = sin(ang);
+      sr += in[2 * t] * c - in[2 * t + 1] * s;
+      si += in[2 * t] * s + in[2 * t + 1] * c;
+    }
+    out[2 * k] = sr;
+    out[2 * k + 1] = si;
+  }
+}
+
+/* FFT for any n: Stockham when n is a power of two, direct DFT otherwise. */
+static void fft_any(int n, double *data, int inverse) {
+  if (is_pow2(n)) {
+    orc_fft(n, data, inverse);
+  } else {
+    double *tmp = (double *)malloc(sizeof(double) * 2 * n);
+    orc_dft(n, data, tmp, inverse);
+    memcpy(data, tmp, sizeof(double) * 2 * n);
+    free(tmp);
+  }
+}
+
+/* Half-plane 2D spectrum: mean -> window -> rows -> cols -> x 1/N;
+ * mode m = ky*NH + kx; src/fft.js:149-183. */
+int orc_spectrum2d_half(const double *img, int N, const double *win, double *out) {
+  int NH = (N >> 1) + 1;
+  double mean = 0;
+  for (long i = 0; i < (long)N * N; i++) mean += img[i];
+  mean /= (double)N * N;
+  double *row = (double *)malloc(sizeof(double) * 2 * N);
+  double *rows = (double *)malloc(sizeof(double) * 2 * (size_t)N * NH);
+  for (int y = 0; y < N; y++) {
+    double wy = win[y];
+    for (int x = 0; x < N; x++) {
+      row[2 * x] = (img[(long)y * N + x] - mean) * win[x] * wy;
+      row[2 * x + 1] = 0;
+    }
+    fft_any(N, row, 0);
+    for (int kx = 0; kx < NH; kx++) {
+      rows[2 * ((long)y * NH + kx)] = row[2 * kx];
+      rows[2 * ((long)y * NH + kx) + 1] = row[2 * kx + 1];
+    }
+  }
+  double s = 1.0 / N;
+  for (int kx = 0; kx < NH; kx++) {
+    for (int y = 0; y < N; y++) {
+      row[2 * y] = rows[2 * ((long)y * NH + kx)];
+      row[2 * y + 1] = rows[2 * ((long)y * NH + kx) + 1];
+    }
+    fft_any(N, row, 0);
+    for (int ky = 0; ky < N; ky++) {
+      out[2 * ((long)ky * NH + kx)] = row[2 * ky] * s;
+      out[2 * ((long)ky * NH + kx) + 1] = row[2 * ky + 1] * s;
+    }
+  }
+  free(row);
+  free(rows);
+  return 0;
+}
+
+/* History.ingest: spectrum of one frame into ring slot `slot`,
+ * hist[(m*T + slot)*2 + {0,1}]; src/ddm_ref.js:196-205. */
+void orc_history_ingest(double *hist, int N, int T, const double *img, const double *win, int slot) {
+  int NH = (N >> 1) + 1;
+  long modes = (long)N * NH;
+  double *spec = (double *)malloc(sizeof(double) * 2 * modes);
+  orc_spectrum2d_half(img, N, win, spec);
+  for (long m = 0; m < modes; m++) {
+    long o = (m * T + slot) * 2;
+    hist[o] = spec[2 * m];
+    hist[o + 1] = spec[2 * m + 1];
+  }
+  free(spec);
+}
+
+/* Batch form used by tests and the CPU baseline: `frames` are nframes
+ * integer/float frames (dtype 0=u8, 1=u16, 2=f32) scaled by `scale`, stored
+ * into consecutive slots (first_slot + i) mod T. */
+void orc_history_ingest_stack(double *hist, int N, int T, const void *frames, int dtype, double scale,
+                              int nframes, int first_slot) {
+  long npx = (long)N * N;
+  double *img = (double *)malloc(sizeof(double) * npx);
+  double *win = (double *)malloc(sizeof(double) * N);
+  orc_blackman_harris(N, win);
+  for (int f = 0; f < nframes; f++) {
+    for (long i = 0; i < npx; i++) {
+      double v;
+      if (dtype == 0) v = ((const uint8_t *)frames)[f * npx + i];
+      else if (dtype == 1) v = ((const uint16_t *)frames)[f * npx + i];
+      else v = ((const float *)frames)[f * npx + i];
+      img[i] = v * scale;
+    }
+    orc_history_ingest(hist, N, T, img, win, (first_slot + f) % T);
+  }
+  free(img);
+  free(win);
+}
+
+/* ------------------------------------------------------------------ temporal */
+
+static int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+/* Masked Wiener-Khinchin structure function of one mode;
+ * src/ddm_ref.js:229-288.  The reference pads to n = 2T (T a power of two);
+ * here n = next power of two >= 2T, which gives the same linear correlation
+ * for every T.  Returns P (static power). */
+double orc_temporal_mode(const double *hist, long mode, int T, const uint32_t *order, const float *mask, int L,
+                         double *outD, double *outC) {
+  int n = next_pow2(2 * T);
+  double *z = (double *)calloc(2 * n, sizeof(double));
+  double *y = (double *)calloc(2 * n, sizeof(double));
+  double *S = (double *)calloc(2 * n, sizeof(double));
+  long base = mode * T * 2;
+  for (int j = 0; j < T; j++) {
+    if (!mask[j]) continue;
+    uint32_t s = order[j];
+    double xr = hist[base + 2 * s], xi = hist[base + 2 * s + 1];
+    z[2 * j] = 1;
+    z[2 * j + 1] = xr * xr + xi * xi;
+    y[2 * j] = xr;
+    y[2 * j + 1] = xi;
+  }
+  orc_fft(n, z, 0);
+  for (int k = 0; k < n; k++) {
+    int kk = (n - k) % n;
+    double zr = z[2 * k], zi = z[2 * k + 1];
+    double cr = z[2 * kk], ci = -z[2 * kk + 1];
+    double Mr = 0.5 * (zr + cr), Mi = 0.5 * (zi + ci);
+    double dr = zr - cr, di = zi - ci;
+    double Pr = 0.5 * di, Pi = -0.5 * dr;
+    S[2 * k] = Mr * Pr + Mi * Pi;
+    S[2 * k + 1] = Mr * Pi - Mi * Pr + (Mr * Mr + Mi * Mi);
+  }
+  orc_fft(n, S, 1);
+  orc_fft(n, y, 0);
+  for (int k = 0; k < n; k++) {
+    double yr = y[2 * k], yi = y[2 * k + 1];
+    y[2 * k] = yr * yr + yi * yi;
+    y[2 * k + 1] = 0;
+  }
+  orc_fft(n, y, 1);
+  double inv = 1.0 / n;
+  double s1 = S[1] * inv;
+  double P = S[0] * inv / (s1 > 1e-300 ? s1 : 1e-300);
+  for (int tau = 0; tau < L; tau++) {
+    double A = S[2 * tau] * inv;
+    double B = S[2 * ((n - tau) % n)] * inv;
+    double Np = round(S[2 * tau + 1] * inv); /* Math.round; the argument is never a half-integer */
+    if (Np > 0) {
+      double Gr = y[2 * tau] * inv, Gi = y[2 * tau + 1] * inv;
+      outD[tau] = (A + B - 2 * Gr) / Np;
+      if (outC) {
+        outC[2 * tau] = Gr / Np;
+        outC[2 * tau + 1] = Gi / Np;
+      }
+    } else {
+      outD[tau] = 0;
+      if (outC) outC[2 * tau] = outC[2 * tau + 1] = 0;
+    }
+  }
+  free(z);
+  free(y);
+  free(S);
+  return s1 > 0.5 ? P : 0;
+}
+
+/* Direct O(T*L) masked sums; src/ddm_ref.js:306-333. */
+double orc_temporal_mode_direct(const double *hist, long mode, int T, const uint32_t *order, const float *mask, int L,
+                                double *outD, double *outC) {
+  long base = mode * T * 2;
+  double *xr = (double *)calloc(T, sizeof(double));
+  double *xi = (double *)calloc(T, sizeof(double));
+  double pSum = 0, nSum = 0;
+  for (int j = 0; j < T; j++) {
+    if (!mask[j]) continue;
+    xr[j] = hist[base + 2 * order[j]];
+    xi[j] = hist[base + 2 * order[j] + 1];
+    pSum += xr[j] * xr[j] + xi[j] * xi[j];
+    nSum++;
+  }
+  for (int tau = 0; tau < L; tau++) {
+    double d = 0, gr = 0, gi = 0, cnt = 0;
+    for (int t = 0; t + tau < T; t++) {
+      if (!mask[t] || !mask[t + tau]) continue;
+      double dr = xr[t + tau] - xr[t], di = xi[t + tau] - xi[t];
+      d += dr * dr + di * di;
+      gr += xr[t] * xr[t + tau] + xi[t] * xi[t + tau];
+      gi += xr[t] * xi[t + tau] - xi[t] * xr[t + tau];
+      cnt++;
+    }
+    outD[tau] = cnt ? d / cnt : 0;
+    if (outC) {
+      outC[2 * tau] = cnt ? gr / cnt : 0;
+      outC[2 * tau + 1] = cnt ? gi / cnt : 0;
+    }
+  }
+  free(xr);
+  free(xi);
+  return nSum ? pSum / nSum : 0;
+}
+
+/* Per-analysis metadata: exact pair counts N(tau) and the FP64 FFT of the
+ * zero-padded mask rounded to f32; src/ddm_ref.js:538-560 (generalised T, L). */
+void orc_temporal_meta(int T, int L, const uint32_t *order, const float *mask, float *npairs, float *mspec) {
+  (void)order;
+  int n = next_pow2(2 * T);
+  for (int tau = 0; tau < L; tau++) {
+    int c = 0;
+    for (int t = 0; t + tau < T; t++)
+      if (mask[t] && mask[t + tau]) c++;
+    npairs[tau] = (float)c;
+  }
+  if (mspec) {
+    double *z = (double *)calloc(2 * n, sizeof(double));
+    for (int j = 0; j < T; j++) z[2 * j] = mask[j] ? 1 : 0;
+    orc_fft(n, z, 0);
+    for (int i = 0; i < 2 * n; i++) mspec[i] = (float)z[i];
+    free(z);
+  }
+}
+
+/* ------------------------------------------------------------------ bins */
+
+/* V8's Math.hypot for two arguments: scale by the max, Kahan-sum the squared
+ * ratios in argument order, sqrt * max.  Not correctly rounded; reproducing it
+ * is what makes the reference's bin edges bit-exact (src/ddm_ref.js:93). */
+double orc_v8_hypot(double a, double b) {
+  double aa = fabs(a), ab = fabs(b);
+  if (isinf(aa) || isinf(ab)) return INFINITY;
+  if (isnan(aa) || isnan(ab)) return NAN;
+  double mx = aa > ab ? aa : ab;
+  if (mx == 0) return 0;
+  double sum = 0, comp = 0;
+  double v[2] = {aa, ab};
+  for (int i = 0; i < 2; i++) {
+    double r = v[i] / mx;
+    double summand = r * r - comp;
+    double prelim = sum + summand;
+    comp = (prelim - sum) - summand;
+    sum = prelim;
+  }
+  return sqrt(sum) * mx;
+}
+
+/* Math.round: nearest integer, ties toward +infinity. */
+static double js_round(double x) {
+  double r = floor(x);
+  if (x - r >= 0.5) r += 1;
+  return r;
+}
+
+/* windowLeakage: |W(d)|^2 of the window at fractional offsets d in
+ * [-4, 4] step 0.5, normalised to sum 1; src/ddm_ref.js:40-55. */
+static int window_leakage(int N, double *d, double *p) {
+  double *w = (double *)malloc(sizeof(double) * N);
+  orc_blackman_harris(N, w);
+  int cnt = 0;
+  for (double x = -4; x <= 4 + 1e-9; x += 0.5) {
+    double re = 0, im = 0;
+    for (int n = 0; n < N; n++) {
+      double a = (-2 * ORC_PI * x * n) / N;
+      re += w[n] * cos(a);
+      im += w[n] * sin(a);
+    }
+    d[cnt] = x;
+    p[cnt] = re * re + im * im;
+    cnt++;
+  }
+  double s = 0;
+  for (int i = 0; i < cnt; i++) s += p[i];
+  for (int i = 0; i < cnt; i++) p[i] /= s;
+  free(w);
+  return cnt;
+}
+
+/* makeBins; src/ddm_ref.js:84-170.  hypot_mode 0 = V8 Math.hypot (reference
+ * parity), 1 = correctly rounded libm hypot.  binModes holds 4 int32 per entry
+ * (mode, kx, ky, 0).  Returns the number of binned modes, or -1 if `cap` is
+ * too small. */
+long orc_make_bins(int N, int NB, double kmin, double kmax, int hypot_mode, int K, uint32_t *binOffsets,
+                   int32_t *binModes, long cap, double *kCenter, uint32_t *nModes, double *kNodes) {
+  int NH = (N >> 1) + 1;
+  long modes = (long)N * NH;
+  double width = (kmax - kmin) / NB;
+  double (*hyp)(double, double) = hypot_mode ? hypot : orc_v8_hypot;
+  int32_t *bin_of = (int32_t *)malloc(sizeof(int32_t) * modes);
+  long *count = (long *)calloc(NB, sizeof(long));
+  for (long m = 0; m < modes; m++) {
+    int kx = (int)(m % NH);
+    int kyi = (int)((m - kx) / NH);
+    int ky = kyi < N / 2 ? kyi : kyi - N;
+    bin_of[m] = -1;
+    if (kx == 0 && ky <= 0) continue;
+    if (kx == N / 2 || ky == -N / 2) continue;
+    double kr = hyp((double)kx, (double)ky);
+    double b = floor((kr - kmin) / width);
+    if (b >= 0 && b < NB) {
+      bin_of[m] = (int32_t)b;
+      count[(int)b]++;
+    }
+  }
+  long total = 0;
+  for (int b = 0; b < NB; b++) {
+    binOffsets[b] = (uint32_t)total;
+    total += count[b];
+  }
+  binOffsets[NB] = (uint32_t)total;
+  if (total > cap) {
+    free(bin_of);
+    free(count);
+    return -1;
+  }
+  long *fill = (long *)calloc(NB, sizeof(long));
+  for (long m = 0; m < modes; m++) {
+    int b = bin_of[m];
+    if (b < 0) continue;
+    int kx = (int)(m % NH);
+    int kyi = (int)((m - kx) / NH);
+    int ky = kyi < N / 2 ? kyi : kyi - N;
+    long o = 4 * (binOffsets[b] + fill[b]++);
+    binModes[o] = (int32_t)m;
+    binModes[o + 1] = kx;
+    binModes[o + 2] = ky;
+    binModes[o + 3] = 0;
+  }
+  double ld[64], lp[64];
+  int nl = window_leakage(N, ld, lp);
+  double h0 = kmin - 6 > 0 ? kmin - 6 : 0, hres = 0.01;
+  long hlen = (long)ceil((kmax + 6 - h0) / hres) + 1;
+  double *hist = (double *)malloc(sizeof(double) * hlen);
+  double *hk = (double *)malloc(sizeof(double) * hlen);
+  for (int b = 0; b < NB; b++) {
+    long i0 = binOffsets[b], i1 = binOffsets[b + 1];
+    double sk = 0;
+    for (long i = i0; i < i1; i++) sk += hyp((double)binModes[4 * i + 1], (double)binModes[4 * i + 2]);
+    nModes[b] = (uint32_t)(i1 - i0);
+    kCenter[b] = (i1 > i0) ? sk / (double)(i1 - i0) : kmin + (b + 0.5) * width;
+    if (!kNodes) continue;
+    memset(hist, 0, sizeof(double) * hlen);
+    memset(hk, 0, sizeof(double) * hlen);
+    for (long i = i0; i < i1; i++) {
+      int kx = binModes[4 * i + 1], ky = binModes[4 * i + 2];
+      for (int a = 0; a < nl; a++) {
+        double ax = kx + ld[a];
+        for (int c = 0; c < nl; c++) {
+          double k = hyp(ax, ky + ld[c]);
+          double wt = lp[a] * lp[c];
+          double cc = js_round((k - h0) / hres);
+          if (cc < 0) cc = 0;
+          if (cc > hlen - 1) cc = (double)(hlen - 1);
+          long ci = (long)cc;
+          hist[ci] += wt;
+          hk[ci] += wt * k;
+        }
+      }
+    }
+    double wsum = 0;
+    for (long c = 0; c < hlen; c++) wsum += hist[c];
+    if (!(wsum > 0)) {
+      for (int n = 0; n < K; n++) kNodes[b * K + n] = kCenter[b];
+      continue;
+    }
+    double acc = 0, sw = 0, swk = 0;
+    int n = 0;
+    for (long c = 0; c < hlen && n < K; c++) {
+      double rem = hist[c];
+      double kc = hist[c] > 0 ? hk[c] / hist[c] : 0;
+      while (rem > 0 && n < K) {
+        double lim = ((double)(n + 1) / K) * wsum - acc;
+        double take = rem < lim ? rem : lim;
+        sw += take;
+        swk += take * kc;
+        acc += take;
+        rem -= take;
+        if (acc >= ((double)(n + 1) / K) * wsum * (1 - 1e-12)) {
+          kNodes[b * K + n] = sw > 0 ? swk / sw : kc;
+          n++;
+          sw = 0;
+          swk = 0;
+        }
+      }
+    }
+    for (; n < K; n++) kNodes[b * K + n] = kmax;
+  }
+  free(hist);
+  free(hk);
+  free(fill);
+  free(bin_of);
+  free(count);
+  return total;
+}
+
+/* analyzeRadial: temporalMode over binned modes, accumulate, divide by the
+ * bin's mode count; src/ddm_ref.js:360-437 (the drift sums are not part of
+ * this path and are not computed).  D[b*L+tau], P[b]. */
+void orc_analyze_radial(const double *hist, int T, const uint32_t *order, const float *mask, const uint32_t *binOffsets,
+                        const int32_t *binModes, int NB, int L, double *D, double *P) {
+  double *d = (double *)malloc(sizeof(double) * L);
+  memset(D, 0, sizeof(double) * NB * L);
+  memset(P, 0, sizeof(double) * NB);
+  for (int b = 0; b < NB; b++) {
+    for (uint32_t i = binOffsets[b]; i < binOffsets[b + 1]; i++) {
+      long m = binModes[4 * i];
+      double p = orc_temporal_mode(hist, m, T, order, mask, L, d, NULL);
+      P[b] += p;
+      for (int t = 0; t < L; t++) D[b * L + t] += d[t];
+    }
+    uint32_t n = binOffsets[b + 1] - binOffsets[b];
+    if (!n) continue;
+    P[b] /= n;
+    for (int t = 0; t < L; t++) D[b * L + t] /= n;
+  }
+  free(d);
+}
+
+/* Same reduction from direct O(T*L) sums (cancellation-free reference for the
+ * accuracy check of D at small lags). */
+void orc_analyze_radial_direct(const double *hist, int T, const uint32_t *order, const float *mask,
+                               const uint32_t *binOffsets, const int32_t *binModes, int NB, int L, double *D,
+                               double *P) {
+  double *d = (double *)malloc(sizeof(double) * L);
+  memset(D, 0, sizeof(double) * NB * L);
+  memset(P, 0, sizeof(double) * NB);
+  for (int b = 0; b < NB; b++) {
+    for (uint32_t i = binOffsets[b]; i < binOffsets[b + 1]; i++) {
+      long m = binModes[4 * i];
+      double p = orc_temporal_mode_direct(hist, m, T, order, mask, L, d, NULL);
+      P[b] += p;
+      for (int t = 0; t < L; t++) D[b * L + t] += d[t];
+    }
+    uint32_t n = binOffsets[b + 1] - binOffsets[b];
+    if (!n) continue;
+    P[b] /= n;
+    for (int t = 0; t < L; t++) D[b * L + t] /= n;
+  }
+  free(d);
+}
+
+/* ------------------------------------------------------------------ special functions */
+
+/* Bessel J0, rational approximations; src/motility_fit.js:54-68 */
+double orc_bessel_j0(double x) {
+  double ax = fabs(x);
+  if (ax < 8) {
+    double y = x * x;
+    double a1 = 57568490574.0 + y * (-13362590354.0 + y * (651619640.7 + y * (-11214424.18 + y * (77392.33017 + y * -184.9052456))));
+    double a2 = 57568490411.0 + y * (1029532985.0 + y * (9494680.718 + y * (59272.64853 + y * (267.8532712 + y))));
+    return a1 / a2;
+  }
+  double z = 8 / ax, y = z * z, xx = ax - 0.785398164;
+  double a1 = 1 + y * (-0.1098628627e-2 + y * (0.2734510407e-4 + y * (-0.2073370639e-5 + y * 0.2093887211e-6)));
+  double a2 = -0.1562499995e-1 + y * (0.1430488765e-3 + y * (-0.6911147651e-5 + y * (0.7621095161e-6 - y * 0.934935152e-7)));
+  return sqrt(0.636619772 / ax) * (cos(xx) * a1 - z * sin(xx) * a2);
+}
+
+/* ln Gamma, Lanczos g=7 n=9; src/motility_fit.js:74-86 */
+double orc_lngamma(double x) {
+  static const double c[9] = {0.99999999999980993, 676.5203681218851, -1259.1392167224028,
+                              771.32342877765313, -176.61502916214059, 12.507343278686905,
+                              -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+  if (x < 0.5) return log(ORC_PI / fabs(sin(ORC_PI * x))) - orc_lngamma(1 - x);
+  x -= 1;
+  double a = c[0], t = x + 7.5;
+  for (int i = 1; i < 9; i++) a += c[i] / (x + i);
+  return 0.5 * log(2 * ORC_PI) + (x + 0.5) * log(t) - t + log(a);
+}
+
+/* Schulz speed pdf; src/motility_fit.js:95-99 */
+double orc_schulz_pdf(double v, double vbar, double Z) {
+  if (v <= 0) return 0;
+  double b = (Z + 1) / vbar;
+  return exp(Z * log(v) - orc_lngamma(Z + 1) + (Z + 1) * log(b) - b * v);
+}
+
+/* Legendre P_nu(c) by Laplace's integral, trapezoid rule;
+ * src/motility_fit.js:120-132 */
+double orc_legendre_p(double nu, double c) {
+  double s = sqrt(fmax(0, 1 - c * c));
+  int n = 16 + (int)ceil(nu * atan2(s, c));
+  double sum = 0;
+  for (int i = 0; i <= n; i++) {
+    double phi = (ORC_PI * i) / n;
+    double im = s * cos(phi);
+    double r2 = c * c + im * im;
+    double val = exp(0.5 * nu * log(r2)) * cos(nu * atan2(im, c));
+    sum += (i == 0 || i == n) ? 0.5 * val : val;
+  }
+  return sum / n;
+}
+
+/* 3D swimmer ISF closed form; src/motility_fit.js:140-144 */
+double orc_isf3d(double x, double Z) {
+  double L = x / (Z + 1);
+  if (L < 1e-4) return 1 - (x * x * (Z + 2)) / (6 * (Z + 1));
+  return sin(Z * atan(L)) / (Z * L * exp(0.5 * Z * log1p(L * L)));
+}
+
+/* 2D swimmer ISF closed form; src/motility_fit.js:152-159 */
+double orc_isf2d(double x, double Z) {
+  double L = x / (Z + 1);
+  if (L < 1e-4) return 1 - (x * x * (Z + 2)) / (4 * (Z + 1));
+  double u = 1 + L * L;
+  double lnPre = -0.5 * (Z + 1) * log(u);
+  if (lnPre < -25) return 0;
+  return orc_legendre_p(Z, 1 / sqrt(u)) * exp(lnPre);
+}
+
+/* Tabulated 2D ISF (Catmull-Rom in ln Lambda, quadratic Lagrange in ln Z);
+ * src/motility_fit.js:161-220. */
+#define T2D_NZ 72
+typedef struct {
+  double Z, u0, du;
+  int n;
+  double *v;
+} t2d_row;
+static t2d_row t2d_rows[T2D_NZ];
+static int t2d_built = 0;
+static double t2d_lz0, t2d_dlz;
+
+static double isf2d_exact_lambda(double L, double Z) {
+  double u = 1 + L * L;
+  double lnPre = -0.5 * (Z + 1) * log(u);
+  if (lnPre < -25) return 0;
+  return orc_legendre_p(Z, 1 / sqrt(u)) * exp(lnPre);
+}
+
+static void build_t2d(void) {
+  t2d_lz0 = log(0.5);
+  double lz1 = log(100.0);
+  t2d_dlz = (lz1 - t2d_lz0) / (T2D_NZ - 1);
+  for (int i = 0; i < T2D_NZ; i++) {
+    double Z = exp(t2d_lz0 + i * t2d_dlz);
+    double Lc = fmin(1e4, sqrt(expm1(50 / (Z + 1))));
+    double u0 = log(1e-4), u1 = log(Lc);
+    double du = fmin(0.04, 0.1 / ((Z + 0.5) / 2));
+    int n = (int)ceil((u1 - u0) / du) + 1;
+    double *v = (double *)malloc(sizeof(double) * (n + 2));
+    for (int k = 0; k < n + 2; k++) v[k] = isf2d_exact_lambda(exp(u0 + k * du), Z);
+    t2d_rows[i] = (t2d_row){Z, u0, du, n, v};
+  }
+  t2d_built = 1;
+}
+
+static double t2d_row_eval(const t2d_row *r, double x) {
+  double L = x / (r->Z + 1);
+  if (L < 1e-4) return 1 - (x * x * (r->Z + 2)) / (4 * (r->Z + 1));
+  double f = (log(L) - r->u0) / r->du;
+  if (f >= r->n - 1) return 0;
+  int i = (int)floor(f);
+  double t = f - i;
+  const double *v = r->v;
+  double p1 = v[i], p2 = v[i + 1], p3 = v[i + 2];
+  double p0 = i > 0 ? v[i - 1] : 2 * p1 - p2;
+  return p1 + 0.5 * t * (p2 - p0 + t * (2 * p0 - 5 * p1 + 4 * p2 - p3 + t * (3 * (p1 - p2) + p3 - p0)));
+}
+
+double orc_isf2d_fast(double x, double Z) {
+  if (!t2d_built) build_t2d();
+  double zc = fmin(100, fmax(0.5, Z));
+  double zf = (log(zc) - t2d_lz0) / t2d_dlz;
+  int i0 = (int)js_round(zf) - 1;
+  if (i0 < 0) i0 = 0;
+  if (i0 > T2D_NZ - 3) i0 = T2D_NZ - 3;
+  double t = zf - i0;
+  double g0 = t2d_row_eval(&t2d_rows[i0], x), g1 = t2d_row_eval(&t2d_rows[i0 + 1], x),
+         g2 = t2d_row_eval(&t2d_rows[i0 + 2], x);
+  return 0.5 * (t - 1) * (t - 2) * g0 - t * (t - 2) * g1 + 0.5 * t * (t - 1) * g2;
+}
+
+/* ------------------------------------------------------------------ fit building blocks */
+
+/* lagGrid: 1..10 then x1.08 steps; src/motility_fit.js:358-369 */
+int orc_lag_grid(int L, int *lags, int cap) {
+  int n = 0;
+  for (int t = 1; t <= (10 < L - 1 ? 10 : L - 1); t++)
+    if (n < cap) lags[n++] = t;
+  double x = 10;
+  for (;;) {
+    x *= 1.08;
+    int t = (int)js_round(x);
+    if (t > L - 1) break;
+    if (n == 0 || t > lags[n - 1]) {
+      if (n < cap) lags[n++] = t;
+    }
+  }
+  return n;
+}
+
+/* Gaussian elimination with partial pivoting, k <= 3; src/motility_fit.js:270-291 */
+static int solve_small(int k, double A[3][3], const double *b, double *x) {
+  double M[3][4];
+  double scale = 1e-300;
+  for (int i = 0; i < k; i++) {
+    for (int j = 0; j < k; j++) M[i][j] = A[i][j];
+    M[i][k] = b[i];
+    if (fabs(A[i][i]) > scale) scale = fabs(A[i][i]);
+  }
+  for (int c = 0; c < k; c++) {
+    int p = c;
+    for (int r = c + 1; r < k; r++)
+      if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
+    if (fabs(M[p][c]) < 1e-11 * scale) return 0;
+    for (int j = 0; j <= k; j++) {
+      double t = M[c][j];
+      M[c][j] = M[p][j];
+      M[p][j] = t;
+    }
+    for (int r = c + 1; r < k; r++) {
+      double f = M[r][c] / M[c][c];
+      for (int j = c; j <= k; j++) M[r][j] -= f * M[c][j];
+    }
+  }
+  for (int r = k - 1; r >= 0; r--) {
+    double s = M[r][k];
+    for (int j = r + 1; j < k; j++) s -= M[r][j] * x[j];
+    x[r] = s / M[r][r];
+  }
+  return 1;
+}
+
+/* Exact NNLS for <= 3 columns by support enumeration; src/motility_fit.js:243-263 */
+void orc_nnls3(const double *G, const double *h, double dd, double *a_out, double *rss_out) {
+  double best_a[3] = {0, 0, 0}, best_rss = dd;
+  for (int mask = 1; mask < 8; mask++) {
+    int idx[3], k = 0;
+    for (int i = 0; i < 3; i++)
+      if (mask & (1 << i)) idx[k++] = i;
+    double A[3][3], b[3], x[3];
+    for (int i = 0; i < k; i++) {
+      for (int j = 0; j < k; j++) A[i][j] = G[3 * idx[i] + idx[j]];
+      b[i] = h[idx[i]];
+    }
+    if (!solve_small(k, A, b, x)) continue;
+    int neg = 0;
+    for (int i = 0; i < k; i++)
+      if (x[i] < 0) neg = 1;
+    if (neg) continue;
+    double rss = dd;
+    for (int i = 0; i < k; i++) rss -= x[i] * b[i];
+    if (rss < best_rss) {
+      best_a[0] = best_a[1] = best_a[2] = 0;
+      for (int i = 0; i < k; i++) best_a[idx[i]] = x[i];
+      best_rss = rss > 0 ? rss : 0;
+    }
+  }
+  for (int i = 0; i < 3; i++) a_out[i] = best_a[i];
+  *rss_out = best_rss;
+}
+
+/* modelColumn: c(tau) = 1 - <E (1 - alpha + alpha g(q vbar tau; Z))>_nodes,
+ * E = exp(-Dd q^2 tau) [* J0(q |V| tau)]; src/motility_fit.js:417-432.
+ * geometry 0 = 3D kernel (isf3d), 1 = 2D exact (isf2d), 2 = 2D table (isf2dFast). */
+void orc_model_column(int ntau, const double *tau, int nq, const double *qn, double vbar, double Z, double Dd,
+                      double Vmag, double alpha, int swim, int geometry, double *out) {
+  double inv = 1.0 / nq;
+  double al = swim ? alpha : 0;
+  for (int j = 0; j < ntau; j++) {
+    double t = tau[j], f = 0;
+    for (int i = 0; i < nq; i++) {
+      double q = qn[i];
+      double E = exp(-Dd * q * q * t);
+      if (Vmag > 0) E *= orc_bessel_j0(q * Vmag * t);
+      if (al > 0) {
+        double x = q * vbar * t;
+        double g = geometry == 0 ? orc_isf3d(x, Z) : geometry == 1 ? orc_isf2d(x, Z) : orc_isf2d_fast(x, Z);
+        f += E * (1 - al + al * g);
+      } else {
+        f += E;
+      }
+    }
+    out[j] = 1 - f * inv;
+  }
+}

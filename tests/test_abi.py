@@ -1,0 +1,46 @@
+"""libddm.so loads and exports every entry point include/ddm.h declares (CPU: no compute calls)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "ddm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ddm_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_expected_entry_points():
+    fns = header_functions()
+    for name in ("ddm_create", "ddm_destroy", "ddm_ingest", "ddm_ingest_device", "ddm_compute_temporal",
+                 "ddm_read_radial", "ddm_read_history", "ddm_last_error"):
+        assert name in fns
+
+
+def test_library_exports_every_declared_symbol():
+    from hackathon_amd.engine import EXPORTS, LIB_PATH, load_library
+    if not os.path.exists(LIB_PATH):
+        pytest.fail("libddm.so not built: run python -m hackathon_amd.build")
+    lib = load_library()
+    fns = header_functions()
+    for name in fns:
+        assert hasattr(lib, name), name
+    assert sorted(EXPORTS) == fns
+    nm = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (ddm_[a-z_0-9]+)", nm))
+    assert set(fns) <= exported
+
+
+def test_create_rejects_bad_geometry_without_gpu_work():
+    """Geometry validation happens on the host before any HIP call."""
+    from hackathon_amd.engine import DdmEngine, DdmError
+    with pytest.raises(DdmError, match="power of two"):
+        DdmEngine.create(N=100, T=8, L=4, NB=4, kmin=0, kmax=8)
+    with pytest.raises(DdmError, match="L must be"):
+        DdmEngine.create(N=64, T=8, L=9, NB=4, kmin=0, kmax=8)
+    with pytest.raises(DdmError, match="kmin"):
+        DdmEngine.create(N=64, T=8, L=4, NB=4, kmin=5, kmax=2)
