@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,6 +30,16 @@ int ilog2(long n) {
   return l;
 }
 bool is_pow2(long n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+// Tuning override from the environment (clamped), or the default.
+long env_long(const char* name, long dflt, long lo, long hi) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char* end = nullptr;
+  long x = std::strtol(v, &end, 10);
+  if (end == v) return dflt;
+  return std::min(hi, std::max(lo, x));
+}
 
 // ------------------------------------------------------------ host tables
 
@@ -358,7 +369,7 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
   const long binned = e->bins.offsets[g.NB];
 
   // Temporal work items: (bin, first mode, count); up to M modes of one bin.
-  const long target_items = 2048;
+  const long target_items = env_long("DDM_TEMPORAL_ITEMS", 2048, 1, 1L << 22);
   int M = (int)std::min<long>(64, std::max<long>(1, (binned + target_items - 1) / target_items));
   std::vector<int> items_h, bin_modes_h(std::max<long>(binned, 1));
   e->bin_items_h.assign(g.NB + 1, 0);
@@ -373,10 +384,12 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
   e->bin_items_h[g.NB] = e->nitems;
   for (long i = 0; i < binned; i++) bin_modes_h[i] = e->bins.modes4[4 * i];
 
-  // Spatial chunk: frames whose row spectra (NH*N*8 B each) fit ~32 MB.
+  // Spatial chunk: frames whose row spectra (NH*N*8 B each) fit the chunk
+  // budget (default 128 MB: the scratch stays in the 256 MB last-level cache).
   const int F = ddm::cols_frames_per_wg(e->logn);
   long per_frame = (long)e->NH * g.N * 8;
-  int chunk = (int)std::max<long>(F, ((32L << 20) / per_frame) / F * F);
+  const long chunk_bytes = env_long("DDM_CHUNK_MB", 128, 1, 1 << 16) << 20;
+  int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
   chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
   e->chunk = chunk;
   e->stage_bytes = (size_t)chunk * g.N * g.N * 4;

@@ -234,8 +234,8 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
   extern __shared__ float2 lds[];
   float2* buf = lds;                  // F * N
   float2* tw = lds + F * N;           // N
-  float* mu = reinterpret_cast<float*>(tw + N);   // F (hi) + F (lo)
-  long long* tot = reinterpret_cast<long long*>(mu + 2 * F);  // F
+  float* mu = reinterpret_cast<float*>(tw + N);   // 4F: frame mean as (hi sum, its compensation, lo, pad)
+  long long* tot = reinterpret_cast<long long*>(mu + 4 * F);  // F
   const int tid = threadIdx.x;
   const int kx = blockIdx.x;
   const int f0 = blockIdx.y * F;
@@ -257,8 +257,11 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
           comp_add(sh, ch, __int_as_float(ri[y].x));
           comp_add(sl, cl, __int_as_float(ri[y].y));
         }
-        mu[tid] = (sh + ch) * (1.0f / N);
-        mu[F + tid] = (sl + cl) * (1.0f / N);
+        // N is a power of two: dividing by it is exact, so the frame mean is
+        // kept unrounded as sh/N + ch/N + (sl + cl)/N
+        mu[tid] = sh * (1.0f / N);
+        mu[F + tid] = ch * (1.0f / N);
+        mu[2 * F + tid] = (sl + cl) * (1.0f / N);
       }
     }
     __syncthreads();
@@ -272,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
         long long num = (long long)N * ri.x - tot[f];
         d = (float)num * (scale / ((float)N * (float)N));
       } else {
-        d = ((__int_as_float(ri.x) - mu[f]) + (__int_as_float(ri.y) - mu[F + f])) * scale;
+        d = (((__int_as_float(ri.x) - mu[f]) - mu[F + f]) + (__int_as_float(ri.y) - mu[2 * F + f])) * scale;
       }
       float c = d * win[y];
       buf[e] = cadd(buf[e], make_float2(c * w.x, c * w.y));
@@ -487,8 +490,8 @@ template <typename Tin>
 static hipError_t launch_rows_dt(int logn, const void* frames, int nframes, float scale, const float* win,
                                  const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
   switch (logn) {
-#define CASE(L) \
-  case L: return launch_rows_t<Tin, L>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
+#define CASE(LG_) \
+  case LG_: return launch_rows_t<Tin, LG_>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
     CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
 #undef CASE
   }
@@ -510,7 +513,7 @@ static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int 
                                 int first_slot, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
   constexpr int F = (LOGN >= 11) ? 4 : (LOGN >= 10 ? 8 : 16);
-  size_t lds = sizeof(float2) * (F * N + N) + sizeof(float) * 2 * F + sizeof(long long) * F;
+  size_t lds = sizeof(float2) * (F * N + N) + sizeof(float) * 4 * F + sizeof(long long) * F;
   auto kern = k_cols<LOGN, F>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -524,8 +527,8 @@ hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int
                        const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
                        hipStream_t st) {
   switch (logn) {
-#define CASE(L) \
-  case L: return launch_cols_t<L>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+#define CASE(LG_) \
+  case LG_: return launch_cols_t<LG_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
     CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
 #undef CASE
   }
@@ -550,9 +553,9 @@ hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* item
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                            float* S_part, float* p_part, float* P_part, hipStream_t st) {
   switch (logt) {
-#define CASE(L)                                                                                                    \
-  case L:                                                                                                          \
-    return launch_temporal_t<L>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
+#define CASE(LG_)                                                                                                    \
+  case LG_:                                                                                                        \
+    return launch_temporal_t<LG_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
                                 P_part, st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
@@ -579,9 +582,9 @@ hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, c
                            const float* npairs, int T, int L, float n0, const float2* tw, float* radD, float* radP,
                            hipStream_t st) {
   switch (logt) {
-#define CASE(L)                                                                                                       \
-  case L:                                                                                                             \
-    return launch_finalize_t<L>(S_part, p_part, P_part, bin_items, NB, nmodes, mask, npairs, T, L, n0, tw, radD, radP, \
+#define CASE(LG_)                                                                                                       \
+  case LG_:                                                                                                           \
+    return launch_finalize_t<LG_>(S_part, p_part, P_part, bin_items, NB, nmodes, mask, npairs, T, L, n0, tw, radD, radP, \
                                 st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
