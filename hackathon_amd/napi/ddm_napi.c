@@ -1,0 +1,328 @@
+/*
+ * ddm_napi.c -- thin N-API addon over libddm.so for the TypeScript facade.
+ *
+ * One JS function per C-ABI entry point (include/ddm.h); no logic of its own.
+ * Errors: a negative status throws Error(ddm_last_error()), the way the
+ * reference engine throws (motility-ddm src/ddm-engine.ts:126-163, :313, :332).
+ * The engine handle is a napi_external whose finalizer calls ddm_destroy.
+ * Built with gcc against node_api.h (N-API >= 6), linked to libddm.so with an
+ * $ORIGIN rpath.
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/ddm.h"
+
+#define NAPI_CALL(env, call)                                          \
+  do {                                                                \
+    if ((call) != napi_ok) {                                          \
+      napi_throw_error((env), NULL, "N-API call failed: " #call);     \
+      return NULL;                                                    \
+    }                                                                 \
+  } while (0)
+
+typedef struct {
+  ddm_engine *e;
+} handle_t;
+
+static void finalize_handle(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  handle_t *h = (handle_t *)data;
+  if (h->e) ddm_destroy(h->e);
+  free(h);
+}
+
+static napi_value throw_status(napi_env env, ddm_engine *e, int rc) {
+  (void)rc;
+  napi_throw_error(env, NULL, ddm_last_error(e));
+  return NULL;
+}
+
+static int get_handle(napi_env env, napi_value v, handle_t **out) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected a DDM engine handle");
+    return 0;
+  }
+  *out = (handle_t *)p;
+  if (!(*out)->e) {
+    napi_throw_error(env, NULL, "engine destroyed");
+    return 0;
+  }
+  return 1;
+}
+
+static double get_prop_num(napi_env env, napi_value obj, const char *name, double dflt) {
+  bool has = false;
+  napi_value v;
+  double d = dflt;
+  if (napi_has_named_property(env, obj, name, &has) == napi_ok && has &&
+      napi_get_named_property(env, obj, name, &v) == napi_ok) {
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_number) napi_get_value_double(env, v, &d);
+  }
+  return d;
+}
+
+/* create({N, T, L, NB, kmin, kmax, hypotMode, knodes}) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1) {
+    napi_throw_type_error(env, NULL, "create expects a geometry object");
+    return NULL;
+  }
+  ddm_geometry g;
+  g.N = (int32_t)get_prop_num(env, argv[0], "N", 256);
+  g.T = (int32_t)get_prop_num(env, argv[0], "T", 256);
+  g.L = (int32_t)get_prop_num(env, argv[0], "L", 128);
+  g.NB = (int32_t)get_prop_num(env, argv[0], "NB", 32);
+  g.kmin = get_prop_num(env, argv[0], "kmin", 4);
+  g.kmax = get_prop_num(env, argv[0], "kmax", 100);
+  g.hypot_mode = (int32_t)get_prop_num(env, argv[0], "hypotMode", 0);
+  g.knodes = (int32_t)get_prop_num(env, argv[0], "knodes", 4);
+  ddm_engine *e = NULL;
+  int rc = ddm_create(&g, &e);
+  if (rc != DDM_OK) return throw_status(env, NULL, rc);
+  handle_t *h = (handle_t *)calloc(1, sizeof(handle_t));
+  h->e = e;
+  napi_value out;
+  NAPI_CALL(env, napi_create_external(env, h, finalize_handle, NULL, &out));
+  return out;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  void *p = NULL;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc >= 1 && napi_get_value_external(env, argv[0], &p) == napi_ok && p) {
+    handle_t *h = (handle_t *)p;
+    if (h->e) ddm_destroy(h->e);
+    h->e = NULL;
+  }
+  return NULL;
+}
+
+static napi_value make_typed(napi_env env, napi_typedarray_type t, size_t count, size_t esize, void **data) {
+  napi_value ab, ta;
+  if (napi_create_arraybuffer(env, count * esize, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, t, count, ab, 0, &ta) != napi_ok) return NULL;
+  return ta;
+}
+
+static void set_prop(napi_env env, napi_value obj, const char *name, napi_value v) {
+  if (v) napi_set_named_property(env, obj, name, v);
+}
+
+static napi_value num(napi_env env, double d) {
+  napi_value v;
+  napi_create_double(env, d, &v);
+  return v;
+}
+
+/* sizes(h) -> {N, NH, T, TPAD, L, NB, knodes, modes, binnedModes, histBytes, workItems, chunkFrames} */
+static napi_value js_sizes(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  ddm_sizes s;
+  int rc = ddm_get_sizes(h->e, &s);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  NAPI_CALL(env, napi_create_object(env, &out));
+  set_prop(env, out, "N", num(env, s.N));
+  set_prop(env, out, "NH", num(env, s.NH));
+  set_prop(env, out, "T", num(env, s.T));
+  set_prop(env, out, "TPAD", num(env, s.TPAD));
+  set_prop(env, out, "L", num(env, s.L));
+  set_prop(env, out, "NB", num(env, s.NB));
+  set_prop(env, out, "knodes", num(env, s.knodes));
+  set_prop(env, out, "modes", num(env, (double)s.modes));
+  set_prop(env, out, "binnedModes", num(env, (double)s.binned_modes));
+  set_prop(env, out, "histBytes", num(env, (double)s.hist_bytes));
+  set_prop(env, out, "workItems", num(env, s.work_items));
+  set_prop(env, out, "chunkFrames", num(env, s.chunk_frames));
+  return out;
+}
+
+/* bins(h) -> {binOffsets, binModes, kCenter, nModes, kNodes} (makeBins layout) */
+static napi_value js_bins(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  void *off, *modes, *kc, *nm, *kn;
+  napi_value voff = make_typed(env, napi_uint32_array, (size_t)s.NB + 1, 4, &off);
+  napi_value vmodes = make_typed(env, napi_int32_array, (size_t)(4 * s.binned_modes), 4, &modes);
+  napi_value vkc = make_typed(env, napi_float64_array, (size_t)s.NB, 8, &kc);
+  napi_value vnm = make_typed(env, napi_uint32_array, (size_t)s.NB, 4, &nm);
+  napi_value vkn = make_typed(env, napi_float64_array, (size_t)s.NB * s.knodes, 8, &kn);
+  if (!voff || !vmodes || !vkc || !vnm || !vkn) {
+    napi_throw_error(env, NULL, "bins: allocation failed");
+    return NULL;
+  }
+  int rc = ddm_get_bins(h->e, (uint32_t *)off, (int32_t *)modes, (double *)kc, (uint32_t *)nm, (double *)kn);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  NAPI_CALL(env, napi_create_object(env, &out));
+  set_prop(env, out, "binOffsets", voff);
+  set_prop(env, out, "binModes", vmodes);
+  set_prop(env, out, "kCenter", vkc);
+  set_prop(env, out, "nModes", vnm);
+  set_prop(env, out, "kNodes", vkn);
+  return out;
+}
+
+/* ingest(h, Uint8Array | Uint16Array | Float32Array, nframes, firstSlot) -> boolean */
+static napi_value js_ingest(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 4 || !get_handle(env, argv[0], &h)) return NULL;
+  napi_typedarray_type t;
+  size_t len;
+  void *data;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off) != napi_ok) {
+    napi_throw_type_error(env, NULL, "ingest expects a Uint8Array, Uint16Array or Float32Array");
+    return NULL;
+  }
+  int32_t dtype = t == napi_uint8_array ? DDM_U8 : t == napi_uint16_array ? DDM_U16 : t == napi_float32_array ? DDM_F32 : -1;
+  if (dtype < 0) {
+    napi_throw_type_error(env, NULL, "ingest expects a Uint8Array, Uint16Array or Float32Array");
+    return NULL;
+  }
+  int32_t nframes, slot;
+  NAPI_CALL(env, napi_get_value_int32(env, argv[2], &nframes));
+  NAPI_CALL(env, napi_get_value_int32(env, argv[3], &slot));
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  if ((int64_t)len != (int64_t)nframes * s.N * s.N) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "ingest expects %lld values", (long long)nframes * s.N * s.N);
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+  }
+  int rc = ddm_ingest(h->e, data, dtype, nframes, slot);
+  if (rc < 0) return throw_status(env, h->e, rc);
+  napi_value out;
+  NAPI_CALL(env, napi_get_boolean(env, rc == DDM_OK, &out));
+  return out;
+}
+
+/* computeTemporal(h, Uint32Array order, Float32Array mask) */
+static napi_value js_compute_temporal(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 3 || !get_handle(env, argv[0], &h)) return NULL;
+  napi_typedarray_type t1, t2;
+  size_t n1, n2, o;
+  void *d1, *d2;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, argv[1], &t1, &n1, &d1, &ab, &o) != napi_ok || t1 != napi_uint32_array ||
+      napi_get_typedarray_info(env, argv[2], &t2, &n2, &d2, &ab, &o) != napi_ok || t2 != napi_float32_array) {
+    napi_throw_type_error(env, NULL, "computeTemporal expects (Uint32Array order, Float32Array mask)");
+    return NULL;
+  }
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  if ((int)n1 != s.T || (int)n2 != s.T) {
+    char msg[96];
+    snprintf(msg, sizeof msg, "temporalMeta: order/mask must have length %d", s.T);
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+  }
+  int rc = ddm_compute_temporal(h->e, (const uint32_t *)d1, (const float *)d2);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  return NULL;
+}
+
+/* readRadial(h) -> {D: Float32Array, P: Float32Array, drift: Float32Array} */
+static napi_value js_read_radial(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  void *D, *P, *dr;
+  napi_value vD = make_typed(env, napi_float32_array, (size_t)s.NB * s.L, 4, &D);
+  napi_value vP = make_typed(env, napi_float32_array, (size_t)s.NB, 4, &P);
+  napi_value vdr = make_typed(env, napi_float32_array, (size_t)s.NB * s.L * 8, 4, &dr);
+  if (!vD || !vP || !vdr) {
+    napi_throw_error(env, NULL, "readRadial: allocation failed");
+    return NULL;
+  }
+  int rc = ddm_read_radial(h->e, (float *)D, (float *)P, (float *)dr);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  NAPI_CALL(env, napi_create_object(env, &out));
+  set_prop(env, out, "D", vD);
+  set_prop(env, out, "P", vP);
+  set_prop(env, out, "drift", vdr);
+  return out;
+}
+
+/* readHistory(h) -> Float32Array hist[(m*T + slot)*2 + {0,1}] */
+static napi_value js_read_history(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  void *d;
+  napi_value v = make_typed(env, napi_float32_array, (size_t)s.hist_bytes / 4, 4, &d);
+  if (!v) {
+    napi_throw_error(env, NULL, "readHistory: allocation failed");
+    return NULL;
+  }
+  int rc = ddm_read_history(h->e, (float *)d);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  return v;
+}
+
+static napi_value js_idle(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  int rc = ddm_idle(h->e);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  return NULL;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  napi_property_descriptor d[] = {
+      {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
+      {"destroy", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
+      {"sizes", NULL, js_sizes, NULL, NULL, NULL, napi_default, NULL},
+      {"bins", NULL, js_bins, NULL, NULL, NULL, napi_default, NULL},
+      {"ingest", NULL, js_ingest, NULL, NULL, NULL, napi_default, NULL},
+      {"computeTemporal", NULL, js_compute_temporal, NULL, NULL, NULL, napi_default, NULL},
+      {"readRadial", NULL, js_read_radial, NULL, NULL, NULL, napi_default, NULL},
+      {"readHistory", NULL, js_read_history, NULL, NULL, NULL, napi_default, NULL},
+      {"idle", NULL, js_idle, NULL, NULL, NULL, napi_default, NULL},
+  };
+  if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)
