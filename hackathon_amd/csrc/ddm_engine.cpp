@@ -220,6 +220,8 @@ struct ddm_engine {
   int NH = 0, TPAD = 0, logn = 0, logt = 0;
   long modes = 0;
   int chunk = 0;
+  int cols_f = 16;        // frames per column workgroup (wavefront kernels)
+  bool wave_spatial = false;
   hipStream_t stream = nullptr;
   std::string err;
   Bins bins;
@@ -247,6 +249,9 @@ struct ddm_engine {
   float* P_part = nullptr;
   float* radD = nullptr;
   float* radP = nullptr;
+  float* Gbin = nullptr;   // NB*L summed Re G per bin
+  float* pbin = nullptr;   // NB*T summed p per bin
+  float* Pbin = nullptr;   // NB summed P per bin
   bool temporal_busy = false;
   // pinned staging of the per-analysis metadata (order, 0/1 mask, N(tau));
   // meta_ev marks when the last upload from it finished
@@ -316,7 +321,7 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
-                  e->p_part, e->P_part, e->radD, e->radP};
+                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -386,7 +391,9 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
 
   // Spatial chunk: frames whose row spectra (NH*N*8 B each) fit the chunk
   // budget (default 128 MB: the scratch stays in the 256 MB last-level cache).
-  const int F = ddm::cols_frames_per_wg(e->logn);
+  e->wave_spatial = (e->logn == 9 || e->logn == 10) && env_long("DDM_WAVE_SPATIAL", 1, 0, 1);
+  e->cols_f = env_long("DDM_COLS_F", 16, 8, 16) >= 16 ? 16 : 8;
+  const int F = 16;   // chunks are a multiple of every column-kernel frame tile
   long per_frame = (long)e->NH * g.N * 8;
   const long chunk_bytes = env_long("DDM_CHUNK_MB", 128, 1, 1 << 16) << 20;
   int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
@@ -406,7 +413,9 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
       (he = dalloc(&e->order, g.T)) != hipSuccess || (he = dalloc(&e->mask, g.T)) != hipSuccess ||
       (he = dalloc(&e->npairs, g.L)) != hipSuccess || (he = dalloc(&e->S_part, nitems * e->TPAD)) != hipSuccess ||
       (he = dalloc(&e->p_part, nitems * g.T)) != hipSuccess || (he = dalloc(&e->P_part, nitems)) != hipSuccess ||
-      (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess) {
+      (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess ||
+      (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
+      (he = dalloc(&e->Pbin, g.NB)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: device allocation");
     return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
   }
@@ -504,11 +513,17 @@ static int check_ingest(ddm_engine* e, const void* frames, int32_t dtype, int32_
 static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframes, int slot) {
   const float scale = dtype == DDM_U8 ? 1.0f / 255.0f : dtype == DDM_U16 ? 1.0f / 65535.0f : 1.0f;
   hipError_t he = timed(e, DDM_K_ROWS, [&] {
+    if (e->wave_spatial)
+      return ddm::launch_rows_w(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
+                                e->stream);
     return ddm::launch_rows(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
                             e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial rows launch");
   he = timed(e, DDM_K_COLS, [&] {
+    if (e->wave_spatial)
+      return ddm::launch_cols_w(e->logn, e->cols_f, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win,
+                                e->tw_n, e->wx, e->hist, e->g.T, slot % e->g.T, e->stream);
     return ddm::launch_cols(e->logn, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
                             e->hist, e->g.T, slot % e->g.T, e->stream);
   });
@@ -586,15 +601,20 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
     rc = hip_fail(e, he, "computeTemporal: metadata upload");
   } else if (e->nitems > 0 &&
              (he = timed(e, DDM_K_TEMPORAL, [&] {
+                const float inv_n0 = n0 > 0.5f ? 1.0f / n0 : 0.0f;
+                if (e->logt <= 11)   // wavefront-per-mode kernel (registers hold TPAD/64 points per lane)
+                  return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
+                                                e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part,
+                                                e->P_part, e->stream);
                 return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
-                                            identity ? 1 : 0, n0 > 0.5f ? 1.0f / n0 : 0.0f, e->tw_t, e->S_part,
-                                            e->p_part, e->P_part, e->stream);
+                                            identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part,
+                                            e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "temporal launch");
   } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
-                return ddm::launch_finalize(e->logt, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
-                                            e->nmodes, e->mask, e->npairs, T, L, n0, e->tw_t, e->radD, e->radP,
-                                            e->stream);
+                return ddm::launch_bin_finalize(e->logt, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
+                                                e->nmodes, e->mask, e->npairs, T, L, n0, e->tw_t, e->Gbin, e->pbin,
+                                                e->Pbin, e->radD, e->radP, e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "radial finalize launch");
   }

@@ -141,3 +141,164 @@ __device__ __forceinline__ void comp_add(float& s, float& c, float x) {
 }
 
 }  // namespace ddm
+
+namespace ddm {
+
+// ----------------------------------------------------------------- wavefront FFTs
+//
+// One 64-lane wavefront computes one n-point FFT (n = 2^LOGN, n >= 256).  Lane
+// l holds P = n/64 points in the "canonical" distribution v[q] = x[l + 64 q],
+// which is both what coalesced global loads produce and what every Stockham
+// pass reads (butterfly j = l + 64 b reads x[j + r n/R] = v[b + r n/(64 R)]).
+// Between passes the outputs are permuted back to canonical through a
+// wave-private LDS buffer; no workgroup barrier is involved.  The last pass
+// writes X[j + s n/R], which is canonical again, straight into registers.
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Padded LDS index for the exchange buffers: one spare slot per 16 entries
+// spreads the strided Stockham writes over the banks.
+__device__ __forceinline__ int xpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int xch_size(int n) { return n + n / 16; }
+
+// First-pass DFTs whose upper half of inputs is zero (zero-padded data).
+template <int R, bool INV> __device__ __forceinline__ void dft_half(float2* a);
+template <> __device__ __forceinline__ void dft_half<4, false>(float2* a) {
+  float2 x0 = a[0], x1 = a[1];
+  a[0] = cadd(x0, x1);
+  a[2] = csub(x0, x1);
+  float2 t = mul_mi<false>(x1);
+  a[1] = cadd(x0, t);
+  a[3] = csub(x0, t);
+}
+template <> __device__ __forceinline__ void dft_half<4, true>(float2* a) {
+  float2 x0 = a[0], x1 = a[1];
+  a[0] = cadd(x0, x1);
+  a[2] = csub(x0, x1);
+  float2 t = mul_mi<true>(x1);
+  a[1] = cadd(x0, t);
+  a[3] = csub(x0, t);
+}
+template <bool INV> __device__ __forceinline__ void dft8_half(float2* a) {
+  // inputs a[0..3], a[4..7] == 0: E = DFT4(a0, a2, 0, 0), O = DFT4(a1, a3, 0, 0)
+  float2 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+  float2 e0 = cadd(a0, a2), e2 = csub(a0, a2), t = mul_mi<INV>(a2);
+  float2 e1 = cadd(a0, t), e3 = csub(a0, t);
+  float2 o0 = cadd(a1, a3), o2 = csub(a1, a3), u = mul_mi<INV>(a3);
+  float2 o1 = cadd(a1, u), o3 = csub(a1, u);
+  const float r = 0.70710678118654752440f;
+  float2 w1o1 = INV ? make_float2(r * (o1.x - o1.y), r * (o1.x + o1.y)) : make_float2(r * (o1.x + o1.y), r * (o1.y - o1.x));
+  float2 w2o2 = mul_mi<INV>(o2);
+  float2 w3o3 = INV ? make_float2(-r * (o3.x + o3.y), r * (o3.x - o3.y)) : make_float2(r * (o3.y - o3.x), -r * (o3.x + o3.y));
+  a[0] = cadd(e0, o0);
+  a[4] = csub(e0, o0);
+  a[1] = cadd(e1, w1o1);
+  a[5] = csub(e1, w1o1);
+  a[2] = cadd(e2, w2o2);
+  a[6] = csub(e2, w2o2);
+  a[3] = cadd(e3, w3o3);
+  a[7] = csub(e3, w3o3);
+}
+template <> __device__ __forceinline__ void dft_half<8, false>(float2* a) { dft8_half<false>(a); }
+template <> __device__ __forceinline__ void dft_half<8, true>(float2* a) { dft8_half<true>(a); }
+
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+// radix (log2) of a wavefront pass: the usual choice, capped so every lane owns
+// at least one butterfly (R <= n/64)
+__host__ __device__ constexpr int wave_lr(int logn, int nslog) {
+  return next_lr(logn - nslog) < (logn - 6) ? next_lr(logn - nslog) : (logn - 6);
+}
+
+// Twiddles w^r, r = 1..R-1, of one butterfly: w^1 and w^4 come from the
+// table, the others are at most two products away from them.
+template <int R, bool INV>
+__device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, float2* w) {
+  float2 w1 = tw[m1];
+  if (INV) w1.y = -w1.y;
+  w[1] = w1;
+  if (R >= 4) {
+    w[2] = cmul(w1, w1);
+    w[3] = cmul(w[2], w1);
+  }
+  if (R >= 8) {
+    float2 w4 = tw[4 * m1];
+    if (INV) w4.y = -w4.y;
+    w[4] = w4;
+    w[5] = cmul(w4, w1);
+    w[6] = cmul(w4, w[2]);
+    w[7] = cmul(w4, w[3]);
+  }
+}
+
+template <int LOGN, int NSLOG, bool INV, bool HALF>
+__device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane) {
+  constexpr int n = 1 << LOGN;
+  constexpr int LR = wave_lr(LOGN, NSLOG);
+  constexpr int R = 1 << LR;
+  constexpr int Ns = 1 << NSLOG;
+  constexpr int NBL = n / (64 * R);          // butterflies per lane
+  constexpr int QS = n / (64 * R);           // canonical index step between a butterfly's inputs
+  constexpr bool LAST = (NSLOG + LR == LOGN);
+  if constexpr (!LAST) wave_sync();          // the previous pass's reads of xch are done
+#pragma unroll
+  for (int b = 0; b < NBL; b++) {
+    const int j = lane + 64 * b;
+    const int k = j & (Ns - 1);
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) a[r] = v[b + r * QS];
+    if (NSLOG > 0) {
+      float2 w[R];
+      butterfly_twiddles<R, INV>(tw, k << (LOGN - NSLOG - LR), w);
+#pragma unroll
+      for (int r = 1; r < R; r++) a[r] = cmul(a[r], w[r]);
+    }
+    if (HALF && NSLOG == 0 && R >= 4) {
+      dft_half<R, INV>(a);
+    } else {
+      dft_small<R, INV>(a);
+    }
+    if constexpr (LAST) {
+#pragma unroll
+      for (int s = 0; s < R; s++) v[b + s * QS] = a[s];
+    } else {
+      const int d0 = ((j - k) << LR) + k;
+#pragma unroll
+      for (int s = 0; s < R; s++) xch[xpad(d0 + s * Ns)] = a[s];
+    }
+  }
+  if constexpr (!LAST) {
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < n / 64; q++) v[q] = xch[xpad(lane + 64 * q)];
+    wave_pass<LOGN, NSLOG + LR, INV, false>(v, xch, tw, lane);
+  }
+}
+
+// In-place canonical-distribution FFT of one wavefront.  HALF: inputs with
+// index >= n/2 are known to be zero (v[q] for q >= n/128 are ignored).
+template <int LOGN, bool INV, bool HALF>
+__device__ __forceinline__ void wave_fft(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane) {
+  wave_pass<LOGN, 0, INV, HALF>(v, xch, tw, lane);
+}
+
+// 2Sum accumulate (branch-free): (s, c) += x, value s + c.
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {
+  float t = s + x;
+  float z = t - s;
+  c += (s - (t - z)) + (x - z);
+  s = t;
+}
+
+// Wavefront sum of a float (all lanes get the result).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace ddm

@@ -464,6 +464,430 @@ __global__ __launch_bounds__(NT) void k_finalize(const float* __restrict__ S_par
   if (tid == 0) radP[b] = (nm > 0 && n0 > 0.5f) ? Ptot.x / n0 / (float)nm : 0.f;
 }
 
+// ----------------------------------------------------------------- k_temporal_w
+//
+// Wavefront-per-mode version of k_temporal: each of the 4 waves takes every
+// 4th mode of the item, keeps the mode's series in registers (canonical
+// distribution, t = lane + 64 q), and runs the TPAD-point FFT with
+// wave-private LDS exchanges only.  The first pass skips the zero-padded upper
+// half.  |Y|^2 and p are accumulated per lane with 2Sum; the 4 waves are
+// combined through LDS at the end of the item.
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_temporal_w(const float2* __restrict__ hist, int T,
+                                                   const int4* __restrict__ items, const int* __restrict__ bin_modes,
+                                                   const uint32_t* __restrict__ order, const float* __restrict__ mask,
+                                                   int identity, float inv_n0, const float2* __restrict__ tw_g,
+                                                   float* __restrict__ S_part, float* __restrict__ p_part,
+                                                   float* __restrict__ P_part) {
+  constexpr int n = 1 << LOGT;
+  constexpr int P = n / 64;        // points per lane
+  constexpr int PT = P / 2;        // time samples per lane (T <= n/2)
+  constexpr int XS = xch_size(n);
+  static_assert(PT <= 32, "mask bits");
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float2* xch = lds + wave * XS;                  // per wave
+  float* red = reinterpret_cast<float*>(lds + 4 * XS);  // 4
+  const float2* tw = tw_g;                        // twiddles through the vector cache
+  const int4 it = items[blockIdx.x];
+  const int start = it.y, count = it.z;
+  uint32_t mbits = 0;                             // bit q: sample t = lane + 64 q is valid
+#pragma unroll
+  for (int q = 0; q < PT; q++) {
+    int t = lane + 64 * q;
+    if (t < T && mask[t] != 0.f) mbits |= 1u << q;
+  }
+  float S[P], Sc[P], pa[PT], pc[PT];
+#pragma unroll
+  for (int q = 0; q < P; q++) S[q] = Sc[q] = 0.f;
+#pragma unroll
+  for (int q = 0; q < PT; q++) pa[q] = pc[q] = 0.f;
+  float aP = 0.f, cP = 0.f;
+  for (int mi = wave; mi < count; mi += NT / 64) {
+    const float2* x = hist + (long)bin_modes[start + mi] * T;
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      int t = lane + 64 * q;
+      v[q] = (t < T) ? x[t] : make_float2(0.f, 0.f);
+    }
+    if (!identity) {
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        int t = lane + 64 * q;
+        if (t < T) xch[xpad(t)] = v[q];
+      }
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        int t = lane + 64 * q;
+        v[q] = (t < T) ? xch[xpad((int)order[t])] : make_float2(0.f, 0.f);
+      }
+      wave_sync();
+    }
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
+      v[q] = make_float2(v[q].x * mq, v[q].y * mq);
+      sx += v[q].x;
+      sy += v[q].y;
+      two_sum_add(aP, cP, v[q].x * v[q].x + v[q].y * v[q].y);
+    }
+    const float mx = wave_sum(sx) * inv_n0, my = wave_sum(sy) * inv_n0;
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
+      v[q] = make_float2((v[q].x - mx) * mq, (v[q].y - my) * mq);
+      two_sum_add(pa[q], pc[q], v[q].x * v[q].x + v[q].y * v[q].y);
+    }
+#pragma unroll
+    for (int q = PT; q < P; q++) v[q] = make_float2(0.f, 0.f);
+    wave_fft<LOGT, false, true>(v, xch, tw, lane);
+#pragma unroll
+    for (int q = 0; q < P; q++) two_sum_add(S[q], Sc[q], v[q].x * v[q].x + v[q].y * v[q].y);
+  }
+  // combine the 4 waves: each wave parks (S + Sc) and (pa + pc) in LDS
+  __syncthreads();
+  float* park = reinterpret_cast<float*>(lds);       // reuse the exchange area: 4 * (n + n/2) floats
+#pragma unroll
+  for (int q = 0; q < P; q++) park[wave * (n + n / 2) + lane + 64 * q] = S[q] + Sc[q];
+#pragma unroll
+  for (int q = 0; q < PT; q++) park[wave * (n + n / 2) + n + lane + 64 * q] = pa[q] + pc[q];
+  float pw = wave_sum(aP + cP);
+  if (lane == 0) red[wave] = pw;
+  __syncthreads();
+  float* Sp = S_part + (long)blockIdx.x * n;
+  for (int i = tid; i < n; i += NT) {
+    float s = 0.f, c = 0.f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) two_sum_add(s, c, park[w * (n + n / 2) + i]);
+    Sp[i] = s + c;
+  }
+  float* pp = p_part + (long)blockIdx.x * T;
+  for (int t = tid; t < T; t += NT) {
+    float s = 0.f, c = 0.f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) two_sum_add(s, c, park[w * (n + n / 2) + n + t]);
+    pp[t] = s + c;
+  }
+  if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// ----------------------------------------------------------------- k_bin_reduce / k_bin_lags
+//
+// Radial finalisation in two launches.  k_bin_reduce (one workgroup per bin)
+// sums the bin's partials, inverse-FFTs the summed |Y|^2 to the bin's summed
+// Re G(tau), and stores G[b][tau], p[b][t] and the bin's summed P.
+// k_bin_lags (one workgroup per (bin, 64 lags)) forms the masked correlations
+// A(tau) = sum_t m(t) p(t+tau), B(tau) = sum_t p(t) m(t+tau) with lanes over
+// tau and waves over quarters of t, and D = (A + B - 2G) / N(tau) / nModes.
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_bin_reduce(const float* __restrict__ S_part, const float* __restrict__ p_part,
+                                                   const float* __restrict__ P_part, const int* __restrict__ bin_items,
+                                                   int T, int L, const float2* __restrict__ tw_g,
+                                                   float* __restrict__ Gbin, float* __restrict__ pbin,
+                                                   float* __restrict__ Pbin) {
+  constexpr int n = 1 << LOGT;
+  extern __shared__ float2 lds[];
+  float2* buf = lds;
+  float2* tw = lds + n;
+  float* red = reinterpret_cast<float*>(tw + n);
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  const int i0 = bin_items[b], i1 = bin_items[b + 1];
+  for (int i = tid; i < n; i += NT) {
+    tw[i] = tw_g[i];
+    float s = 0.f, c = 0.f;
+    for (int it = i0; it < i1; it++) two_sum_add(s, c, S_part[(long)it * n + i]);
+    buf[i] = make_float2(s + c, 0.f);
+  }
+  for (int t = tid; t < T; t += NT) {
+    float s = 0.f, c = 0.f;
+    for (int it = i0; it < i1; it++) two_sum_add(s, c, p_part[(long)it * T + t]);
+    pbin[(long)b * T + t] = s + c;
+  }
+  float ps = 0.f, pc = 0.f;
+  for (int it = i0 + tid; it < i1; it += NT) two_sum_add(ps, pc, P_part[it]);
+  float2 Ptot = block_sum2(make_float2(ps + pc, 0.f), red, tid);  // barriers inside
+  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
+  const float invn = 1.0f / n;
+  for (int tau = tid; tau < L; tau += NT) Gbin[(long)b * L + tau] = buf[tau].x * invn;
+  if (tid == 0) Pbin[b] = Ptot.x;
+}
+
+__global__ __launch_bounds__(NT) void k_bin_lags(const float* __restrict__ Gbin, const float* __restrict__ pbin,
+                                                 const float* __restrict__ Pbin, const uint32_t* __restrict__ nmodes,
+                                                 const float* __restrict__ mask01, const float* __restrict__ npairs,
+                                                 int T, int L, float n0, float* __restrict__ radD,
+                                                 float* __restrict__ radP) {
+  extern __shared__ float2 lds[];
+  float* p = reinterpret_cast<float*>(lds);      // T
+  float* m = p + T;                              // T
+  float* part = m + T;                           // 4 waves x 64 lanes x 4 (sa, ca, sb, cb)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  const int tau = blockIdx.y * 64 + lane;
+  for (int t = tid; t < T; t += NT) {
+    p[t] = pbin[(long)b * T + t];
+    m[t] = mask01[t];
+  }
+  __syncthreads();
+  float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
+  if (tau < L) {
+    const int tend = T - tau;                 // t + tau < T
+    const int q = (tend + 3) / 4;
+    const int t0 = wave * q, t1 = min(tend, t0 + q);
+    for (int t = t0; t < t1; t++) {
+      two_sum_add(sa, ca, m[t] * p[t + tau]);
+      two_sum_add(sb, cb, p[t] * m[t + tau]);
+    }
+  }
+  float* pw = part + 4 * (wave * 64 + lane);
+  pw[0] = sa;
+  pw[1] = ca;
+  pw[2] = sb;
+  pw[3] = cb;
+  __syncthreads();
+  if (wave == 0 && tau < L) {
+    float s = 0.f, c = 0.f;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      const float* q4 = part + 4 * (w * 64 + lane);
+      two_sum_add(s, c, q4[0]);
+      two_sum_add(s, c, q4[2]);
+      c += q4[1] + q4[3];
+    }
+    const float np = npairs[tau];
+    const uint32_t nm = nmodes[b];
+    const float g = Gbin[(long)b * L + tau];
+    float d = 0.f;
+    if (np > 0.5f && nm > 0) d = ((s + c) - 2.0f * g) / np / (float)nm;
+    radD[(long)b * L + tau] = d;
+  }
+  if (tid == 0 && blockIdx.y == 0) {
+    const uint32_t nm = nmodes[b];
+    radP[b] = (nm > 0 && n0 > 0.5f) ? Pbin[b] / n0 / (float)nm : 0.f;
+  }
+}
+
+// ----------------------------------------------------------------- k_rows_w / k_cols_w
+//
+// Wavefront versions of the spatial passes for N >= 512.  k_rows_w: each wave
+// transforms two real rows at a time (packed as one complex FFT, canonical
+// x = lane + 64 q), untangles them through its LDS exchange buffer and parks
+// the kx values of its rows in an LDS tile; the tile [kx][16 rows] is stored
+// with 16-row (128 B) runs.  k_cols_w: each wave transforms F/4 frames' column
+// kx in registers; the results meet in an LDS tile [ky][F] that is stored as
+// F-frame runs of the history (the corner turn).
+
+template <typename T> __device__ __forceinline__ float px_to_float(T v) { return (float)v; }
+
+template <int LOGN, typename Tin>
+__global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, int nframes, float scale,
+                                               const float* __restrict__ win, const float2* __restrict__ tw,
+                                               float2* __restrict__ scratch, int2* __restrict__ rowinfo) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
+  constexpr int RW = 16;                       // rows per workgroup (4 per wave)
+  constexpr int XS = xch_size(N);
+  constexpr int TS = RW + 1;                   // padded tile row
+  constexpr bool IS_INT = !std::is_same<Tin, float>::value;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float2* xch = lds + wave * XS;
+  float2* tile = lds + 4 * XS;                 // NH x TS
+  const int groups = N / RW;
+  const int f = blockIdx.x / groups;
+  const int y0 = (blockIdx.x % groups) * RW;
+  if (f >= nframes) return;
+#pragma unroll 1
+  for (int pr = 0; pr < 2; pr++) {
+    const int ra = wave * 4 + 2 * pr;          // rows ra, ra+1 of the tile
+    const Tin* rowa = frames + ((long)f * N + y0 + ra) * N;
+    const Tin* rowb = rowa + N;
+    float xa[P], xb[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      xa[q] = px_to_float(rowa[lane + 64 * q]);
+      xb[q] = px_to_float(rowb[lane + 64 * q]);
+    }
+    float ha = 0.f, hb = 0.f, la = 0.f, lb = 0.f;
+    int Sa = 0, Sb = 0;
+    if constexpr (IS_INT) {
+      // exact integer row sums
+      int ia = 0, ib = 0;
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        ia += (int)xa[q];
+        ib += (int)xb[q];
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ia += __shfl_xor(ia, o, 64);
+        ib += __shfl_xor(ib, o, 64);
+      }
+      Sa = ia;
+      Sb = ib;
+    } else {
+      // row mean as an unevaluated (hi, lo) pair: hi from the sum, lo from the residuals
+      float sa = 0.f, sb = 0.f;
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        sa += xa[q];
+        sb += xb[q];
+      }
+      ha = wave_sum(sa) * (1.0f / N);
+      hb = wave_sum(sb) * (1.0f / N);
+      float ra2 = 0.f, rb2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        ra2 += xa[q] - ha;
+        rb2 += xb[q] - hb;
+      }
+      la = wave_sum(ra2) * (1.0f / N);
+      lb = wave_sum(rb2) * (1.0f / N);
+    }
+    const float wya = win[y0 + ra], wyb = win[y0 + ra + 1];
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      const int x = lane + 64 * q;
+      const float wx = win[x];
+      float da, db;
+      if constexpr (IS_INT) {
+        da = (float)((int)xa[q] * N - Sa) * (scale / N);
+        db = (float)((int)xb[q] * N - Sb) * (scale / N);
+      } else {
+        da = ((xa[q] - ha) - la) * scale;
+        db = ((xb[q] - hb) - lb) * scale;
+      }
+      v[q] = make_float2(da * wx * wya, db * wx * wyb);
+    }
+    if (lane == 0) {
+      if constexpr (IS_INT) {
+        rowinfo[(long)f * N + y0 + ra] = make_int2(Sa, 0);
+        rowinfo[(long)f * N + y0 + ra + 1] = make_int2(Sb, 0);
+      } else {
+        rowinfo[(long)f * N + y0 + ra] = make_int2(__float_as_int(ha), __float_as_int(la));
+        rowinfo[(long)f * N + y0 + ra + 1] = make_int2(__float_as_int(hb), __float_as_int(lb));
+      }
+    }
+    wave_fft<LOGN, false, false>(v, xch, tw, lane);
+    // untangle: X_a(k) = (Z(k) + conj Z(N-k))/2, X_b(k) = (Z(k) - conj Z(N-k))/(2i)
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < P; q++) xch[xpad(lane + 64 * q)] = v[q];
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q <= P / 2; q++) {
+      const int k = lane + 64 * q;
+      if (k <= N / 2) {
+        const float2 z = v[q];
+        const float2 zc = cconj(xch[xpad((N - k) & (N - 1))]);
+        const float2 u = csub(z, zc);
+        tile[k * TS + ra] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        tile[k * TS + ra + 1] = make_float2(0.5f * u.y, -0.5f * u.x);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < NH * RW; e += NT) {
+    const int r = e % RW, kx = e / RW;
+    scratch[((long)f * NH + kx) * N + y0 + r] = tile[kx * TS + r];
+  }
+}
+
+template <int LOGN, int F>
+__global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratch, const int2* __restrict__ rowinfo,
+                                               int nframes, int is_int, float scale, const float* __restrict__ win,
+                                               const float2* __restrict__ tw, const float2* __restrict__ wx,
+                                               float2* __restrict__ hist, int T, int first_slot) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
+  constexpr int FW = F / 4;                    // frames per wave
+  constexpr int XS = xch_size(N);
+  constexpr int TS = F + 1;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float2* xch = lds + wave * XS;
+  float2* tile = lds;                          // N x TS, reused after every wave's FFTs
+  const int kx = blockIdx.x;
+  const int f0 = blockIdx.y * F;
+  float2 res[FW][P];
+#pragma unroll
+  for (int i = 0; i < FW; i++) {
+    const int fl = wave * FW + i;
+    const int fr = f0 + fl;
+    float2* v = res[i];
+    if (fr < nframes) {
+      const float2* col = scratch + ((long)fr * NH + kx) * N;
+#pragma unroll
+      for (int q = 0; q < P; q++) v[q] = col[lane + 64 * q];
+      if (kx <= 3) {
+        const int2* ri = rowinfo + (long)fr * N;
+        const float2 w = wx[kx];
+        if (is_int) {
+          long long s = 0;
+#pragma unroll
+          for (int q = 0; q < P; q++) s += ri[lane + 64 * q].x;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            const int y = lane + 64 * q;
+            const float d = (float)((long long)N * ri[y].x - s) * (scale / ((float)N * (float)N));
+            const float c = d * win[y];
+            v[q] = cadd(v[q], make_float2(c * w.x, c * w.y));
+          }
+        } else {
+          float sh = 0.f, ch = 0.f, sl = 0.f, cl = 0.f;
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            two_sum_add(sh, ch, __int_as_float(ri[lane + 64 * q].x));
+            two_sum_add(sl, cl, __int_as_float(ri[lane + 64 * q].y));
+          }
+          // wave-wide compensated sums: reduce (s, c) pairs lane by lane
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            float osh = __shfl_xor(sh, o, 64), och = __shfl_xor(ch, o, 64);
+            float osl = __shfl_xor(sl, o, 64), ocl = __shfl_xor(cl, o, 64);
+            two_sum_add(sh, ch, osh);
+            ch += och;
+            two_sum_add(sl, cl, osl);
+            cl += ocl;
+          }
+          const float mh = sh * (1.0f / N), mc = ch * (1.0f / N), ml = (sl + cl) * (1.0f / N);
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            const int y = lane + 64 * q;
+            const float d = (((__int_as_float(ri[y].x) - mh) - mc) + (__int_as_float(ri[y].y) - ml)) * scale;
+            const float c = d * win[y];
+            v[q] = cadd(v[q], make_float2(c * w.x, c * w.y));
+          }
+        }
+      }
+      wave_fft<LOGN, false, false>(res[i], xch, tw, lane);
+    }
+  }
+  __syncthreads();   // every wave is done with its exchange buffer: the tile reuses the space
+  const float sc = 1.0f / N;
+#pragma unroll
+  for (int i = 0; i < FW; i++) {
+    const int fl = wave * FW + i;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      const float2 z = res[i][q];
+      tile[(lane + 64 * q) * TS + fl] = make_float2(z.x * sc, z.y * sc);
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < N * F; e += NT) {
+    const int fl = e % F, ky = e / F;
+    if (f0 + fl >= nframes) continue;
+    const int slot = (first_slot + f0 + fl) % T;
+    hist[((long)ky * NH + kx) * T + slot] = tile[ky * TS + fl];
+  }
+}
+
 // ----------------------------------------------------------------- launchers
 
 template <typename Tin, int LOGN>
@@ -589,6 +1013,129 @@ hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, c
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
   }
+  return hipErrorInvalidValue;
+}
+
+template <int LOGT>
+static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+                                      const uint32_t* order, const float* mask, int identity, float inv_n0,
+                                      const float2* tw, float* S_part, float* p_part, float* P_part, hipStream_t st) {
+  constexpr int n = 1 << LOGT;
+  size_t lds = sizeof(float2) * (4 * xch_size(n)) + sizeof(float) * 4;
+  static_assert(4 * (n + n / 2) <= 2 * 4 * xch_size(n), "park area must fit in the exchange buffers");
+  auto kern = k_temporal_w<LOGT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
+                     tw, S_part, p_part, P_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+                             const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
+                             float* S_part, float* p_part, float* P_part, hipStream_t st) {
+  switch (logt) {
+#define CASE(LG_)                                                                                               \
+  case LG_:                                                                                                   \
+    return launch_temporal_w_t<LG_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, \
+                                    p_part, P_part, st);
+    CASE(8) CASE(9) CASE(10) CASE(11)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int LOGT>
+static hipError_t launch_bin_reduce_t(const float* S_part, const float* p_part, const float* P_part,
+                                      const int* bin_items, int NB, int T, int L, const float2* tw, float* Gbin,
+                                      float* pbin, float* Pbin, hipStream_t st) {
+  constexpr int n = 1 << LOGT;
+  size_t lds = sizeof(float2) * 2 * n + sizeof(float) * 2 * (NT / 64);
+  auto kern = k_bin_reduce<LOGT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, S_part, p_part, P_part, bin_items, T, L, tw, Gbin, pbin, Pbin);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_finalize(int logt, const float* S_part, const float* p_part, const float* P_part,
+                               const int* bin_items, int NB, const uint32_t* nmodes, const float* mask,
+                               const float* npairs, int T, int L, float n0, const float2* tw, float* Gbin,
+                               float* pbin, float* Pbin, float* radD, float* radP, hipStream_t st) {
+  hipError_t e = hipErrorInvalidValue;
+  switch (logt) {
+#define CASE(LG_)                                                                                                \
+  case LG_:                                                                                                    \
+    e = launch_bin_reduce_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, tw, Gbin, pbin, Pbin, st);         \
+    break;
+    CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+#undef CASE
+  }
+  if (e != hipSuccess) return e;
+  size_t lds = sizeof(float) * (2 * T + 4 * NT);
+  e = hipFuncSetAttribute((const void*)k_bin_lags, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bin_lags, dim3(NB, (L + 63) / 64), dim3(NT), lds, st, Gbin, pbin, Pbin, nmodes, mask, npairs,
+                     T, L, n0, radD, radP);
+  return hipGetLastError();
+}
+
+template <typename Tin, int LOGN>
+static hipError_t launch_rows_w_t(const void* frames, int nframes, float scale, const float* win, const float2* tw,
+                                  float2* scratch, int2* rowinfo, hipStream_t st) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
+  size_t lds = sizeof(float2) * (4 * xch_size(N) + NH * 17);
+  auto kern = k_rows_w<LOGN, Tin>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(nframes * (N / 16)), dim3(NT), lds, st, (const Tin*)frames, nframes, scale, win, tw,
+                     scratch, rowinfo);
+  return hipGetLastError();
+}
+
+template <int LOGN, int F>
+static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+                                  const float* win, const float2* tw, const float2* wx, float2* hist, int T,
+                                  int first_slot, hipStream_t st) {
+  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
+  size_t tile = sizeof(float2) * N * (F + 1), xch = sizeof(float2) * 4 * xch_size(N);
+  size_t lds = tile > xch ? tile : xch;
+  auto kern = k_cols_w<LOGN, F>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(NH, (nframes + F - 1) / F), dim3(NT), lds, st, scratch, rowinfo, nframes, is_int, scale,
+                     win, tw, wx, hist, T, first_slot);
+  return hipGetLastError();
+}
+
+// Wavefront spatial kernels: N = 512 and 1024 (registers hold N/64 points
+// per lane); F = frames per column workgroup (8 or 16).
+hipError_t launch_rows_w(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
+                         const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
+  if (logn == 9) {
+    return dtype == 0 ? launch_rows_w_t<uint8_t, 9>(frames, nframes, scale, win, tw, scratch, rowinfo, st)
+           : dtype == 1 ? launch_rows_w_t<uint16_t, 9>(frames, nframes, scale, win, tw, scratch, rowinfo, st)
+                        : launch_rows_w_t<float, 9>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  }
+  if (logn == 10) {
+    return dtype == 0 ? launch_rows_w_t<uint8_t, 10>(frames, nframes, scale, win, tw, scratch, rowinfo, st)
+           : dtype == 1 ? launch_rows_w_t<uint16_t, 10>(frames, nframes, scale, win, tw, scratch, rowinfo, st)
+                        : launch_rows_w_t<float, 10>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
+                         float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
+                         int first_slot, hipStream_t st) {
+  if (logn == 9 && F == 16)
+    return launch_cols_w_t<9, 16>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+  if (logn == 9 && F == 8)
+    return launch_cols_w_t<9, 8>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+  if (logn == 10 && F == 8)
+    return launch_cols_w_t<10, 8>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+  if (logn == 10 && F == 16)
+    return launch_cols_w_t<10, 16>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
   return hipErrorInvalidValue;
 }
 
