@@ -236,6 +236,7 @@ struct ddm_engine {
   float* win = nullptr;
   float2* tw_n = nullptr;
   float2* tw_t = nullptr;
+  float2* tw_f = nullptr;   // twiddles of the temporal sub-transform length (min(TPAD, 512))
   float2* wx = nullptr;
   int4* items = nullptr;
   int* bin_modes = nullptr;
@@ -319,7 +320,7 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 }
 
 void free_all(ddm_engine* e) {
-  void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->wx,
+  void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
                   e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin};
   for (void* p : ptrs)
@@ -407,6 +408,7 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
       (he = dalloc(&e->rowinfo, (size_t)chunk * g.N)) != hipSuccess ||
       (he = hipMalloc(&e->stage, e->stage_bytes)) != hipSuccess || (he = dalloc(&e->win, g.N)) != hipSuccess ||
       (he = dalloc(&e->tw_n, g.N)) != hipSuccess || (he = dalloc(&e->tw_t, e->TPAD)) != hipSuccess ||
+      (he = dalloc(&e->tw_f, std::min(e->TPAD, 512))) != hipSuccess ||
       (he = dalloc(&e->wx, 4)) != hipSuccess || (he = dalloc(&e->items, nitems)) != hipSuccess ||
       (he = dalloc(&e->bin_modes, bin_modes_h.size())) != hipSuccess ||
       (he = dalloc(&e->bin_items, g.NB + 1)) != hipSuccess || (he = dalloc(&e->nmodes, g.NB)) != hipSuccess ||
@@ -443,12 +445,13 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
     wx_h[2 * kx] = (float)re;
     wx_h[2 * kx + 1] = (float)im;
   }
-  std::vector<float> twn = twiddles(g.N), twt = twiddles(e->TPAD);
+  std::vector<float> twn = twiddles(g.N), twt = twiddles(e->TPAD), twf = twiddles(std::min(e->TPAD, 512));
   std::vector<int4> items4(nitems, int4{0, 0, 0, 0});
   std::memcpy(items4.data(), items_h.data(), items_h.size() * sizeof(int));
   if ((he = hipMemcpy(e->win, w32.data(), g.N * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->tw_n, twn.data(), twn.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->tw_t, twt.data(), twt.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->tw_f, twf.data(), twf.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->wx, wx_h.data(), 32, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->items, items4.data(), items4.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->bin_modes, bin_modes_h.data(), bin_modes_h.size() * 4, hipMemcpyHostToDevice)) !=
@@ -602,10 +605,10 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
   } else if (e->nitems > 0 &&
              (he = timed(e, DDM_K_TEMPORAL, [&] {
                 const float inv_n0 = n0 > 0.5f ? 1.0f / n0 : 0.0f;
-                if (e->logt <= 11)   // wavefront-per-mode kernel (registers hold TPAD/64 points per lane)
+                if (e->logt <= 11)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
                   return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
-                                                e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part,
-                                                e->P_part, e->stream);
+                                                e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->tw_f, e->S_part,
+                                                e->p_part, e->P_part, e->stream);
                 return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
                                             identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part,
                                             e->stream);

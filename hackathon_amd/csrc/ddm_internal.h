@@ -21,7 +21,7 @@ hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, c
 
 hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                             float* S_part, float* p_part, float* P_part, hipStream_t st);
+                             const float2* tw_f, float* S_part, float* p_part, float* P_part, hipStream_t st);
 hipError_t launch_bin_finalize(int logt, const float* S_part, const float* p_part, const float* P_part,
                                const int* bin_items, int NB, const uint32_t* nmodes, const float* mask,
                                const float* npairs, int T, int L, float n0, const float2* tw, float* Gbin,

@@ -32,6 +32,14 @@ __host__ __device__ constexpr int rows_pairs_per_wg(int logn) {
                                                                 : (1 << (logn - 1)));
 }
 
+// float2 slots per wave of k_temporal_w: the park area (n floats of |Y|^2 +
+// n/2 floats of p) behind the sub-transform exchange buffer when the
+// transform is split, else whichever of the two is larger (they alias).
+__host__ __device__ constexpr int temporal_wave_lds(int logt, int lm) {
+  return lm ? xch_size(1 << (logt - lm)) + (3 << logt) / 4
+            : (xch_size(1 << logt) > (3 << logt) / 4 ? xch_size(1 << logt) : (3 << logt) / 4);
+}
+
 // ----------------------------------------------------------------- helpers
 
 template <typename T> struct vec8b;
@@ -466,111 +474,180 @@ __global__ __launch_bounds__(NT) void k_finalize(const float* __restrict__ S_par
 
 // ----------------------------------------------------------------- k_temporal_w
 //
-// Wavefront-per-mode version of k_temporal: each of the 4 waves takes every
-// 4th mode of the item, keeps the mode's series in registers (canonical
-// distribution, t = lane + 64 q), and runs the TPAD-point FFT with
-// wave-private LDS exchanges only.  The first pass skips the zero-padded upper
-// half.  |Y|^2 and p are accumulated per lane with 2Sum; the 4 waves are
-// combined through LDS at the end of the item.
-template <int LOGT>
-__global__ __launch_bounds__(NT) void k_temporal_w(const float2* __restrict__ hist, int T,
-                                                   const int4* __restrict__ items, const int* __restrict__ bin_modes,
-                                                   const uint32_t* __restrict__ order, const float* __restrict__ mask,
-                                                   int identity, float inv_n0, const float2* __restrict__ tw_g,
-                                                   float* __restrict__ S_part, float* __restrict__ p_part,
-                                                   float* __restrict__ P_part) {
+// Wavefront-per-mode version of k_temporal.  The item's modes are processed in
+// rounds of 4, one per wave.  The wave keeps the mode's series y(t), t < n/2,
+// in registers (canonical distribution t = lane + 64 q).  Because y is zero
+// for t >= n/2, the n-point transform splits exactly into m = 2^LM independent
+// (n/m)-point transforms (first decimation-in-frequency stage):
+//   Y(m k + r) = FFT_{n/m}[ u_r ](k),  u_r(t1) = w_n^{r t1} sum_{t2} y(t1 + t2 n/m) w_m^{r t2}
+// (only t2 < m/2 contribute), so a wave never holds more than n/m points of a
+// transform (m = 1 for n <= 512, whose first pass skips the zero half
+// instead; 512-point sub-transforms above).  Each sub-transform uses wave-private LDS exchanges; |Y|^2 and p
+// are parked in the wave's buffer and, after a workgroup barrier, every thread
+// adds its slice of the 4 parked spectra into 2Sum accumulators.  The next
+// round's series is loaded while the current one is transformed.
+template <int LOGT, int LM>
+__global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__ hist, int T,
+                                                      const int4* __restrict__ items,
+                                                      const int* __restrict__ bin_modes,
+                                                      const uint32_t* __restrict__ order,
+                                                      const float* __restrict__ mask, int identity, float inv_n0,
+                                                      const float2* __restrict__ tw_g, const float2* __restrict__ tw_f,
+                                                      float* __restrict__ S_part, float* __restrict__ p_part,
+                                                      float* __restrict__ P_part) {
   constexpr int n = 1 << LOGT;
-  constexpr int P = n / 64;        // points per lane
-  constexpr int PT = P / 2;        // time samples per lane (T <= n/2)
-  constexpr int XS = xch_size(n);
+  constexpr int m = 1 << LM;
+  constexpr int LOGF = LOGT - LM;  // sub-transform size
+  constexpr int NF = 1 << LOGF;
+  constexpr int PF = NF / 64;      // points per lane of a sub-transform
+  constexpr int PT = n / 128;      // time samples per lane (T <= n/2)
+  constexpr int WS = temporal_wave_lds(LOGT, LM);  // float2 per wave
+  constexpr int KS = n / NT;       // frequencies per thread in the accumulation
+  constexpr int TS = (n / 2) / NT > 0 ? (n / 2) / NT : 1;
   static_assert(PT <= 32, "mask bits");
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  float2* xch = lds + wave * XS;                  // per wave
-  float* red = reinterpret_cast<float*>(lds + 4 * XS);  // 4
-  const float2* tw = tw_g;                        // twiddles through the vector cache
+  float2* xch = lds + wave * WS;
+  float* red = reinterpret_cast<float*>(lds + 4 * WS);  // 4
   const int4 it = items[blockIdx.x];
   const int start = it.y, count = it.z;
-  uint32_t mbits = 0;                             // bit q: sample t = lane + 64 q is valid
+  uint32_t mbits = 0;
 #pragma unroll
   for (int q = 0; q < PT; q++) {
     int t = lane + 64 * q;
     if (t < T && mask[t] != 0.f) mbits |= 1u << q;
   }
-  float S[P], Sc[P], pa[PT], pc[PT];
+  float S[KS], Sc[KS], pa[TS], pc[TS];
 #pragma unroll
-  for (int q = 0; q < P; q++) S[q] = Sc[q] = 0.f;
+  for (int i = 0; i < KS; i++) S[i] = Sc[i] = 0.f;
 #pragma unroll
-  for (int q = 0; q < PT; q++) pa[q] = pc[q] = 0.f;
+  for (int i = 0; i < TS; i++) pa[i] = pc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
-  for (int mi = wave; mi < count; mi += NT / 64) {
-    const float2* x = hist + (long)bin_modes[start + mi] * T;
-    float2 v[P];
+  const int rounds = (count + 3) / 4;
+  // next round's series, loaded while the current one is transformed (up to
+  // n = 1024; above that the registers are needed by the split transforms)
+  constexpr bool PREFETCH = LOGT <= 10;
+  float2 nx[PREFETCH ? PT : 1];
+  if constexpr (PREFETCH) {
+    const int mi = wave;
+    const float2* x = hist + (long)bin_modes[start + (mi < count ? mi : 0)] * T;
 #pragma unroll
     for (int q = 0; q < PT; q++) {
       int t = lane + 64 * q;
-      v[q] = (t < T) ? x[t] : make_float2(0.f, 0.f);
+      nx[q] = (t < T && mi < count) ? x[t] : make_float2(0.f, 0.f);
+    }
+  }
+  float* park = reinterpret_cast<float*>(xch);
+  for (int rd = 0; rd < rounds; rd++) {
+    const int mi = rd * 4 + wave;
+    const bool active = mi < count;
+    float2 y[PT];
+    if constexpr (PREFETCH) {
+#pragma unroll
+      for (int q = 0; q < PT; q++) y[q] = nx[q];
+      const int mn = mi + 4;
+      const float2* x = hist + (long)bin_modes[start + (mn < count ? mn : 0)] * T;
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        int t = lane + 64 * q;
+        nx[q] = (t < T && mn < count) ? x[t] : make_float2(0.f, 0.f);
+      }
+    } else {
+      const float2* x = hist + (long)bin_modes[start + (active ? mi : 0)] * T;
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        int t = lane + 64 * q;
+        y[q] = (t < T && active) ? x[t] : make_float2(0.f, 0.f);
+      }
     }
     if (!identity) {
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         int t = lane + 64 * q;
-        if (t < T) xch[xpad(t)] = v[q];
+        if (t < T) xch[xpad(t)] = y[q];
       }
       wave_sync();
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         int t = lane + 64 * q;
-        v[q] = (t < T) ? xch[xpad((int)order[t])] : make_float2(0.f, 0.f);
+        y[q] = (t < T) ? xch[xpad((int)order[t])] : make_float2(0.f, 0.f);
       }
-      wave_sync();
     }
     float sx = 0.f, sy = 0.f;
 #pragma unroll
     for (int q = 0; q < PT; q++) {
       const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
-      v[q] = make_float2(v[q].x * mq, v[q].y * mq);
-      sx += v[q].x;
-      sy += v[q].y;
-      two_sum_add(aP, cP, v[q].x * v[q].x + v[q].y * v[q].y);
+      y[q] = make_float2(y[q].x * mq, y[q].y * mq);
+      sx += y[q].x;
+      sy += y[q].y;
+      two_sum_add(aP, cP, y[q].x * y[q].x + y[q].y * y[q].y);
     }
     const float mx = wave_sum(sx) * inv_n0, my = wave_sum(sy) * inv_n0;
 #pragma unroll
     for (int q = 0; q < PT; q++) {
       const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
-      v[q] = make_float2((v[q].x - mx) * mq, (v[q].y - my) * mq);
-      two_sum_add(pa[q], pc[q], v[q].x * v[q].x + v[q].y * v[q].y);
+      y[q] = make_float2((y[q].x - mx) * mq, (y[q].y - my) * mq);
     }
+    if constexpr (LM == 0) {
+      float2 v[PF];
 #pragma unroll
-    for (int q = PT; q < P; q++) v[q] = make_float2(0.f, 0.f);
-    wave_fft<LOGT, false, true>(v, xch, tw, lane);
+      for (int q = 0; q < PT; q++) v[q] = y[q];
 #pragma unroll
-    for (int q = 0; q < P; q++) two_sum_add(S[q], Sc[q], v[q].x * v[q].x + v[q].y * v[q].y);
+      for (int q = PT; q < PF; q++) v[q] = make_float2(0.f, 0.f);
+      wave_fft<LOGF, false, true>(v, xch, tw_f, lane);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < PF; q++) park[lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
+#pragma unroll
+      for (int q = 0; q < PT; q++) park[n + lane + 64 * q] = active ? y[q].x * y[q].x + y[q].y * y[q].y : 0.f;
+    } else {
+      // u_r(t1) = w_n^{r t1} sum_{t2 < m/2} y(t1 + t2 n/m) w_m^{r t2}: for
+      // t1 = lane + 64 q the samples t1 + t2 n/m = lane + 64 (q + t2 PF) are
+      // in this lane's own registers.  |Y(m k + r)|^2 is parked after each
+      // sub-transform in the area behind the exchange buffer.
+      float* parky = reinterpret_cast<float*>(xch + xch_size(NF));
+#pragma unroll 1
+      for (int r = 0; r < m; r++) {
+        float2 v[PF];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+          float2 u = y[q];
+#pragma unroll
+          for (int t2 = 1; t2 < m / 2; t2++) u = cadd(u, cmul(y[q + t2 * PF], tw_g[((r * t2) & (m - 1)) * NF]));
+          v[q] = r ? cmul(u, tw_g[r * (lane + 64 * q)]) : u;
+        }
+        wave_fft<LOGF, false, false>(v, xch, tw_f, lane);
+#pragma unroll
+        for (int q = 0; q < PF; q++) parky[m * (lane + 64 * q) + r] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < PT; q++) parky[n + lane + 64 * q] = active ? y[q].x * y[q].x + y[q].y * y[q].y : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+      const float* pw = reinterpret_cast<const float*>(lds + w * WS + (LM ? xch_size(NF) : 0));
+#pragma unroll
+      for (int i = 0; i < KS; i++) two_sum_add(S[i], Sc[i], pw[tid + NT * i]);
+#pragma unroll
+      for (int i = 0; i < TS; i++)
+        if (tid + NT * i < n / 2) two_sum_add(pa[i], pc[i], pw[n + tid + NT * i]);
+    }
+    __syncthreads();
   }
-  // combine the 4 waves: each wave parks (S + Sc) and (pa + pc) in LDS
-  __syncthreads();
-  float* park = reinterpret_cast<float*>(lds);       // reuse the exchange area: 4 * (n + n/2) floats
+  float* Sp = S_part + (long)blockIdx.x * n;
 #pragma unroll
-  for (int q = 0; q < P; q++) park[wave * (n + n / 2) + lane + 64 * q] = S[q] + Sc[q];
+  for (int i = 0; i < KS; i++) Sp[tid + NT * i] = S[i] + Sc[i];
+  float* pp = p_part + (long)blockIdx.x * T;
 #pragma unroll
-  for (int q = 0; q < PT; q++) park[wave * (n + n / 2) + n + lane + 64 * q] = pa[q] + pc[q];
-  float pw = wave_sum(aP + cP);
+  for (int i = 0; i < TS; i++) {
+    const int t = tid + NT * i;
+    if (t < T) pp[t] = pa[i] + pc[i];
+  }
+  const float pw = wave_sum(aP + cP);
   if (lane == 0) red[wave] = pw;
   __syncthreads();
-  float* Sp = S_part + (long)blockIdx.x * n;
-  for (int i = tid; i < n; i += NT) {
-    float s = 0.f, c = 0.f;
-#pragma unroll
-    for (int w = 0; w < NT / 64; w++) two_sum_add(s, c, park[w * (n + n / 2) + i]);
-    Sp[i] = s + c;
-  }
-  float* pp = p_part + (long)blockIdx.x * T;
-  for (int t = tid; t < T; t += NT) {
-    float s = 0.f, c = 0.f;
-#pragma unroll
-    for (int w = 0; w < NT / 64; w++) two_sum_add(s, c, park[w * (n + n / 2) + n + t]);
-    pp[t] = s + c;
-  }
   if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
@@ -1016,30 +1093,31 @@ hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, c
   return hipErrorInvalidValue;
 }
 
-template <int LOGT>
+template <int LOGT, int LM>
 static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
-                                      const float2* tw, float* S_part, float* p_part, float* P_part, hipStream_t st) {
-  constexpr int n = 1 << LOGT;
-  size_t lds = sizeof(float2) * (4 * xch_size(n)) + sizeof(float) * 4;
-  static_assert(4 * (n + n / 2) <= 2 * 4 * xch_size(n), "park area must fit in the exchange buffers");
-  auto kern = k_temporal_w<LOGT>;
+                                      const float2* tw, const float2* tw_f, float* S_part, float* p_part,
+                                      float* P_part, hipStream_t st) {
+  size_t lds = sizeof(float2) * 4 * temporal_wave_lds(LOGT, LM) + sizeof(float) * 4;
+  auto kern = k_temporal_w<LOGT, LM>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
-                     tw, S_part, p_part, P_part);
+                     tw, tw_f, S_part, p_part, P_part);
   return hipGetLastError();
 }
 
+// tw: exp(-2 pi i j/n) for the TPAD length n; tw_f: the same for the
+// sub-transform length n/m (m = 1 for n <= 512, n/512 above).
 hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                             float* S_part, float* p_part, float* P_part, hipStream_t st) {
+                             const float2* tw_f, float* S_part, float* p_part, float* P_part, hipStream_t st) {
   switch (logt) {
-#define CASE(LG_)                                                                                               \
-  case LG_:                                                                                                   \
-    return launch_temporal_w_t<LG_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, \
-                                    p_part, P_part, st);
-    CASE(8) CASE(9) CASE(10) CASE(11)
+#define CASE(LG_, LM_)                                                                                              \
+  case LG_:                                                                                                         \
+    return launch_temporal_w_t<LG_, LM_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, tw_f, \
+                                         S_part, p_part, P_part, st);
+    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
 #undef CASE
   }
   return hipErrorInvalidValue;
