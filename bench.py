@@ -105,30 +105,35 @@ def main():
     fpx = T * N * N
     value = fpx / (ms_step * 1e-3)
 
-    # algorithmic bytes per launch (per frame-pixel: raw read once, the f32
-    # complex half-plane spectrum written once and read once)
+    # Algorithmic bytes per frame-pixel: the raw frame read once plus the f32
+    # complex half-plane spectrum written once and read once.  Per launch:
+    # spatial rows = frames read, spatial cols = spectrum write, temporal =
+    # spectrum read (all modes, by definition; it reads only the binned ones).
     NH = N // 2 + 1
     spec_bytes = T * N * NH * 8
     raw_bytes = T * N * N * 2
-    algo = {"spatial_rows": None, "spatial_cols_cornerturn": None, "temporal_wk": spec_bytes,
-            "radial_finalize": None}
+    per_step_algo = {"spatial_rows": raw_bytes, "spatial_cols_cornerturn": spec_bytes, "temporal_wk": spec_bytes,
+                     "radial_finalize": 0}
+    kernel_symbol = {"spatial_rows": "ddm::k_rows_w<9, unsigned short>", "spatial_cols_cornerturn": "ddm::k_cols_w<9, 16>",
+                     "temporal_wk": "ddm::k_temporal_w<11, 2>", "radial_finalize": "ddm::k_bin_reduce<11>"}
     per_kernel = {}
     for name, (ms, cnt) in times.items():
         per_kernel[name] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps,
                             "avg_launch_ms": ms / max(cnt, 1)}
-    spatial_ms = (times["spatial_rows"][0] + times["spatial_cols_cornerturn"][0]) / a.steps
-    temporal_ms = times["temporal_wk"][0] / a.steps
-    dom = "temporal_wk" if temporal_ms >= spatial_ms else "spatial"
-    if dom == "temporal_wk":
-        ms_launch = times["temporal_wk"][0] / max(times["temporal_wk"][1], 1)
-        achieved = spec_bytes / (ms_launch * 1e-3) / 1e9
-        traffic = None
-        kern = "k_temporal (masked WK FFT + |Y|^2 + per-bin partials)"
-    else:
-        ms_launch = spatial_ms
-        achieved = (raw_bytes + spec_bytes) / (spatial_ms * 1e-3) / 1e9
-        traffic = None
-        kern = "k_rows + k_cols (R2C 2D FFT + corner turn), per step"
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
+    launches = max(per_kernel[dom]["launches_per_step"], 1)
+    algo_launch = per_step_algo[dom] / launches
+    achieved = algo_launch / (per_kernel[dom]["avg_launch_ms"] * 1e-3) / 1e9
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c2_1024x512_hbm_traffic.json")
+    if (T, N, NB) == (1024, 512, 256) and os.path.exists(prof):
+        k = json.load(open(prof))["kernels"].get(kernel_symbol[dom])
+        if k:
+            traffic = (k["fetch_MiB"] + k["write_MiB"]) * 2 ** 20
+    kern = {"temporal_wk": "k_temporal_w (masked WK: gather, mean removal, TPAD FFT, |Y|^2, per-bin partials)",
+            "spatial_cols_cornerturn": "k_cols_w (column FFT + corner turn into the history)",
+            "spatial_rows": "k_rows_w (decode, row-mean removal, window, R2C row FFT)",
+            "radial_finalize": "k_bin_reduce + k_bin_lags"}[dom]
     pipeline_gbs = (raw_bytes + 2 * spec_bytes) / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -146,7 +151,9 @@ def main():
         "config": {"workload": f"plain DDM, {T} frames of {N}x{N} u16, lags 1..{L - 1}, {NB} unit-width |q| bins",
                    "frames": T, "N": N, "lags": L, "bins": NB},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": algo_launch,
+                     "avg_launch_ms": per_kernel[dom]["avg_launch_ms"]},
         "pipeline": {"algorithmic_bytes": raw_bytes + 2 * spec_bytes, "GBs": pipeline_gbs,
                      "frac_of_hbm_peak": pipeline_gbs / HBM_PEAK_GBS, "kernels": per_kernel},
     }
