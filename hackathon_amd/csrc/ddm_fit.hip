@@ -1,0 +1,463 @@
+// Batched per-q Levenberg-Marquardt motility fits (gfx950).
+//
+// One wavefront per fit (one q-bin of one video); lanes own lag points.  The
+// per-bin model is the reference's modelColumn (motility-ddm
+// src/motility_fit.js:417-432) with explicit amplitudes:
+//   f(tau) = A c(tau) + B,   c = (1/K) sum_i [ -expm1(-D q_i^2 tau) + E_i alpha (1 - g(q_i vbar tau; Z)) ]
+// (= 1 - <E (1 - alpha + alpha g)> without the 1 - (~1) cancellation), g the
+// Schulz-averaged 3D swimmer ISF (isf3d, :140-144).  chi^2 = w^2 sum (d - f)^2
+// with w = 1/max d (:392-398, 469).  Parameters theta = (A, B, alpha, ln vbar,
+// ln Z, ln D) (swimmer) or (A, B, ln D) (Brownian), box-constrained by
+// projection with the reference's bounds (:480-491).
+//
+// Precision: the model and Jacobian are FP32, except 1 - g and its
+// derivatives at small u = q vbar tau/(Z+1) (< 0.1), where 1 - g cancels in
+// FP32 and is evaluated in FP64.  Normal equations are accumulated and solved
+// (Cholesky) in FP64.  Marquardt scaling, Nielsen's gain-ratio damping.
+//
+// Seeding: "plain" = the reference's start grid per bin (49-point log-D
+// Brownian grid; 16 vbar x Z{1.5, 8} x alpha{0.3, 0.8} at that D; amplitudes by
+// 2-column NNLS), LM from the best `nstarts`; "seeded" = LM from one given
+// start (the surrogate's), no grid.
+#include "ddm_internal.h"
+#include "ddm_fft.h"
+
+namespace ddm {
+
+namespace {
+
+constexpr int FNT = 256;          // 4 fits per workgroup
+constexpr int MAXT = 128;         // lag points per fit
+constexpr int MAXK = 8;           // q nodes per bin
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return __shfl(v, 0, 64);        // one lane's sum for everybody: identical branches
+}
+
+// 1 - g(x; Z) for the 3D Schulz ISF and its derivatives with respect to ln x
+// and ln Z (x held fixed).  g = sin(Z atan u) / (Z u (1+u^2)^(Z/2)), u = x/(Z+1).
+struct Omg {
+  float v, dlx, dlz;
+};
+
+__device__ __noinline__ Omg one_minus_isf3d(float xf, float Zf) {
+  const float uf = xf / (Zf + 1.f);
+  Omg o;
+  if (uf < 1e-4f) {
+    // series of the reference's small-argument branch: g = 1 - x^2 (Z+2) / (6 (Z+1))
+    const double x = xf, Z = Zf, u = x / (Z + 1);
+    o.v = (float)(x * x * (Z + 2) / (6 * (Z + 1)));
+    o.dlx = (float)(2 * x * x * (Z + 2) / (6 * (Z + 1)));          // d(1-g)/d ln x
+    o.dlz = (float)(-Z * u * u / 6);                                 // d(1-g)/d ln Z at fixed x
+    return o;
+  }
+  if (uf < 0.1f) {
+    // 1 - g cancels in FP32 here: FP64
+    const double Z = Zf, u = (double)xf / (Z + 1);
+    const double phi = atan(u), s = sin(Z * phi), cz = cos(Z * phi);
+    const double pre = exp(-0.5 * Z * log1p(u * u));
+    const double g = s * pre / (Z * u);
+    const double udg = pre / Z * (Z * cz / (1 + u * u) - s * (Z * u / (1 + u * u) + 1 / u));
+    const double zdgz = pre / u * (phi * cz - s * (0.5 * log1p(u * u) + 1 / Z));
+    o.v = (float)(1 - g);
+    o.dlx = (float)(-udg);
+    o.dlz = (float)(-(zdgz - Z / (Z + 1) * udg));
+    return o;
+  }
+  const float u = uf, Z = Zf;
+  const float phi = atanf(u);
+  float s, cz;
+  sincosf(Z * phi, &s, &cz);
+  const float pre = expf(-0.5f * Z * log1pf(u * u));
+  const float g = s * pre / (Z * u);
+  const float udg = pre / Z * (Z * cz / (1.f + u * u) - s * (Z * u / (1.f + u * u) + 1.f / u));
+  const float zdgz = pre / u * (phi * cz - s * (0.5f * log1pf(u * u) + 1.f / Z));
+  o.v = 1.f - g;
+  o.dlx = -udg;
+  o.dlz = -(zdgz - Z / (Z + 1.f) * udg);
+  return o;
+}
+
+struct FitIn {
+  const float* tau;     // [nfits * MAXT]
+  const float* d;       // [nfits * MAXT]
+  const int* ntau;      // [nfits]
+  const float* w;       // [nfits]
+  const float* qn;      // [nfits * K]
+  const float* bounds;  // [nfits * 4] vmin, vmax, Dmin, Dmax
+  const float* seeds;   // [nfits * 6] or null
+  int K;
+};
+
+// Column value (and derivatives w.r.t. alpha, ln vbar, ln Z, ln D) at one lag.
+__device__ __forceinline__ void column(float t, const float* q, int K, float alpha, float vbar, float Z, float Dd,
+                                       bool swim, bool deriv, float& c, float& da, float& dlv, float& dlz,
+                                       float& dld) {
+  c = da = dlv = dlz = dld = 0.f;
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) {
+    if (i >= K) break;
+    const float qi = q[i];
+    const float s = Dd * qi * qi * t;
+    const float E = expf(-s), omE = -expm1f(-s);
+    Omg og = {0.f, 0.f, 0.f};
+    if (swim) og = one_minus_isf3d(qi * vbar * t, Z);
+    c += omE + E * alpha * og.v;
+    if (deriv) {
+      da += E * og.v;
+      dlv += E * alpha * og.dlx;
+      dlz += E * alpha * og.dlz;
+      dld += s * E * (1.f - alpha * og.v);
+    }
+  }
+  const float inv = 1.f / K;
+  c *= inv;
+  da *= inv;
+  dlv *= inv;
+  dlz *= inv;
+  dld *= inv;
+}
+
+// chi^2 at theta (FP64 sum over the wave).
+__device__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q, int K,
+                          bool swim, int lane) {
+  double s = 0;
+  const float alpha = swim ? th[2] : 0.f, vbar = expf(th[3]), Z = expf(th[4]), Dd = expf(th[5]);
+#pragma unroll
+  for (int r = 0; r < MAXT / 64; r++) {
+    const int j = lane + 64 * r;
+    if (j < nt) {
+      float c, a, b, e, f;
+      column(tv[r], q, K, alpha, vbar, Z, Dd, swim, false, c, a, b, e, f);
+      const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
+      s += res * res;
+    }
+  }
+  return wave_sum_d(s);
+}
+
+// Exact 2-column NNLS (A, B >= 0) for a fixed column c, from the wave sums
+// g11 = sum c^2, g12 = sum c, n, h1 = sum c d, h2 = sum d, dd = sum d^2.
+__device__ double nnls2(double g11, double g12, double n, double h1, double h2, double dd, float& A, float& B) {
+  double best = dd, bA = 0, bB = 0;
+  if (g11 > 0) {  // A only
+    const double a = h1 / g11;
+    const double r = dd - a * h1;
+    if (a >= 0 && r < best) {
+      best = r;
+      bA = a;
+      bB = 0;
+    }
+  }
+  if (n > 0) {  // B only
+    const double b = h2 / n;
+    const double r = dd - b * h2;
+    if (b >= 0 && r < best) {
+      best = r;
+      bA = 0;
+      bB = b;
+    }
+  }
+  const double det = g11 * n - g12 * g12;
+  if (det > 1e-11 * fmax(g11, n) * fmax(g11, n)) {
+    const double a = (h1 * n - g12 * h2) / det, b = (g11 * h2 - g12 * h1) / det;
+    const double r = dd - a * h1 - b * h2;
+    if (a >= 0 && b >= 0 && r < best) {
+      best = r;
+      bA = a;
+      bB = b;
+    }
+  }
+  A = (float)bA;
+  B = (float)bB;
+  return best > 0 ? best : 0;
+}
+
+__device__ double grid_point(const float* tv, const float* dv, int nt, const float* q, int K, float alpha, float vbar,
+                             float Z, float Dd, bool swim, int lane, float& A, float& B) {
+  double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
+#pragma unroll
+  for (int r = 0; r < MAXT / 64; r++) {
+    const int j = lane + 64 * r;
+    if (j < nt) {
+      float c, a, b, e, f;
+      column(tv[r], q, K, alpha, vbar, Z, Dd, swim, false, c, a, b, e, f);
+      g11 += (double)c * c;
+      g12 += c;
+      h1 += (double)c * dv[r];
+      h2 += dv[r];
+      dd += (double)dv[r] * dv[r];
+    }
+  }
+  g11 = wave_sum_d(g11);
+  g12 = wave_sum_d(g12);
+  h1 = wave_sum_d(h1);
+  h2 = wave_sum_d(h2);
+  dd = wave_sum_d(dd);
+  return nnls2(g11, g12, (double)nt, h1, h2, dd, A, B);
+}
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+
+// LM from theta (in/out); returns chi^2, adds the steps taken to *steps.
+template <bool SWIM>
+__device__ double lm_run(float* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
+                         const float* q, int K, int maxit, int lane, int* steps) {
+  constexpr bool swim = SWIM;
+  constexpr int p = SWIM ? 6 : 3;
+  constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
+  for (int k = 0; k < 6; k++) th[k] = clampf(th[k], lo[k], hi[k]);
+  double chi = chi2_at(th, tv, dv, nt, w, q, K, swim, lane);
+  double lam = 1e-3, nu = 2;
+  int it = 0;
+  bool done = false;
+  while (it < maxit && !done) {
+    double H[21], g[6];
+#pragma unroll
+    for (int k = 0; k < 21; k++) H[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) g[k] = 0;
+    const float alpha = swim ? th[2] : 0.f, vbar = expf(th[3]), Z = expf(th[4]), Dd = expf(th[5]);
+#pragma unroll
+    for (int r = 0; r < MAXT / 64; r++) {
+      const int j = lane + 64 * r;
+      if (j < nt) {
+        float c, da, dlv, dlz, dld;
+        column(tv[r], q, K, alpha, vbar, Z, Dd, swim, true, c, da, dlv, dlz, dld);
+        double F[6];
+        F[0] = (double)w * c;
+        F[1] = w;
+        F[2] = (double)w * th[0] * da;
+        F[3] = (double)w * th[0] * dlv;
+        F[4] = (double)w * th[0] * dlz;
+        F[5] = (double)w * th[0] * dld;
+        const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
+        int h = 0;
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+          if (a < p) {
+            g[a] += F[idx[a]] * res;
+#pragma unroll
+            for (int b = 0; b <= a; b++) H[h++] += F[idx[a]] * F[idx[b]];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 21; k++) H[k] = wave_sum_d(H[k]);
+#pragma unroll
+    for (int k = 0; k < 6; k++) g[k] = wave_sum_d(g[k]);
+    for (;;) {
+      it++;
+      // Cholesky of H + lam diag(H), packed lower triangle, FP64
+      double Lc[21], diag[6];
+      int h = 0;
+#pragma unroll
+      for (int a = 0; a < p; a++)
+#pragma unroll
+        for (int b = 0; b <= a; b++, h++) {
+          Lc[h] = H[h];
+          if (a == b) {
+            diag[a] = H[h] > 1e-300 ? H[h] : 1e-300;
+            Lc[h] += lam * diag[a];
+          }
+        }
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < p; a++) {
+        const int ra = a * (a + 1) / 2;
+#pragma unroll
+        for (int b = 0; b <= a; b++) {
+          const int rb = b * (b + 1) / 2;
+          double s = Lc[ra + b];
+#pragma unroll
+          for (int k = 0; k < b; k++) s -= Lc[ra + k] * Lc[rb + k];
+          if (a == b) {
+            ok = ok && (s > 0);
+            Lc[ra + a] = sqrt(s > 0 ? s : 1.0);
+          } else {
+            Lc[ra + b] = s / Lc[rb + b];
+          }
+        }
+      }
+      if (ok) {
+        double y[6], x[6];
+#pragma unroll
+        for (int a = 0; a < p; a++) {
+          const int ra = a * (a + 1) / 2;
+          double s = g[a];
+#pragma unroll
+          for (int k = 0; k < a; k++) s -= Lc[ra + k] * y[k];
+          y[a] = s / Lc[ra + a];
+        }
+#pragma unroll
+        for (int a = p - 1; a >= 0; a--) {
+          double s = y[a];
+#pragma unroll
+          for (int k = a + 1; k < p; k++) s -= Lc[k * (k + 1) / 2 + a] * x[k];
+          x[a] = s / Lc[a * (a + 1) / 2 + a];
+        }
+        float nth[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) nth[k] = th[k];
+        double pred = 0;
+#pragma unroll
+        for (int a = 0; a < p; a++) {
+          const int k = idx[a];
+          nth[k] = clampf((float)(th[k] + x[a]), lo[k], hi[k]);
+          const double dk = (double)nth[k] - th[k];
+          pred += dk * (lam * diag[a] * dk + g[a]);
+        }
+        const double nchi = chi2_at(nth, tv, dv, nt, w, q, K, swim, lane);
+        const double rho = pred > 0 ? (chi - nchi) / pred : -1;
+        if (nchi < chi && rho > 0) {
+          const double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
+          for (int k = 0; k < 6; k++) th[k] = nth[k];
+          chi = nchi;
+          const double f = 1 - (2 * rho - 1) * (2 * rho - 1) * (2 * rho - 1);
+          lam *= f > 1.0 / 3 ? f : 1.0 / 3;
+          if (lam < 1e-15) lam = 1e-15;
+          nu = 2;
+          if (rel < 1e-10 || chi < 1e-300) done = true;
+          break;
+        }
+      }
+      lam *= nu;
+      nu *= 2;
+      if (lam > 1e15 || it >= maxit) {
+        done = true;
+        break;
+      }
+    }
+  }
+  *steps += it;
+  return chi;
+}
+
+}  // namespace
+
+// grid != 0: plain grid-seeded LM from the best `nstarts` starts; else LM from seeds.
+template <bool SWIM>
+__global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int grid, int nstarts, int maxit,
+                                                float* __restrict__ theta_out, double* __restrict__ chi_out,
+                                                int* __restrict__ steps_out) {
+  const int lane = threadIdx.x & 63;
+  const int f = blockIdx.x * (FNT / 64) + (threadIdx.x >> 6);
+  if (f >= nfits) return;
+  const int nt = min(in.ntau[f], MAXT);
+  const int K = min(in.K, MAXK);
+  float tv[MAXT / 64], dv[MAXT / 64], q[MAXK];
+#pragma unroll
+  for (int r = 0; r < MAXT / 64; r++) {
+    const int j = lane + 64 * r;
+    tv[r] = j < nt ? in.tau[(long)f * MAXT + j] : 0.f;
+    dv[r] = j < nt ? in.d[(long)f * MAXT + j] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) q[i] = i < K ? in.qn[(long)f * in.K + i] : 0.f;
+  const float w = in.w[f];
+  const float vmin = in.bounds[4 * f], vmax = in.bounds[4 * f + 1], Dmin = in.bounds[4 * f + 2],
+              Dmax = in.bounds[4 * f + 3];
+  const float lo[6] = {0.f, 0.f, 0.f, logf(vmin), logf(0.5f), logf(Dmin)};
+  const float hi[6] = {3.4e38f, 3.4e38f, 1.f, logf(vmax), logf(100.f), logf(Dmax)};
+  int steps = 0;
+  float st[3][6];
+  int nrun = 1;
+  if (!grid) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) st[0][k] = in.seeds[(long)f * 6 + k];
+  } else {
+    // Brownian 49-point log-D grid
+    float bA = 0, bB = 0, blD = lo[5];
+    double bchi = 1e300;
+    for (int i = 0; i <= 48; i++) {
+      const float lD = lo[5] + ((hi[5] - lo[5]) * i) / 48.f;
+      float A, B;
+      const double r = grid_point(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), false, lane, A, B);
+      if (r < bchi) {
+        bchi = r;
+        bA = A;
+        bB = B;
+        blD = lD;
+      }
+    }
+    if (!SWIM) {
+      st[0][0] = bA;
+      st[0][1] = bB;
+      st[0][2] = 0.f;
+      st[0][3] = 0.5f * (lo[3] + hi[3]);
+      st[0][4] = logf(4.f);
+      st[0][5] = blD;
+    } else {
+      double sf[3] = {1e300, 1e300, 1e300};
+      const float lZs[2] = {logf(1.5f), logf(8.f)}, as[2] = {0.3f, 0.8f};
+      for (int i = 0; i < 16; i++) {
+        const float lv = lo[3] + ((hi[3] - lo[3]) * (i + 0.5f)) / 16.f;
+#pragma unroll
+        for (int z = 0; z < 2; z++)
+#pragma unroll
+          for (int a = 0; a < 2; a++) {
+            float A, B;
+            const double r =
+                grid_point(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), true, lane, A, B) * w * w;
+            // keep the best 3 (stable: earlier grid points win ties); an
+            // insertion step with static register indices
+            float cand[6] = {A, B, as[a], lv, lZs[z], blD};
+            double cr = r;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+              if (cr < sf[k]) {
+                const double tf = sf[k];
+                sf[k] = cr;
+                cr = tf;
+#pragma unroll
+                for (int m = 0; m < 6; m++) {
+                  const float tm = st[k][m];
+                  st[k][m] = cand[m];
+                  cand[m] = tm;
+                }
+              }
+            }
+          }
+      }
+      nrun = min(nstarts, 3);
+    }
+  }
+  float best[6];
+  double bestchi = 1e300;
+  for (int s = 0; s < nrun; s++) {
+    float th[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
+    const double chi = lm_run<SWIM>(th, lo, hi, tv, dv, nt, w, q, K, maxit, lane, &steps);
+    if (chi < bestchi) {
+      bestchi = chi;
+#pragma unroll
+      for (int k = 0; k < 6; k++) best[k] = th[k];
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 6; k++) theta_out[(long)f * 6 + k] = best[k];
+    chi_out[f] = bestchi;
+    steps_out[f] = steps;
+  }
+}
+
+hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
+                         const float* bounds, const float* seeds, int nfits, int swim, int grid, int nstarts,
+                         int maxit, float* theta, double* chi, int* steps, hipStream_t st) {
+  if (K > MAXK) return hipErrorInvalidValue;
+  FitIn in{tau, d, ntau, w, qn, bounds, seeds, K};
+  const int blocks = (nfits + FNT / 64 - 1) / (FNT / 64);
+  if (swim)
+    hipLaunchKernelGGL(k_lm_fit<true>, dim3(blocks), dim3(FNT), 0, st, in, nfits, grid, nstarts, maxit, theta, chi,
+                       steps);
+  else
+    hipLaunchKernelGGL(k_lm_fit<false>, dim3(blocks), dim3(FNT), 0, st, in, nfits, grid, nstarts, maxit, theta, chi,
+                       steps);
+  return hipGetLastError();
+}
+
+}  // namespace ddm

@@ -1,0 +1,152 @@
+// Host side of the batched per-q LM fitter (ddm_fit in include/ddm.h): packs
+// the batch into device arrays (f32 model data, one row of <= 128 lags per
+// fit), launches k_lm_fit on the fitter's stream and times it with HIP events.
+#include "../../include/ddm.h"
+#include "ddm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+struct ddm_fitter {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  size_t cap = 0;  // fits the buffers hold
+  float *tau = nullptr, *d = nullptr, *w = nullptr, *qn = nullptr, *bounds = nullptr, *seeds = nullptr;
+  int* ntau = nullptr;
+  float* theta = nullptr;
+  double* chi = nullptr;
+  int* steps = nullptr;
+};
+
+namespace {
+constexpr int kMaxT = 128;
+constexpr int kMaxK = 8;
+
+void release(ddm_fitter* f) {
+  void* ptrs[] = {f->tau, f->d, f->w, f->qn, f->bounds, f->seeds, f->ntau, f->theta, f->chi, f->steps};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  f->tau = f->d = f->w = f->qn = f->bounds = f->seeds = f->theta = nullptr;
+  f->ntau = f->steps = nullptr;
+  f->chi = nullptr;
+  f->cap = 0;
+}
+
+int ffail(ddm_fitter* f, int code, const std::string& m) {
+  f->err = m;
+  return code;
+}
+
+hipError_t grow(ddm_fitter* f, size_t n) {
+  if (n <= f->cap) return hipSuccess;
+  release(f);
+  hipError_t e;
+  if ((e = hipMalloc((void**)&f->tau, n * kMaxT * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->d, n * kMaxT * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->w, n * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->qn, n * kMaxK * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->bounds, n * 16)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->seeds, n * 24)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->ntau, n * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->theta, n * 24)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->chi, n * 8)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->steps, n * 4)) != hipSuccess) return e;
+  f->cap = n;
+  return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int ddm_fitter_create(ddm_fitter** out) {
+  if (!out) return DDM_EINVAL;
+  ddm_fitter* f = new ddm_fitter();
+  if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&f->ev0) != hipSuccess || hipEventCreate(&f->ev1) != hipSuccess) {
+    delete f;
+    return DDM_EHIP;
+  }
+  *out = f;
+  return DDM_OK;
+}
+
+void ddm_fitter_destroy(ddm_fitter* f) {
+  if (!f) return;
+  hipStreamSynchronize(f->stream);
+  release(f);
+  hipEventDestroy(f->ev0);
+  hipEventDestroy(f->ev1);
+  hipStreamDestroy(f->stream);
+  delete f;
+}
+
+const char* ddm_fitter_last_error(const ddm_fitter* f) { return f ? f->err.c_str() : "null fitter"; }
+
+int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, int32_t* iters, double* kernel_ms) {
+  if (!f || !b) return DDM_EINVAL;
+  if (b->nfits < 0) return ffail(f, DDM_EINVAL, "fit: negative batch size");
+  if (b->nfits == 0) return DDM_OK;
+  if (b->ntau_max < 1 || b->ntau_max > kMaxT) return ffail(f, DDM_EINVAL, "fit: ntau_max must be in [1, 128]");
+  if (b->knodes < 1 || b->knodes > kMaxK) return ffail(f, DDM_EINVAL, "fit: knodes must be in [1, 8]");
+  if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D) return ffail(f, DDM_EINVAL, "fit: model");
+  if (b->seed_mode == DDM_SEED_GIVEN && !b->seeds) return ffail(f, DDM_EINVAL, "fit: seeds required");
+  if (!b->ntau || !b->tau || !b->data || !b->weight || !b->qnodes || !b->bounds || !theta || !chi2 || !iters)
+    return ffail(f, DDM_EINVAL, "fit: null array");
+  const size_t n = (size_t)b->nfits;
+  std::vector<float> tau(n * kMaxT, 0.f), d(n * kMaxT, 0.f), w(n), qn(n * b->knodes), bd(n * 4), sd(n * 6, 0.f);
+  std::vector<int> nt(n);
+  for (size_t i = 0; i < n; i++) {
+    const int m = b->ntau[i];
+    if (m < 1 || m > b->ntau_max) return ffail(f, DDM_EINVAL, "fit: ntau[i] out of range");
+    nt[i] = m;
+    for (int j = 0; j < m; j++) {
+      tau[i * kMaxT + j] = (float)b->tau[i * b->ntau_max + j];
+      d[i * kMaxT + j] = (float)b->data[i * b->ntau_max + j];
+    }
+    w[i] = (float)b->weight[i];
+    for (int k = 0; k < b->knodes; k++) qn[i * b->knodes + k] = (float)b->qnodes[i * b->knodes + k];
+    for (int k = 0; k < 4; k++) bd[i * 4 + k] = (float)b->bounds[i * 4 + k];
+    if (!(bd[i * 4] > 0 && bd[i * 4 + 1] > bd[i * 4] && bd[i * 4 + 2] > 0 && bd[i * 4 + 3] > bd[i * 4 + 2]))
+      return ffail(f, DDM_EINVAL, "fit: bounds must satisfy 0 < vmin < vmax, 0 < Dmin < Dmax");
+    if (b->seed_mode == DDM_SEED_GIVEN)
+      for (int k = 0; k < 6; k++) sd[i * 6 + k] = (float)b->seeds[i * 6 + k];
+  }
+  hipError_t e = grow(f, n);
+  if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: allocation: ") + hipGetErrorString(e));
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, f->stream);
+  };
+  if ((e = up(f->tau, tau.data(), tau.size() * 4)) != hipSuccess || (e = up(f->d, d.data(), d.size() * 4)) != hipSuccess ||
+      (e = up(f->w, w.data(), n * 4)) != hipSuccess || (e = up(f->qn, qn.data(), qn.size() * 4)) != hipSuccess ||
+      (e = up(f->bounds, bd.data(), n * 16)) != hipSuccess || (e = up(f->seeds, sd.data(), n * 24)) != hipSuccess ||
+      (e = up(f->ntau, nt.data(), n * 4)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("fit: upload: ") + hipGetErrorString(e));
+  hipEventRecord(f->ev0, f->stream);
+  e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds, b->nfits,
+                         b->model == DDM_MODEL_SWIMMER_3D, b->seed_mode == DDM_SEED_GRID,
+                         std::max(1, std::min(3, b->nstarts)), std::max(1, b->max_iter), f->theta, f->chi, f->steps,
+                         f->stream);
+  if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: launch: ") + hipGetErrorString(e));
+  hipEventRecord(f->ev1, f->stream);
+  std::vector<float> th(n * 6);
+  std::vector<int> st(n);
+  if ((e = hipMemcpyAsync(th.data(), f->theta, n * 24, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(chi2, f->chi, n * 8, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(st.data(), f->steps, n * 4, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(f->stream)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("fit: ") + hipGetErrorString(e));
+  for (size_t i = 0; i < n * 6; i++) theta[i] = th[i];
+  for (size_t i = 0; i < n; i++) iters[i] = st[i];
+  if (kernel_ms) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, f->ev0, f->ev1);
+    *kernel_ms = ms;
+  }
+  return DDM_OK;
+}
+
+}  // extern "C"

@@ -33,4 +33,8 @@ hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* row
                          float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                          int first_slot, hipStream_t st);
 
+hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
+                         const float* bounds, const float* seeds, int nfits, int swim, int grid, int nstarts,
+                         int maxit, float* theta, double* chi, int* steps, hipStream_t st);
+
 }  // namespace ddm

@@ -1,0 +1,209 @@
+"""Per-q motility fits on the GPU (libddm.so `ddm_fit`).
+
+`prepare` and `bounds` restate the reference's data preparation
+(motility-ddm src/motility_fit.js:383-401 and :480-491): bins with at least 3
+modes and a positive maximum, the quasi-logarithmic lag grid (:358-369), q from
+kCenter, q-nodes from kNodes, weight w = 1/max D.  `Fitter.fit` runs one
+batched Levenberg-Marquardt launch over any number of (video, bin) problems;
+`analyzeMotilityLM` is the per-q counterpart of `analyzeMotility`'s fitting
+part (:870-911) for one Radial record.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .engine import DdmError, Radial, load_library
+
+MAXT = 128
+MAXK = 8
+BROWNIAN, SWIMMER_3D = 0, 1
+SEED_GRID, SEED_GIVEN = 0, 1
+SWIM_MIN_UM_S = 2.0   # GATES.SWIM_MIN_UM_S (src/motility_fit.js:37)
+
+
+class FitBatch(C.Structure):
+    _fields_ = [("nfits", C.c_int32), ("ntau_max", C.c_int32), ("knodes", C.c_int32), ("model", C.c_int32),
+                ("seed_mode", C.c_int32), ("nstarts", C.c_int32), ("max_iter", C.c_int32),
+                ("ntau", C.c_void_p), ("tau", C.c_void_p), ("data", C.c_void_p), ("weight", C.c_void_p),
+                ("qnodes", C.c_void_p), ("bounds", C.c_void_p), ("seeds", C.c_void_p)]
+
+
+def _declare(L):
+    if getattr(L, "_fit_declared", False):
+        return L
+    vp = C.c_void_p
+    L.ddm_fitter_create.argtypes = [C.POINTER(vp)]
+    L.ddm_fitter_create.restype = C.c_int
+    L.ddm_fitter_destroy.argtypes = [vp]
+    L.ddm_fitter_destroy.restype = None
+    L.ddm_fitter_last_error.argtypes = [vp]
+    L.ddm_fitter_last_error.restype = C.c_char_p
+    L.ddm_fit.argtypes = [vp, C.POINTER(FitBatch), vp, vp, vp, vp]
+    L.ddm_fit.restype = C.c_int
+    L._fit_declared = True
+    return L
+
+
+def lag_grid(L: int) -> np.ndarray:
+    """Lags 1..10, then steps of x1.08 (src/motility_fit.js:358-369)."""
+    lags = list(range(1, min(10, L - 1) + 1))
+    x = 10.0
+    while True:
+        x *= 1.08
+        t = int(np.floor(x + 0.5))          # Math.round for positive x
+        if t > L - 1:
+            break
+        if t > lags[-1]:
+            lags.append(t)
+    return np.array(lags, dtype=np.int64)
+
+
+@dataclass
+class PreparedBin:
+    b: int
+    q: float
+    qn: np.ndarray
+    d: np.ndarray
+    w: float
+
+
+@dataclass
+class Prepared:
+    bins: list
+    lags: np.ndarray
+    tau: np.ndarray
+    dt: float
+    L: int
+    tauMax: float
+
+
+def prepare(radial: Radial, dxUm: float, fps: float) -> Prepared:
+    """prepare (src/motility_fit.js:383-401)."""
+    N, NB, L = radial.N, radial.NB, radial.L
+    dt = 1.0 / fps
+    lags = lag_grid(L)
+    D = np.asarray(radial.D, dtype=np.float64).reshape(NB, L)
+    qs = 2 * np.pi / (N * dxUm)
+    kn = radial.kNodes
+    K = len(kn) // NB if kn is not None and len(kn) else 0
+    bins = []
+    for b in range(NB):
+        if radial.nModes[b] < 3:
+            continue
+        d = D[b, lags]
+        scale = float(d.max())
+        if not (scale > 0) or not np.isfinite(scale):
+            continue
+        q = qs * float(radial.kCenter[b])
+        qn = np.asarray(kn[b * K:(b + 1) * K], dtype=np.float64) * qs if K else np.array([q])
+        bins.append(PreparedBin(b, q, qn, d, 1.0 / scale))
+    return Prepared(bins, lags, lags * dt, dt, L, (L - 1) * dt)
+
+
+def bounds(pr: Prepared) -> np.ndarray:
+    """vmin, vmax, Dmin, Dmax (src/motility_fit.js:480-491)."""
+    qs = np.array([b.q for b in pr.bins])
+    qmin, qmax = qs.min(), qs.max()
+    return np.array([max(SWIM_MIN_UM_S, 0.3 / (qmax * pr.tauMax)), 3 / (qmin * pr.dt),
+                     0.01 / (qmax * qmax * pr.tauMax), 10 / (qmin * qmin * pr.dt)])
+
+
+@dataclass
+class FitResult:
+    theta: np.ndarray     # (nfits, 6): A, B, alpha, ln vbar, ln Z, ln D
+    chi2: np.ndarray
+    iters: np.ndarray
+    kernel_ms: float
+
+    @property
+    def alpha(self):
+        return self.theta[:, 2]
+
+    @property
+    def vbar(self):
+        return np.exp(self.theta[:, 3])
+
+    @property
+    def Z(self):
+        return np.exp(self.theta[:, 4])
+
+    @property
+    def D(self):
+        return np.exp(self.theta[:, 5])
+
+
+class Fitter:
+    """A fitter context (one HIP stream, device buffers reused across calls)."""
+
+    def __init__(self):
+        self._lib = _declare(load_library())
+        h = C.c_void_p()
+        if self._lib.ddm_fitter_create(C.byref(h)) != 0:
+            raise DdmError("ddm_fitter_create failed")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.ddm_fitter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fit(self, tau, data, weight, qnodes, bnds, model=SWIMMER_3D, seeds=None, nstarts=3,
+            max_iter=100) -> FitResult:
+        """tau/data: lists (or 2D arrays) of per-fit lag times and D values; weight (nfits,);
+        qnodes (nfits, K); bnds (nfits, 4); seeds (nfits, 6) for seeded LM, else grid seeding."""
+        n = len(data)
+        ntau = np.array([len(x) for x in data], dtype=np.int32)
+        if n == 0:
+            return FitResult(np.zeros((0, 6)), np.zeros(0), np.zeros(0, np.int32), 0.0)
+        tmax = int(ntau.max())
+        if tmax > MAXT:
+            raise DdmError(f"at most {MAXT} lags per fit")
+        T = np.zeros((n, tmax))
+        Dd = np.zeros((n, tmax))
+        for i in range(n):
+            T[i, :ntau[i]] = tau[i]
+            Dd[i, :ntau[i]] = data[i]
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        qn = np.ascontiguousarray(qnodes, dtype=np.float64).reshape(n, -1)
+        bd = np.ascontiguousarray(bnds, dtype=np.float64).reshape(n, 4)
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float64).reshape(n, 6)
+        b = FitBatch(n, tmax, qn.shape[1], model, SEED_GRID if sd is None else SEED_GIVEN, nstarts, max_iter,
+                     ntau.ctypes.data, T.ctypes.data, Dd.ctypes.data, w.ctypes.data, qn.ctypes.data, bd.ctypes.data,
+                     None if sd is None else sd.ctypes.data)
+        th = np.zeros((n, 6))
+        chi = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        ms = C.c_double()
+        rc = self._lib.ddm_fit(self._h, C.byref(b), th.ctypes.data, chi.ctypes.data, it.ctypes.data, C.byref(ms))
+        if rc != 0:
+            raise DdmError(self._lib.ddm_fitter_last_error(self._h).decode())
+        return FitResult(th, chi, it, ms.value)
+
+
+def problems_from(pr: Prepared):
+    """Per-bin fit problems of one prepared Radial: (tau, data, weight, qnodes, bounds) lists."""
+    bd = bounds(pr)
+    K = max(len(b.qn) for b in pr.bins)
+    tau = [pr.tau for _ in pr.bins]
+    data = [b.d for b in pr.bins]
+    w = np.array([b.w for b in pr.bins])
+    qn = np.array([np.resize(b.qn, K) for b in pr.bins])
+    return tau, data, w, qn, np.tile(bd, (len(pr.bins), 1))
+
+
+def analyzeMotilityLM(radial: Radial, dxUm: float, fps: float, model=SWIMMER_3D, fitter: Fitter | None = None,
+                      seeds=None, nstarts=3, max_iter=100):
+    """Per-q LM fits of every usable bin of one Radial record -> (Prepared, FitResult)."""
+    pr = prepare(radial, dxUm, fps)
+    f = fitter or Fitter()
+    res = f.fit(*problems_from(pr), model=model, seeds=seeds, nstarts=nstarts, max_iter=max_iter)
+    return pr, res

@@ -136,6 +136,48 @@ enum { DDM_K_ROWS = 0, DDM_K_COLS = 1, DDM_K_TEMPORAL = 2, DDM_K_FINALIZE = 3, D
 int ddm_set_profiling(ddm_engine *e, int32_t enable);
 int ddm_kernel_times(ddm_engine *e, double *ms, int32_t *count, int32_t n);
 
+/* ------------------------------------------------------------------ per-q motility fits
+ *
+ * Batched Levenberg-Marquardt fits of the reference's per-bin model
+ * (modelColumn, motility-ddm src/motility_fit.js:417-432; prepare :383-401
+ * defines the data, lags, weights and q nodes; bounds :480-491):
+ *   D_b(tau) = A c(tau) + B,  c = 1 - <exp(-D q^2 tau) (1 - alpha + alpha g(q vbar tau; Z))>_nodes
+ * one fit per q-bin, all bins of all videos in one launch.  The reference fits
+ * alpha, vbar, Z, D globally with Nelder-Mead over VarPro
+ * (fitSwimmer/fitBrownian :500-611); this entry point is the per-q LM of the
+ * MI355X design (no reference counterpart for the LM itself).
+ * theta = (A, B, alpha, ln vbar, ln Z, ln D); chi^2 = w^2 sum (d - f)^2. */
+typedef struct ddm_fitter ddm_fitter;
+
+enum { DDM_MODEL_BROWNIAN = 0, DDM_MODEL_SWIMMER_3D = 1 };
+enum { DDM_SEED_GRID = 0, DDM_SEED_GIVEN = 1 };
+
+typedef struct ddm_fit_batch {
+  int32_t nfits;
+  int32_t ntau_max;      /* row stride of tau/data, <= 128 */
+  int32_t knodes;        /* q nodes per bin, <= 8 */
+  int32_t model;         /* DDM_MODEL_* */
+  int32_t seed_mode;     /* DDM_SEED_GRID: the reference's start grid, LM from the best nstarts;
+                            DDM_SEED_GIVEN: LM from seeds[] */
+  int32_t nstarts;       /* 1..3 */
+  int32_t max_iter;      /* LM steps per start */
+  const int32_t *ntau;   /* [nfits] lags used by each fit */
+  const double *tau;     /* [nfits * ntau_max] lag times (s) */
+  const double *data;    /* [nfits * ntau_max] D_b(tau) */
+  const double *weight;  /* [nfits] w = 1 / max_tau D_b */
+  const double *qnodes;  /* [nfits * knodes] effective |q| nodes (rad/um) */
+  const double *bounds;  /* [nfits * 4] vmin, vmax, Dmin, Dmax */
+  const double *seeds;   /* [nfits * 6] theta seeds for DDM_SEED_GIVEN, else NULL */
+} ddm_fit_batch;
+
+int ddm_fitter_create(ddm_fitter **out);
+void ddm_fitter_destroy(ddm_fitter *f);
+const char *ddm_fitter_last_error(const ddm_fitter *f);
+/* Runs the batch; blocks until done.  theta (nfits*6), chi2 (nfits), iters
+ * (nfits, LM steps summed over starts) are host outputs; *kernel_ms (may be
+ * NULL) is the fit kernel's duration from HIP events. */
+int ddm_fit(ddm_fitter *f, const ddm_fit_batch *b, double *theta, double *chi2, int32_t *iters, double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -782,3 +782,533 @@ void orc_model_column(int ntau, const double *tau, int nq, const double *qn, dou
     out[j] = 1 - f * inv;
   }
 }
+
+/* ------------------------------------------------------------------ global VarPro fit (reference) */
+
+/* evalModel: per bin the model column c(tau), exact 2-column NNLS for (A, B),
+ * rss = sum_b rss_b w_b^2; src/motility_fit.js:450-474.  amps (2*nb) may be NULL. */
+double orc_eval_model(int nb, int ntau, int K, const double *tau, const double *d, const double *w, const double *qn,
+                      double vbar, double Z, double Dd, double Vmag, double alpha, int swim, int geometry,
+                      double *amps) {
+  double rss = 0;
+  double *c = (double *)malloc(sizeof(double) * ntau);
+  for (int b = 0; b < nb; b++) {
+    orc_model_column(ntau, tau, K, qn + (long)b * K, vbar, Z, Dd, Vmag, alpha, swim, geometry, c);
+    double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
+    const double *db = d + (long)b * ntau;
+    for (int j = 0; j < ntau; j++) {
+      g11 += c[j] * c[j];
+      g12 += c[j];
+      h1 += c[j] * db[j];
+      h2 += db[j];
+      dd += db[j] * db[j];
+    }
+    double G[9] = {g11, g12, 0, g12, (double)ntau, 0, 0, 0, 0}, h[3] = {h1, h2, 0}, a[3], r;
+    orc_nnls3(G, h, dd, a, &r);
+    rss += r * w[b] * w[b];
+    if (amps) {
+      amps[2 * b] = a[0];
+      amps[2 * b + 1] = a[1];
+    }
+  }
+  free(c);
+  return rss;
+}
+
+/* bounds: src/motility_fit.js:480-491 (GATES.SWIM_MIN_UM_S = 2). out: vmin, vmax, Dmin, Dmax, Zmin, Zmax */
+void orc_bounds(int nb, const double *q, double dt, double tauMax, double *out) {
+  double qmin = q[0], qmax = q[0];
+  for (int b = 1; b < nb; b++) {
+    if (q[b] < qmin) qmin = q[b];
+    if (q[b] > qmax) qmax = q[b];
+  }
+  double v0 = 0.3 / (qmax * tauMax);
+  out[0] = v0 > 2 ? v0 : 2;
+  out[1] = 3 / (qmin * dt);
+  out[2] = 0.01 / (qmax * qmax * tauMax);
+  out[3] = 10 / (qmin * qmin * dt);
+  out[4] = 0.5;
+  out[5] = 100;
+}
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+typedef double (*orc_objective)(const double *u, void *ctx);
+
+/* Bounded Nelder-Mead (bounds by clamping inside the objective);
+ * src/motility_fit.js:302-350.  x0/x (n <= 8). Returns f at the best vertex. */
+double orc_nelder_mead(orc_objective f, void *ctx, int n, const double *x0, const double *step, int maxIter, double tol,
+                       double *xbest) {
+  double S[9][8], fs[9];
+  for (int j = 0; j < n; j++) S[0][j] = x0[j];
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) S[i + 1][j] = x0[j];
+    S[i + 1][i] += step[i];
+  }
+  for (int i = 0; i <= n; i++) fs[i] = f(S[i], ctx);
+  for (int it = 0; it < maxIter; it++) {
+    /* stable sort by f (insertion sort keeps equal elements in order, as Array.sort does) */
+    for (int i = 1; i <= n; i++) {
+      double fv = fs[i], xv[8];
+      for (int j = 0; j < n; j++) xv[j] = S[i][j];
+      int k = i - 1;
+      while (k >= 0 && fs[k] > fv) {
+        fs[k + 1] = fs[k];
+        for (int j = 0; j < n; j++) S[k + 1][j] = S[k][j];
+        k--;
+      }
+      fs[k + 1] = fv;
+      for (int j = 0; j < n; j++) S[k + 1][j] = xv[j];
+    }
+    if (fabs(fs[n] - fs[0]) <= tol * (fabs(fs[0]) + 1e-30)) break;
+    double c[8] = {0};
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) c[j] += S[i][j] / n;
+    double xr[8], xe[8], xc[8];
+    for (int j = 0; j < n; j++) xr[j] = c[j] + -1 * (S[n][j] - c[j]);
+    double fr = f(xr, ctx);
+    if (fr < fs[0]) {
+      for (int j = 0; j < n; j++) xe[j] = c[j] + -2 * (S[n][j] - c[j]);
+      double fe = f(xe, ctx);
+      if (fe < fr) {
+        for (int j = 0; j < n; j++) S[n][j] = xe[j];
+        fs[n] = fe;
+      } else {
+        for (int j = 0; j < n; j++) S[n][j] = xr[j];
+        fs[n] = fr;
+      }
+    } else if (fr < fs[n - 1]) {
+      for (int j = 0; j < n; j++) S[n][j] = xr[j];
+      fs[n] = fr;
+    } else {
+      int outside = fr < fs[n];
+      double t = outside ? -0.5 : 0.5;
+      for (int j = 0; j < n; j++) xc[j] = c[j] + t * (S[n][j] - c[j]);
+      double fc = f(xc, ctx);
+      if (fc < (outside ? fr : fs[n])) {
+        for (int j = 0; j < n; j++) S[n][j] = xc[j];
+        fs[n] = fc;
+      } else {
+        for (int i = 1; i <= n; i++) {
+          for (int j = 0; j < n; j++) S[i][j] = S[0][j] + 0.5 * (S[i][j] - S[0][j]);
+          fs[i] = f(S[i], ctx);
+        }
+      }
+    }
+  }
+  int ib = 0;
+  for (int i = 1; i <= n; i++)
+    if (fs[i] < fs[ib]) ib = i;
+  for (int j = 0; j < n; j++) xbest[j] = S[ib][j];
+  return fs[ib];
+}
+
+typedef struct {
+  int nb, ntau, K, geometry;
+  const double *tau, *d, *w, *qn;
+  double B[6], Vmag;
+} fit_ctx;
+
+static void swim_unpack(const fit_ctx *c, const double *u, double *p) {
+  p[0] = exp(clampd(u[0], log(c->B[0]), log(c->B[1])));
+  p[1] = exp(clampd(u[1], log(c->B[4]), log(c->B[5])));
+  p[2] = exp(clampd(u[2], log(c->B[2]), log(c->B[3])));
+  p[3] = clampd(u[3], 0, 1);
+}
+
+static double swim_obj(const double *u, void *vc) {
+  fit_ctx *c = (fit_ctx *)vc;
+  double p[4];
+  swim_unpack(c, u, p);
+  return orc_eval_model(c->nb, c->ntau, c->K, c->tau, c->d, c->w, c->qn, p[0], p[1], p[2], c->Vmag, p[3], 1,
+                        c->geometry, NULL);
+}
+
+static double brown_obj(double lnD, fit_ctx *c) {
+  return orc_eval_model(c->nb, c->ntau, c->K, c->tau, c->d, c->w, c->qn, 0, 1, exp(lnD), c->Vmag, 0, 0, 0, NULL);
+}
+
+/* fitBrownian: 49-point log-D grid + golden section; src/motility_fit.js:500-536.
+ * bounds6 from orc_bounds.  Returns Dd; *rss the final rss. */
+double orc_fit_brownian(int nb, int ntau, int K, const double *tau, const double *d, const double *w,
+                        const double *qn, const double *bounds6, double Vmag, double *rss) {
+  fit_ctx c = {nb, ntau, K, 0, tau, d, w, qn, {0}, Vmag};
+  for (int i = 0; i < 6; i++) c.B[i] = bounds6[i];
+  double lo = log(c.B[2]), hi = log(c.B[3]);
+  int n = 48, bi = 0;
+  double bf = INFINITY, xs[49];
+  for (int i = 0; i <= n; i++) {
+    double x = lo + ((hi - lo) * i) / n;
+    xs[i] = x;
+    double v = brown_obj(x, &c);
+    if (v < bf) {
+      bf = v;
+      bi = i;
+    }
+  }
+  double a = xs[bi > 0 ? bi - 1 : 0], b = xs[bi < n ? bi + 1 : n];
+  double gr = (sqrt(5.0) - 1) / 2;
+  double cc = b - gr * (b - a), dd = a + gr * (b - a);
+  double fc = brown_obj(cc, &c), fd = brown_obj(dd, &c);
+  for (int it = 0; it < 60 && b - a > 1e-6; it++) {
+    if (fc < fd) {
+      b = dd;
+      dd = cc;
+      fd = fc;
+      cc = b - gr * (b - a);
+      fc = brown_obj(cc, &c);
+    } else {
+      a = cc;
+      cc = dd;
+      fc = fd;
+      dd = a + gr * (b - a);
+      fd = brown_obj(dd, &c);
+    }
+  }
+  double lnD = fc < fd ? cc : dd;
+  double Dd = exp(lnD);
+  if (rss) *rss = orc_eval_model(nb, ntau, K, tau, d, w, qn, 0, 1, Dd, Vmag, 0, 0, 0, NULL);
+  return Dd;
+}
+
+/* fitSwimmer: 64 grid starts (16 vbar x Z{1.5, 8} x alpha{0.3, 0.8}), Nelder-Mead
+ * from the best 3 (<= 400 iterations, tol 1e-10); src/motility_fit.js:577-611.
+ * out4 = (vbar, Z, Dd, alpha); returns rss. */
+double orc_fit_swimmer(int nb, int ntau, int K, const double *tau, const double *d, const double *w, const double *qn,
+                       const double *bounds6, int geometry, double Vmag, double Dstart, double *out4) {
+  fit_ctx c = {nb, ntau, K, geometry, tau, d, w, qn, {0}, Vmag};
+  for (int i = 0; i < 6; i++) c.B[i] = bounds6[i];
+  double D0 = Dstart > 0 ? clampd(Dstart, c.B[2], c.B[3]) : sqrt(c.B[2] * c.B[3]);
+  double su[64][4], sf[64];
+  int ns = 0;
+  for (int i = 0; i < 16; i++) {
+    double lv = log(c.B[0]) + ((log(c.B[1]) - log(c.B[0])) * (i + 0.5)) / 16;
+    double lZs[2] = {log(1.5), log(8.0)}, as[2] = {0.3, 0.8};
+    for (int z = 0; z < 2; z++)
+      for (int q = 0; q < 2; q++) {
+        su[ns][0] = lv;
+        su[ns][1] = lZs[z];
+        su[ns][2] = log(D0);
+        su[ns][3] = as[q];
+        sf[ns] = swim_obj(su[ns], &c);
+        ns++;
+      }
+  }
+  /* stable sort of the starts by f */
+  int idx[64];
+  for (int i = 0; i < 64; i++) idx[i] = i;
+  for (int i = 1; i < 64; i++) {
+    int v = idx[i], k = i - 1;
+    while (k >= 0 && sf[idx[k]] > sf[v]) {
+      idx[k + 1] = idx[k];
+      k--;
+    }
+    idx[k + 1] = v;
+  }
+  double best[4], bestf = INFINITY, step[4] = {0.3, 0.7, 0.7, 0.2};
+  for (int s = 0; s < 3; s++) {
+    double x[4];
+    double fr = orc_nelder_mead(swim_obj, &c, 4, su[idx[s]], step, 400, 1e-10, x);
+    if (fr < bestf) {
+      bestf = fr;
+      for (int j = 0; j < 4; j++) best[j] = x[j];
+    }
+  }
+  swim_unpack(&c, best, out4);
+  /* out4 order: vbar, Z, Dd, alpha */
+  return orc_eval_model(nb, ntau, K, tau, d, w, qn, out4[0], out4[1], out4[2], Vmag, out4[3], 1, geometry, NULL);
+}
+
+/* ------------------------------------------------------------------ per-q Levenberg-Marquardt (FP64) */
+/*
+ * The per-bin model of the batched GPU fitter, in FP64: for one q-bin with K
+ * effective-|q| nodes q_i and lags tau_j,
+ *   f_j = A c_j + B,  c_j = (1/K) sum_i [ -expm1(-D q_i^2 tau_j) + E_ij alpha (1 - g(q_i vbar tau_j; Z)) ]
+ * which equals the reference's modelColumn 1 - <E (1 - alpha + alpha g)> (src/motility_fit.js:417-432)
+ * written without the 1 - (~1) cancellation.  Parameters theta = (A, B, alpha,
+ * ln vbar, ln Z, ln D) (swimmer) or (A, B, ln D) (Brownian); chi^2 =
+ * w^2 sum_j (d_j - f_j)^2 with w = 1/max_j d_j (the reference's rss weighting,
+ * src/motility_fit.js:392-398,469).  Box constraints by projection.  This
+ * oracle uses the exact 2D kernel (isf2d) where the GPU uses the tabulated one.
+ */
+static double one_minus_isf(double x, double Z, int geometry) {
+  double L = x / (Z + 1);
+  if (geometry == 0) {
+    if (L < 1e-4) return (x * x * (Z + 2)) / (6 * (Z + 1));
+    return 1 - sin(Z * atan(L)) / (Z * L * exp(0.5 * Z * log1p(L * L)));
+  }
+  if (L < 1e-4) return (x * x * (Z + 2)) / (4 * (Z + 1));
+  return 1 - (geometry == 1 ? orc_isf2d(x, Z) : orc_isf2d_fast(x, Z));
+}
+
+/* model column and analytic/numeric derivatives; dc[k][j] for k over the
+ * nonlinear parameters (alpha, ln vbar, ln Z, ln D) */
+static void lm_column(int ntau, const double *tau, int K, const double *qn, double alpha, double vbar, double Z,
+                      double Dd, int swim, int geometry, double *c, double (*dc)[128]) {
+  for (int j = 0; j < ntau; j++) {
+    double t = tau[j], cj = 0, da = 0, dlv = 0, dlz = 0, dld = 0;
+    for (int i = 0; i < K; i++) {
+      double q = qn[i], s = Dd * q * q * t;
+      double E = exp(-s), omE = -expm1(-s);
+      double x = q * vbar * t;
+      double omg = swim ? one_minus_isf(x, Z, geometry) : 0;
+      cj += omE + E * alpha * omg;
+      da += E * omg;
+      dld += s * E * (1 - alpha * omg);   /* d/dlnD of [1 - E(1 - alpha omg)] = s E (1 - alpha omg) */
+      if (swim) {
+        double hv = 1e-5, hz = 1e-5;
+        double omgp = one_minus_isf(x * exp(hv), Z, geometry), omgm = one_minus_isf(x * exp(-hv), Z, geometry);
+        dlv += E * alpha * (omgp - omgm) / (2 * hv);
+        double omzp = one_minus_isf(x, Z * exp(hz), geometry), omzm = one_minus_isf(x, Z * exp(-hz), geometry);
+        dlz += E * alpha * (omzp - omzm) / (2 * hz);
+      }
+    }
+    c[j] = cj / K;
+    if (dc) {
+      dc[0][j] = da / K;
+      dc[1][j] = dlv / K;
+      dc[2][j] = dlz / K;
+      dc[3][j] = dld / K;
+    }
+  }
+}
+
+static int cholesky_solve(int p, double *H, const double *g, double *x) {
+  double Lm[8][8] = {{0}};
+  for (int i = 0; i < p; i++) {
+    for (int j = 0; j <= i; j++) {
+      double s = H[i * p + j];
+      for (int k = 0; k < j; k++) s -= Lm[i][k] * Lm[j][k];
+      if (i == j) {
+        if (!(s > 0)) return 0;
+        Lm[i][i] = sqrt(s);
+      } else {
+        Lm[i][j] = s / Lm[j][j];
+      }
+    }
+  }
+  double y[8];
+  for (int i = 0; i < p; i++) {
+    double s = g[i];
+    for (int k = 0; k < i; k++) s -= Lm[i][k] * y[k];
+    y[i] = s / Lm[i][i];
+  }
+  for (int i = p - 1; i >= 0; i--) {
+    double s = y[i];
+    for (int k = i + 1; k < p; k++) s -= Lm[k][i] * x[k];
+    x[i] = s / Lm[i][i];
+  }
+  return 1;
+}
+
+/* theta layout (6): A, B, alpha, ln vbar, ln Z, ln D.  Brownian uses A, B, ln D
+ * (alpha = 0; vbar, Z unused).  lo/hi: box (6). */
+static double lm_chi2(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                      const double *th, int swim, int geometry, double *c) {
+  lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, c, NULL);
+  double s = 0;
+  for (int j = 0; j < ntau; j++) {
+    double r = w * (d[j] - (th[0] * c[j] + th[1]));
+    s += r * r;
+  }
+  return s;
+}
+
+/* 2-column NNLS (A, B >= 0) for a fixed column c: weighted LS in closed form */
+static double nnls_ab(int ntau, const double *c, const double *d, double w, double *A, double *B) {
+  double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
+  for (int j = 0; j < ntau; j++) {
+    g11 += c[j] * c[j];
+    g12 += c[j];
+    h1 += c[j] * d[j];
+    h2 += d[j];
+    dd += d[j] * d[j];
+  }
+  double G[9] = {g11, g12, 0, g12, (double)ntau, 0, 0, 0, 0}, h[3] = {h1, h2, 0}, a[3], r;
+  orc_nnls3(G, h, dd, a, &r);
+  *A = a[0];
+  *B = a[1];
+  return r * w * w;
+}
+
+/* Grid seed, the reference's start grid applied to one bin: Brownian 49-point
+ * log-D grid, then (swimmer) 16 vbar x Z{1.5, 8} x alpha{0.3, 0.8} at that D;
+ * A, B by NNLS at every grid point. */
+void orc_lm_seed(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                 const double *bounds4, int swim, int geometry, double *th) {
+  double c[128];
+  double lo = log(bounds4[2]), hi = log(bounds4[3]);
+  double best = INFINITY;
+  for (int i = 0; i <= 48; i++) {
+    double lD = lo + ((hi - lo) * i) / 48, A, B;
+    lm_column(ntau, tau, K, qn, 0, 1, 1, exp(lD), 0, geometry, c, NULL);
+    double r = nnls_ab(ntau, c, d, w, &A, &B);
+    if (r < best) {
+      best = r;
+      th[0] = A;
+      th[1] = B;
+      th[2] = 0;
+      th[3] = log(sqrt(bounds4[0] * bounds4[1]));
+      th[4] = log(4.0);
+      th[5] = lD;
+    }
+  }
+  if (!swim) return;
+  double lD = th[5];
+  best = INFINITY;
+  double lZs[2] = {log(1.5), log(8.0)}, as[2] = {0.3, 0.8};
+  for (int i = 0; i < 16; i++) {
+    double lv = log(bounds4[0]) + ((log(bounds4[1]) - log(bounds4[0])) * (i + 0.5)) / 16;
+    for (int z = 0; z < 2; z++)
+      for (int q = 0; q < 2; q++) {
+        double A, B;
+        lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(lD), 1, geometry, c, NULL);
+        double r = nnls_ab(ntau, c, d, w, &A, &B);
+        if (r < best) {
+          best = r;
+          th[0] = A;
+          th[1] = B;
+          th[2] = as[q];
+          th[3] = lv;
+          th[4] = lZs[z];
+          th[5] = lD;
+        }
+      }
+  }
+}
+
+/* One per-q LM fit (ntau <= 128, K <= 16).  bounds4: vmin, vmax, Dmin, Dmax.
+ * th: in = seed, out = fit.  Marquardt scaling D = diag(F^T F), Nielsen's
+ * gain-ratio damping update.  Returns chi^2; *iters = LM steps taken. */
+double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                  const double *bounds4, int swim, int geometry, int maxit, double *th, int *iters) {
+  double lo[6] = {0, 0, 0, log(bounds4[0]), log(0.5), log(bounds4[2])};
+  double hi[6] = {INFINITY, INFINITY, 1, log(bounds4[1]), log(100.0), log(bounds4[3])};
+  int idx[6] = {0, 1, 2, 3, 4, 5}, p = 6;
+  if (!swim) {
+    idx[2] = 5;
+    p = 3;
+  }
+  for (int k = 0; k < 6; k++) th[k] = clampd(th[k], lo[k], hi[k]);
+  double c[128], dc[4][128], cn[128];
+  double chi = lm_chi2(ntau, tau, d, w, K, qn, th, swim, geometry, c);
+  double lam = 1e-3, nu = 2;
+  int it = 0, done = 0;
+  while (it < maxit && !done) {
+    lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, c, dc);
+    double H[36] = {0}, g[6] = {0};
+    for (int j = 0; j < ntau; j++) {
+      double F[6];
+      F[0] = w * c[j];
+      F[1] = w;
+      F[2] = w * th[0] * dc[0][j];
+      F[3] = w * th[0] * dc[1][j];
+      F[4] = w * th[0] * dc[2][j];
+      F[5] = w * th[0] * dc[3][j];
+      double r = w * (d[j] - (th[0] * c[j] + th[1]));
+      for (int a = 0; a < p; a++) {
+        g[a] += F[idx[a]] * r;
+        for (int b = 0; b <= a; b++) H[a * p + b] += F[idx[a]] * F[idx[b]];
+      }
+    }
+    for (int a = 0; a < p; a++)
+      for (int b = 0; b < a; b++) H[b * p + a] = H[a * p + b];
+    /* one damped step (retried with larger damping until it lowers chi^2) */
+    for (;;) {
+      it++;
+      double Hd[36], delta[6], ntheta[6];
+      memcpy(Hd, H, sizeof(Hd));
+      for (int a = 0; a < p; a++) Hd[a * p + a] += lam * (H[a * p + a] > 1e-300 ? H[a * p + a] : 1e-300);
+      int ok = cholesky_solve(p, Hd, g, delta);
+      if (ok) {
+        memcpy(ntheta, th, sizeof(ntheta));
+        double pred = 0;
+        for (int a = 0; a < p; a++) {
+          int k = idx[a];
+          ntheta[k] = clampd(th[k] + delta[a], lo[k], hi[k]);
+          double dk = ntheta[k] - th[k];
+          pred += dk * (lam * (H[a * p + a] > 1e-300 ? H[a * p + a] : 1e-300) * dk + g[a]);
+        }
+        double nchi = lm_chi2(ntau, tau, d, w, K, qn, ntheta, swim, geometry, cn);
+        double rho = pred > 0 ? (chi - nchi) / pred : -1;
+        if (nchi < chi && rho > 0) {
+          double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
+          memcpy(th, ntheta, sizeof(ntheta));
+          chi = nchi;
+          double f = 1 - (2 * rho - 1) * (2 * rho - 1) * (2 * rho - 1);
+          lam *= f > 1.0 / 3 ? f : 1.0 / 3;
+          if (lam < 1e-15) lam = 1e-15;
+          nu = 2;
+          if (rel < 1e-13 || chi < 1e-300) done = 1;
+          break;
+        }
+      }
+      lam *= nu;
+      nu *= 2;
+      if (lam > 1e15 || it >= maxit) {
+        done = 1;
+        break;
+      }
+    }
+  }
+  if (iters) *iters = it;
+  return chi;
+}
+
+/* Plain grid-seeded per-q LM: the 64 grid starts of orc_lm_seed are ranked by
+ * chi^2 at NNLS amplitudes, LM runs from the best `nstarts`, the lowest final
+ * chi^2 wins.  out6 = theta; returns chi^2; *iters = total LM steps. */
+double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                    const double *bounds4, int swim, int geometry, int nstarts, int maxit, double *out6,
+                    int *iters) {
+  double c[128];
+  double starts[64][6], sf[64];
+  int ns = 0;
+  double th0[6];
+  orc_lm_seed(ntau, tau, d, w, K, qn, bounds4, 0, geometry, th0);   /* Brownian seed: D, A, B */
+  if (!swim) {
+    starts[0][0] = th0[0];
+    for (int k = 0; k < 6; k++) starts[0][k] = th0[k];
+    sf[0] = 0;
+    ns = 1;
+  } else {
+    double lZs[2] = {log(1.5), log(8.0)}, as[2] = {0.3, 0.8};
+    for (int i = 0; i < 16; i++) {
+      double lv = log(bounds4[0]) + ((log(bounds4[1]) - log(bounds4[0])) * (i + 0.5)) / 16;
+      for (int z = 0; z < 2; z++)
+        for (int q = 0; q < 2; q++) {
+          double A, B;
+          lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(th0[5]), 1, geometry, c, NULL);
+          sf[ns] = nnls_ab(ntau, c, d, w, &A, &B);
+          double s[6] = {A, B, as[q], lv, lZs[z], th0[5]};
+          memcpy(starts[ns], s, sizeof(s));
+          ns++;
+        }
+    }
+  }
+  int idx[64];
+  for (int i = 0; i < ns; i++) idx[i] = i;
+  for (int i = 1; i < ns; i++) {
+    int v = idx[i], k = i - 1;
+    while (k >= 0 && sf[idx[k]] > sf[v]) {
+      idx[k + 1] = idx[k];
+      k--;
+    }
+    idx[k + 1] = v;
+  }
+  double best = INFINITY;
+  int tot = 0;
+  for (int s = 0; s < (nstarts < ns ? nstarts : ns); s++) {
+    double th[6];
+    int it;
+    memcpy(th, starts[idx[s]], sizeof(th));
+    double chi = orc_lm_fit(ntau, tau, d, w, K, qn, bounds4, swim, geometry, maxit, th, &it);
+    tot += it;
+    if (chi < best) {
+      best = chi;
+      memcpy(out6, th, sizeof(th));
+    }
+  }
+  if (iters) *iters = tot;
+  return best;
+}

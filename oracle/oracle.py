@@ -243,3 +243,126 @@ def shuffle_order(order, mask, rand):
         order[i], order[j] = order[j], order[i]
         mask[i], mask[j] = mask[j], mask[i]
     return order, mask
+
+
+# ------------------------------------------------------------------ fits
+
+def _fit_decls():
+    L = lib()
+    if getattr(L, "_fit_declared", False):
+        return L
+    L.orc_eval_model.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p]
+    L.orc_eval_model.restype = C.c_double
+    L.orc_bounds.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp]
+    L.orc_fit_brownian.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                   C.POINTER(C.c_double)]
+    L.orc_fit_brownian.restype = C.c_double
+    L.orc_fit_swimmer.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                  _dp]
+    L.orc_fit_swimmer.restype = C.c_double
+    L.orc_lm_seed.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+    L.orc_lm_fit.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
+                             C.POINTER(C.c_int)]
+    L.orc_lm_fit.restype = C.c_double
+    L._fit_declared = True
+    return L
+
+
+def prepare(radial: dict, dxUm: float, fps: float) -> dict:
+    """prepare (src/motility_fit.js:383-401): bins with nModes >= 3 and max D > 0,
+    q, q-nodes from kNodes, data on the lag grid, weight 1/max d."""
+    N, NB, L = radial["N"], radial["NB"], radial["L"]
+    dt = 1.0 / fps
+    lags = lag_grid(L)
+    tau = lags * dt
+    D = np.asarray(radial["D"]).reshape(NB, L)
+    qs = 2 * np.pi / (N * dxUm)
+    kn = radial.get("kNodes")
+    K = len(kn) // NB if kn is not None and len(kn) else 0
+    out = []
+    for b in range(NB):
+        if radial["nModes"][b] < 3:
+            continue
+        d = D[b, lags]
+        scale = d.max()
+        if not (scale > 0) or not np.isfinite(scale):
+            continue
+        q = qs * radial["kCenter"][b]
+        qn = np.asarray(kn[b * K:(b + 1) * K]) * qs if K else np.array([q])
+        out.append(dict(b=b, q=q, qn=qn, d=d.astype(np.float64), w=1.0 / scale))
+    return dict(bins=out, lags=lags, tau=tau, dt=dt, L=L, tauMax=(L - 1) * dt)
+
+
+def _stack(pr):
+    bins = pr["bins"]
+    K = len(bins[0]["qn"])
+    return (len(bins), len(pr["tau"]), K, np.ascontiguousarray(pr["tau"], dtype=np.float64),
+            np.ascontiguousarray(np.concatenate([b["d"] for b in bins])),
+            np.ascontiguousarray([b["w"] for b in bins], dtype=np.float64),
+            np.ascontiguousarray(np.concatenate([b["qn"] for b in bins]), dtype=np.float64))
+
+
+def bounds(pr) -> np.ndarray:
+    q = np.array([b["q"] for b in pr["bins"]], dtype=np.float64)
+    out = np.zeros(6)
+    _fit_decls().orc_bounds(len(q), q, pr["dt"], pr["tauMax"], out)
+    return out
+
+
+def fit_brownian(pr, Vmag=0.0):
+    """fitBrownian (src/motility_fit.js:500-536) -> (Dd, rss)."""
+    nb, nt, K, tau, d, w, qn = _stack(pr)
+    rss = C.c_double()
+    Dd = _fit_decls().orc_fit_brownian(nb, nt, K, tau, d, w, qn, bounds(pr), Vmag, C.byref(rss))
+    return Dd, rss.value
+
+
+def fit_swimmer(pr, geometry="3d", Vmag=0.0, Dstart=0.0):
+    """fitSwimmer (src/motility_fit.js:577-611) -> dict(vbar, Z, Dd, alpha, rss); 2d uses the table kernel."""
+    nb, nt, K, tau, d, w, qn = _stack(pr)
+    out = np.zeros(4)
+    geo = 0 if geometry == "3d" else 2
+    rss = _fit_decls().orc_fit_swimmer(nb, nt, K, tau, d, w, qn, bounds(pr), geo, Vmag, Dstart, out)
+    return dict(vbar=out[0], Z=out[1], Dd=out[2], alpha=out[3], rss=rss)
+
+
+GEOMETRY = {"3d": 0, "2d-exact": 1, "2d": 2}
+
+
+def lm_seed(tau, d, w, qn, bounds4, swim=True, geometry="3d"):
+    th = np.zeros(6)
+    _fit_decls().orc_lm_seed(len(tau), np.ascontiguousarray(tau, dtype=np.float64),
+                             np.ascontiguousarray(d, dtype=np.float64), w, len(qn),
+                             np.ascontiguousarray(qn, dtype=np.float64), np.ascontiguousarray(bounds4, dtype=np.float64),
+                             int(swim), GEOMETRY[geometry], th)
+    return th
+
+
+def lm_fit(tau, d, w, qn, bounds4, seed, swim=True, geometry="3d", maxit=200):
+    """Per-q LM (FP64) from theta = (A, B, alpha, ln vbar, ln Z, ln D) -> (theta, chi2, iterations)."""
+    th = np.ascontiguousarray(seed, dtype=np.float64).copy()
+    it = C.c_int()
+    chi = _fit_decls().orc_lm_fit(len(tau), np.ascontiguousarray(tau, dtype=np.float64),
+                                  np.ascontiguousarray(d, dtype=np.float64), w, len(qn),
+                                  np.ascontiguousarray(qn, dtype=np.float64),
+                                  np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], maxit,
+                                  th, C.byref(it))
+    return th, chi, it.value
+
+
+def lm_plain(tau, d, w, qn, bounds4, swim=True, geometry="3d", nstarts=3, maxit=100):
+    """Grid-seeded per-q LM from the best `nstarts` grid starts -> (theta, chi2, total iterations)."""
+    L = _fit_decls()
+    if not hasattr(L, "_plain"):
+        L.orc_lm_plain.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, _dp, C.POINTER(C.c_int)]
+        L.orc_lm_plain.restype = C.c_double
+        L._plain = True
+    th = np.zeros(6)
+    it = C.c_int()
+    chi = L.orc_lm_plain(len(tau), np.ascontiguousarray(tau, dtype=np.float64), np.ascontiguousarray(d, dtype=np.float64),
+                         w, len(qn), np.ascontiguousarray(qn, dtype=np.float64),
+                         np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], nstarts, maxit,
+                         th, C.byref(it))
+    return th, chi, it.value
