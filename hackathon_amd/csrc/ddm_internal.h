@@ -37,4 +37,15 @@ hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, cons
                          const float* bounds, const float* seeds, int nfits, int swim, int grid, int nstarts,
                          int maxit, float* theta, double* chi, int* steps, hipStream_t st);
 
+hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
+                            float* C, long ldc, int M, int N, int K, int acc, hipStream_t st);
+hipError_t launch_bias_act(float* Z, const float* bias, unsigned short* H, int M, int N, int gelu, hipStream_t st);
+hipError_t launch_act_bwd(const float* dH, const float* Z, unsigned short* dZ, float* dZf, float* dbias, int M, int N,
+                          int gelu, hipStream_t st);
+hipError_t launch_mse(const float* Y, const float* Tg, const float* wt, float* dY, float* rowloss, int M, int N,
+                      hipStream_t st);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, unsigned short* pb, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t st);
+hipError_t launch_to_bf16(const float* x, unsigned short* y, long n, hipStream_t st);
+
 }  // namespace ddm

@@ -31,7 +31,8 @@ KERNEL_NAMES = ("spatial_rows", "spatial_cols_cornerturn", "temporal_wk", "radia
 EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_get_bins", "ddm_ingest",
            "ddm_ingest_device", "ddm_compute_temporal", "ddm_read_radial", "ddm_read_history", "ddm_idle",
            "ddm_history_device", "ddm_set_profiling", "ddm_kernel_times", "ddm_fitter_create", "ddm_fitter_destroy",
-           "ddm_fitter_last_error", "ddm_fit")
+           "ddm_fitter_last_error", "ddm_fit", "ddm_mlp_create", "ddm_mlp_destroy", "ddm_mlp_last_error",
+           "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step")
 
 
 class DdmError(RuntimeError):
@@ -78,7 +79,8 @@ def load_library(path: str = LIB_PATH):
     L.ddm_set_profiling.argtypes = [vp, C.c_int32]
     L.ddm_kernel_times.argtypes = [vp, vp, vp, C.c_int32]
     for name in EXPORTS:
-        if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error"):
+        if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
+                        "ddm_mlp_destroy", "ddm_mlp_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -178,6 +178,31 @@ const char *ddm_fitter_last_error(const ddm_fitter *f);
  * NULL) is the fit kernel's duration from HIP events. */
 int ddm_fit(ddm_fitter *f, const ddm_fit_batch *b, double *theta, double *chi2, int32_t *iters, double *kernel_ms);
 
+/* ------------------------------------------------------------------ LM-seeding surrogate (MLP)
+ *
+ * 1536 -> 256 -> 256 -> 256 -> 68 with GELU after the hidden layers, BF16
+ * weights and activations on MFMA with FP32 accumulation, FP32 master weights
+ * and Adam.  No reference counterpart.  Parameters are one flat FP32 array of
+ * DDM_MLP_PARAMS values: W0[1536][256], b0[256], W1[256][256], b1, W2[256][256],
+ * b2, W3[256][68], b3[68] (row-major, y = x W + b). */
+typedef struct ddm_mlp ddm_mlp;
+enum { DDM_MLP_IN = 1536, DDM_MLP_HIDDEN = 256, DDM_MLP_OUT = 68, DDM_MLP_PARAMS = 542532 };
+
+int ddm_mlp_create(int32_t max_batch, ddm_mlp **out);
+void ddm_mlp_destroy(ddm_mlp *m);
+const char *ddm_mlp_last_error(const ddm_mlp *m);
+int ddm_mlp_set_params(ddm_mlp *m, const float *params);
+int ddm_mlp_get_params(ddm_mlp *m, float *params);
+/* inference: X[M][1536] -> Y[M][68] (host arrays) */
+int ddm_mlp_forward(ddm_mlp *m, const float *X, int32_t M, float *Y);
+/* weighted MSE loss 0.5/M sum_i sum_j wt[j] (y_ij - t_ij)^2 and its gradient
+ * w.r.t. every parameter (grads[DDM_MLP_PARAMS], may be NULL), no update */
+int ddm_mlp_loss_grad(ddm_mlp *m, const float *X, const float *Tg, const float *wt, int32_t M, float *loss,
+                      float *grads);
+/* one Adam step (t counts steps from 1) on a batch; *loss before the update */
+int ddm_mlp_train_step(ddm_mlp *m, const float *X, const float *Tg, const float *wt, int32_t M, float lr,
+                       float *loss);
+
 #ifdef __cplusplus
 }
 #endif
