@@ -1,0 +1,73 @@
+// Type declarations of the MI355X DDM engine facade (ddm-engine.mjs).  The
+// class surface is the reference's WebGPU engine (motility-ddm
+// src/ddm-engine.ts:62-426) and the Radial record of src/ddm_ref.js:336-348.
+
+export declare const DDM: Readonly<{
+  N: number; T: number; L: number; NB: number; KMIN: number; KMAX: number; KNODES: number; DRIFT_STRIDE: number;
+}>;
+
+export interface Radial {
+  N: number;
+  NB: number;
+  L: number;
+  kCenter: Float64Array;
+  kNodes: Float64Array;
+  nModes: Uint32Array;
+  /** D[b*L + tau] */
+  D: Float64Array;
+  P: Float64Array;
+  /** NB*L*8 phase-drift sums (zero-filled by this engine) */
+  drift: Float64Array;
+}
+
+export interface Bins {
+  N: number;
+  NB: number;
+  kmin: number;
+  kmax: number;
+  binOffsets: Uint32Array;
+  binModes: Int32Array;
+  kCenter: Float64Array;
+  nModes: Uint32Array;
+  kNodes: Float64Array;
+}
+
+export interface EngineOptions {
+  N?: number;
+  T?: number;
+  L?: number;
+  NB?: number;
+  kmin?: number;
+  kmax?: number;
+  /** 0: V8 Math.hypot bin rule (reference parity), 1: correctly rounded */
+  hypotMode?: number;
+  knodes?: number;
+  /** accepted for signature compatibility; the kernels are compiled HIP */
+  shaderCode?: string;
+  powerPreference?: string;
+  onLost?: (info: unknown) => void;
+  onError?: (err: Error) => void;
+}
+
+export declare class DdmEngine {
+  static create(opts?: EngineOptions): Promise<DdmEngine>;
+  readonly N: number;
+  readonly T: number;
+  readonly L: number;
+  readonly NB: number;
+  readonly bins: Bins;
+  readonly adapterInfo: { vendor: string; architecture: string; description: string };
+  throttled: number;
+  readonly pendingSubmits: number;
+  readonly saturated: boolean;
+  waitForCapacity(): Promise<void>;
+  ingestLuma(luma: Float32Array, slot: number): boolean;
+  ingestFrames(frames: Uint8Array | Uint16Array | Float32Array, nframes: number, firstSlot?: number): boolean;
+  computeTemporal(order: Uint32Array, mask: Float32Array, opts?: { yieldBetweenChunks?: boolean }): Promise<void>;
+  readRadial(): Promise<Radial>;
+  readHistory(): Promise<Float32Array>;
+  idle(): Promise<void>;
+  destroy(): void;
+}
+
+export declare function linearOrder(valid: ArrayLike<boolean | number>): { order: Uint32Array; mask: Float32Array };
