@@ -48,4 +48,8 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, unsigned sh
                        float b2, float eps, float c1, float c2, hipStream_t st);
 hipError_t launch_to_bf16(const float* x, unsigned short* y, long n, hipStream_t st);
 
+hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp, double bg, double vbar, double Z,
+                        double D, uint64_t seed, int dtype, int noise, double4* part, float2* pos, void* out,
+                        hipStream_t st);
+
 }  // namespace ddm

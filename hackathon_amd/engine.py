@@ -32,7 +32,8 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_ingest_device", "ddm_compute_temporal", "ddm_read_radial", "ddm_read_history", "ddm_idle",
            "ddm_history_device", "ddm_set_profiling", "ddm_kernel_times", "ddm_fitter_create", "ddm_fitter_destroy",
            "ddm_fitter_last_error", "ddm_fit", "ddm_mlp_create", "ddm_mlp_destroy", "ddm_mlp_last_error",
-           "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step")
+           "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step",
+           "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks")
 
 
 class DdmError(RuntimeError):
@@ -80,7 +81,7 @@ def load_library(path: str = LIB_PATH):
     L.ddm_kernel_times.argtypes = [vp, vp, vp, C.c_int32]
     for name in EXPORTS:
         if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
-                        "ddm_mlp_destroy", "ddm_mlp_last_error"):
+                        "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -8,11 +8,14 @@ N(0, 2D) steps per axis per frame.  The structure follows the reference's movie
 simulator (test/sim.mjs:90-186) with the workload's units (px, px/frame) and
 Poisson instead of Gaussian camera noise.
 
-`video_numpy` runs on the CPU (tests); `video_torch` renders the same model on
-the GPU with torch (bench input only; the rendering is not part of the timed
-path).
+`GpuSynth` / `video_gpu` run the HIP generator of libddm.so (ddm_synth_video:
+Philox4x32-10 streams, Marsaglia-Tsang speeds, PTRS Poisson noise) into device
+memory; `video_numpy` is a small CPU renderer of the same movie model for
+host-side tests, and `video_torch` a torch rendering kept for comparison.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
@@ -106,3 +109,86 @@ def config1(seed=1):
 
 C2 = dict(T=1024, N=512, n_spots=4000, sigma=2.0, amp=100.0, background=200.0, D=0.3, alpha=0.6,
           vbar_px_frame=15.0 * 0.010, Z=3.0)
+
+
+class VideoParams(C.Structure):
+    _fields_ = [("N", C.c_int32), ("T", C.c_int32), ("n_spots", C.c_int32), ("dtype", C.c_int32),
+                ("noise", C.c_int32), ("pad_", C.c_int32), ("sigma", C.c_double), ("amp", C.c_double),
+                ("background", C.c_double), ("alpha", C.c_double), ("vbar", C.c_double), ("Z", C.c_double),
+                ("D", C.c_double), ("seed", C.c_uint64)]
+
+
+_DT = {"uint8": 0, "uint16": 1, "float32": 2}
+
+
+class GpuSynth:
+    """The HIP video generator (ddm_synth_*); outputs go to torch device tensors."""
+
+    def __init__(self):
+        from .engine import DdmError, load_library
+        self._err = DdmError
+        L = load_library()
+        if not getattr(L, "_synth_declared", False):
+            vp = C.c_void_p
+            L.ddm_synth_create.argtypes = [C.POINTER(vp)]
+            L.ddm_synth_create.restype = C.c_int
+            L.ddm_synth_destroy.argtypes = [vp]
+            L.ddm_synth_destroy.restype = None
+            L.ddm_synth_last_error.argtypes = [vp]
+            L.ddm_synth_last_error.restype = C.c_char_p
+            L.ddm_synth_video.argtypes = [vp, C.POINTER(VideoParams), vp]
+            L.ddm_synth_video.restype = C.c_int
+            L.ddm_synth_tracks.argtypes = [vp, C.POINTER(VideoParams), vp]
+            L.ddm_synth_tracks.restype = C.c_int
+            L._synth_declared = True
+        self._lib = L
+        h = C.c_void_p()
+        if L.ddm_synth_create(C.byref(h)) != 0:
+            raise DdmError("ddm_synth_create failed")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.ddm_synth_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def params(T, N, n_spots, sigma, amp, background, D=0.0, alpha=0.0, vbar_px_frame=0.0, Z=3.0,
+               dtype="uint16", seed=1, noise=True) -> VideoParams:
+        return VideoParams(int(N), int(T), int(n_spots), _DT[dtype], 1 if noise else 0, 0, float(sigma), float(amp),
+                           float(background), float(alpha), float(vbar_px_frame), float(Z), float(D), int(seed))
+
+    def video(self, p: VideoParams, out=None):
+        """(T, N, N) device tensor (uint8 / int16 holding uint16 bits / float32)."""
+        import torch
+        tdt = {0: torch.uint8, 1: torch.int16, 2: torch.float32}[p.dtype]
+        if out is None:
+            out = torch.empty((p.T, p.N, p.N), dtype=tdt, device="cuda")
+        elif out.dtype != tdt or out.numel() < p.T * p.N * p.N or not out.is_cuda:
+            raise self._err("output tensor does not fit the video")
+        rc = self._lib.ddm_synth_video(self._h, C.byref(p), C.c_void_p(out.data_ptr()))
+        if rc != 0:
+            raise self._err(self._lib.ddm_synth_last_error(self._h).decode())
+        return out
+
+    def tracks(self, p: VideoParams):
+        """(T, n_spots, 2) device float32 positions."""
+        import torch
+        out = torch.empty((p.T, max(p.n_spots, 1), 2), dtype=torch.float32, device="cuda")
+        rc = self._lib.ddm_synth_tracks(self._h, C.byref(p), C.c_void_p(out.data_ptr()))
+        if rc != 0:
+            raise self._err(self._lib.ddm_synth_last_error(self._h).decode())
+        return out[:, :p.n_spots]
+
+
+def video_gpu(T, N, n_spots, sigma, amp, background, D=0.0, alpha=0.0, vbar_px_frame=0.0, Z=3.0, dtype="uint16",
+              seed=2, synth: GpuSynth | None = None):
+    """HIP-rendered (T, N, N) frame stack in device memory."""
+    g = synth or GpuSynth()
+    return g.video(GpuSynth.params(T, N, n_spots, sigma, amp, background, D, alpha, vbar_px_frame, Z, dtype, seed))

@@ -203,6 +203,39 @@ int ddm_mlp_loss_grad(ddm_mlp *m, const float *X, const float *Tg, const float *
 int ddm_mlp_train_step(ddm_mlp *m, const float *X, const float *Tg, const float *wt, int32_t M, float lr,
                        float *loss);
 
+/* ------------------------------------------------------------------ synthetic videos
+ *
+ * Seeded video generator on the GPU (the benchmark and training workloads):
+ * n_spots Gaussian spots (width sigma px, peak amp) on a constant background,
+ * the first round(alpha * n_spots) swim in straight lines (uniform heading,
+ * speed ~ Gamma(Z+1, vbar/(Z+1)) px/frame), the rest take N(0, 2D) steps per
+ * axis per frame (D in px^2/frame), periodic wrap, Poisson photon noise
+ * (noise = 0 stores the expected intensity instead), clipped to the pixel type.
+ * The model follows the reference's movie simulator (motility-ddm
+ * test/sim.mjs:90-186) in pixel units; random draws are Philox4x32-10 streams
+ * keyed by `seed`, so a CPU restatement reproduces them. */
+typedef struct ddm_synth ddm_synth;
+typedef struct ddm_video_params {
+  int32_t N;        /* frame side, multiple of 16, 16..4096 */
+  int32_t T;        /* frames */
+  int32_t n_spots;
+  int32_t dtype;    /* DDM_U8 / DDM_U16 / DDM_F32 */
+  int32_t noise;    /* 1: Poisson counts, 0: expected intensity */
+  int32_t pad_;
+  double sigma, amp, background;
+  double alpha, vbar, Z, D;
+  uint64_t seed;
+} ddm_video_params;
+
+int ddm_synth_create(ddm_synth **out);
+void ddm_synth_destroy(ddm_synth *s);
+const char *ddm_synth_last_error(const ddm_synth *s);
+/* Renders T frames of N*N pixels into device memory `dframes` (frame-major,
+ * row-major); returns when the frames are written. */
+int ddm_synth_video(ddm_synth *s, const ddm_video_params *p, void *dframes);
+/* Particle positions only: dpos[(t * n_spots + i) * 2 + {x, y}] (device, f32). */
+int ddm_synth_tracks(ddm_synth *s, const ddm_video_params *p, float *dpos);
+
 #ifdef __cplusplus
 }
 #endif

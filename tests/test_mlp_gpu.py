@@ -38,7 +38,7 @@ def test_forward_matches_oracle(gpu, M):
     # pass shows the same few rows at M=200).  The BF16 format itself: loose.
     row = np.linalg.norm(y - yb, axis=1) / np.linalg.norm(yb, axis=1)
     assert np.median(row) < 1e-5
-    assert np.mean(row > 1e-5) < 0.05
+    assert np.mean(row > 1e-5) < 0.2 and np.max(row) < 5e-3
     assert eb < 1e-3
     assert np.max(np.abs(y - yb)) < 3e-3 * np.max(np.abs(yb))
     assert e64 < 3e-2

@@ -1,0 +1,92 @@
+"""HIP video generator against its CPU restatement, and the moments of its draws."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import synth_oracle as SO
+from hackathon_amd.engine import DdmError
+from hackathon_amd.synth import GpuSynth
+
+pytestmark = pytest.mark.gpu
+
+
+def _wrapdiff(a, b, N):
+    d = np.mod(a - b + N / 2, N) - N / 2
+    return np.abs(d)
+
+
+def test_tracks_match_restatement(gpu):
+    g = GpuSynth()
+    N, T, n = 64, 12, 40
+    p = g.params(T, N, n, 1.5, 100, 20, D=0.5, alpha=0.5, vbar_px_frame=1.7, Z=3.0, seed=0x1234_5678_9ABC)
+    got = g.tracks(p).cpu().numpy().astype(np.float64)
+    ref = SO.tracks(N, T, n, 0.5, 1.7, 3.0, 0.5, 0x1234_5678_9ABC)
+    assert np.max(_wrapdiff(got, ref, N)) < 1e-4
+
+
+def test_render_expected_intensity_matches(gpu):
+    g = GpuSynth()
+    N, T, n = 48, 3, 30
+    p = g.params(T, N, n, 1.5, 100, 20, D=0.5, alpha=0.3, vbar_px_frame=2.0, Z=4.0, dtype="float32", seed=11,
+                 noise=False)
+    vid = g.video(p).cpu().numpy().astype(np.float64)
+    pos = g.tracks(p).cpu().numpy()
+    for t in range(T):
+        lam = SO.intensity(N, pos[t], 1.5, 100.0, 20.0)
+        assert np.max(np.abs(vid[t] - lam) / lam) < 1e-5
+
+
+def test_poisson_frames_match_restatement(gpu):
+    g = GpuSynth()
+    N, T, n = 32, 2, 25
+    p = g.params(T, N, n, 2.0, 100, 200, D=0.3, alpha=0.6, vbar_px_frame=1.0, Z=3.0, dtype="uint16", seed=77)
+    vid = g.video(p).cpu().numpy().astype(np.int64) & 0xFFFF
+    pos = g.tracks(p).cpu().numpy()
+    for t in range(T):
+        ref = SO.noisy_frame(SO.intensity(N, pos[t], 2.0, 100.0, 200.0), t, 77, 65535)
+        assert np.mean(vid[t] == ref) > 0.999
+
+
+@pytest.mark.parametrize("lam", [0.4, 6.0, 9.99, 10.0, 37.0, 200.0, 3000.0])
+def test_poisson_moments(gpu, lam):
+    g = GpuSynth()
+    v = g.video(g.params(16, 256, 0, 1.0, 0, lam, dtype="float32", seed=5)).cpu().numpy().astype(np.float64).ravel()
+    n = v.size
+    assert np.all(v == np.floor(v))
+    assert abs(v.mean() - lam) < 5 * math.sqrt(lam / n)
+    assert abs(v.var() - lam) < 5 * math.sqrt((lam + 2 * lam * lam) / n)
+    if lam < 1:
+        assert abs(np.mean(v == 0) - math.exp(-lam)) < 5 * math.sqrt(math.exp(-lam) / n)
+
+
+def test_schulz_speed_moments(gpu):
+    g = GpuSynth()
+    vbar, Z, n = 3.0, 3.0, 20000
+    pos = g.tracks(g.params(2, 256, n, 1.0, 0, 0, alpha=1.0, vbar_px_frame=vbar, Z=Z, seed=9)).cpu().numpy()
+    d = np.mod(pos[1] - pos[0] + 128, 256) - 128
+    s = np.hypot(d[:, 0], d[:, 1])
+    assert abs(s.mean() / vbar - 1) < 4 * math.sqrt(1 / (Z + 1) / n)
+    assert abs(s.var() / (vbar * vbar / (Z + 1)) - 1) < 0.05
+    # uniform heading
+    ang = np.arctan2(d[:, 1], d[:, 0])
+    assert abs(np.mean(np.cos(ang))) < 0.03 and abs(np.mean(np.sin(ang))) < 0.03
+
+
+def test_brownian_msd(gpu):
+    g = GpuSynth()
+    D, T, n = 0.5, 33, 4000
+    pos = g.tracks(g.params(T, 256, n, 1.0, 0, 0, D=D, seed=4)).cpu().numpy()
+    d = np.mod(pos[32] - pos[0] + 128, 256) - 128
+    msd = np.mean(np.sum(d * d, axis=1))
+    assert abs(msd / (4 * D * 32) - 1) < 0.1
+
+
+def test_u8_clip_and_errors(gpu):
+    g = GpuSynth()
+    v = g.video(g.params(2, 32, 5, 1.5, 300, 250, dtype="uint8", seed=1)).cpu().numpy()
+    assert v.max() == 255
+    with pytest.raises(DdmError, match="multiple of 16"):
+        g.video(g.params(2, 40, 5, 1.5, 100, 20, dtype="uint8"))
+    with pytest.raises(DdmError, match="out of range"):
+        g.video(g.params(2, 32, 5, 1.5, 100, 20, alpha=1.5, dtype="uint8"))
