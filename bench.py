@@ -10,7 +10,11 @@ timed region starts.
 Prints one JSON line.  value = frame-pixels/s through to D(q, tau).
 `roofline` is for the dominant kernel, timed live with HIP events on the
 engine's stream; `cpu_baseline` is the FP64 oracle (one thread) on a bounded
-slice of the same workload.
+slice of the same workload.  `fits` is the second half of the metric: per-q
+Schulz-model LM fits per second on held-out simulated videos (256x256x512,
+32 bins each), plain grid-seeded LM against surrogate-seeded LM, with the
+acceptance rule chi^2_seeded <= chi^2_plain (1 + 1e-6) and fallback to the
+plain result for rejected fits.
 """
 from __future__ import annotations
 
@@ -36,7 +40,28 @@ def parse():
     p.add_argument("--bins", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-frames", type=int, default=256, help="frames in the CPU-baseline slice")
+    p.add_argument("--fit-videos", type=int, default=320, help="held-out videos for the fits/s comparison")
+    p.add_argument("--no-fits", action="store_true")
     return p.parse_args()
+
+
+def fits_section(nvideos):
+    """Plain vs surrogate-seeded per-q LM on held-out videos (10,240 curves at 320 videos)."""
+    from hackathon_amd import surrogate as S
+    from hackathon_amd.fit import Fitter
+    if not os.path.exists(S.WEIGHTS_PATH):
+        return {"skipped": "no trained surrogate weights"}
+    sim = S.Simulator()
+    fitter = Fitter()
+    X, _, truth, probs = S.build_dataset(nvideos, S.HELDOUT_SEED0, sim, fitter, keep_problems=True,
+                                         log=lambda m: print("[bench] fits: " + m, file=sys.stderr, flush=True))
+    sur = S.Surrogate()
+    pred, mlp_ms = S.timed_predict(sur, X)
+    res, _, _ = S.compare(fitter, probs, S.Surrogate.seeds(pred, probs), truth, mlp_ms)
+    res["workload"] = (f"{nvideos} held-out videos of 256x256x512 u8 (seeds {S.HELDOUT_SEED0}..), "
+                       f"{len(probs)} per-q Schulz-model fits, plain = 3 best of the reference start grid, "
+                       f"seeded = 1 start from the MLP; fits/s from HIP-event kernel times (+ MLP inference)")
+    return res
 
 
 def cpu_baseline(N, frames, nbins):
@@ -75,7 +100,7 @@ def main():
     c2 = dict(synth.C2)
     c2.update(T=T, N=N)
     print(f"[bench] generating {T} frames of {N}x{N} u16 on the GPU", file=sys.stderr, flush=True)
-    frames = synth.video_torch(dtype="uint16", seed=2, device=dev, **c2)
+    frames = synth.video_gpu(dtype="uint16", seed=2, **c2)
     torch.cuda.synchronize()
     eng = DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=0, kmax=N // 2)
     order = np.arange(T, dtype=np.uint32)
@@ -147,7 +172,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded spots: 60% Schulz swimmers, 40% diffusers, Poisson noise; GPU-rendered)",
+        "data": "synthetic (seeded spots: 60% Schulz swimmers, 40% diffusers, Poisson noise; rendered by the "
+                "library's HIP generator)",
         "config": {"workload": f"plain DDM, {T} frames of {N}x{N} u16, lags 1..{L - 1}, {NB} unit-width |q| bins",
                    "frames": T, "N": N, "lags": L, "bins": NB},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -159,6 +185,9 @@ def main():
     }
     eng.destroy()
     del frames
+    if not a.no_fits:
+        print("[bench] fits: plain vs seeded LM", file=sys.stderr, flush=True)
+        out["fits"] = fits_section(a.fit_videos)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(N, min(a.cpu_frames, T), NB)

@@ -120,17 +120,42 @@ __device__ __forceinline__ void column(float t, const float* q, int K, float alp
   dld *= inv;
 }
 
-// chi^2 at theta (FP64 sum over the wave).
-__device__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q, int K,
+// 1 - g(x; Z) in FP64 (value only), the series below u = 1e-4 as above.
+__device__ __noinline__ double one_minus_isf3d_d(double x, double Z) {
+  const double u = x / (Z + 1);
+  if (u < 1e-4) return x * x * (Z + 2) / (6 * (Z + 1));
+  const double phi = atan(u);
+  return 1 - sin(Z * phi) * exp(-0.5 * Z * log1p(u * u)) / (Z * u);
+}
+
+// Column value at one lag in FP64 (the objective; the Jacobian stays FP32).
+__device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd, bool swim) {
+  double c = 0;
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) {
+    if (i >= K) break;
+    const double qi = q[i];
+    const double s = Dd * qi * qi * t;
+    const double og = swim ? one_minus_isf3d_d(qi * vbar * t, Z) : 0.0;
+    c += -expm1(-s) + exp(-s) * alpha * og;
+  }
+  return c / K;
+}
+
+// chi^2 at theta with the model in FP64 and an FP64 sum over the wave: the
+// objective is resolved far below the 1e-6 relative level at which fits are
+// compared, so LM accept/reject and termination decisions are not driven by
+// FP32 rounding of the model.
+__device__ __noinline__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q, int K,
                           bool swim, int lane) {
   double s = 0;
-  const float alpha = swim ? th[2] : 0.f, vbar = expf(th[3]), Z = expf(th[4]), Dd = expf(th[5]);
+  const double alpha = swim ? th[2] : 0.0, vbar = exp((double)th[3]), Z = exp((double)th[4]),
+               Dd = exp((double)th[5]);
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
-      float c, a, b, e, f;
-      column(tv[r], q, K, alpha, vbar, Z, Dd, swim, false, c, a, b, e, f);
+      const double c = column_d(tv[r], q, K, alpha, vbar, Z, Dd, swim);
       const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
       s += res * res;
     }
@@ -249,20 +274,36 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
     for (int k = 0; k < 21; k++) H[k] = wave_sum_d(H[k]);
 #pragma unroll
     for (int k = 0; k < 6; k++) g[k] = wave_sum_d(g[k]);
+    // Active set: a parameter on a bound whose gradient points out of the box
+    // is held for this step (its row and column drop out of the system).
+    // Marquardt scaling with a floor of 1e-12 x the largest diagonal, so a
+    // parameter the data barely see cannot take an unbounded step.
+    bool frozen[6];
+    double diag[6], dmax = 0;
+#pragma unroll
+    for (int a = 0; a < p; a++) {
+      const int k = idx[a];
+      frozen[a] = (th[k] <= lo[k] && g[a] < 0) || (th[k] >= hi[k] && g[a] > 0);
+      const double Haa = H[a * (a + 1) / 2 + a];
+      dmax = Haa > dmax ? Haa : dmax;
+    }
+#pragma unroll
+    for (int a = 0; a < p; a++) {
+      const double Haa = H[a * (a + 1) / 2 + a], fl = fmax(1e-12 * dmax, 1e-300);
+      diag[a] = Haa > fl ? Haa : fl;
+      if (frozen[a]) g[a] = 0;
+    }
     for (;;) {
       it++;
-      // Cholesky of H + lam diag(H), packed lower triangle, FP64
-      double Lc[21], diag[6];
+      // Cholesky of H + lam diag, packed lower triangle, FP64
+      double Lc[21];
       int h = 0;
 #pragma unroll
       for (int a = 0; a < p; a++)
 #pragma unroll
         for (int b = 0; b <= a; b++, h++) {
-          Lc[h] = H[h];
-          if (a == b) {
-            diag[a] = H[h] > 1e-300 ? H[h] : 1e-300;
-            Lc[h] += lam * diag[a];
-          }
+          if (frozen[a] || frozen[b]) Lc[h] = a == b ? 1.0 : 0.0;
+          else Lc[h] = H[h] + (a == b ? lam * diag[a] : 0.0);
         }
       bool ok = true;
 #pragma unroll
@@ -320,7 +361,9 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
           lam *= f > 1.0 / 3 ? f : 1.0 / 3;
           if (lam < 1e-15) lam = 1e-15;
           nu = 2;
-          if (rel < 1e-10 || chi < 1e-300) done = true;
+          // converged: the actual and the predicted relative decrease are both
+          // below 1e-9 (MINPACK's ftol test), or the decrease vanished
+          if ((rel < 1e-9 && pred < 1e-9 * chi) || rel < 1e-12 || chi < 1e-300) done = true;
           break;
         }
       }
@@ -338,9 +381,10 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
 
 }  // namespace
 
-// grid != 0: plain grid-seeded LM from the best `nstarts` starts; else LM from seeds.
+// mode 0: LM from the seeds; 1: plain grid-seeded LM from the best `nstarts`
+// starts; 2: nonlinear parameters from the seeds, A and B by NNLS there.
 template <bool SWIM>
-__global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int grid, int nstarts, int maxit,
+__global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
                                                 float* __restrict__ theta_out, double* __restrict__ chi_out,
                                                 int* __restrict__ steps_out) {
   const int lane = threadIdx.x & 63;
@@ -365,9 +409,19 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int grid, i
   int steps = 0;
   float st[3][6];
   int nrun = 1;
-  if (!grid) {
+  if (mode != 1) {
 #pragma unroll
     for (int k = 0; k < 6; k++) st[0][k] = in.seeds[(long)f * 6 + k];
+    if (mode == 2) {
+      st[0][2] = fminf(1.f, fmaxf(0.f, st[0][2]));
+#pragma unroll
+      for (int k = 3; k < 6; k++) st[0][k] = fminf(hi[k], fmaxf(lo[k], st[0][k]));
+      float A, B;
+      grid_point(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), SWIM, lane,
+                 A, B);
+      st[0][0] = A;
+      st[0][1] = B;
+    }
   } else {
     // Brownian 49-point log-D grid
     float bA = 0, bB = 0, blD = lo[5];
@@ -446,16 +500,18 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int grid, i
 }
 
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int swim, int grid, int nstarts,
+                         const float* bounds, const float* seeds, int nfits, int swim, int seed_mode, int nstarts,
                          int maxit, float* theta, double* chi, int* steps, hipStream_t st) {
+  // seed_mode: 0 grid, 1 given, 2 nonlinear given (the ABI's DDM_SEED_*) -> kernel mode
+  const int mode = seed_mode == 0 ? 1 : (seed_mode == 2 ? 2 : 0);
   if (K > MAXK) return hipErrorInvalidValue;
   FitIn in{tau, d, ntau, w, qn, bounds, seeds, K};
   const int blocks = (nfits + FNT / 64 - 1) / (FNT / 64);
   if (swim)
-    hipLaunchKernelGGL(k_lm_fit<true>, dim3(blocks), dim3(FNT), 0, st, in, nfits, grid, nstarts, maxit, theta, chi,
+    hipLaunchKernelGGL(k_lm_fit<true>, dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta, chi,
                        steps);
   else
-    hipLaunchKernelGGL(k_lm_fit<false>, dim3(blocks), dim3(FNT), 0, st, in, nfits, grid, nstarts, maxit, theta, chi,
+    hipLaunchKernelGGL(k_lm_fit<false>, dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta, chi,
                        steps);
   return hipGetLastError();
 }

@@ -93,7 +93,10 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
   if (b->ntau_max < 1 || b->ntau_max > kMaxT) return ffail(f, DDM_EINVAL, "fit: ntau_max must be in [1, 128]");
   if (b->knodes < 1 || b->knodes > kMaxK) return ffail(f, DDM_EINVAL, "fit: knodes must be in [1, 8]");
   if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D) return ffail(f, DDM_EINVAL, "fit: model");
-  if (b->seed_mode == DDM_SEED_GIVEN && !b->seeds) return ffail(f, DDM_EINVAL, "fit: seeds required");
+  if (b->seed_mode != DDM_SEED_GRID && b->seed_mode != DDM_SEED_GIVEN && b->seed_mode != DDM_SEED_NONLINEAR)
+    return ffail(f, DDM_EINVAL, "fit: seed_mode");
+  const bool seeded = b->seed_mode != DDM_SEED_GRID;
+  if (seeded && !b->seeds) return ffail(f, DDM_EINVAL, "fit: seeds required");
   if (!b->ntau || !b->tau || !b->data || !b->weight || !b->qnodes || !b->bounds || !theta || !chi2 || !iters)
     return ffail(f, DDM_EINVAL, "fit: null array");
   const size_t n = (size_t)b->nfits;
@@ -112,7 +115,7 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
     for (int k = 0; k < 4; k++) bd[i * 4 + k] = (float)b->bounds[i * 4 + k];
     if (!(bd[i * 4] > 0 && bd[i * 4 + 1] > bd[i * 4] && bd[i * 4 + 2] > 0 && bd[i * 4 + 3] > bd[i * 4 + 2]))
       return ffail(f, DDM_EINVAL, "fit: bounds must satisfy 0 < vmin < vmax, 0 < Dmin < Dmax");
-    if (b->seed_mode == DDM_SEED_GIVEN)
+    if (seeded)
       for (int k = 0; k < 6; k++) sd[i * 6 + k] = (float)b->seeds[i * 6 + k];
   }
   hipError_t e = grow(f, n);
@@ -127,8 +130,8 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
     return ffail(f, DDM_EHIP, std::string("fit: upload: ") + hipGetErrorString(e));
   hipEventRecord(f->ev0, f->stream);
   e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds, b->nfits,
-                         b->model == DDM_MODEL_SWIMMER_3D, b->seed_mode == DDM_SEED_GRID,
-                         std::max(1, std::min(3, b->nstarts)), std::max(1, b->max_iter), f->theta, f->chi, f->steps,
+                         b->model == DDM_MODEL_SWIMMER_3D, b->seed_mode,
+                         std::max(1, std::min(3, b->nstarts)), std::max(0, b->max_iter), f->theta, f->chi, f->steps,
                          f->stream);
   if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: launch: ") + hipGetErrorString(e));
   hipEventRecord(f->ev1, f->stream);

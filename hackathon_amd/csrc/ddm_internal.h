@@ -34,7 +34,7 @@ hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* row
                          int first_slot, hipStream_t st);
 
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int swim, int grid, int nstarts,
+                         const float* bounds, const float* seeds, int nfits, int swim, int seed_mode, int nstarts,
                          int maxit, float* theta, double* chi, int* steps, hipStream_t st);
 
 hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
@@ -49,7 +49,7 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, unsigned sh
 hipError_t launch_to_bf16(const float* x, unsigned short* y, long n, hipStream_t st);
 
 hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp, double bg, double vbar, double Z,
-                        double D, uint64_t seed, int dtype, int noise, double4* part, float2* pos, void* out,
-                        hipStream_t st);
+                        double D, int heading3d, int swim_diffuse, uint64_t seed, int dtype, int noise,
+                        double4* part, float2* pos, void* out, hipStream_t st);
 
 }  // namespace ddm

@@ -113,7 +113,10 @@ __device__ double poisson(Philox& g, double lam) {
 }  // namespace
 
 // particle p: start (x0, y0) uniform on the torus, velocity (px/frame) for swimmers
-__global__ void k_synth_init(int n, int n_sw, int N, double vbar, double Z, uint64_t seed, double4* __restrict__ part) {
+// (the reference simulator's draw order: position, heading, speed, then the
+// polar direction when headings are 3D, test/sim.mjs:109-121)
+__global__ void k_synth_init(int n, int n_sw, int N, double vbar, double Z, int heading3d, uint64_t seed,
+                             double4* __restrict__ part) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   Philox g((uint32_t)p, 0u, S_INIT, seed);
@@ -121,22 +124,29 @@ __global__ void k_synth_init(int n, int n_sw, int N, double vbar, double Z, uint
   double vx = 0.0, vy = 0.0;
   if (p < n_sw) {
     const double phi = 6.283185307179586 * g.uniform();
-    const double s = gamma_mt(g, Z + 1.0, vbar / (Z + 1.0));
+    double s = gamma_mt(g, Z + 1.0, vbar / (Z + 1.0));
+    if (heading3d) {   // isotropic direction in 3D, projected on the image plane
+      const double cz = 2.0 * g.uniform() - 1.0;
+      s *= sqrt(1.0 - cz * cz);
+    }
     vx = s * cos(phi);
     vy = s * sin(phi);
   }
   part[p] = make_double4(x0, y0, vx, vy);
 }
 
-// positions pos[t][p] = wrap(start + t v + sum_{s=1..t} step_s), steps N(0, 2D) per axis for diffusers
-__global__ void k_synth_tracks(int n, int n_sw, int N, int T, double D, uint64_t seed, const double4* __restrict__ part,
-                               float2* __restrict__ pos) {
+// positions pos[t][p] = wrap(start + t v + sum_{s=1..t} step_s), steps N(0, 2D) per axis
+// for the non-swimmers, and for the swimmers too when swim_diffuse is set (the
+// reference simulator moves every mobile particle by a diffusive step,
+// test/sim.mjs:162-168)
+__global__ void k_synth_tracks(int n, int n_sw, int N, int T, double D, int swim_diffuse, uint64_t seed,
+                               const double4* __restrict__ part, float2* __restrict__ pos) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double4 s0 = part[p];
   double bx = 0.0, by = 0.0;
   const double sd = sqrt(2.0 * D);
-  const bool diff = p >= n_sw && D > 0.0;
+  const bool diff = (p >= n_sw || swim_diffuse) && D > 0.0;
   for (int t = 0; t < T; t++) {
     if (diff && t > 0) {
       Philox g((uint32_t)p, (uint32_t)t, S_STEP, seed);
@@ -220,11 +230,12 @@ __global__ __launch_bounds__(256) void k_synth_render(int n, int N, int W, float
 }
 
 hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp, double bg, double vbar, double Z,
-                        double D, uint64_t seed, int dtype, int noise, double4* part, float2* pos, void* out,
-                        hipStream_t st) {
+                        double D, int heading3d, int swim_diffuse, uint64_t seed, int dtype, int noise,
+                        double4* part, float2* pos, void* out, hipStream_t st) {
   if (n > 0) {
-    hipLaunchKernelGGL(k_synth_init, dim3((n + 255) / 256), dim3(256), 0, st, n, n_sw, N, vbar, Z, seed, part);
-    hipLaunchKernelGGL(k_synth_tracks, dim3((n + 63) / 64), dim3(64), 0, st, n, n_sw, N, T, D, seed,
+    hipLaunchKernelGGL(k_synth_init, dim3((n + 255) / 256), dim3(256), 0, st, n, n_sw, N, vbar, Z, heading3d, seed,
+                       part);
+    hipLaunchKernelGGL(k_synth_tracks, dim3((n + 63) / 64), dim3(64), 0, st, n, n_sw, N, T, D, swim_diffuse, seed,
                        (const double4*)part, pos);
   }
   if (!out) return hipGetLastError();

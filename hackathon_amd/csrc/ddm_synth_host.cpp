@@ -61,7 +61,7 @@ int run(ddm_synth* s, const ddm_video_params* p, void* dframes, float2* dpos) {
     pos = s->pos;
   }
   const int n_sw = (int)std::floor(p->alpha * n + 0.5);
-  e = ddm::launch_synth(p->N, p->T, (int)n, n_sw, p->sigma, p->amp, p->background, p->vbar, p->Z, p->D, p->seed,
+  e = ddm::launch_synth(p->N, p->T, (int)n, n_sw, p->sigma, p->amp, p->background, p->vbar, p->Z, p->D, p->heading3d, p->swim_diffuse, p->seed,
                         p->dtype, p->noise, s->part, pos, dframes, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return sfail(s, DDM_EHIP, std::string("synth: ") + hipGetErrorString(e));

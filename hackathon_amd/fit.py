@@ -20,7 +20,7 @@ from .engine import DdmError, Radial, load_library
 MAXT = 128
 MAXK = 8
 BROWNIAN, SWIMMER_3D = 0, 1
-SEED_GRID, SEED_GIVEN = 0, 1
+SEED_GRID, SEED_GIVEN, SEED_NONLINEAR = 0, 1, 2
 SWIM_MIN_UM_S = 2.0   # GATES.SWIM_MIN_UM_S (src/motility_fit.js:37)
 
 
@@ -157,9 +157,10 @@ class Fitter:
             pass
 
     def fit(self, tau, data, weight, qnodes, bnds, model=SWIMMER_3D, seeds=None, nstarts=3,
-            max_iter=100) -> FitResult:
+            max_iter=100, seed_ab=True) -> FitResult:
         """tau/data: lists (or 2D arrays) of per-fit lag times and D values; weight (nfits,);
-        qnodes (nfits, K); bnds (nfits, 4); seeds (nfits, 6) for seeded LM, else grid seeding."""
+        qnodes (nfits, K); bnds (nfits, 4); seeds (nfits, 6) for seeded LM, else grid seeding.
+        seed_ab=False takes only the nonlinear parameters from the seeds and solves A, B by NNLS."""
         n = len(data)
         ntau = np.array([len(x) for x in data], dtype=np.int32)
         if n == 0:
@@ -176,7 +177,8 @@ class Fitter:
         qn = np.ascontiguousarray(qnodes, dtype=np.float64).reshape(n, -1)
         bd = np.ascontiguousarray(bnds, dtype=np.float64).reshape(n, 4)
         sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float64).reshape(n, 6)
-        b = FitBatch(n, tmax, qn.shape[1], model, SEED_GRID if sd is None else SEED_GIVEN, nstarts, max_iter,
+        mode = SEED_GRID if sd is None else (SEED_GIVEN if seed_ab else SEED_NONLINEAR)
+        b = FitBatch(n, tmax, qn.shape[1], model, mode, nstarts, max_iter,
                      ntau.ctypes.data, T.ctypes.data, Dd.ctypes.data, w.ctypes.data, qn.ctypes.data, bd.ctypes.data,
                      None if sd is None else sd.ctypes.data)
         th = np.zeros((n, 6))

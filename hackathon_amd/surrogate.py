@@ -1,0 +1,363 @@
+"""Learned LM seeding: an MLP maps a video's normalised log D(q, tau) to an LM start.
+
+Pipeline (all GPU work in libddm.so):
+  1. `Simulator.radial`: the HIP video generator renders a seeded
+     256x256x512 u8 video (200 spots, sigma 1.5 px, amplitude 100 on
+     background 20; a fraction alpha of Schulz swimmers with 3D headings,
+     every particle diffusing with D, as in the reference simulator; Poisson
+     noise), the engine computes its radial D(q, tau)
+     (32 bins on |k| in [4, 100), lags 0..255).
+  2. `features`: log D resampled to 32 q x 48 log-spaced lags, each q's curve
+     z-normalised (1536 features).
+  3. Targets: alpha, ln vbar, ln Z, ln D of the truth, plus per-bin A_b w_b and
+     B_b w_b, the exact NNLS amplitudes of the per-q model at the truth
+     (w_b = 1 / max_tau D_b, the reference's weight, src/motility_fit.js:383-401).
+  4. `train`: Adam on the GPU with the library's BF16 MFMA forward/backward.
+  5. `Surrogate.seeds` turns a prediction into per-bin LM starts; `compare`
+     runs the plain grid-seeded LM (the fallback and the judge) and the seeded
+     LM on the same problems: a seeded fit is accepted only if its chi^2 is at
+     most plain chi^2 (1 + 1e-6), rejected fits fall back to the plain result.
+
+Units: dx = 1 um per pixel and 100 frames/s, so vbar is in um/s (= px/s) and
+D in um^2/s (= 100 x px^2/frame).  `python -m hackathon_amd.surrogate train`
+generates the data, trains, evaluates on held-out videos and writes the
+weights (`data/surrogate_lm_seed.npz`, plain arrays, no pickle) and a report.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from .engine import DdmEngine, DdmError
+from .fit import SWIMMER_3D, Fitter, bounds, prepare
+from .mlp import IN, OUT, Mlp, init_params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS_PATH = os.path.join(_HERE, "data", "surrogate_lm_seed.npz")
+
+FPS = 100.0
+DX_UM = 1.0
+GEOM = dict(N=256, T=512, L=256, NB=32, kmin=4.0, kmax=100.0)
+SPOTS = dict(n_spots=200, sigma=1.5, amp=100.0, background=20.0)
+NQ, NTAU = 32, 48
+# alpha, mean speed (px/s), Z, D (px^2/frame): uniform ranges of the training and held-out draws
+RANGES = dict(alpha=(0.0, 1.0), vbar=(5.0, 30.0), Z=(1.0, 10.0), D=(0.05, 1.0))
+TRAIN_SEED0, HELDOUT_SEED0 = 1000, 50_000_000
+ACCEPT_RTOL = 1e-6
+assert NQ == GEOM["NB"] and IN == NQ * NTAU and OUT == 4 + 2 * NQ
+
+
+def sample_truth(rng) -> np.ndarray:
+    """(alpha, vbar px/s, Z, D px^2/frame) uniform in RANGES."""
+    return np.array([rng.uniform(*RANGES[k]) for k in ("alpha", "vbar", "Z", "D")])
+
+
+def truth_theta(tr) -> np.ndarray:
+    """(alpha, ln vbar um/s, ln Z, ln D um^2/s)."""
+    return np.array([tr[0], math.log(tr[1] * DX_UM), math.log(tr[2]), math.log(tr[3] * DX_UM * DX_UM * FPS)])
+
+
+def feature_lags(L: int) -> np.ndarray:
+    return np.exp(np.linspace(0.0, math.log(L - 1), NTAU))
+
+
+def features(D: np.ndarray) -> np.ndarray:
+    """(NB, L) structure function -> 1536 z-normalised log-D features."""
+    NB, L = D.shape
+    lag = feature_lags(L)
+    lo = np.minimum(np.floor(lag).astype(np.int64), L - 2)
+    f = (np.log(lag) - np.log(lo)) / (np.log(lo + 1) - np.log(lo))
+    lD = np.log(np.maximum(D, 1e-30))
+    x = (1 - f) * lD[:, lo] + f * lD[:, lo + 1]
+    # each q's curve z-normalised on its own: removes the bin's amplitude A(q)
+    # (the spot's form factor) and keeps the shape that carries the dynamics
+    x = (x - x.mean(axis=1, keepdims=True)) / np.maximum(x.std(axis=1, keepdims=True), 1e-12)
+    return x.astype(np.float32).ravel()
+
+
+class Simulator:
+    """Video generator + engine for the training geometry (reused across videos)."""
+
+    def __init__(self):
+        import torch
+        from .synth import GpuSynth
+        g = GEOM
+        self.synth = GpuSynth()
+        self.eng = DdmEngine.create(N=g["N"], T=g["T"], L=g["L"], NB=g["NB"], kmin=g["kmin"], kmax=g["kmax"])
+        self.buf = torch.empty((g["T"], g["N"], g["N"]), dtype=torch.uint8, device="cuda")
+        self.order = np.arange(g["T"], dtype=np.uint32)
+        self.mask = np.ones(g["T"], dtype=np.float32)
+
+    def radial(self, tr, seed: int):
+        g = GEOM
+        p = self.synth.params(g["T"], g["N"], SPOTS["n_spots"], SPOTS["sigma"], SPOTS["amp"], SPOTS["background"],
+                              D=tr[3], alpha=tr[0], vbar_px_frame=tr[1] / FPS, Z=tr[2], dtype="uint8", seed=seed,
+                              heading3d=True, swim_diffuse=True)
+        self.synth.video(p, out=self.buf)
+        self.eng.ingestDevice(self.buf.data_ptr(), np.uint8, g["T"], 0)
+        self.eng.computeTemporal(self.order, self.mask)
+        return self.eng.readRadial()
+
+
+class Problems:
+    """Per-bin fit problems of many videos (all bins share the lag grid)."""
+
+    def __init__(self):
+        self.tau = None
+        self.data, self.w, self.qn, self.bd, self.video = [], [], [], [], []
+
+    def add(self, radial, vid: int):
+        pr = prepare(radial, DX_UM, FPS)
+        if len(pr.bins) != NQ:
+            raise DdmError(f"video {vid}: {len(pr.bins)} usable bins, expected {NQ}")
+        self.tau = pr.tau
+        bd = bounds(pr)
+        K = max(len(b.qn) for b in pr.bins)
+        for b in pr.bins:
+            self.data.append(b.d)
+            self.w.append(b.w)
+            self.qn.append(np.resize(b.qn, K))
+            self.bd.append(bd)
+            self.video.append(vid)
+
+    def arrays(self, sel=None):
+        idx = np.arange(len(self.data)) if sel is None else np.asarray(sel)
+        d = np.array(self.data)[idx]
+        return ([self.tau] * len(idx), list(d), np.array(self.w)[idx], np.array(self.qn)[idx], np.array(self.bd)[idx])
+
+    def __len__(self):
+        return len(self.data)
+
+
+def amplitudes_at(fitter: Fitter, probs: Problems, theta4: np.ndarray):
+    """Exact NNLS (A, B) of every bin at per-video nonlinear parameters (no LM steps)."""
+    vid = np.array(probs.video)
+    seeds = np.zeros((len(probs), 6))
+    seeds[:, 2:] = theta4[vid]
+    res = fitter.fit(*probs.arrays(), model=SWIMMER_3D, seeds=seeds, max_iter=0, seed_ab=False)
+    return res.theta[:, 0], res.theta[:, 1]
+
+
+def build_dataset(n: int, seed0: int, sim: Simulator, fitter: Fitter, keep_problems=False, log=print, chunk=256):
+    """Features X (n, 1536), targets Y (n, 68), truth (n, 4) and optionally the fit problems."""
+    rng = np.random.default_rng(seed0)
+    X = np.zeros((n, IN), np.float32)
+    Y = np.zeros((n, OUT), np.float32)
+    truth = np.zeros((n, 4))
+    allp = Problems() if keep_problems else None
+    t0 = time.time()
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        probs = Problems()
+        th4 = np.zeros((c1 - c0, 4))
+        for i in range(c0, c1):
+            tr = sample_truth(rng)
+            truth[i] = tr
+            r = sim.radial(tr, seed0 + i)
+            X[i] = features(r.D.reshape(r.NB, r.L))
+            probs.add(r, i - c0)
+            if allp is not None:
+                allp.add(r, i)
+            th4[i - c0] = truth_theta(tr)
+        A, B = amplitudes_at(fitter, probs, th4)
+        w = np.array(probs.w)
+        Y[c0:c1, :4] = th4
+        Y[c0:c1, 4:4 + NQ] = (A * w).reshape(c1 - c0, NQ)
+        Y[c0:c1, 4 + NQ:] = (B * w).reshape(c1 - c0, NQ)
+        log(f"  videos {c1}/{n}  {time.time() - t0:.1f}s")
+    return X, Y, truth, allp
+
+
+def target_weights() -> np.ndarray:
+    wt = np.full(OUT, 0.05, np.float32)
+    wt[:4] = 4.0
+    return wt
+
+
+def train(X, Y, epochs=40, batch=256, lr=1e-3, seed=0, val_frac=0.05, log=print):
+    """Adam with cosine decay on standardised targets; returns (params, ymean, ystd, history)."""
+    ymean = Y.mean(axis=0)
+    ystd = Y.std(axis=0) + 1e-6
+    Yn = ((Y - ymean) / ystd).astype(np.float32)
+    n = X.shape[0]
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    nval = max(1, int(val_frac * n))
+    val, tr = perm[:nval], perm[nval:]
+    net = Mlp(max_batch=batch, params=init_params(seed))
+    wt = target_weights()
+    steps_per_epoch = max(1, len(tr) // batch)
+    total = epochs * steps_per_epoch
+    step = 0
+    hist = []
+    best = None
+    for ep in range(epochs):
+        order = rng.permutation(tr)
+        s = 0.0
+        for b in range(steps_per_epoch):
+            idx = order[b * batch:(b + 1) * batch]
+            lr_t = 1e-5 + 0.5 * (lr - 1e-5) * (1 + math.cos(math.pi * step / total))
+            s += net.train_step(X[idx], Yn[idx], wt, lr_t)
+            step += 1
+        pv = net.forward(X[val])
+        vloss = float(0.5 * np.mean(np.sum(wt * (pv - Yn[val]) ** 2, axis=1)))
+        hist.append((s / steps_per_epoch, vloss))
+        if best is None or vloss < best[0]:
+            best = (vloss, ep, net.get_params())   # early stopping on the validation split
+        if ep % 5 == 0 or ep == epochs - 1:
+            log(f"  epoch {ep}: train {s / steps_per_epoch:.4f}  val {vloss:.4f}")
+    log(f"  best validation loss {best[0]:.4f} at epoch {best[1]}")
+    return best[2], ymean.astype(np.float32), ystd.astype(np.float32), hist
+
+
+class Surrogate:
+    """A trained network plus its target standardisation."""
+
+    def __init__(self, params=None, ymean=None, ystd=None, path: str = WEIGHTS_PATH, max_batch=1024):
+        if params is None:
+            if not os.path.exists(path):
+                raise DdmError(f"no surrogate weights at {path}")
+            z = np.load(path, allow_pickle=False)
+            params, ymean, ystd = z["params"], z["ymean"], z["ystd"]
+        self.ymean, self.ystd = np.asarray(ymean, np.float32), np.asarray(ystd, np.float32)
+        self.net = Mlp(max_batch=max_batch, params=params)
+
+    def predict(self, X) -> np.ndarray:
+        return self.net.forward(X) * self.ystd + self.ymean
+
+    @staticmethod
+    def seeds(pred: np.ndarray, probs: Problems) -> np.ndarray:
+        """Per-bin (A, B, alpha, ln vbar, ln Z, ln D) starts from per-video predictions."""
+        vid = np.array(probs.video)
+        w = np.array(probs.w)
+        bin_of = np.arange(len(probs)) % NQ
+        p = pred[vid]
+        th = np.zeros((len(probs), 6))
+        th[:, 0] = np.maximum(p[np.arange(len(vid)), 4 + bin_of], 0) / w
+        th[:, 1] = np.maximum(p[np.arange(len(vid)), 4 + NQ + bin_of], 0) / w
+        th[:, 2] = np.clip(p[:, 0], 0, 1)
+        th[:, 3:] = p[:, 1:4]
+        return th
+
+
+def compare(fitter: Fitter, probs: Problems, seeds: np.ndarray, truth4=None, mlp_ms=0.0, seed_ab=False,
+            max_iter=100, nstarts=3):
+    """Plain grid-seeded LM vs surrogate-seeded LM on the same problems."""
+    args = probs.arrays()
+    plain = fitter.fit(*args, model=SWIMMER_3D, nstarts=nstarts, max_iter=max_iter)
+    seeded = fitter.fit(*args, model=SWIMMER_3D, seeds=seeds, max_iter=max_iter, seed_ab=seed_ab)
+    acc = seeded.chi2 <= plain.chi2 * (1 + ACCEPT_RTOL)
+    rej = np.nonzero(~acc)[0]
+    fb_ms = 0.0
+    if len(rej):
+        fb = fitter.fit(*probs.arrays(rej), model=SWIMMER_3D, nstarts=nstarts, max_iter=max_iter)
+        fb_ms = fb.kernel_ms
+    n = len(probs)
+
+    def params(r):
+        return np.stack([r.alpha, r.vbar, r.Z, r.D], axis=1)
+
+    pp, ps = params(plain), params(seeded)
+    rel = np.abs(ps - pp) / np.maximum(np.abs(pp), 1e-3)
+    out = dict(
+        fits=n,
+        plain_kernel_ms=plain.kernel_ms, seeded_kernel_ms=seeded.kernel_ms, mlp_ms=mlp_ms,
+        fallback_kernel_ms=fb_ms,
+        plain_fits_per_s=n / (plain.kernel_ms * 1e-3),
+        seeded_fits_per_s=n / ((seeded.kernel_ms + mlp_ms) * 1e-3),
+        seeded_with_fallback_fits_per_s=n / ((seeded.kernel_ms + mlp_ms + fb_ms) * 1e-3),
+        accept_rate=float(acc.mean()),
+        plain_iters_median=float(np.median(plain.iters)), seeded_iters_median=float(np.median(seeded.iters)),
+        plain_iters_mean=float(np.mean(plain.iters)), seeded_iters_mean=float(np.mean(seeded.iters)),
+        accepted_param_agreement_p50=float(np.median(rel[acc].max(axis=1))) if acc.any() else None,
+        accepted_param_within_1e3=float(np.mean(rel[acc].max(axis=1) < 1e-3)) if acc.any() else None,
+        seeded_better_chi2=float(np.mean(seeded.chi2 < plain.chi2 * (1 - ACCEPT_RTOL))),
+        seeded_converged=float(np.mean(seeded.iters < max_iter)),
+        plain_converged_all_starts=float(np.mean(plain.iters < nstarts * max_iter)),
+    )
+    out["speedup"] = out["seeded_fits_per_s"] / out["plain_fits_per_s"]
+    out["speedup_with_fallback"] = out["seeded_with_fallback_fits_per_s"] / out["plain_fits_per_s"]
+    if truth4 is not None:
+        vid = np.array(probs.video)
+        t = np.asarray(truth4)[vid]
+        tp = np.stack([t[:, 0], t[:, 1] * DX_UM, t[:, 2], t[:, 3] * DX_UM ** 2 * FPS], axis=1)
+        for name, r in (("plain", pp), ("seeded", ps)):
+            err = np.abs(r - tp) / np.maximum(np.abs(tp), 1e-2)
+            out[f"{name}_param_err_median"] = dict(zip(("alpha", "vbar", "Z", "D"), map(float, np.median(err, 0))))
+    return out, plain, seeded
+
+
+def timed_predict(sur: Surrogate, X):
+    sur.predict(X[:1])   # warm
+    t0 = time.perf_counter()
+    p = sur.predict(X)
+    return p, (time.perf_counter() - t0) * 1e3
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--videos", type=int, default=6250)
+    t.add_argument("--heldout", type=int, default=320)
+    t.add_argument("--epochs", type=int, default=40)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--out", default=WEIGHTS_PATH)
+    t.add_argument("--report", default=None)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--heldout", type=int, default=320)
+    e.add_argument("--weights", default=WEIGHTS_PATH)
+    e.add_argument("--report", default=None)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    sim = Simulator()
+    fitter = Fitter()
+    report = dict(geometry=GEOM, spots=SPOTS, ranges=RANGES, fps=FPS, dx_um=DX_UM)
+    if a.cmd == "train":
+        log(f"generating {a.videos} training videos")
+        t0 = time.time()
+        X, Y, truth, _ = build_dataset(a.videos, TRAIN_SEED0, sim, fitter, log=log)
+        report["train_videos"] = a.videos
+        report["train_curves"] = a.videos * NQ
+        report["generate_s"] = time.time() - t0
+        log("training")
+        t0 = time.time()
+        params, ymean, ystd, hist = train(X, Y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        report["train_s"] = time.time() - t0
+        report["loss_history"] = hist
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        np.savez(a.out, params=params, ymean=ymean, ystd=ystd)
+        sur = Surrogate(params, ymean, ystd)
+        del X, Y
+    else:
+        sur = Surrogate(path=a.weights)
+    log(f"held-out: {a.heldout} videos")
+    Xh, Yh, th, probs = build_dataset(a.heldout, HELDOUT_SEED0, sim, fitter, keep_problems=True, log=log)
+    pred, mlp_ms = timed_predict(sur, Xh)
+    err = np.abs(pred[:, :4] - Yh[:, :4])
+    report["heldout_videos"] = a.heldout
+    report["heldout_curves"] = len(probs)
+    report["surrogate_abs_err_median"] = dict(zip(("alpha", "ln_vbar", "ln_Z", "ln_D"), map(float, np.median(err, 0))))
+    seeds = Surrogate.seeds(pred, probs)
+    for mode, ab in (("nonlinear_nnls_ab", False), ("given_ab", True)):
+        res, _, _ = compare(fitter, probs, seeds, th, mlp_ms, seed_ab=ab)
+        report[f"compare_{mode}"] = res
+        log(f"{mode}: " + json.dumps({k: v for k, v in res.items() if not isinstance(v, dict)}))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1, default=float)
+    return report
+
+
+if __name__ == "__main__":
+    sys.exit(0 if main() else 1)

@@ -113,7 +113,8 @@ C2 = dict(T=1024, N=512, n_spots=4000, sigma=2.0, amp=100.0, background=200.0, D
 
 class VideoParams(C.Structure):
     _fields_ = [("N", C.c_int32), ("T", C.c_int32), ("n_spots", C.c_int32), ("dtype", C.c_int32),
-                ("noise", C.c_int32), ("pad_", C.c_int32), ("sigma", C.c_double), ("amp", C.c_double),
+                ("noise", C.c_int32), ("heading3d", C.c_int32),
+                ("swim_diffuse", C.c_int32), ("pad_", C.c_int32), ("sigma", C.c_double), ("amp", C.c_double),
                 ("background", C.c_double), ("alpha", C.c_double), ("vbar", C.c_double), ("Z", C.c_double),
                 ("D", C.c_double), ("seed", C.c_uint64)]
 
@@ -160,8 +161,9 @@ class GpuSynth:
 
     @staticmethod
     def params(T, N, n_spots, sigma, amp, background, D=0.0, alpha=0.0, vbar_px_frame=0.0, Z=3.0,
-               dtype="uint16", seed=1, noise=True) -> VideoParams:
-        return VideoParams(int(N), int(T), int(n_spots), _DT[dtype], 1 if noise else 0, 0, float(sigma), float(amp),
+               dtype="uint16", seed=1, noise=True, heading3d=False, swim_diffuse=False) -> VideoParams:
+        return VideoParams(int(N), int(T), int(n_spots), _DT[dtype], 1 if noise else 0, 1 if heading3d else 0,
+                           1 if swim_diffuse else 0, 0, float(sigma), float(amp),
                            float(background), float(alpha), float(vbar_px_frame), float(Z), float(D), int(seed))
 
     def video(self, p: VideoParams, out=None):

@@ -150,7 +150,7 @@ int ddm_kernel_times(ddm_engine *e, double *ms, int32_t *count, int32_t n);
 typedef struct ddm_fitter ddm_fitter;
 
 enum { DDM_MODEL_BROWNIAN = 0, DDM_MODEL_SWIMMER_3D = 1 };
-enum { DDM_SEED_GRID = 0, DDM_SEED_GIVEN = 1 };
+enum { DDM_SEED_GRID = 0, DDM_SEED_GIVEN = 1, DDM_SEED_NONLINEAR = 2 };
 
 typedef struct ddm_fit_batch {
   int32_t nfits;
@@ -158,16 +158,18 @@ typedef struct ddm_fit_batch {
   int32_t knodes;        /* q nodes per bin, <= 8 */
   int32_t model;         /* DDM_MODEL_* */
   int32_t seed_mode;     /* DDM_SEED_GRID: the reference's start grid, LM from the best nstarts;
-                            DDM_SEED_GIVEN: LM from seeds[] */
+                            DDM_SEED_GIVEN: LM from seeds[];
+                            DDM_SEED_NONLINEAR: alpha, ln vbar, ln Z, ln D from seeds[],
+                            A and B by the exact 2-column NNLS at those values */
   int32_t nstarts;       /* 1..3 */
-  int32_t max_iter;      /* LM steps per start */
+  int32_t max_iter;      /* LM steps per start; 0 evaluates chi^2 at the (clamped) start */
   const int32_t *ntau;   /* [nfits] lags used by each fit */
   const double *tau;     /* [nfits * ntau_max] lag times (s) */
   const double *data;    /* [nfits * ntau_max] D_b(tau) */
   const double *weight;  /* [nfits] w = 1 / max_tau D_b */
   const double *qnodes;  /* [nfits * knodes] effective |q| nodes (rad/um) */
   const double *bounds;  /* [nfits * 4] vmin, vmax, Dmin, Dmax */
-  const double *seeds;   /* [nfits * 6] theta seeds for DDM_SEED_GIVEN, else NULL */
+  const double *seeds;   /* [nfits * 6] theta seeds (DDM_SEED_GIVEN / _NONLINEAR), else NULL */
 } ddm_fit_batch;
 
 int ddm_fitter_create(ddm_fitter **out);
@@ -208,8 +210,11 @@ int ddm_mlp_train_step(ddm_mlp *m, const float *X, const float *Tg, const float 
  * Seeded video generator on the GPU (the benchmark and training workloads):
  * n_spots Gaussian spots (width sigma px, peak amp) on a constant background,
  * the first round(alpha * n_spots) swim in straight lines (uniform heading,
- * speed ~ Gamma(Z+1, vbar/(Z+1)) px/frame), the rest take N(0, 2D) steps per
- * axis per frame (D in px^2/frame), periodic wrap, Poisson photon noise
+ * speed ~ Gamma(Z+1, vbar/(Z+1)) px/frame; with heading3d the direction is
+ * isotropic in 3D and the velocity its projection on the image plane, as in
+ * the reference simulator's default), the rest take N(0, 2D) steps per
+ * axis per frame (D in px^2/frame; with swim_diffuse the swimmers too, as in
+ * the reference simulator), periodic wrap, Poisson photon noise
  * (noise = 0 stores the expected intensity instead), clipped to the pixel type.
  * The model follows the reference's movie simulator (motility-ddm
  * test/sim.mjs:90-186) in pixel units; random draws are Philox4x32-10 streams
@@ -221,6 +226,8 @@ typedef struct ddm_video_params {
   int32_t n_spots;
   int32_t dtype;    /* DDM_U8 / DDM_U16 / DDM_F32 */
   int32_t noise;    /* 1: Poisson counts, 0: expected intensity */
+  int32_t heading3d; /* 1: isotropic 3D headings projected on the image plane */
+  int32_t swim_diffuse; /* 1: swimmers also take the diffusive steps */
   int32_t pad_;
   double sigma, amp, background;
   double alpha, vbar, Z, D;

@@ -1213,13 +1213,32 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
     }
     for (int a = 0; a < p; a++)
       for (int b = 0; b < a; b++) H[b * p + a] = H[a * p + b];
+    /* active set: a parameter on a bound whose gradient points out of the box
+       is held for this step (its row and column drop out of the system);
+       Marquardt scaling with a floor of 1e-12 x the largest diagonal */
+    int frozen[6];
+    double diag[6], dmax = 0;
+    for (int a = 0; a < p; a++) {
+      int k = idx[a];
+      frozen[a] = (th[k] <= lo[k] && g[a] < 0) || (th[k] >= hi[k] && g[a] > 0);
+      if (H[a * p + a] > dmax) dmax = H[a * p + a];
+    }
+    for (int a = 0; a < p; a++) {
+      double f = 1e-12 * dmax;
+      diag[a] = H[a * p + a] > f ? H[a * p + a] : (f > 1e-300 ? f : 1e-300);
+    }
     /* one damped step (retried with larger damping until it lowers chi^2) */
     for (;;) {
       it++;
-      double Hd[36], delta[6], ntheta[6];
+      double Hd[36], delta[6], ntheta[6], gf[6];
       memcpy(Hd, H, sizeof(Hd));
-      for (int a = 0; a < p; a++) Hd[a * p + a] += lam * (H[a * p + a] > 1e-300 ? H[a * p + a] : 1e-300);
-      int ok = cholesky_solve(p, Hd, g, delta);
+      for (int a = 0; a < p; a++) {
+        gf[a] = frozen[a] ? 0 : g[a];
+        for (int b = 0; b < p; b++)
+          if (frozen[a] || frozen[b]) Hd[a * p + b] = a == b ? 1 : 0;
+        if (!frozen[a]) Hd[a * p + a] += lam * diag[a];
+      }
+      int ok = cholesky_solve(p, Hd, gf, delta);
       if (ok) {
         memcpy(ntheta, th, sizeof(ntheta));
         double pred = 0;
@@ -1227,7 +1246,7 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
           int k = idx[a];
           ntheta[k] = clampd(th[k] + delta[a], lo[k], hi[k]);
           double dk = ntheta[k] - th[k];
-          pred += dk * (lam * (H[a * p + a] > 1e-300 ? H[a * p + a] : 1e-300) * dk + g[a]);
+          pred += dk * (lam * diag[a] * dk + gf[a]);
         }
         double nchi = lm_chi2(ntau, tau, d, w, K, qn, ntheta, swim, geometry, cn);
         double rho = pred > 0 ? (chi - nchi) / pred : -1;
@@ -1239,7 +1258,8 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
           lam *= f > 1.0 / 3 ? f : 1.0 / 3;
           if (lam < 1e-15) lam = 1e-15;
           nu = 2;
-          if (rel < 1e-13 || chi < 1e-300) done = 1;
+          /* converged: actual and predicted relative decrease below 1e-9 */
+          if ((rel < 1e-9 && pred < 1e-9 * chi) || rel < 1e-13 || chi < 1e-300) done = 1;
           break;
         }
       }

@@ -110,8 +110,8 @@ def n_swimmers(alpha, n):
     return int(math.floor(alpha * n + 0.5))
 
 
-def particles(N, n, alpha, vbar, Z, seed):
-    """(n, 4): x0, y0, vx, vy."""
+def particles(N, n, alpha, vbar, Z, seed, heading3d=False):
+    """(n, 4): x0, y0, vx, vy (draw order of test/sim.mjs:109-121)."""
     n_sw = n_swimmers(alpha, n)
     out = np.zeros((n, 4))
     for p in range(n):
@@ -121,20 +121,23 @@ def particles(N, n, alpha, vbar, Z, seed):
         if p < n_sw:
             phi = 6.283185307179586 * g.uniform()
             s = gamma_mt(g, Z + 1.0, vbar / (Z + 1.0))
+            if heading3d:
+                cz = 2.0 * g.uniform() - 1.0
+                s *= math.sqrt(1.0 - cz * cz)
             vx, vy = s * math.cos(phi), s * math.sin(phi)
         out[p] = (x0, y0, vx, vy)
     return out
 
 
-def tracks(N, T, n, alpha, vbar, Z, D, seed):
+def tracks(N, T, n, alpha, vbar, Z, D, seed, heading3d=False, swim_diffuse=False):
     """(T, n, 2) wrapped positions, FP64."""
-    part = particles(N, n, alpha, vbar, Z, seed)
+    part = particles(N, n, alpha, vbar, Z, seed, heading3d)
     n_sw = n_swimmers(alpha, n)
     sd = math.sqrt(2.0 * D)
     pos = np.zeros((T, n, 2))
     for p in range(n):
         bx = by = 0.0
-        diff = p >= n_sw and D > 0.0
+        diff = (p >= n_sw or swim_diffuse) and D > 0.0
         for t in range(T):
             if diff and t > 0:
                 n1, n2 = Stream(p, t, S_STEP, seed).normal2()

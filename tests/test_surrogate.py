@@ -1,0 +1,97 @@
+"""Learned LM seeding: feature/seed plumbing on the CPU; on the GPU, what the
+trained network gains over the plain grid-seeded LM on held-out videos."""
+import os
+
+import numpy as np
+import pytest
+
+from hackathon_amd import surrogate as S
+
+
+def test_feature_lags_and_normalisation():
+    lags = S.feature_lags(256)
+    assert len(lags) == S.NTAU and lags[0] == pytest.approx(1.0) and lags[-1] == pytest.approx(255.0)
+    assert np.all(np.diff(lags) > 0)
+    rng = np.random.default_rng(0)
+    tau = np.arange(256)
+    A = rng.uniform(10, 1000, (32, 1))
+    D = A * (1 - np.exp(-tau / rng.uniform(5, 50, (32, 1)))) + 1.0
+    x = S.features(D).reshape(32, S.NTAU)
+    assert np.allclose(x.mean(axis=1), 0, atol=1e-5) and np.allclose(x.std(axis=1), 1, atol=1e-4)
+    # amplitude-free: scaling a q's curve leaves its features unchanged
+    D2 = D.copy()
+    D2[3] *= 7.5
+    assert np.allclose(S.features(D2), S.features(D), atol=1e-5)
+    # exact at integer lags: log-linear interpolation reproduces a power law
+    Dp = np.tile(np.arange(256, dtype=float) ** 0.7, (32, 1))
+    Dp[:, 0] = 1.0
+    xp = S.features(Dp).reshape(32, S.NTAU)[0]
+    ref = 0.7 * np.log(lags)
+    assert np.allclose(xp, (ref - ref.mean()) / ref.std(), atol=1e-4)
+
+
+def test_seeds_from_prediction():
+    class P:   # two videos' worth of per-bin weights
+        video = [0] * S.NQ + [1] * S.NQ
+        w = list(np.linspace(0.5, 2.0, 2 * S.NQ))
+
+        def __len__(self):
+            return 2 * S.NQ
+    pred = np.zeros((2, S.OUT if hasattr(S, "OUT") else 68))
+    pred[:, 0] = [0.3, 1.4]
+    pred[:, 1:4] = [[1.0, 2.0, 3.0], [1.5, 2.5, 3.5]]
+    pred[:, 4:4 + S.NQ] = np.arange(S.NQ)
+    pred[:, 4 + S.NQ:] = -1.0
+    th = S.Surrogate.seeds(pred, P())
+    w = np.array(P.w)
+    assert np.allclose(th[:S.NQ, 0], np.arange(S.NQ) / w[:S.NQ])
+    assert np.all(th[:, 1] == 0)                      # negative B clipped to 0
+    assert th[0, 2] == 0.3 and th[S.NQ, 2] == 1.0     # alpha clipped to [0, 1]
+    assert np.allclose(th[S.NQ:, 3:], [1.5, 2.5, 3.5])
+
+
+def test_truth_units():
+    th = S.truth_theta([0.5, 15.0, 3.0, 0.3])
+    assert th[0] == 0.5 and np.isclose(np.exp(th[1]), 15.0) and np.isclose(np.exp(th[3]), 30.0)
+
+
+@pytest.mark.skipif(not os.path.exists(S.WEIGHTS_PATH), reason="no trained surrogate weights")
+def test_weights_file_is_plain_arrays():
+    z = np.load(S.WEIGHTS_PATH, allow_pickle=False)
+    from hackathon_amd.mlp import NPARAMS
+    assert z["params"].shape == (NPARAMS,) and z["params"].dtype == np.float32
+    assert z["ymean"].shape == (68,) and np.all(z["ystd"] > 0)
+    assert np.all(np.isfinite(z["params"]))
+
+
+@pytest.mark.gpu
+def test_trained_network_gains_over_plain_lm(gpu):
+    """Held-out videos (disjoint seeds): seeded LM needs far fewer iterations and
+    runs faster than the plain grid-seeded LM; most seeded fits reach the plain
+    chi^2 and agree with its parameters; the surrogate's ln D is informative."""
+    from hackathon_amd.fit import Fitter
+    sim, fitter = S.Simulator(), Fitter()
+    X, Y, truth, probs = S.build_dataset(32, S.HELDOUT_SEED0 + 7777, sim, fitter, keep_problems=True,
+                                         log=lambda m: None)
+    sur = S.Surrogate()
+    pred, mlp_ms = S.timed_predict(sur, X)
+    err_lnD = np.median(np.abs(pred[:, 3] - Y[:, 3]))
+    prior = np.median(np.abs(Y[:, 3] - np.mean(np.log(np.array(S.RANGES["D"]) * 100))))
+    res, plain, seeded = S.compare(fitter, probs, S.Surrogate.seeds(pred, probs), truth, mlp_ms)
+    print({k: v for k, v in res.items() if not isinstance(v, dict)}, "lnD err", err_lnD, "prior", prior)
+    assert err_lnD < 0.5 * prior
+    assert res["seeded_iters_median"] <= 0.5 * res["plain_iters_median"]
+    assert res["speedup"] > 2.0
+    assert res["accept_rate"] > 0.5
+    assert res["accepted_param_agreement_p50"] < 1e-3
+
+
+@pytest.mark.gpu
+def test_short_training_lowers_validation_loss(gpu):
+    from hackathon_amd.fit import Fitter
+    sim, fitter = S.Simulator(), Fitter()
+    X, Y, _, _ = S.build_dataset(384, 777, sim, fitter, log=lambda m: None)
+    params, ymean, ystd, hist = S.train(X, Y, epochs=6, batch=64, lr=1e-3, log=lambda m: None)
+    # predicting the training mean scores 0.5 sum(wt) on standardised targets
+    assert min(h[1] for h in hist) < 0.5 * float(np.sum(S.target_weights()))
+    assert np.all(np.isfinite(params))

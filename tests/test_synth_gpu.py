@@ -25,6 +25,16 @@ def test_tracks_match_restatement(gpu):
     assert np.max(_wrapdiff(got, ref, N)) < 1e-4
 
 
+def test_tracks_match_restatement_3d_headings(gpu):
+    g = GpuSynth()
+    N, T, n = 64, 6, 40
+    p = g.params(T, N, n, 1.5, 100, 20, D=0.2, alpha=0.7, vbar_px_frame=1.2, Z=2.0, seed=31, heading3d=True,
+                 swim_diffuse=True)
+    got = g.tracks(p).cpu().numpy().astype(np.float64)
+    ref = SO.tracks(N, T, n, 0.7, 1.2, 2.0, 0.2, 31, heading3d=True, swim_diffuse=True)
+    assert np.max(_wrapdiff(got, ref, N)) < 1e-4
+
+
 def test_render_expected_intensity_matches(gpu):
     g = GpuSynth()
     N, T, n = 48, 3, 30
@@ -71,6 +81,12 @@ def test_schulz_speed_moments(gpu):
     # uniform heading
     ang = np.arctan2(d[:, 1], d[:, 0])
     assert abs(np.mean(np.cos(ang))) < 0.03 and abs(np.mean(np.sin(ang))) < 0.03
+    # 3D headings: the projected speed has mean vbar * pi/4
+    pos = g.tracks(g.params(2, 256, n, 1.0, 0, 0, alpha=1.0, vbar_px_frame=vbar, Z=Z, seed=9,
+                            heading3d=True)).cpu().numpy()
+    d = np.mod(pos[1] - pos[0] + 128, 256) - 128
+    s3 = np.hypot(d[:, 0], d[:, 1])
+    assert abs(s3.mean() / (vbar * math.pi / 4) - 1) < 0.02
 
 
 def test_brownian_msd(gpu):
