@@ -1,0 +1,106 @@
+"""Launch-shape autotuner for one geometry.
+
+Candidates cover the engine's tunables (include/ddm.h `ddm_tuning`): the
+temporal work split (target item count: modes per workgroup), the spatial
+chunk (row-spectrum scratch budget, i.e. how long a chunk stays in the
+last-level cache), workgroup- vs wavefront-level spatial FFTs (N = 512, 1024)
+and frames per column workgroup.  Every candidate first has to reproduce the
+default configuration's D(q, tau) and P(q) on the caller's frames (relative
+1e-5 of the largest value, the FP32 bar of the path: candidates differ only in
+FP32 summation order); only then is it timed.  Results can be kept in a JSON
+cache keyed by geometry and pixel type (`tuned_create`).
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+from .engine import DdmEngine
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE_PATH = os.path.join(_HERE, "data", "tuning_cache.json")
+PARITY_RTOL = 1e-5
+
+
+def candidates(N: int, quick: bool = False):
+    items = (2048, 8192) if quick else (1024, 2048, 4096, 8192)
+    chunks = (128,) if quick else (64, 128, 256)
+    waves = (1, 2) if N in (512, 1024) else (1,)
+    out = []
+    for it, ch, wv in itertools.product(items, chunks, waves):
+        for cf in ((8, 16) if wv == 2 else (16,)):
+            out.append(dict(temporal_items=it, chunk_mb=ch, wave_spatial=wv, cols_f=cf))
+    return out
+
+
+def _run(eng: DdmEngine, dptr, dtype, nframes, order, mask):
+    eng.ingestDevice(dptr, dtype, nframes, 0)
+    eng.computeTemporal(order, mask)
+    eng.idle()
+
+
+def key(N, T, L, NB, dtype) -> str:
+    return f"N{N}_T{T}_L{L}_NB{NB}_{np.dtype(dtype).name}"
+
+
+def autotune(geometry: dict, dptr: int, dtype, nframes: int, reps: int = 3, quick: bool = False, log=None):
+    """geometry: DdmEngine.create kwargs; dptr: device frame stack.  Returns (best tuning, table)."""
+    T = geometry["T"]
+    order = np.arange(T, dtype=np.uint32)
+    mask = (np.arange(T) < nframes).astype(np.float32)
+    ref = DdmEngine.create(**geometry)
+    _run(ref, dptr, dtype, nframes, order, mask)
+    r0 = ref.readRadial()
+    default = ref.tuning()
+    ref.destroy()
+    scaleD = max(np.max(np.abs(r0.D)), 1e-30)
+    scaleP = max(np.max(np.abs(r0.P)), 1e-30)
+    table = []
+    for cand in candidates(geometry["N"], quick):
+        eng = DdmEngine.create(**geometry, tuning=cand)
+        _run(eng, dptr, dtype, nframes, order, mask)
+        r = eng.readRadial()
+        err = max(np.max(np.abs(r.D - r0.D)) / scaleD, np.max(np.abs(r.P - r0.P)) / scaleP)
+        row = dict(tuning=eng.tuning(), parity_err=float(err), ok=bool(err <= PARITY_RTOL), ms=None)
+        if row["ok"]:
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                _run(eng, dptr, dtype, nframes, order, mask)
+                best = min(best, (time.perf_counter() - t0) * 1e3)
+            row["ms"] = best
+        eng.destroy()
+        table.append(row)
+        if log:
+            log(f"  {row}")
+    ok = [r for r in table if r["ok"]]
+    if not ok:
+        return default, table
+    return min(ok, key=lambda r: r["ms"])["tuning"], table
+
+
+def load_cache(path: str = CACHE_PATH) -> dict:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_cache(entries: dict, path: str = CACHE_PATH) -> None:
+    cur = load_cache(path)
+    cur.update(entries)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(cur, f, indent=1, sort_keys=True)
+
+
+def tuned_create(dtype, path: str = CACHE_PATH, **geometry) -> DdmEngine:
+    """DdmEngine.create with the cached tuning of this geometry (defaults when none is cached)."""
+    g = dict(geometry)
+    k = key(g["N"], g["T"], g["L"], g["NB"], dtype)
+    tn = load_cache(path).get(k)
+    return DdmEngine.create(**g, tuning=tn)

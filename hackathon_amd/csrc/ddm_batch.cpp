@@ -1,0 +1,182 @@
+// Batches of uneven videos (ddm_batch_* in include/ddm.h): one engine per
+// video; the videos' ingest + temporal + radial work is recorded into one HIP
+// graph with `chains` concurrent branches (fork/join through events on the
+// batch stream), or enqueued eagerly with the same dependencies.
+#include "../../include/ddm.h"
+#include "ddm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+struct ddm_batch {
+  std::vector<ddm_engine*> eng;
+  std::vector<ddm_batch_video> vids;
+  std::vector<std::vector<uint32_t>> order;
+  std::vector<std::vector<float>> mask;
+  std::vector<hipEvent_t> evs;
+  int mode = DDM_BATCH_EAGER, chains = 1;
+  hipStream_t s0 = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+namespace {
+int bfail(ddm_batch* b, int code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+hipEvent_t new_event(ddm_batch* b) {
+  hipEvent_t ev = nullptr;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return nullptr;
+  b->evs.push_back(ev);
+  return ev;
+}
+
+// Enqueue every video on its engine's stream, chained per `chains`, forked from
+// and joined back into b->s0 (identical when recorded into a graph).
+int enqueue_all(ddm_batch* b, bool capturing) {
+  const int n = (int)b->eng.size();
+  hipEvent_t fork = new_event(b);
+  if (!fork || hipEventRecord(fork, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: fork event");
+  std::vector<hipEvent_t> tail(b->chains, fork);
+  for (int i = 0; i < n; i++) {
+    ddm_engine* e = b->eng[i];
+    hipStream_t es = ddm::engine_stream(e);
+    const int c = i % b->chains;
+    if (hipStreamWaitEvent(es, tail[c], 0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain wait");
+    ddm::engine_set_capturing(e, capturing);
+    const ddm_batch_video& v = b->vids[i];
+    int rc = ddm_ingest_device(e, v.dframes, v.dtype, v.nframes, 0);
+    if (!rc) rc = ddm_compute_temporal(e, b->order[i].data(), b->mask[i].data());
+    ddm::engine_set_capturing(e, false);
+    if (rc) return bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(e));
+    hipEvent_t done = new_event(b);
+    if (!done || hipEventRecord(done, es) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain event");
+    tail[c] = done;
+  }
+  for (int c = 0; c < b->chains; c++)
+    if (tail[c] != fork && hipStreamWaitEvent(b->s0, tail[c], 0) != hipSuccess)
+      return bfail(b, DDM_EHIP, "batch: join");
+  return DDM_OK;
+}
+
+void release(ddm_batch* b) {
+  if (b->s0) hipStreamSynchronize(b->s0);
+  for (auto* e : b->eng) ddm_destroy(e);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (auto ev : b->evs) hipEventDestroy(ev);
+  if (b->t0) hipEventDestroy(b->t0);
+  if (b->t1) hipEventDestroy(b->t1);
+  if (b->s0) hipStreamDestroy(b->s0);
+}
+}  // namespace
+
+extern "C" {
+
+static std::string g_batch_create_error;
+
+int ddm_batch_create(const ddm_batch_video* v, int32_t n, int32_t mode, int32_t chains, ddm_batch** out) {
+  if (!out) return DDM_EINVAL;
+  *out = nullptr;
+  if (!v || n < 1 || (mode != DDM_BATCH_EAGER && mode != DDM_BATCH_GRAPH) || chains < 1) {
+    g_batch_create_error = "batch: bad arguments";
+    return DDM_EINVAL;
+  }
+  ddm_batch* b = new ddm_batch();
+  b->mode = mode;
+  b->chains = std::min<int>(chains, n);
+  auto bail = [&](int rc) {
+    g_batch_create_error = b->err;
+    release(b);
+    delete b;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess)
+    return (bfail(b, DDM_EHIP, "batch: stream/event creation"), bail(DDM_EHIP));
+  for (int i = 0; i < n; i++) {
+    const ddm_batch_video& vi = v[i];
+    if (!vi.dframes || vi.nframes < 1 || vi.nframes > vi.g.T)
+      return (bfail(b, DDM_EINVAL, "batch: video " + std::to_string(i) + ": need 1 <= nframes <= T and frames"),
+              bail(DDM_EINVAL));
+    ddm_engine* e = nullptr;
+    int rc = ddm_create_tuned(&vi.g, &vi.tuning, &e);
+    if (rc) return (bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(nullptr)), bail(rc));
+    b->eng.push_back(e);
+    b->vids.push_back(vi);
+    std::vector<uint32_t> o(vi.g.T);
+    std::vector<float> m(vi.g.T, 1.f);
+    for (int j = 0; j < vi.g.T; j++) {
+      o[j] = vi.order ? vi.order[j] : (uint32_t)j;
+      if (vi.mask) m[j] = vi.mask[j];
+      else if (j >= vi.nframes) m[j] = 0.f;   // slots never written are invalid
+    }
+    b->order.push_back(std::move(o));
+    b->mask.push_back(std::move(m));
+  }
+  if (mode == DDM_BATCH_GRAPH) {
+    if (hipStreamBeginCapture(b->s0, hipStreamCaptureModeRelaxed) != hipSuccess)
+      return (bfail(b, DDM_EHIP, "batch: begin capture"), bail(DDM_EHIP));
+    int rc = enqueue_all(b, true);
+    hipGraph_t g = nullptr;
+    hipError_t he = hipStreamEndCapture(b->s0, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return bail(rc);
+    }
+    if (he != hipSuccess) return (bfail(b, DDM_EHIP, std::string("batch: end capture: ") + hipGetErrorString(he)),
+                                  bail(DDM_EHIP));
+    b->graph = g;
+    if ((he = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0)) != hipSuccess)
+      return (bfail(b, DDM_EHIP, std::string("batch: instantiate: ") + hipGetErrorString(he)), bail(DDM_EHIP));
+  }
+  *out = b;
+  return DDM_OK;
+}
+
+int ddm_batch_run(ddm_batch* b, double* gpu_ms) {
+  if (!b) return DDM_EINVAL;
+  if (hipEventRecord(b->t0, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: timing event");
+  int rc = DDM_OK;
+  if (b->mode == DDM_BATCH_GRAPH) {
+    hipError_t he = hipGraphLaunch(b->exec, b->s0);
+    if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: graph launch: ") + hipGetErrorString(he));
+  } else {
+    // events of earlier eager runs are no longer waited on: recycle them
+    hipStreamSynchronize(b->s0);
+    for (auto ev : b->evs) hipEventDestroy(ev);
+    b->evs.clear();
+    if ((rc = enqueue_all(b, false))) return rc;
+  }
+  if (hipEventRecord(b->t1, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: timing event");
+  hipError_t he = hipEventSynchronize(b->t1);
+  if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: run: ") + hipGetErrorString(he));
+  if (gpu_ms) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, b->t0, b->t1);
+    *gpu_ms = ms;
+  }
+  return DDM_OK;
+}
+
+int ddm_batch_engine(ddm_batch* b, int32_t i, ddm_engine** e) {
+  if (!b || !e || i < 0 || i >= (int)b->eng.size()) return DDM_EINVAL;
+  *e = b->eng[i];
+  return DDM_OK;
+}
+
+const char* ddm_batch_last_error(const ddm_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
+
+void ddm_batch_destroy(ddm_batch* b) {
+  if (!b) return;
+  release(b);
+  delete b;
+}
+
+}  // extern "C"

@@ -254,6 +254,10 @@ struct ddm_engine {
   float* pbin = nullptr;   // NB*T summed p per bin
   float* Pbin = nullptr;   // NB summed P per bin
   bool temporal_busy = false;
+  // set while the engine's work is being recorded into a HIP graph (batch):
+  // no host waits, the pinned metadata staging then belongs to the graph
+  bool capturing = false;
+  ddm_tuning tune{};
   // pinned staging of the per-analysis metadata (order, 0/1 mask, N(tau));
   // meta_ev marks when the last upload from it finished
   uint32_t* meta_order = nullptr;
@@ -309,7 +313,7 @@ hipEvent_t take_event(ddm_engine* e) {
 // Wrap a launch with events when profiling is on.
 template <typename F>
 hipError_t timed(ddm_engine* e, int kind, F&& launch) {
-  if (!e->prof) return launch();
+  if (!e->prof || e->capturing) return launch();
   hipEvent_t a = take_event(e), b = take_event(e);
   if (!a || !b) return launch();
   hipEventRecord(a, e->stream);
@@ -343,8 +347,15 @@ extern "C" {
 
 const char* ddm_last_error(const ddm_engine* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 
-int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
+int ddm_create(const ddm_geometry* gp, ddm_engine** out) { return ddm_create_tuned(gp, nullptr, out); }
+
+int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** out) {
   if (!gp || !out) return fail(nullptr, DDM_EINVAL, "ddm_create: null argument");
+  ddm_tuning tn{};
+  if (tp) tn = *tp;
+  if (tn.temporal_items < 0 || tn.chunk_mb < 0 || tn.wave_spatial < 0 || tn.wave_spatial > 2 ||
+      (tn.cols_f != 0 && tn.cols_f != 8 && tn.cols_f != 16))
+    return fail(nullptr, DDM_EINVAL, "ddm_create: tuning out of range");
   *out = nullptr;
   const ddm_geometry g = *gp;
   if (!is_pow2(g.N) || g.N < 16 || g.N > 2048)
@@ -375,7 +386,8 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
   const long binned = e->bins.offsets[g.NB];
 
   // Temporal work items: (bin, first mode, count); up to M modes of one bin.
-  const long target_items = env_long("DDM_TEMPORAL_ITEMS", 2048, 1, 1L << 22);
+  const long target_items =
+      tn.temporal_items > 0 ? tn.temporal_items : env_long("DDM_TEMPORAL_ITEMS", 2048, 1, 1L << 22);
   int M = (int)std::min<long>(64, std::max<long>(1, (binned + target_items - 1) / target_items));
   std::vector<int> items_h, bin_modes_h(std::max<long>(binned, 1));
   e->bin_items_h.assign(g.NB + 1, 0);
@@ -392,11 +404,18 @@ int ddm_create(const ddm_geometry* gp, ddm_engine** out) {
 
   // Spatial chunk: frames whose row spectra (NH*N*8 B each) fit the chunk
   // budget (default 128 MB: the scratch stays in the 256 MB last-level cache).
-  e->wave_spatial = (e->logn == 9 || e->logn == 10) && env_long("DDM_WAVE_SPATIAL", 1, 0, 1);
-  e->cols_f = env_long("DDM_COLS_F", 16, 8, 16) >= 16 ? 16 : 8;
+  // wave_spatial: 0 default (environment, else on where built), 1 off, 2 on
+  const long ws = tn.wave_spatial ? tn.wave_spatial - 1 : env_long("DDM_WAVE_SPATIAL", 1, 0, 1);
+  e->wave_spatial = (e->logn == 9 || e->logn == 10) && ws;
+  e->cols_f = (tn.cols_f ? tn.cols_f : env_long("DDM_COLS_F", 16, 8, 16)) >= 16 ? 16 : 8;
   const int F = 16;   // chunks are a multiple of every column-kernel frame tile
   long per_frame = (long)e->NH * g.N * 8;
-  const long chunk_bytes = env_long("DDM_CHUNK_MB", 128, 1, 1 << 16) << 20;
+  const long chunk_mb = tn.chunk_mb > 0 ? tn.chunk_mb : env_long("DDM_CHUNK_MB", 128, 1, 1 << 16);
+  const long chunk_bytes = chunk_mb << 20;
+  e->tune.temporal_items = (int32_t)target_items;
+  e->tune.chunk_mb = (int32_t)chunk_mb;
+  e->tune.wave_spatial = e->wave_spatial ? 2 : 1;
+  e->tune.cols_f = e->cols_f;
   int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
   chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
   e->chunk = chunk;
@@ -473,6 +492,12 @@ void ddm_destroy(ddm_engine* e) {
   hipStreamSynchronize(e->stream);
   free_all(e);
   delete e;
+}
+
+int ddm_get_tuning(const ddm_engine* e, ddm_tuning* t) {
+  if (!e || !t) return DDM_EINVAL;
+  *t = e->tune;
+  return DDM_OK;
 }
 
 int ddm_get_sizes(const ddm_engine* e, ddm_sizes* s) {
@@ -576,7 +601,7 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
     }
   }
   // the pinned staging buffers may still feed the previous upload
-  hipError_t he = hipEventSynchronize(e->meta_ev);
+  hipError_t he = e->capturing ? hipSuccess : hipEventSynchronize(e->meta_ev);
   if (he != hipSuccess) {
     e->temporal_busy = false;
     return hip_fail(e, he, "computeTemporal: metadata staging");
@@ -600,7 +625,7 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
   if ((he = hipMemcpyAsync(e->order, e->meta_order, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
       (he = hipMemcpyAsync(e->mask, m01, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
       (he = hipMemcpyAsync(e->npairs, np, L * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-      (he = hipEventRecord(e->meta_ev, e->stream)) != hipSuccess) {
+      (!e->capturing && (he = hipEventRecord(e->meta_ev, e->stream)) != hipSuccess)) {
     rc = hip_fail(e, he, "computeTemporal: metadata upload");
   } else if (e->nitems > 0 &&
              (he = timed(e, DDM_K_TEMPORAL, [&] {
@@ -685,3 +710,8 @@ int ddm_kernel_times(ddm_engine* e, double* ms, int32_t* count, int32_t n) {
 }
 
 }  // extern "C"
+
+namespace ddm {
+hipStream_t engine_stream(ddm_engine* e) { return e->stream; }
+void engine_set_capturing(ddm_engine* e, bool on) { e->capturing = on; }
+}  // namespace ddm

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+struct ddm_engine;
+
 namespace ddm {
 
 hipError_t launch_rows(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
@@ -51,5 +53,9 @@ hipError_t launch_to_bf16(const float* x, unsigned short* y, long n, hipStream_t
 hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp, double bg, double vbar, double Z,
                         double D, int heading3d, int swim_diffuse, uint64_t seed, int dtype, int noise,
                         double4* part, float2* pos, void* out, hipStream_t st);
+
+// engine internals used by the batch runner (ddm_batch.cpp)
+hipStream_t engine_stream(::ddm_engine* e);
+void engine_set_capturing(::ddm_engine* e, bool on);
 
 }  // namespace ddm

@@ -33,7 +33,9 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_history_device", "ddm_set_profiling", "ddm_kernel_times", "ddm_fitter_create", "ddm_fitter_destroy",
            "ddm_fitter_last_error", "ddm_fit", "ddm_mlp_create", "ddm_mlp_destroy", "ddm_mlp_last_error",
            "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step",
-           "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks")
+           "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
+           "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
+           "ddm_batch_last_error", "ddm_batch_destroy")
 
 
 class DdmError(RuntimeError):
@@ -43,6 +45,15 @@ class DdmError(RuntimeError):
 class Geometry(C.Structure):
     _fields_ = [("N", C.c_int32), ("T", C.c_int32), ("L", C.c_int32), ("NB", C.c_int32),
                 ("kmin", C.c_double), ("kmax", C.c_double), ("hypot_mode", C.c_int32), ("knodes", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    """Launch-shape tuning (0 = default): see ddm_tuning in include/ddm.h."""
+    _fields_ = [("temporal_items", C.c_int32), ("chunk_mb", C.c_int32), ("wave_spatial", C.c_int32),
+                ("cols_f", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Sizes(C.Structure):
@@ -79,9 +90,12 @@ def load_library(path: str = LIB_PATH):
     L.ddm_history_device.argtypes = [vp, C.POINTER(vp)]
     L.ddm_set_profiling.argtypes = [vp, C.c_int32]
     L.ddm_kernel_times.argtypes = [vp, vp, vp, C.c_int32]
+    L.ddm_create_tuned.argtypes = [C.POINTER(Geometry), C.POINTER(Tuning), C.POINTER(vp)]
+    L.ddm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     for name in EXPORTS:
         if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
-                        "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error"):
+                        "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error",
+                        "ddm_batch_last_error", "ddm_batch_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -125,9 +139,10 @@ _DTYPES = {np.dtype(np.uint8): DDM_U8, np.dtype(np.uint16): DDM_U16, np.dtype(np
 class DdmEngine:
     """One geometry, one HIP stream, engine-owned device buffers."""
 
-    def __init__(self, handle, lib):
+    def __init__(self, handle, lib, owned: bool = True):
         self._h = handle
         self._lib = lib
+        self._owned = owned
         s = Sizes()
         self._check(lib.ddm_get_sizes(self._h, C.byref(s)))
         self.sizes = s
@@ -140,19 +155,22 @@ class DdmEngine:
     # -- lifetime ---------------------------------------------------------
     @classmethod
     def create(cls, N: int = 256, T: int = 256, L: int = 128, NB: int = 32, kmin: float = 4.0,
-               kmax: float = 100.0, hypot_mode: int = 0, knodes: int = 4) -> "DdmEngine":
-        """DdmEngine.create (src/ddm-engine.ts:125-149); defaults = the reference DDM geometry."""
+               kmax: float = 100.0, hypot_mode: int = 0, knodes: int = 4, tuning: dict | None = None) -> "DdmEngine":
+        """DdmEngine.create (src/ddm-engine.ts:125-149); defaults = the reference DDM geometry.
+        tuning: optional launch-shape choices (ddm_tuning fields), e.g. from hackathon_amd.autotune."""
         lib = load_library()
         g = Geometry(N, T, L, NB, float(kmin), float(kmax), hypot_mode, knodes)
         h = C.c_void_p()
-        rc = lib.ddm_create(C.byref(g), C.byref(h))
+        tn = Tuning(**(tuning or {}))
+        rc = lib.ddm_create_tuned(C.byref(g), C.byref(tn), C.byref(h))
         if rc != DDM_OK:
             raise DdmError(lib.ddm_last_error(None).decode())
         return cls(h, lib)
 
     def destroy(self) -> None:
         if self._h:
-            self._lib.ddm_destroy(self._h)
+            if self._owned:
+                self._lib.ddm_destroy(self._h)
             self._h = None
 
     def __enter__(self):
@@ -177,6 +195,11 @@ class DdmEngine:
     def _alive(self):
         if not self._h:
             raise DdmError("engine destroyed")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self._lib.ddm_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
 
     # -- tables -----------------------------------------------------------
     def _read_bins(self) -> Bins:

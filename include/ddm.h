@@ -73,10 +73,26 @@ typedef struct ddm_sizes {
   int32_t chunk_frames;   /* frames per spatial chunk */
 } ddm_sizes;
 
+/* Launch-shape tuning of an engine (0 in any field = the default, which the
+ * environment variables DDM_TEMPORAL_ITEMS, DDM_CHUNK_MB, DDM_WAVE_SPATIAL,
+ * DDM_COLS_F may override).  Every choice computes the same D(q, tau) up to
+ * FP32 summation order; the autotuner (hackathon_amd/autotune.py) checks that
+ * before it times a candidate. */
+typedef struct ddm_tuning {
+  int32_t temporal_items;  /* target count of temporal work items (default 2048; <= 64 modes each) */
+  int32_t chunk_mb;        /* spatial chunk budget in MiB of row spectra (default 128) */
+  int32_t wave_spatial;    /* 1: workgroup-level spatial FFTs, 2: wavefront-level (N = 512, 1024 only) */
+  int32_t cols_f;          /* frames per column workgroup of the wavefront kernel: 8 or 16 */
+} ddm_tuning;
+
 /* DdmEngine.create (src/ddm-engine.ts:125-149): allocate device buffers for the
  * geometry, build the bin tables (makeBins, src/ddm_ref.js:84-170) and the
  * twiddle / window tables on the host in FP64 (src/ddm-engine.ts:193-197). */
 int ddm_create(const ddm_geometry *g, ddm_engine **out);
+/* Same with explicit tuning (t may be NULL); ddm_get_tuning reports the
+ * values in effect. */
+int ddm_create_tuned(const ddm_geometry *g, const ddm_tuning *t, ddm_engine **out);
+int ddm_get_tuning(const ddm_engine *e, ddm_tuning *t);
 
 /* DdmEngine.destroy (src/ddm-engine.ts:421-425).  NULL is ignored. */
 void ddm_destroy(ddm_engine *e);
@@ -204,6 +220,40 @@ int ddm_mlp_loss_grad(ddm_mlp *m, const float *X, const float *Tg, const float *
 /* one Adam step (t counts steps from 1) on a batch; *loss before the update */
 int ddm_mlp_train_step(ddm_mlp *m, const float *X, const float *Tg, const float *wt, int32_t M, float lr,
                        float *loss);
+
+/* ------------------------------------------------------------------ batches of videos
+ *
+ * Many videos (or ROIs) of uneven frame size and length through the spatial,
+ * temporal and radial stages in one submission: one engine per video, every
+ * video's kernels recorded once into a HIP graph whose branches run
+ * concurrently (DDM_BATCH_GRAPH), or enqueued on the videos' streams on every
+ * run (DDM_BATCH_EAGER).  Each video's results are bit-identical to running
+ * its engine alone.  No reference counterpart (the reference analyses one
+ * video at a time). */
+typedef struct ddm_batch ddm_batch;
+typedef struct ddm_batch_video {
+  ddm_geometry g;
+  ddm_tuning tuning;        /* zeros: defaults */
+  int32_t dtype;            /* DDM_U8 / DDM_U16 / DDM_F32 */
+  int32_t nframes;          /* frames ingested into slots 0..nframes-1 (<= T) */
+  const void *dframes;      /* device frame stack; must stay valid while the batch exists */
+  const uint32_t *order;    /* T entries (host), NULL: linear order */
+  const float *mask;        /* T entries (host), NULL: every frame valid */
+} ddm_batch_video;
+enum { DDM_BATCH_EAGER = 0, DDM_BATCH_GRAPH = 1 };
+
+/* Creates the engines (and records the graph).  `chains` (>= 1) splits the
+ * videos round-robin into that many independent chains that may run
+ * concurrently; videos within a chain run one after the other. */
+int ddm_batch_create(const ddm_batch_video *v, int32_t n, int32_t mode, int32_t chains, ddm_batch **out);
+/* One pass over every video; blocks until done; *gpu_ms (may be NULL) is the
+ * pass's duration from HIP events. */
+int ddm_batch_run(ddm_batch *b, double *gpu_ms);
+/* The engine of video i (owned by the batch): read results with
+ * ddm_read_radial etc. after ddm_batch_run. */
+int ddm_batch_engine(ddm_batch *b, int32_t i, ddm_engine **e);
+const char *ddm_batch_last_error(const ddm_batch *b);
+void ddm_batch_destroy(ddm_batch *b);
 
 /* ------------------------------------------------------------------ synthetic videos
  *
