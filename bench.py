@@ -5,7 +5,10 @@ One step = one pass of the hot path over one batch: the resident frame stack
 (1024 frames of 512x512 uint16) -> windowed R2C 2D FFT + corner turn into the
 spectral history -> masked Wiener-Khinchin D(q, tau) for all lags 1..1023 with
 radial binning into 256 unit-width |q| bins.  Inputs are in HBM before the
-timed region starts.
+timed region starts.  Steps alternate between `--inflight` engines (default 2,
+each with its own stream and history), so consecutive analyses overlap on the
+GPU; every step is a complete pass.  Per-kernel times and the roofline come
+from serial steps on one engine (no overlap), reported beside.
 
 Prints one JSON line.  value = frame-pixels/s through to D(q, tau).
 `roofline` is for the dominant kernel, timed live with HIP events on the
@@ -42,6 +45,7 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=256, help="frames in the CPU-baseline slice")
     p.add_argument("--fit-videos", type=int, default=320, help="held-out videos for the fits/s comparison")
     p.add_argument("--no-fits", action="store_true")
+    p.add_argument("--inflight", type=int, default=2, help="analyses in flight (engines, one stream each)")
     return p.parse_args()
 
 
@@ -102,30 +106,46 @@ def main():
     print(f"[bench] generating {T} frames of {N}x{N} u16 on the GPU", file=sys.stderr, flush=True)
     frames = synth.video_gpu(dtype="uint16", seed=2, **c2)
     torch.cuda.synchronize()
-    eng = DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=0, kmax=N // 2)
+    # `inflight` engines of the same geometry: consecutive steps go to different
+    # engines (own stream, own history), so one analysis's compute-bound
+    # temporal stage overlaps the next one's memory-bound spatial stage
+    engs = [DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=0, kmax=N // 2) for _ in range(a.inflight)]
     order = np.arange(T, dtype=np.uint32)
     mask = np.ones(T, np.float32)
 
-    def step():
+    def step(eng):
         eng.ingestDevice(frames.data_ptr(), np.uint16, T, 0)
         eng.computeTemporal(order, mask)
 
     print(f"[bench] warmup {a.warmup}", file=sys.stderr, flush=True)
-    for _ in range(a.warmup):
-        step()
-    eng.idle()
+    for i in range(max(a.warmup, a.inflight)):
+        step(engs[i % a.inflight])
+    for e in engs:
+        e.idle()
+    # per-kernel times: serial steps on one engine, HIP events on its stream
+    prof_steps = max(1, min(a.steps, 5))
+    eng = engs[0]
     eng.setProfiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
+    ts0 = time.perf_counter()
+    for _ in range(prof_steps):
+        step(eng)
     eng.idle()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    serial_ms = (time.perf_counter() - ts0) * 1e3 / prof_steps
     times = eng.kernelTimes()
     eng.setProfiling(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step(engs[i % a.inflight])
+    for e in engs:
+        e.idle()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     r = eng.readRadial()
     assert np.all(np.isfinite(r.D)), "non-finite D"
+    for e in engs[1:]:   # every in-flight engine computed the same (deterministic) result
+        r2 = e.readRadial()
+        assert np.array_equal(r2.D, r.D) and np.array_equal(r2.P, r.P), "in-flight engines disagree"
     ms_step = (t1 - t0) * 1e3 / a.steps
     fpx = T * N * N
     value = fpx / (ms_step * 1e-3)
@@ -143,7 +163,7 @@ def main():
                      "temporal_wk": "ddm::k_temporal_w<11, 2>", "radial_finalize": "ddm::k_bin_reduce<11>"}
     per_kernel = {}
     for name, (ms, cnt) in times.items():
-        per_kernel[name] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps,
+        per_kernel[name] = {"ms_per_step": ms / prof_steps, "launches_per_step": cnt / prof_steps,
                             "avg_launch_ms": ms / max(cnt, 1)}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
     launches = max(per_kernel[dom]["launches_per_step"], 1)
@@ -181,9 +201,13 @@ def main():
                      "algorithmic_bytes_per_launch": algo_launch,
                      "avg_launch_ms": per_kernel[dom]["avg_launch_ms"]},
         "pipeline": {"algorithmic_bytes": raw_bytes + 2 * spec_bytes, "GBs": pipeline_gbs,
-                     "frac_of_hbm_peak": pipeline_gbs / HBM_PEAK_GBS, "kernels": per_kernel},
+                     "frac_of_hbm_peak": pipeline_gbs / HBM_PEAK_GBS, "inflight": a.inflight,
+                     "serial_ms_per_step": serial_ms,
+                     "serial_frame_pixels_per_s": fpx / (serial_ms * 1e-3),
+                     "kernels_serial": per_kernel},
     }
-    eng.destroy()
+    for e in engs:
+        e.destroy()
     del frames
     if not a.no_fits:
         print("[bench] fits: plain vs seeded LM", file=sys.stderr, flush=True)

@@ -250,6 +250,7 @@ struct ddm_engine {
   float* P_part = nullptr;
   float* radD = nullptr;
   float* radP = nullptr;
+  float* Sbin = nullptr;   // NB*TPAD summed |Y|^2 per bin (frequency order)
   float* Gbin = nullptr;   // NB*L summed Re G per bin
   float* pbin = nullptr;   // NB*T summed p per bin
   float* Pbin = nullptr;   // NB summed P per bin
@@ -326,7 +327,7 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
-                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin};
+                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -435,7 +436,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->npairs, g.L)) != hipSuccess || (he = dalloc(&e->S_part, nitems * e->TPAD)) != hipSuccess ||
       (he = dalloc(&e->p_part, nitems * g.T)) != hipSuccess || (he = dalloc(&e->P_part, nitems)) != hipSuccess ||
       (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess ||
-      (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
+      (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess ||
+      (he = dalloc(&e->Sbin, (size_t)g.NB * e->TPAD)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
       (he = dalloc(&e->Pbin, g.NB)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: device allocation");
     return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
@@ -606,10 +608,11 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
     e->temporal_busy = false;
     return hip_fail(e, he, "computeTemporal: metadata staging");
   }
-  bool identity = true;
+  bool identity = true, full = true;
   float* m01 = e->meta_mask;
   for (int j = 0; j < T; j++) {
     identity = identity && order[j] == (uint32_t)j;
+    full = full && mask[j] != 0.f;
     e->meta_order[j] = order[j];
     m01[j] = mask[j] != 0.f ? 1.f : 0.f;
   }
@@ -640,9 +643,12 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
               })) != hipSuccess) {
     rc = hip_fail(e, he, "temporal launch");
   } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
-                return ddm::launch_bin_finalize(e->logt, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
-                                                e->nmodes, e->mask, e->npairs, T, L, n0, e->tw_t, e->Gbin, e->pbin,
-                                                e->Pbin, e->radD, e->radP, e->stream);
+                // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
+                // sub-transforms and stores their spectra r-major
+                const int lm = e->logt <= 11 && e->logt > 9 ? e->logt - 9 : 0;
+                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
+                                                e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t,
+                                                e->Sbin, e->Gbin, e->pbin, e->Pbin, e->radD, e->radP, e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "radial finalize launch");
   }

@@ -424,53 +424,6 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 
 // ----------------------------------------------------------------- k_finalize
 
-template <int LOGT>
-__global__ __launch_bounds__(NT) void k_finalize(const float* __restrict__ S_part, const float* __restrict__ p_part,
-                                                 const float* __restrict__ P_part, const int* __restrict__ bin_items,
-                                                 const uint32_t* __restrict__ nmodes, const float* __restrict__ mask01,
-                                                 const float* __restrict__ npairs, int T, int L, float n0,
-                                                 const float2* __restrict__ tw_g, float* __restrict__ radD,
-                                                 float* __restrict__ radP) {
-  constexpr int n = 1 << LOGT;
-  extern __shared__ float2 lds[];
-  float2* buf = lds;                                  // n
-  float2* tw = lds + n;                               // n
-  float* p = reinterpret_cast<float*>(tw + n);        // n/2 (T <= n/2)
-  float* red = p + n / 2;                             // 2*NT/64
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  const int i0 = bin_items[b], i1 = bin_items[b + 1];
-  for (int i = tid; i < n; i += NT) {
-    tw[i] = tw_g[i];
-    float s = 0.f, c = 0.f;
-    for (int it = i0; it < i1; it++) comp_add(s, c, S_part[(long)it * n + i]);
-    buf[i] = make_float2(s + c, 0.f);
-  }
-  for (int t = tid; t < T; t += NT) {
-    float s = 0.f, c = 0.f;
-    for (int it = i0; it < i1; it++) comp_add(s, c, p_part[(long)it * T + t]);
-    p[t] = s + c;
-  }
-  float ps = 0.f, pc = 0.f;
-  for (int it = i0 + tid; it < i1; it += NT) comp_add(ps, pc, P_part[it]);
-  float2 Ptot = block_sum2(make_float2(ps + pc, 0.f), red, tid);  // has barriers
-  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
-  const uint32_t nm = nmodes[b];
-  const float invn = 1.0f / n;
-  for (int tau = tid; tau < L; tau += NT) {
-    float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
-    for (int t = 0; t + tau < T; t++) {
-      comp_add(sa, ca, mask01[t] * p[t + tau]);
-      comp_add(sb, cb, p[t] * mask01[t + tau]);
-    }
-    float np = npairs[tau];
-    float g = buf[tau].x * invn;
-    float d = 0.f;
-    if (np > 0.5f && nm > 0) d = (((sa + ca) + (sb + cb)) - 2.0f * g) / np / (float)nm;
-    radD[(long)b * L + tau] = d;
-  }
-  if (tid == 0) radP[b] = (nm > 0 && n0 > 0.5f) ? Ptot.x / n0 / (float)nm : 0.f;
-}
 
 // ----------------------------------------------------------------- k_temporal_w
 //
@@ -618,21 +571,31 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
           v[q] = r ? cmul(u, tw_g[r * (lane + 64 * q)]) : u;
         }
         wave_fft<LOGF, false, false>(v, xch, tw_f, lane);
+        // |Y(m j + r)|^2 parked r-major (index r NF + j): consecutive lanes,
+        // consecutive words, no bank conflicts; k_bin_sum undoes the order
 #pragma unroll
-        for (int q = 0; q < PF; q++) parky[m * (lane + 64 * q) + r] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
+        for (int q = 0; q < PF; q++) parky[r * NF + lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
       }
 #pragma unroll
       for (int q = 0; q < PT; q++) parky[n + lane + 64 * q] = active ? y[q].x * y[q].x + y[q].y * y[q].y : 0.f;
     }
     __syncthreads();
+    {
+      // the 4 waves' values are added pairwise (positive terms: relative error
+      // <= 2 ulp), then compensated into the running sums
+      const int po = LM ? 2 * xch_size(NF) : 0;   // parked area, in floats
+      const float* p0 = reinterpret_cast<const float*>(lds) + po;
+      constexpr int WF = 2 * WS;                   // wave stride, in floats
 #pragma unroll
-    for (int w = 0; w < NT / 64; w++) {
-      const float* pw = reinterpret_cast<const float*>(lds + w * WS + (LM ? xch_size(NF) : 0));
+      for (int i = 0; i < KS; i++) {
+        const int k = tid + NT * i;
+        two_sum_add(S[i], Sc[i], (p0[k] + p0[WF + k]) + (p0[2 * WF + k] + p0[3 * WF + k]));
+      }
 #pragma unroll
-      for (int i = 0; i < KS; i++) two_sum_add(S[i], Sc[i], pw[tid + NT * i]);
-#pragma unroll
-      for (int i = 0; i < TS; i++)
-        if (tid + NT * i < n / 2) two_sum_add(pa[i], pc[i], pw[n + tid + NT * i]);
+      for (int i = 0; i < TS; i++) {
+        const int t = n + tid + NT * i;
+        if (tid + NT * i < n / 2) two_sum_add(pa[i], pc[i], (p0[t] + p0[WF + t]) + (p0[2 * WF + t] + p0[3 * WF + t]));
+      }
     }
     __syncthreads();
   }
@@ -651,46 +614,137 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// ----------------------------------------------------------------- k_bin_reduce / k_bin_lags
+// ----------------------------------------------------------------- radial finalisation
 //
-// Radial finalisation in two launches.  k_bin_reduce (one workgroup per bin)
-// sums the bin's partials, inverse-FFTs the summed |Y|^2 to the bin's summed
-// Re G(tau), and stores G[b][tau], p[b][t] and the bin's summed P.
-// k_bin_lags (one workgroup per (bin, 64 lags)) forms the masked correlations
-// A(tau) = sum_t m(t) p(t+tau), B(tau) = sum_t p(t) m(t+tau) with lanes over
-// tau and waves over quarters of t, and D = (A + B - 2G) / N(tau) / nModes.
+// Three launches.  k_bin_sum adds the work items' partials of each bin
+// (|Y|^2 per frequency, p per time sample, P); k_bin_ifft inverse-FFTs the
+// bin's summed |Y|^2 to its summed Re G(tau); k_bin_lags (any mask) or
+// k_bin_lags_full (every frame valid) forms the masked correlations
+// A(tau) = sum_t m(t) p(t+tau), B(tau) = sum_t p(t) m(t+tau) and
+// D = (A + B - 2G) / N(tau) / nModes.
+// k_bin_sum: grid (NB, n/NT + ceil(T/NT)): one thread per frequency (first
+// n/NT block rows) or per time sample (the rest) sums the bin's item partials
+// with 2Sum, four independent loads in flight; frequency sums are written in
+// natural order (undoing the r-major interleave, see above).  Block (b, 0) also
+// sums the items' P.
 template <int LOGT>
-__global__ __launch_bounds__(NT) void k_bin_reduce(const float* __restrict__ S_part, const float* __restrict__ p_part,
-                                                   const float* __restrict__ P_part, const int* __restrict__ bin_items,
-                                                   int T, int L, const float2* __restrict__ tw_g,
-                                                   float* __restrict__ Gbin, float* __restrict__ pbin,
-                                                   float* __restrict__ Pbin) {
+__global__ __launch_bounds__(NT) void k_bin_sum(const float* __restrict__ S_part, const float* __restrict__ p_part,
+                                                const float* __restrict__ P_part, const int* __restrict__ bin_items,
+                                                int T, int lm, float* __restrict__ Sbin, float* __restrict__ pbin,
+                                                float* __restrict__ Pbin) {
+  constexpr int n = 1 << LOGT;
+  extern __shared__ float2 lds[];
+  float* red = reinterpret_cast<float*>(lds);   // 2 x (NT / 64)
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int i0 = bin_items[b], i1 = bin_items[b + 1];
+  const int fb = n / NT;
+  const float* src;
+  long stride;
+  int idx, lim;
+  if ((int)blockIdx.y < fb) {
+    idx = blockIdx.y * NT + tid;
+    src = S_part + idx;
+    stride = n;
+    lim = n;
+  } else {
+    idx = (blockIdx.y - fb) * NT + tid;
+    src = p_part + idx;
+    stride = T;
+    lim = T;
+  }
+  if (idx < lim) {
+    float s = 0.f, c = 0.f;
+    int it = i0;
+    for (; it + 4 <= i1; it += 4) {
+      const float a0 = src[(long)it * stride], a1 = src[(long)(it + 1) * stride];
+      const float a2 = src[(long)(it + 2) * stride], a3 = src[(long)(it + 3) * stride];
+      two_sum_add(s, c, a0);
+      two_sum_add(s, c, a1);
+      two_sum_add(s, c, a2);
+      two_sum_add(s, c, a3);
+    }
+    for (; it < i1; it++) two_sum_add(s, c, src[(long)it * stride]);
+    if ((int)blockIdx.y < fb) {
+      const int k = ((idx & ((n >> lm) - 1)) << lm) | (idx >> (LOGT - lm));
+      Sbin[(long)b * n + k] = s + c;
+    } else {
+      pbin[(long)b * T + idx] = s + c;
+    }
+  }
+  if (blockIdx.y == 0) {
+    float ps = 0.f, pc = 0.f;
+    for (int it = i0 + tid; it < i1; it += NT) two_sum_add(ps, pc, P_part[it]);
+    const float2 Ptot = block_sum2(make_float2(ps + pc, 0.f), red, tid);  // barriers inside
+    if (tid == 0) Pbin[b] = Ptot.x;
+  }
+}
+
+// k_bin_ifft: one workgroup per bin, Re G(tau) = IFFT(sum |Y|^2)(tau) / n.
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_bin_ifft(const float* __restrict__ Sbin, int L, const float2* __restrict__ tw_g,
+                                                 float* __restrict__ Gbin) {
   constexpr int n = 1 << LOGT;
   extern __shared__ float2 lds[];
   float2* buf = lds;
   float2* tw = lds + n;
-  float* red = reinterpret_cast<float*>(tw + n);
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  const int i0 = bin_items[b], i1 = bin_items[b + 1];
+  const int tid = threadIdx.x, b = blockIdx.x;
   for (int i = tid; i < n; i += NT) {
     tw[i] = tw_g[i];
-    float s = 0.f, c = 0.f;
-    for (int it = i0; it < i1; it++) two_sum_add(s, c, S_part[(long)it * n + i]);
-    buf[i] = make_float2(s + c, 0.f);
+    buf[i] = make_float2(Sbin[(long)b * n + i], 0.f);
   }
-  for (int t = tid; t < T; t += NT) {
-    float s = 0.f, c = 0.f;
-    for (int it = i0; it < i1; it++) two_sum_add(s, c, p_part[(long)it * T + t]);
-    pbin[(long)b * T + t] = s + c;
-  }
-  float ps = 0.f, pc = 0.f;
-  for (int it = i0 + tid; it < i1; it += NT) two_sum_add(ps, pc, P_part[it]);
-  float2 Ptot = block_sum2(make_float2(ps + pc, 0.f), red, tid);  // barriers inside
+  __syncthreads();
   lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
   const float invn = 1.0f / n;
   for (int tau = tid; tau < L; tau += NT) Gbin[(long)b * L + tau] = buf[tau].x * invn;
-  if (tid == 0) Pbin[b] = Ptot.x;
+}
+
+// k_bin_lags_full: every frame valid (m = 1 on [0, T)).  Then
+// A(tau) + B(tau) = sum_{t >= tau} p(t) + sum_{t < T - tau} p(t)
+//                 = (S(T) - S(tau)) + S(T - tau)
+// with S the exclusive prefix sum of the bin's p, scanned in FP64 (one
+// workgroup per bin); D = (A + B - 2G) / N(tau) / nModes.
+__global__ __launch_bounds__(NT) void k_bin_lags_full(const float* __restrict__ Gbin, const float* __restrict__ pbin,
+                                                      const float* __restrict__ Pbin,
+                                                      const uint32_t* __restrict__ nmodes,
+                                                      const float* __restrict__ npairs, int T, int L, float n0,
+                                                      float* __restrict__ radD, float* __restrict__ radP) {
+  extern __shared__ float2 lds[];
+  double* S = reinterpret_cast<double*>(lds);   // T + 1
+  double* wsum = S + T + 1;                      // NT / 64
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
+  // blocked exclusive scan: thread tid owns the contiguous run [c0, c1)
+  const int per = (T + NT - 1) / NT;
+  const int c0 = min(T, tid * per), c1 = min(T, c0 + per);
+  double loc = 0;
+  for (int t = c0; t < c1; t++) loc += (double)pbin[(long)b * T + t];
+  // inclusive scan of the per-thread totals across the block
+  double v = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  double base = 0;
+  for (int w = 0; w < wave; w++) base += wsum[w];
+  double run = base + v - loc;                   // exclusive prefix at c0
+  for (int t = c0; t < c1; t++) {
+    S[t] = run;
+    run += (double)pbin[(long)b * T + t];
+  }
+  if (tid == NT - 1) S[T] = run;
+  __syncthreads();
+  const uint32_t nm = nmodes[b];
+  for (int tau = tid; tau < L; tau += NT) {
+    const double ab = (S[T] - S[tau]) + S[T - tau];
+    const float np = npairs[tau];
+    const double g = Gbin[(long)b * L + tau];
+    float d = 0.f;
+    if (np > 0.5f && nm > 0) d = (float)((ab - 2.0 * g) / (double)np / (double)nm);
+    radD[(long)b * L + tau] = d;
+  }
+  if (tid == 0) radP[b] = (nm > 0 && n0 > 0.5f) ? Pbin[b] / n0 / (float)nm : 0.f;
 }
 
 __global__ __launch_bounds__(NT) void k_bin_lags(const float* __restrict__ Gbin, const float* __restrict__ pbin,
@@ -1064,34 +1118,6 @@ hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* item
   return hipErrorInvalidValue;
 }
 
-template <int LOGT>
-static hipError_t launch_finalize_t(const float* S_part, const float* p_part, const float* P_part, const int* bin_items,
-                                    int NB, const uint32_t* nmodes, const float* mask, const float* npairs, int T,
-                                    int L, float n0, const float2* tw, float* radD, float* radP, hipStream_t st) {
-  constexpr int n = 1 << LOGT;
-  size_t lds = sizeof(float2) * 2 * n + sizeof(float) * (n / 2) + sizeof(float) * 2 * (NT / 64);
-  auto kern = k_finalize<LOGT>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, S_part, p_part, P_part, bin_items, nmodes, mask, npairs, T, L,
-                     n0, tw, radD, radP);
-  return hipGetLastError();
-}
-
-hipError_t launch_finalize(int logt, const float* S_part, const float* p_part, const float* P_part,
-                           const int* bin_items, int NB, const uint32_t* nmodes, const float* mask,
-                           const float* npairs, int T, int L, float n0, const float2* tw, float* radD, float* radP,
-                           hipStream_t st) {
-  switch (logt) {
-#define CASE(LG_)                                                                                                       \
-  case LG_:                                                                                                           \
-    return launch_finalize_t<LG_>(S_part, p_part, P_part, bin_items, NB, nmodes, mask, npairs, T, L, n0, tw, radD, radP, \
-                                st);
-    CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
-#undef CASE
-  }
-  return hipErrorInvalidValue;
-}
 
 template <int LOGT, int LM>
 static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
@@ -1124,32 +1150,45 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
 }
 
 template <int LOGT>
-static hipError_t launch_bin_reduce_t(const float* S_part, const float* p_part, const float* P_part,
-                                      const int* bin_items, int NB, int T, int L, const float2* tw, float* Gbin,
-                                      float* pbin, float* Pbin, hipStream_t st) {
+static hipError_t launch_bin_sum_ifft_t(const float* S_part, const float* p_part, const float* P_part,
+                                       const int* bin_items, int NB, int T, int L, int lm, const float2* tw,
+                                       float* Sbin, float* Gbin, float* pbin, float* Pbin, hipStream_t st) {
   constexpr int n = 1 << LOGT;
-  size_t lds = sizeof(float2) * 2 * n + sizeof(float) * 2 * (NT / 64);
-  auto kern = k_bin_reduce<LOGT>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_bin_sum<LOGT>, dim3(NB, n / NT + (T + NT - 1) / NT), dim3(NT), sizeof(float) * 2 * (NT / 64), st,
+                     S_part, p_part, P_part, bin_items, T, lm, Sbin, pbin, Pbin);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, S_part, p_part, P_part, bin_items, T, L, tw, Gbin, pbin, Pbin);
+  size_t lds = sizeof(float2) * 2 * n;
+  auto kern = k_bin_ifft<LOGT>;
+  e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, (const float*)Sbin, L, tw, Gbin);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_finalize(int logt, const float* S_part, const float* p_part, const float* P_part,
-                               const int* bin_items, int NB, const uint32_t* nmodes, const float* mask,
-                               const float* npairs, int T, int L, float n0, const float2* tw, float* Gbin,
-                               float* pbin, float* Pbin, float* radD, float* radP, hipStream_t st) {
+hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const float* p_part, const float* P_part,
+                               const int* bin_items, int NB, const uint32_t* nmodes, const float* mask, int full,
+                               const float* npairs, int T, int L, float n0, const float2* tw, float* Sbin,
+                               float* Gbin, float* pbin, float* Pbin, float* radD, float* radP, hipStream_t st) {
   hipError_t e = hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                \
   case LG_:                                                                                                    \
-    e = launch_bin_reduce_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, tw, Gbin, pbin, Pbin, st);         \
+    e = launch_bin_sum_ifft_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, lm, tw, Sbin, Gbin, pbin, Pbin,   \
+                                   st);                                                                        \
     break;
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
   }
   if (e != hipSuccess) return e;
+  if (full) {
+    size_t lds = sizeof(double) * (T + 1 + NT / 64);
+    e = hipFuncSetAttribute((const void*)k_bin_lags_full, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_lags_full, dim3(NB), dim3(NT), lds, st, (const float*)Gbin, (const float*)pbin,
+                       (const float*)Pbin, nmodes, npairs, T, L, n0, radD, radP);
+    return hipGetLastError();
+  }
   size_t lds = sizeof(float) * (2 * T + 4 * NT);
   e = hipFuncSetAttribute((const void*)k_bin_lags, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
