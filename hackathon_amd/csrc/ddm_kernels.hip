@@ -462,6 +462,16 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * WS;
   float* red = reinterpret_cast<float*>(lds + 4 * WS);  // 4
+  // the sub-transform's twiddle table, staged once per workgroup: the FFT
+  // passes read it from LDS instead of waiting on global loads every pass
+  float2* twl = lds + 4 * WS + 2;
+  for (int i = tid; i < NF; i += NT) twl[i] = tw_f[i];
+  // DIF pre-twiddles w_n^{r (lane + 64 q)} = w_n^{r lane} w_n^{64 r q}: the
+  // per-lane factor is loaded once here, the second is wave-uniform
+  float2 tbase[m > 1 ? m : 1];
+#pragma unroll
+  for (int r = 0; r < (m > 1 ? m : 1); r++) tbase[r] = tw_g[r * lane];
+  __syncthreads();
   const int4 it = items[blockIdx.x];
   const int start = it.y, count = it.z;
   uint32_t mbits = 0;
@@ -470,6 +480,9 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     int t = lane + 64 * q;
     if (t < T && mask[t] != 0.f) mbits |= 1u << q;
   }
+  constexpr uint32_t all_bits = PT == 32 ? 0xffffffffu : ((1u << PT) - 1u);
+  // wave-uniform: every lane's PT samples valid (full mask and T = n/2)
+  const bool dense = __all(mbits == all_bits);
   float S[KS], Sc[KS], pa[TS], pc[TS];
 #pragma unroll
   for (int i = 0; i < KS; i++) S[i] = Sc[i] = 0.f;
@@ -478,8 +491,8 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   float aP = 0.f, cP = 0.f;
   const int rounds = (count + 3) / 4;
   // next round's series, loaded while the current one is transformed (up to
-  // n = 1024; above that the registers are needed by the split transforms)
-  constexpr bool PREFETCH = LOGT <= 10;
+  // n = 2048; above that the registers are needed by the split transforms)
+  constexpr bool PREFETCH = LOGT <= 11;
   float2 nx[PREFETCH ? PT : 1];
   if constexpr (PREFETCH) {
     const int mi = wave;
@@ -526,20 +539,37 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
         y[q] = (t < T) ? xch[xpad((int)order[t])] : make_float2(0.f, 0.f);
       }
     }
-    float sx = 0.f, sy = 0.f;
+    float sx = 0.f, sy = 0.f, ps = 0.f;
+    if (dense) {   // every sample of every lane valid: no masking
 #pragma unroll
-    for (int q = 0; q < PT; q++) {
-      const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
-      y[q] = make_float2(y[q].x * mq, y[q].y * mq);
-      sx += y[q].x;
-      sy += y[q].y;
-      two_sum_add(aP, cP, y[q].x * y[q].x + y[q].y * y[q].y);
+      for (int q = 0; q < PT; q++) {
+        sx += y[q].x;
+        sy += y[q].y;
+        ps += y[q].x * y[q].x + y[q].y * y[q].y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
+        y[q] = make_float2(y[q].x * mq, y[q].y * mq);
+        sx += y[q].x;
+        sy += y[q].y;
+        ps += y[q].x * y[q].x + y[q].y * y[q].y;
+      }
     }
+    // one compensated add per mode and lane: the lane's 16 positive terms are
+    // summed plainly (relative error <= 15 ulp of this mode's share)
+    two_sum_add(aP, cP, ps);
     const float mx = wave_sum(sx) * inv_n0, my = wave_sum(sy) * inv_n0;
+    if (dense) {
 #pragma unroll
-    for (int q = 0; q < PT; q++) {
-      const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
-      y[q] = make_float2((y[q].x - mx) * mq, (y[q].y - my) * mq);
+      for (int q = 0; q < PT; q++) y[q] = make_float2(y[q].x - mx, y[q].y - my);
+    } else {
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        const float mq = ((mbits >> q) & 1u) ? 1.f : 0.f;
+        y[q] = make_float2((y[q].x - mx) * mq, (y[q].y - my) * mq);
+      }
     }
     if constexpr (LM == 0) {
       float2 v[PF];
@@ -547,7 +577,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
       for (int q = 0; q < PT; q++) v[q] = y[q];
 #pragma unroll
       for (int q = PT; q < PF; q++) v[q] = make_float2(0.f, 0.f);
-      wave_fft<LOGF, false, true>(v, xch, tw_f, lane);
+      wave_fft<LOGF, false, true>(v, xch, twl, lane);
       wave_sync();
 #pragma unroll
       for (int q = 0; q < PF; q++) park[lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
@@ -563,14 +593,18 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
       for (int r = 0; r < m; r++) {
         float2 v[PF];
         __builtin_amdgcn_sched_barrier(0);
+        float2 tb = tbase[0];
+#pragma unroll
+        for (int k = 1; k < m; k++)
+          if (r == k) tb = tbase[k];
 #pragma unroll
         for (int q = 0; q < PF; q++) {
           float2 u = y[q];
 #pragma unroll
           for (int t2 = 1; t2 < m / 2; t2++) u = cadd(u, cmul(y[q + t2 * PF], tw_g[((r * t2) & (m - 1)) * NF]));
-          v[q] = r ? cmul(u, tw_g[r * (lane + 64 * q)]) : u;
+          v[q] = r ? cmul(u, cmul(tb, tw_g[(64 * r * q) & (n - 1)])) : u;
         }
-        wave_fft<LOGF, false, false>(v, xch, tw_f, lane);
+        wave_fft<LOGF, false, false>(v, xch, twl, lane);
         // |Y(m j + r)|^2 parked r-major (index r NF + j): consecutive lanes,
         // consecutive words, no bank conflicts; k_bin_sum undoes the order
 #pragma unroll
@@ -1124,7 +1158,7 @@ static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* ite
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
                                       const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                                       float* P_part, hipStream_t st) {
-  size_t lds = sizeof(float2) * 4 * temporal_wave_lds(LOGT, LM) + sizeof(float) * 4;
+  size_t lds = sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + ((1 << LOGT) >> LM));
   auto kern = k_temporal_w<LOGT, LM>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
