@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=256, help="frames in the CPU-baseline slice")
     p.add_argument("--fit-videos", type=int, default=320, help="held-out videos for the fits/s comparison")
     p.add_argument("--no-fits", action="store_true")
+    p.add_argument("--no-batch", action="store_true")
     p.add_argument("--inflight", type=int, default=2, help="analyses in flight (engines, one stream each)")
     return p.parse_args()
 
@@ -66,6 +67,55 @@ def fits_section(nvideos):
                        f"{len(probs)} per-q Schulz-model fits, plain = 3 best of the reference start grid, "
                        f"seeded = 1 start from the MLP; fits/s from HIP-event kernel times (+ MLP inference)")
     return res
+
+
+def batch_section(nvideos=64, seed=3, reps=3):
+    """Batched uneven workload: `nvideos` seeded videos, frame side in {128, 256, 512}, length uniform in
+    [200, 2000], all analysed in one HIP-graph launch, then all their q-bins in one batched LM launch."""
+    from hackathon_amd import batch as BA
+    from hackathon_amd.synth import GpuSynth
+    rng = np.random.default_rng(seed)
+    g = GpuSynth()
+    vids, fpx = [], 0
+    for i in range(nvideos):
+        N = int(rng.choice([128, 256, 512]))
+        T = int(rng.integers(200, 2001))
+        al, vb, Z, D = rng.uniform(0, 1), rng.uniform(5, 30), rng.uniform(1, 10), rng.uniform(0.05, 1)
+        fr = g.video(g.params(T, N, max(50, 200 * N * N // (256 * 256)), 1.5, 100.0, 20.0, D=D, alpha=al,
+                              vbar_px_frame=vb / 100.0, Z=Z, dtype="uint8", seed=seed * 100000 + i, heading3d=True,
+                              swim_diffuse=True))
+        vids.append(dict(frames=fr, dtype=np.uint8, T=T, L=T, NB=N // 2, kmin=0, kmax=N // 2))
+        fpx += T * N * N
+    b = BA.Batch(vids, mode=BA.GRAPH, chains=8)
+    b.run()                                   # warm
+    ms = min(b.run() for _ in range(reps))
+    rads = b.radials()
+    prs, res = BA.fit_all(rads, 1.0, 100.0, nstarts=3)
+    b.close()
+    return {"workload": f"{nvideos} videos, N in {{128, 256, 512}}, T uniform in [200, 2000], u8, lags 1..T-1, "
+                        f"N/2 unit-width bins; one HIP-graph launch (8 concurrent chains); N = 384 is not a power "
+                        f"of two and is rejected as by the reference's FFT",
+            "frame_pixels": fpx, "ms": ms, "frame_pixels_per_s": fpx / (ms * 1e-3),
+            "fits": int(len(res.chi2)), "fit_kernel_ms": res.kernel_ms,
+            "fits_per_s": len(res.chi2) / (res.kernel_ms * 1e-3)}
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / s, GB/s): the achievable HBM figure beside the 8 TB/s peak."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(N, frames, nbins):
@@ -109,7 +159,9 @@ def main():
     # `inflight` engines of the same geometry: consecutive steps go to different
     # engines (own stream, own history), so one analysis's compute-bound
     # temporal stage overlaps the next one's memory-bound spatial stage
-    engs = [DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=0, kmax=N // 2) for _ in range(a.inflight)]
+    # launch shapes from the autotuner's cache for this geometry (defaults if none)
+    from hackathon_amd.autotune import tuned_create
+    engs = [tuned_create(np.uint16, N=N, T=T, L=L, NB=NB, kmin=0, kmax=N // 2) for _ in range(a.inflight)]
     order = np.arange(T, dtype=np.uint32)
     mask = np.ones(T, np.float32)
 
@@ -206,9 +258,14 @@ def main():
                      "serial_frame_pixels_per_s": fpx / (serial_ms * 1e-3),
                      "kernels_serial": per_kernel},
     }
+    out["pipeline"]["tuning"] = engs[0].tuning()
     for e in engs:
         e.destroy()
     del frames
+    out["roofline"]["achievable_copy_GBs"] = copy_bandwidth()
+    if not a.no_batch:
+        print("[bench] batched uneven videos", file=sys.stderr, flush=True)
+        out["batch"] = batch_section()
     if not a.no_fits:
         print("[bench] fits: plain vs seeded LM", file=sys.stderr, flush=True)
         out["fits"] = fits_section(a.fit_videos)

@@ -104,3 +104,35 @@ def tuned_create(dtype, path: str = CACHE_PATH, **geometry) -> DdmEngine:
     k = key(g["N"], g["T"], g["L"], g["NB"], dtype)
     tn = load_cache(path).get(k)
     return DdmEngine.create(**g, tuning=tn)
+
+
+def main(argv=None):
+    """Tune one geometry on a seeded synthetic video and (optionally) cache the result."""
+    import argparse
+    import torch  # noqa: F401  (device memory for the frame stack)
+    from .synth import GpuSynth
+    ap = argparse.ArgumentParser(description="autotune the engine's launch shapes for one geometry")
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--T", type=int, default=1024)
+    ap.add_argument("--L", type=int, default=0, help="lags (default T)")
+    ap.add_argument("--NB", type=int, default=256)
+    ap.add_argument("--dtype", default="uint16", choices=("uint8", "uint16", "float32"))
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--save", action="store_true", help=f"merge the winner into {CACHE_PATH}")
+    a = ap.parse_args(argv)
+    L = a.L or a.T
+    g = GpuSynth()
+    fr = g.video(g.params(a.T, a.N, max(200, a.N * a.N // 64), 2.0, 100.0, 200.0, D=0.3, alpha=0.6,
+                          vbar_px_frame=0.15, Z=3.0, dtype=a.dtype, seed=7))
+    geom = dict(N=a.N, T=a.T, L=L, NB=a.NB, kmin=0, kmax=a.N // 2)
+    best, table = autotune(geom, fr.data_ptr(), np.dtype(a.dtype), a.T, reps=a.reps, quick=a.quick,
+                           log=lambda m: print(m, flush=True))
+    print(json.dumps({"best": best, "key": key(a.N, a.T, L, a.NB, a.dtype)}), flush=True)
+    if a.save:
+        save_cache({key(a.N, a.T, L, a.NB, a.dtype): best})
+    return best
+
+
+if __name__ == "__main__":
+    main()

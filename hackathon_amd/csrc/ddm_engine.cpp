@@ -251,6 +251,8 @@ struct ddm_engine {
   float* radD = nullptr;
   float* radP = nullptr;
   float* Sbin = nullptr;   // NB*TPAD summed |Y|^2 per bin (frequency order)
+  float* G_part = nullptr; // items*KT direct Re G sums of the tail lags
+  int KT = 1;              // tail lags T-KT..T-1 summed directly (ceil(T/32))
   float* Gbin = nullptr;   // NB*L summed Re G per bin
   float* pbin = nullptr;   // NB*T summed p per bin
   float* Pbin = nullptr;   // NB summed P per bin
@@ -327,7 +329,7 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
-                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin};
+                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -373,6 +375,10 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->modes = (long)g.N * e->NH;
   e->logn = ilog2(g.N);
   e->TPAD = std::max(256, 1 << ilog2(2L * g.T));
+  // Lags whose valid-pair count is below T/32 take Re G from direct sums: the
+  // FFT route's error is ~ulp x sum_t |y|^2 whatever the lag, i.e. ~ulp x T/N(tau)
+  // relative to D(tau) there
+  e->KT = std::max(1, (g.T + 31) / 32);
   e->logt = ilog2(e->TPAD);
   auto bail = [&](int code) {
     g_create_error = e->err;
@@ -437,7 +443,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->p_part, nitems * g.T)) != hipSuccess || (he = dalloc(&e->P_part, nitems)) != hipSuccess ||
       (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess ||
       (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess ||
-      (he = dalloc(&e->Sbin, (size_t)g.NB * e->TPAD)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
+      (he = dalloc(&e->Sbin, (size_t)g.NB * e->TPAD)) != hipSuccess ||
+      (he = dalloc(&e->G_part, nitems * (size_t)e->KT)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
       (he = dalloc(&e->Pbin, g.NB)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: device allocation");
     return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
@@ -636,17 +643,18 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
                 if (e->logt <= 11)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
                   return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
                                                 e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->tw_f, e->S_part,
-                                                e->p_part, e->P_part, e->stream);
+                                                e->p_part, e->P_part, e->KT, e->G_part, e->stream);
                 return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
                                             identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part,
-                                            e->stream);
+                                            e->KT, e->G_part, e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "temporal launch");
   } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
                 // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
                 // sub-transforms and stores their spectra r-major
                 const int lm = e->logt <= 11 && e->logt > 9 ? e->logt - 9 : 0;
-                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->bin_items, e->g.NB,
+                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, e->KT,
+                                                e->bin_items, e->g.NB,
                                                 e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t,
                                                 e->Sbin, e->Gbin, e->pbin, e->Pbin, e->radD, e->radP, e->stream);
               })) != hipSuccess) {

@@ -15,16 +15,20 @@ hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int
                        hipStream_t st);
 hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                           float* S_part, float* p_part, float* P_part, hipStream_t st);
+                           float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st);
 hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                             const float2* tw_f, float* S_part, float* p_part, float* P_part, hipStream_t st);
-// lm: log2 of the r-major interleave of S_part rows (0 = frequency order);
-// full != 0: every frame valid (prefix-sum form of the masked correlations)
+                             const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
+                             hipStream_t st);
+// KT: the temporal kernels sum Re G of the last KT lags (T-KT..T-1) directly
+// into G_part[item][KT]; lm: log2 of the r-major interleave of S_part rows
+// (0 = frequency order); full != 0: every frame valid (prefix-sum form of the
+// masked correlations)
 hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const float* p_part, const float* P_part,
-                               const int* bin_items, int NB, const uint32_t* nmodes, const float* mask, int full,
-                               const float* npairs, int T, int L, float n0, const float2* tw, float* Sbin,
-                               float* Gbin, float* pbin, float* Pbin, float* radD, float* radP, hipStream_t st);
+                               const float* G_part, int KT, const int* bin_items, int NB, const uint32_t* nmodes,
+                               const float* mask, int full, const float* npairs, int T, int L, float n0,
+                               const float2* tw, float* Sbin, float* Gbin, float* pbin, float* Pbin, float* radD,
+                               float* radP, hipStream_t st);
 
 hipError_t launch_rows_w(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
                          const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);

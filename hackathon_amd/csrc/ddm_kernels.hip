@@ -328,7 +328,8 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
                                                  const int* __restrict__ bin_modes, const uint32_t* __restrict__ order,
                                                  const float* __restrict__ mask, int identity, float inv_n0,
                                                  const float2* __restrict__ tw_g, float* __restrict__ S_part,
-                                                 float* __restrict__ p_part, float* __restrict__ P_part) {
+                                                 float* __restrict__ p_part, float* __restrict__ P_part, int KT,
+                                                 float* __restrict__ G_part) {
   constexpr int n = 1 << LOGT;
   constexpr int KJ = n / NT;           // frequency bins per thread (n >= 256 enforced by the host: n >= 2T, T >= 2,
                                        // and the host pads n to at least NT)
@@ -357,6 +358,7 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 #pragma unroll
   for (int i = 0; i < KJ; i++) aS[i] = cS[i] = 0.f;
   float aP = 0.f, cP = 0.f;
+  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + tid (threads < KT), summed directly
   __syncthreads();
 
   for (int mi = 0; mi < count; mi++) {
@@ -401,6 +403,15 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
     }
     for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
     __syncthreads();
+    if (tid < KT) {
+      float g = 0.f;
+      for (int t = 0; t < KT - tid; t++) {
+        const float2 a = buf[t], b = buf[T - KT + tid + t];
+        g += a.x * b.x + a.y * b.y;
+      }
+      two_sum_add(Gt, Gtc, g);
+    }
+    __syncthreads();
     lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
 #pragma unroll
     for (int i = 0; i < KJ; i++) {
@@ -420,6 +431,7 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
   }
   float2 tp = block_sum2(make_float2(aP + cP, 0.f), red, tid);
   if (tid == 0) P_part[blockIdx.x] = tp.x;
+  if (tid < KT) G_part[(long)blockIdx.x * KT + tid] = Gt + Gtc;
 }
 
 // ----------------------------------------------------------------- k_finalize
@@ -447,7 +459,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
                                                       const float* __restrict__ mask, int identity, float inv_n0,
                                                       const float2* __restrict__ tw_g, const float2* __restrict__ tw_f,
                                                       float* __restrict__ S_part, float* __restrict__ p_part,
-                                                      float* __restrict__ P_part) {
+                                                      float* __restrict__ P_part, int KT, float* __restrict__ G_part) {
   constexpr int n = 1 << LOGT;
   constexpr int m = 1 << LM;
   constexpr int LOGF = LOGT - LM;  // sub-transform size
@@ -489,6 +501,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
 #pragma unroll
   for (int i = 0; i < TS; i++) pa[i] = pc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
+  float Gt = 0.f, Gtc = 0.f;   // this lane's tail lag T - KT + lane (lanes < KT)
   const int rounds = (count + 3) / 4;
   // next round's series, loaded while the current one is transformed (up to
   // n = 2048; above that the registers are needed by the split transforms)
@@ -571,6 +584,24 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
         y[q] = make_float2((y[q].x - mx) * mq, (y[q].y - my) * mq);
       }
     }
+    // Tail lags tau = T - KT + j (few valid pairs) directly: the FFT route's
+    // error is ~ulp x sum_t |y|^2, far above sum over the N(tau) <= KT pairs
+    // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer.
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const int t = lane + 64 * q;
+      if (t < KT) xch[t] = y[q];
+      if (t >= T - KT && t < T) xch[KT + t - (T - KT)] = y[q];
+    }
+    wave_sync();
+    if (lane < KT) {
+      float g = 0.f;
+      for (int t = 0; t < KT - lane; t++) {
+        const float2 a = xch[t], b = xch[KT + t + lane];
+        g += a.x * b.x + a.y * b.y;
+      }
+      two_sum_add(Gt, Gtc, g);
+    }
     if constexpr (LM == 0) {
       float2 v[PF];
 #pragma unroll
@@ -644,8 +675,15 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   }
   const float pw = wave_sum(aP + cP);
   if (lane == 0) red[wave] = pw;
+  // the 4 waves' tail lags meet in their (now idle) exchange buffers
+  if (lane < KT) reinterpret_cast<float*>(xch)[lane] = Gt + Gtc;
   __syncthreads();
   if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (tid < KT) {
+    const float* g0 = reinterpret_cast<const float*>(lds);
+    constexpr int WF = 2 * WS;
+    G_part[(long)blockIdx.x * KT + tid] = (g0[tid] + g0[WF + tid]) + (g0[2 * WF + tid] + g0[3 * WF + tid]);
+  }
 }
 
 // ----------------------------------------------------------------- radial finalisation
@@ -716,7 +754,8 @@ __global__ __launch_bounds__(NT) void k_bin_sum(const float* __restrict__ S_part
 // k_bin_ifft: one workgroup per bin, Re G(tau) = IFFT(sum |Y|^2)(tau) / n.
 template <int LOGT>
 __global__ __launch_bounds__(NT) void k_bin_ifft(const float* __restrict__ Sbin, int L, const float2* __restrict__ tw_g,
-                                                 float* __restrict__ Gbin) {
+                                                 const float* __restrict__ G_part, const int* __restrict__ bin_items,
+                                                 int T, int KT, float* __restrict__ Gbin) {
   constexpr int n = 1 << LOGT;
   extern __shared__ float2 lds[];
   float2* buf = lds;
@@ -729,7 +768,16 @@ __global__ __launch_bounds__(NT) void k_bin_ifft(const float* __restrict__ Sbin,
   __syncthreads();
   lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
   const float invn = 1.0f / n;
-  for (int tau = tid; tau < L; tau += NT) Gbin[(long)b * L + tau] = buf[tau].x * invn;
+  const int i0 = bin_items[b], i1 = bin_items[b + 1];
+  for (int tau = tid; tau < L; tau += NT) {
+    float g = buf[tau].x * invn;
+    if (tau >= T - KT) {   // tail lags: the direct sums of the temporal kernel
+      float s = 0.f, c = 0.f;
+      for (int it = i0; it < i1; it++) two_sum_add(s, c, G_part[(long)it * KT + tau - (T - KT)]);
+      g = s + c;
+    }
+    Gbin[(long)b * L + tau] = g;
+  }
 }
 
 // k_bin_lags_full: every frame valid (m = 1 on [0, T)).  Then
@@ -1127,25 +1175,27 @@ hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int
 template <int LOGT>
 static hipError_t launch_temporal_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                                     const uint32_t* order, const float* mask, int identity, float inv_n0,
-                                    const float2* tw, float* S_part, float* p_part, float* P_part, hipStream_t st) {
+                                    const float2* tw, float* S_part, float* p_part, float* P_part, int KT,
+                                    float* G_part, hipStream_t st) {
   constexpr int n = 1 << LOGT;
   size_t lds = sizeof(float2) * 2 * n + sizeof(float) * 2 * (NT / 64);
   auto kern = k_temporal<LOGT>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
-                     tw, S_part, p_part, P_part);
+                     tw, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
 }
 
 hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                           float* S_part, float* p_part, float* P_part, hipStream_t st) {
+                           float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st) {
+  if (KT < 1 || KT > NT || KT > T) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                    \
   case LG_:                                                                                                        \
     return launch_temporal_t<LG_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
-                                P_part, st);
+                                P_part, KT, G_part, st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
   }
@@ -1157,13 +1207,13 @@ template <int LOGT, int LM>
 static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
                                       const float2* tw, const float2* tw_f, float* S_part, float* p_part,
-                                      float* P_part, hipStream_t st) {
+                                      float* P_part, int KT, float* G_part, hipStream_t st) {
   size_t lds = sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + ((1 << LOGT) >> LM));
   auto kern = k_temporal_w<LOGT, LM>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
-                     tw, tw_f, S_part, p_part, P_part);
+                     tw, tw_f, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
 }
 
@@ -1171,12 +1221,15 @@ static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* ite
 // sub-transform length n/m (m = 1 for n <= 512, n/512 above).
 hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
-                             const float2* tw_f, float* S_part, float* p_part, float* P_part, hipStream_t st) {
+                             const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
+                             hipStream_t st) {
+  // KT tail lags: at most one per lane, and their samples must fit the exchange buffer
+  if (KT < 1 || KT > 64 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_, LM_)                                                                                              \
   case LG_:                                                                                                         \
     return launch_temporal_w_t<LG_, LM_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, tw_f, \
-                                         S_part, p_part, P_part, st);
+                                         S_part, p_part, P_part, KT, G_part, st);
     CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
 #undef CASE
   }
@@ -1186,7 +1239,8 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
 template <int LOGT>
 static hipError_t launch_bin_sum_ifft_t(const float* S_part, const float* p_part, const float* P_part,
                                        const int* bin_items, int NB, int T, int L, int lm, const float2* tw,
-                                       float* Sbin, float* Gbin, float* pbin, float* Pbin, hipStream_t st) {
+                                       const float* G_part, int KT, float* Sbin, float* Gbin, float* pbin,
+                                       float* Pbin, hipStream_t st) {
   constexpr int n = 1 << LOGT;
   hipLaunchKernelGGL(k_bin_sum<LOGT>, dim3(NB, n / NT + (T + NT - 1) / NT), dim3(NT), sizeof(float) * 2 * (NT / 64), st,
                      S_part, p_part, P_part, bin_items, T, lm, Sbin, pbin, Pbin);
@@ -1196,20 +1250,21 @@ static hipError_t launch_bin_sum_ifft_t(const float* S_part, const float* p_part
   auto kern = k_bin_ifft<LOGT>;
   e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, (const float*)Sbin, L, tw, Gbin);
+  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, (const float*)Sbin, L, tw, G_part, bin_items, T, KT, Gbin);
   return hipGetLastError();
 }
 
 hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const float* p_part, const float* P_part,
-                               const int* bin_items, int NB, const uint32_t* nmodes, const float* mask, int full,
-                               const float* npairs, int T, int L, float n0, const float2* tw, float* Sbin,
-                               float* Gbin, float* pbin, float* Pbin, float* radD, float* radP, hipStream_t st) {
+                               const float* G_part, int KT, const int* bin_items, int NB, const uint32_t* nmodes,
+                               const float* mask, int full, const float* npairs, int T, int L, float n0,
+                               const float2* tw, float* Sbin, float* Gbin, float* pbin, float* Pbin, float* radD,
+                               float* radP, hipStream_t st) {
   hipError_t e = hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                \
   case LG_:                                                                                                    \
-    e = launch_bin_sum_ifft_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, lm, tw, Sbin, Gbin, pbin, Pbin,   \
-                                   st);                                                                        \
+    e = launch_bin_sum_ifft_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, lm, tw, G_part, KT, Sbin, Gbin,   \
+                                   pbin, Pbin, st);                                                            \
     break;
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE

@@ -275,6 +275,63 @@ double orc_temporal_mode(const double *hist, long mode, int T, const uint32_t *o
   return s1 > 0.5 ? P : 0;
 }
 
+/* The same WK formulas with the mask and p transformed separately (three
+ * FP64 FFTs instead of the reference's packed z = m + i p): packing the O(1)
+ * mask with p = |x|^2 in one transform limits p's accuracy to ~ulp x (the
+ * mask's energy), which at T = 1024 and |x|^2 ~ 1e-9 is a 1e-4 relative error
+ * at the last lags; the split form keeps each term's error relative to its
+ * own magnitude.  It is the FP64 truth the engine is held to at full size;
+ * orc_temporal_mode keeps the reference's algorithm. */
+double orc_temporal_mode_split(const double *hist, long mode, int T, const uint32_t *order, const float *mask, int L,
+                               double *outD) {
+  int n = next_pow2(2 * T);
+  double *M = (double *)calloc(2 * n, sizeof(double));
+  double *Pp = (double *)calloc(2 * n, sizeof(double));
+  double *y = (double *)calloc(2 * n, sizeof(double));
+  long base = mode * T * 2;
+  for (int j = 0; j < T; j++) {
+    if (!mask[j]) continue;
+    uint32_t s = order[j];
+    double xr = hist[base + 2 * s], xi = hist[base + 2 * s + 1];
+    M[2 * j] = 1;
+    Pp[2 * j] = xr * xr + xi * xi;
+    y[2 * j] = xr;
+    y[2 * j + 1] = xi;
+  }
+  orc_fft(n, M, 0);
+  orc_fft(n, Pp, 0);
+  orc_fft(n, y, 0);
+  /* A(tau) = sum_t m(t) p(t + tau) = IFFT(conj(M) P); B(tau) = A(-tau); N(tau) = IFFT(|M|^2) */
+  double *A = (double *)calloc(2 * n, sizeof(double));
+  double *Nn = (double *)calloc(2 * n, sizeof(double));
+  for (int k = 0; k < n; k++) {
+    double mr = M[2 * k], mi = M[2 * k + 1], pr = Pp[2 * k], pi = Pp[2 * k + 1];
+    A[2 * k] = mr * pr + mi * pi;
+    A[2 * k + 1] = mr * pi - mi * pr;
+    Nn[2 * k] = mr * mr + mi * mi;
+    double yr = y[2 * k], yi = y[2 * k + 1];
+    y[2 * k] = yr * yr + yi * yi;
+    y[2 * k + 1] = 0;
+  }
+  orc_fft(n, A, 1);
+  orc_fft(n, Nn, 1);
+  orc_fft(n, y, 1);
+  double inv = 1.0 / n;
+  double n0 = round(Nn[0] * inv), P = 0;
+  if (n0 > 0.5) P = A[0] * inv / n0;
+  for (int tau = 0; tau < L; tau++) {
+    double Av = A[2 * tau] * inv, Bv = A[2 * ((n - tau) % n)] * inv;
+    double Np = round(Nn[2 * tau] * inv);
+    outD[tau] = Np > 0 ? (Av + Bv - 2 * y[2 * tau] * inv) / Np : 0;
+  }
+  free(M);
+  free(Pp);
+  free(y);
+  free(A);
+  free(Nn);
+  return P;
+}
+
 /* Direct O(T*L) masked sums; src/ddm_ref.js:306-333. */
 double orc_temporal_mode_direct(const double *hist, long mode, int T, const uint32_t *order, const float *mask, int L,
                                 double *outD, double *outC) {
@@ -504,6 +561,26 @@ long orc_make_bins(int N, int NB, double kmin, double kmax, int hypot_mode, int 
 /* analyzeRadial: temporalMode over binned modes, accumulate, divide by the
  * bin's mode count; src/ddm_ref.js:360-437 (the drift sums are not part of
  * this path and are not computed).  D[b*L+tau], P[b]. */
+void orc_analyze_radial_split(const double *hist, int T, const uint32_t *order, const float *mask,
+                              const uint32_t *binOffsets, const int32_t *binModes, int NB, int L, double *D,
+                              double *P) {
+  double *d = (double *)malloc(sizeof(double) * L);
+  memset(D, 0, sizeof(double) * NB * L);
+  memset(P, 0, sizeof(double) * NB);
+  for (int b = 0; b < NB; b++) {
+    for (uint32_t i = binOffsets[b]; i < binOffsets[b + 1]; i++) {
+      double p = orc_temporal_mode_split(hist, binModes[4 * i], T, order, mask, L, d);
+      P[b] += p;
+      for (int t = 0; t < L; t++) D[b * L + t] += d[t];
+    }
+    uint32_t n = binOffsets[b + 1] - binOffsets[b];
+    if (!n) continue;
+    P[b] /= n;
+    for (int t = 0; t < L; t++) D[b * L + t] /= n;
+  }
+  free(d);
+}
+
 void orc_analyze_radial(const double *hist, int T, const uint32_t *order, const float *mask, const uint32_t *binOffsets,
                         const int32_t *binModes, int NB, int L, double *D, double *P) {
   double *d = (double *)malloc(sizeof(double) * L);

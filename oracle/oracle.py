@@ -51,6 +51,7 @@ def lib():
                                     C.c_long, _dp, _u32p, C.c_void_p]
         L.orc_make_bins.restype = C.c_long
         L.orc_analyze_radial.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
+        L.orc_analyze_radial_split.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
         L.orc_analyze_radial_direct.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp,
                                                 _dp]
         for name in ("orc_bessel_j0", "orc_lngamma"):
@@ -165,14 +166,20 @@ def make_bins(N: int, NB: int, kmin: float, kmax: float, hypot_mode: int = 0, kn
                 kCenter=kc, nModes=nm, kNodes=kn if with_nodes else None)
 
 
-def analyze_radial(hist: np.ndarray, order, mask, bins: dict, L: int, direct: bool = False):
-    """Bin-averaged D[NB, L] and P[NB] (FP64); hist complex (modes, T)."""
+def analyze_radial(hist: np.ndarray, order, mask, bins: dict, L: int, direct: bool = False, split: bool = False):
+    """Bin-averaged D[NB, L] and P[NB] (FP64); hist complex (modes, T).
+
+    Default: the reference's algorithm (packed z = m + i p transform).  split:
+    mask and p transformed separately (the FP64 truth at long series, where
+    the packed form's p inherits ~ulp x the mask's energy).  direct: O(T L)
+    sums."""
     h = np.ascontiguousarray(hist, dtype=np.complex128).view(np.float64).ravel()
     T = hist.shape[1]
     NB = bins["NB"]
     D = np.zeros(NB * L)
     P = np.zeros(NB)
-    f = lib().orc_analyze_radial_direct if direct else lib().orc_analyze_radial
+    f = lib().orc_analyze_radial_direct if direct else (lib().orc_analyze_radial_split if split else
+                                                         lib().orc_analyze_radial)
     f(h, T, np.ascontiguousarray(order, dtype=np.uint32), np.ascontiguousarray(mask, dtype=np.float32),
       bins["binOffsets"], np.ascontiguousarray(bins["binModes"].ravel()), NB, L, D, P)
     return D.reshape(NB, L), P

@@ -302,3 +302,39 @@ def test_nnls3_kkt():
         g = G @ a - h
         assert np.all(a >= 0)
         assert np.all(np.where(a > 1e-12, np.abs(g) < 1e-9, g > -1e-9))
+
+
+def test_split_transform_matches_direct_sums():
+    """The split FP64 WK (mask and p transformed separately) against the direct
+    O(T L) sums, with masks and a shuffled order, including the last lags."""
+    rng = np.random.default_rng(21)
+    T, L, N = 300, 300, 32
+    fr = rng.integers(0, 255, (T, N, N)).astype(np.uint8)
+    h = O.history(fr, T)
+    bins = O.make_bins(N, 8, 0, 16, with_nodes=False)
+    order = rng.permutation(T).astype(np.uint32)
+    mask = (rng.uniform(size=T) > 0.2).astype(np.float32)
+    Ds, Ps = O.analyze_radial(h, order, mask, bins, L, split=True)
+    Dd, Pd = O.analyze_radial(h, order, mask, bins, L, direct=True)
+    ok = bins["nModes"] > 0
+    scale = np.max(np.abs(Dd[ok]), axis=1, keepdims=True)
+    assert np.max(np.abs(Ds[ok] - Dd[ok]) / scale) < 1e-12
+    assert np.max(np.abs(Ps[ok] - Pd[ok]) / Pd[ok]) < 1e-12
+
+
+def test_packed_transform_loses_p_precision_at_small_power():
+    """The reference's packed z = m + i p transform: with |x|^2 << 1 the last
+    lags' A, B carry ~ulp x (mask energy) errors; the split form does not."""
+    rng = np.random.default_rng(3)
+    T, L = 1024, 1024
+    x = (rng.standard_normal((4, T)) + 1j * rng.standard_normal((4, T))) * 3e-5
+    h = x.astype(np.complex128)
+    bins = dict(NB=1, binOffsets=np.array([0, 4], np.uint32),
+                binModes=np.array([[m, 0, 0, 0] for m in range(4)], np.int32), nModes=np.array([4], np.uint32))
+    order, mask = np.arange(T, dtype=np.uint32), np.ones(T, np.float32)
+    Dp, _ = O.analyze_radial(h, order, mask, bins, L)
+    Ds, _ = O.analyze_radial(h, order, mask, bins, L, split=True)
+    Dd, _ = O.analyze_radial(h, order, mask, bins, L, direct=True)
+    err_split = np.abs(Ds[0, -8:] - Dd[0, -8:]) / np.max(Dd[0])
+    err_packed = np.abs(Dp[0, -8:] - Dd[0, -8:]) / np.max(Dd[0])
+    assert err_split.max() < 1e-11 and err_packed.max() > 100 * err_split.max()

@@ -95,3 +95,16 @@ def test_short_training_lowers_validation_loss(gpu):
     # predicting the training mean scores 0.5 sum(wt) on standardised targets
     assert min(h[1] for h in hist) < 0.5 * float(np.sum(S.target_weights()))
     assert np.all(np.isfinite(params))
+
+
+def test_autotune_candidates_and_cache(tmp_path):
+    from hackathon_amd import autotune as AT
+    c = AT.candidates(512)
+    assert len(c) == len({tuple(sorted(x.items())) for x in c})          # no duplicates
+    assert all(x["cols_f"] in (8, 16) and x["wave_spatial"] in (1, 2) for x in c)
+    assert all(x["wave_spatial"] == 1 for x in AT.candidates(256))       # wavefront spatial only at 512/1024
+    path = str(tmp_path / "cache.json")
+    AT.save_cache({AT.key(512, 1024, 1024, 256, "uint16"): c[0]}, path)
+    AT.save_cache({AT.key(256, 64, 64, 32, "uint8"): c[1]}, path)
+    got = AT.load_cache(path)
+    assert got[AT.key(512, 1024, 1024, 256, "uint16")] == c[0] and len(got) == 2
