@@ -358,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 #pragma unroll
   for (int i = 0; i < KJ; i++) aS[i] = cS[i] = 0.f;
   float aP = 0.f, cP = 0.f;
-  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + tid (threads < KT), summed directly
+  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + tid / 4 (threads 4 j, j < KT), summed directly
   __syncthreads();
 
   for (int mi = 0; mi < count; mi++) {
@@ -403,13 +403,18 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
     }
     for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
     __syncthreads();
-    if (tid < KT) {
+    {
+      // 4 threads per lag (KT <= 64): thread 4 j + s sums pairs t = s, s + 4, ...
+      const int j = tid >> 2, s4 = tid & 3;
       float g = 0.f;
-      for (int t = 0; t < KT - tid; t++) {
-        const float2 a = buf[t], b = buf[T - KT + tid + t];
-        g += a.x * b.x + a.y * b.y;
-      }
-      two_sum_add(Gt, Gtc, g);
+      if (j < KT)
+        for (int t = s4; t < KT - j; t += 4) {
+          const float2 a = buf[t], b = buf[T - KT + j + t];
+          g += a.x * b.x + a.y * b.y;
+        }
+      g += __shfl_xor(g, 1, 64);
+      g += __shfl_xor(g, 2, 64);
+      if (s4 == 0 && j < KT) two_sum_add(Gt, Gtc, g);
     }
     __syncthreads();
     lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
   }
   float2 tp = block_sum2(make_float2(aP + cP, 0.f), red, tid);
   if (tid == 0) P_part[blockIdx.x] = tp.x;
-  if (tid < KT) G_part[(long)blockIdx.x * KT + tid] = Gt + Gtc;
+  if ((tid & 3) == 0 && (tid >> 2) < KT) G_part[(long)blockIdx.x * KT + (tid >> 2)] = Gt + Gtc;
 }
 
 // ----------------------------------------------------------------- k_finalize
@@ -501,7 +506,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
 #pragma unroll
   for (int i = 0; i < TS; i++) pa[i] = pc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
-  float Gt = 0.f, Gtc = 0.f;   // this lane's tail lag T - KT + lane (lanes < KT)
+  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / 4 (lanes 4 j, j < KT)
   const int rounds = (count + 3) / 4;
   // next round's series, loaded while the current one is transformed (up to
   // n = 2048; above that the registers are needed by the split transforms)
@@ -594,13 +599,18 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
       if (t >= T - KT && t < T) xch[KT + t - (T - KT)] = y[q];
     }
     wave_sync();
-    if (lane < KT) {
+    {
+      // 4 lanes per lag (KT <= 16): lane 4 j + s sums pairs t = s, s + 4, ...
+      const int j = lane >> 2, s4 = lane & 3;
       float g = 0.f;
-      for (int t = 0; t < KT - lane; t++) {
-        const float2 a = xch[t], b = xch[KT + t + lane];
-        g += a.x * b.x + a.y * b.y;
-      }
-      two_sum_add(Gt, Gtc, g);
+      if (j < KT)
+        for (int t = s4; t < KT - j; t += 4) {
+          const float2 a = xch[t], b = xch[KT + t + j];
+          g += a.x * b.x + a.y * b.y;
+        }
+      g += __shfl_xor(g, 1, 64);
+      g += __shfl_xor(g, 2, 64);
+      if (s4 == 0 && j < KT) two_sum_add(Gt, Gtc, g);
     }
     if constexpr (LM == 0) {
       float2 v[PF];
@@ -676,7 +686,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   const float pw = wave_sum(aP + cP);
   if (lane == 0) red[wave] = pw;
   // the 4 waves' tail lags meet in their (now idle) exchange buffers
-  if (lane < KT) reinterpret_cast<float*>(xch)[lane] = Gt + Gtc;
+  if ((lane & 3) == 0 && (lane >> 2) < KT) reinterpret_cast<float*>(xch)[lane >> 2] = Gt + Gtc;
   __syncthreads();
   if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
   if (tid < KT) {
@@ -1190,7 +1200,7 @@ static hipError_t launch_temporal_t(const float2* hist, int T, const int4* items
 hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                            float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st) {
-  if (KT < 1 || KT > NT || KT > T) return hipErrorInvalidValue;
+  if (KT < 1 || KT > NT / 4 || KT > T) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                    \
   case LG_:                                                                                                        \
@@ -1223,8 +1233,8 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st) {
-  // KT tail lags: at most one per lane, and their samples must fit the exchange buffer
-  if (KT < 1 || KT > 64 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
+  // KT tail lags: four lanes each, and their samples must fit the exchange buffer
+  if (KT < 1 || KT > 16 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_, LM_)                                                                                              \
   case LG_:                                                                                                         \
