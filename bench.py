@@ -211,8 +211,10 @@ def main():
     raw_bytes = T * N * N * 2
     per_step_algo = {"spatial_rows": raw_bytes, "spatial_cols_cornerturn": spec_bytes, "temporal_wk": spec_bytes,
                      "radial_finalize": 0}
-    kernel_symbol = {"spatial_rows": "ddm::k_rows_w<9, unsigned short>", "spatial_cols_cornerturn": "ddm::k_cols_w<9, 16>",
-                     "temporal_wk": "ddm::k_temporal_w<11, 2>", "radial_finalize": "ddm::k_bin_reduce<11>"}
+    cols_f = engs[0].tuning()["cols_f"]
+    kernel_symbol = {"spatial_rows": "ddm::k_rows_w<9, unsigned short>",
+                     "spatial_cols_cornerturn": f"ddm::k_cols_w<9, {cols_f}>",
+                     "temporal_wk": "ddm::k_temporal_w<11, 2>", "radial_finalize": "ddm::k_bin_sum<11>"}
     per_kernel = {}
     for name, (ms, cnt) in times.items():
         per_kernel[name] = {"ms_per_step": ms / prof_steps, "launches_per_step": cnt / prof_steps,
@@ -230,7 +232,7 @@ def main():
     kern = {"temporal_wk": "k_temporal_w (masked WK: gather, mean removal, TPAD FFT, |Y|^2, per-bin partials)",
             "spatial_cols_cornerturn": "k_cols_w (column FFT + corner turn into the history)",
             "spatial_rows": "k_rows_w (decode, row-mean removal, window, R2C row FFT)",
-            "radial_finalize": "k_bin_reduce + k_bin_lags"}[dom]
+            "radial_finalize": "k_bin_sum + k_bin_ifft + k_bin_lags(_full)"}[dom]
     pipeline_gbs = (raw_bytes + 2 * spec_bytes) / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
