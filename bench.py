@@ -224,7 +224,10 @@ def main():
     algo_launch = per_step_algo[dom] / launches
     achieved = algo_launch / (per_kernel[dom]["avg_launch_ms"] * 1e-3) / 1e9
     traffic = None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c2_1024x512_hbm_traffic.json")
+    # calibrated FETCH_SIZE + WRITE_SIZE per launch of this workload (rocprofv3 passes summarised in
+    # profiles/c2_1024x512_hbm_traffic.md)
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hackathon_amd", "data",
+                        "hbm_traffic_1024x512_u16.json")
     if (T, N, NB) == (1024, 512, 256) and os.path.exists(prof):
         k = json.load(open(prof))["kernels"].get(kernel_symbol[dom])
         if k:
