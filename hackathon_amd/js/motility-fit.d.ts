@@ -1,0 +1,19 @@
+// Type declarations of the per-q motility fit module (motility-fit.mjs).
+import { Radial } from './ddm-engine';
+
+export declare const MODEL: Readonly<{ BROWNIAN: 0; SWIMMER_3D: 1 }>;
+export declare const SWIM_MIN_UM_S: number;
+
+export interface PreparedBin { b: number; q: number; qn: Float64Array; d: Float64Array; w: number }
+export interface Prepared { bins: PreparedBin[]; lags: number[]; tau: Float64Array; dt: number; L: number; tauMax: number }
+
+export declare function lagGrid(L: number): number[];
+export declare function prepare(radial: Radial, dxUm: number, fps: number): Prepared;
+export declare function bounds(pr: Prepared): [number, number, number, number];
+
+export interface BinFit { b: number; q: number; A: number; B: number; alpha: number; vbar: number; Z: number; D: number; chi2: number; iters: number }
+export declare function analyzeMotilityLM(radial: Radial, opt?: {
+  dxUm?: number; fps?: number; model?: number; nstarts?: number; maxIter?: number;
+  /** 6 per usable bin: A, B, alpha, ln vbar, ln Z, ln D */
+  seeds?: Float64Array; nnlsAmplitudes?: boolean;
+}): { pr: Prepared; fits: BinFit[]; kernelMs: number };

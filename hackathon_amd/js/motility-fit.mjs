@@ -1,0 +1,105 @@
+// Per-q motility fits for the TypeScript-facing API (ES module, ES2019).
+//
+// prepare / bounds / lagGrid keep the reference's data preparation
+// (motility-ddm src/motility_fit.js:358-369, :383-401, :480-491): usable bins
+// (nModes >= 3, positive maximum), the quasi-logarithmic lag grid, q and the
+// q-nodes from kCenter / kNodes, weight w = 1 / max D.  analyzeMotilityLM runs
+// every usable bin of a Radial record as one batched Levenberg-Marquardt launch
+// on the MI355X (libddm.so ddm_fit through the N-API addon) and returns the
+// fitted per-q parameters.  Types are declared in motility-fit.d.ts.
+import { createRequire } from 'module';
+import { fileURLToPath } from 'url';
+import path from 'path';
+
+const here = path.dirname(fileURLToPath(import.meta.url));
+const require = createRequire(import.meta.url);
+const addon = require(path.join(here, '..', 'napi', 'ddm_napi.node'));
+
+export const MODEL = Object.freeze({ BROWNIAN: 0, SWIMMER_3D: 1 });
+const SEED = Object.freeze({ GRID: 0, GIVEN: 1, NONLINEAR: 2 });
+/** GATES.SWIM_MIN_UM_S (src/motility_fit.js:37) */
+export const SWIM_MIN_UM_S = 2;
+const pick = (a, b) => (a === undefined || a === null ? b : a);
+
+/** Lags 1..10, then steps of x1.08 (src/motility_fit.js:358-369). */
+export function lagGrid(L) {
+  const lags = [];
+  for (let t = 1; t <= Math.min(10, L - 1); t++) lags.push(t);
+  let x = 10;
+  for (;;) {
+    x *= 1.08;
+    const t = Math.round(x);
+    if (t > L - 1) break;
+    if (t > lags[lags.length - 1]) lags.push(t);
+  }
+  return lags;
+}
+
+/** prepare(radial, dxUm, fps) (src/motility_fit.js:383-401). */
+export function prepare(radial, dxUm, fps) {
+  const { N, NB, L } = radial;
+  const dt = 1 / fps;
+  const lags = lagGrid(L);
+  const tau = Float64Array.from(lags, (t) => t * dt);
+  const qs = (2 * Math.PI) / (N * dxUm);
+  const K = radial.kNodes && radial.kNodes.length ? radial.kNodes.length / NB : 0;
+  const bins = [];
+  for (let b = 0; b < NB; b++) {
+    if (radial.nModes[b] < 3) continue;
+    const d = Float64Array.from(lags, (t) => radial.D[b * L + t]);
+    let scale = -Infinity;
+    for (const v of d) scale = Math.max(scale, v);
+    if (!(scale > 0) || !Number.isFinite(scale)) continue;
+    const q = qs * radial.kCenter[b];
+    const qn = K ? Float64Array.from(radial.kNodes.subarray(b * K, (b + 1) * K), (k) => k * qs) : Float64Array.of(q);
+    bins.push({ b, q, qn, d, w: 1 / scale });
+  }
+  return { bins, lags, tau, dt, L, tauMax: (L - 1) * dt };
+}
+
+/** vmin, vmax, Dmin, Dmax (src/motility_fit.js:480-491). */
+export function bounds(pr) {
+  let qmin = Infinity, qmax = -Infinity;
+  for (const b of pr.bins) { qmin = Math.min(qmin, b.q); qmax = Math.max(qmax, b.q); }
+  return [Math.max(SWIM_MIN_UM_S, 0.3 / (qmax * pr.tauMax)), 3 / (qmin * pr.dt),
+    0.01 / (qmax * qmax * pr.tauMax), 10 / (qmin * qmin * pr.dt)];
+}
+
+let fitter = null;
+
+/**
+ * Per-q LM fits of every usable bin of one Radial record.
+ * opt: {dxUm, fps, model (MODEL.*), nstarts, maxIter, seeds (Float64Array 6 per usable bin), nnlsAmplitudes}
+ * -> {pr, fits: [{b, q, A, B, alpha, vbar, Z, D, chi2, iters}], kernelMs}
+ */
+export function analyzeMotilityLM(radial, opt = {}) {
+  const pr = prepare(radial, pick(opt.dxUm, 1), pick(opt.fps, 30));
+  const n = pr.bins.length;
+  if (!n) return { pr, fits: [], kernelMs: 0 };
+  const ntauMax = pr.tau.length;
+  const K = Math.max(...pr.bins.map((b) => b.qn.length));
+  const bd = bounds(pr);
+  const batch = {
+    nfits: n, ntauMax, knodes: K, model: pick(opt.model, MODEL.SWIMMER_3D),
+    seedMode: opt.seeds ? (opt.nnlsAmplitudes ? SEED.NONLINEAR : SEED.GIVEN) : SEED.GRID,
+    nstarts: pick(opt.nstarts, 3), maxIter: pick(opt.maxIter, 100),
+    ntau: new Int32Array(n).fill(ntauMax), tau: new Float64Array(n * ntauMax), data: new Float64Array(n * ntauMax),
+    weight: new Float64Array(n), qnodes: new Float64Array(n * K), bounds: new Float64Array(4 * n),
+    seeds: opt.seeds || null,
+  };
+  pr.bins.forEach((bin, i) => {
+    batch.tau.set(pr.tau, i * ntauMax);
+    batch.data.set(bin.d, i * ntauMax);
+    batch.weight[i] = bin.w;
+    for (let k = 0; k < K; k++) batch.qnodes[i * K + k] = bin.qn[k % bin.qn.length];
+    batch.bounds.set(bd, 4 * i);
+  });
+  if (!fitter) fitter = addon.fitterCreate();
+  const r = addon.fit(fitter, batch);
+  const fits = pr.bins.map((bin, i) => {
+    const t = r.theta.subarray(6 * i, 6 * i + 6);
+    return { b: bin.b, q: bin.q, A: t[0], B: t[1], alpha: t[2], vbar: Math.exp(t[3]), Z: Math.exp(t[4]),
+      D: Math.exp(t[5]), chi2: r.chi2[i], iters: r.iters[i] };
+  });
+  return { pr, fits, kernelMs: r.kernelMs };
+}

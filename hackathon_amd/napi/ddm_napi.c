@@ -309,6 +309,114 @@ static napi_value js_idle(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* ------------------------------------------------------------------ per-q fits */
+
+static void finalize_fitter(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  ddm_fitter_destroy((ddm_fitter *)data);
+}
+
+/* fitterCreate() -> external handle of a ddm_fitter */
+static napi_value js_fitter_create(napi_env env, napi_callback_info info) {
+  (void)info;
+  ddm_fitter *f = NULL;
+  if (ddm_fitter_create(&f) != DDM_OK) {
+    napi_throw_error(env, NULL, "ddm_fitter_create failed");
+    return NULL;
+  }
+  napi_value out;
+  NAPI_CALL(env, napi_create_external(env, f, finalize_fitter, NULL, &out));
+  return out;
+}
+
+static int get_typed(napi_env env, napi_value obj, const char *name, napi_typedarray_type want, void **data,
+                     size_t *len, int required) {
+  napi_value v;
+  bool has = false;
+  *data = NULL;
+  *len = 0;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return !required;
+  if (napi_get_named_property(env, obj, name, &v) != napi_ok) return 0;
+  napi_valuetype vt;
+  napi_typeof(env, v, &vt);
+  if (vt == napi_undefined || vt == napi_null) return !required;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &t, len, data, &ab, &off) != napi_ok || t != want) return 0;
+  return 1;
+}
+
+/* fit(fitter, {nfits, ntauMax, knodes, model, seedMode, nstarts, maxIter, ntau: Int32Array,
+ *              tau, data, weight, qnodes, bounds: Float64Array, seeds?: Float64Array})
+ *   -> {theta: Float64Array(6 nfits), chi2: Float64Array, iters: Int32Array, kernelMs} */
+static napi_value js_fit(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2], out;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  void *fp = NULL;
+  if (argc < 2 || napi_get_value_external(env, argv[0], &fp) != napi_ok || !fp) {
+    napi_throw_type_error(env, NULL, "fit(fitter, batch)");
+    return NULL;
+  }
+  ddm_fitter *f = (ddm_fitter *)fp;
+  ddm_fit_batch b;
+  memset(&b, 0, sizeof b);
+  b.nfits = (int32_t)get_prop_num(env, argv[1], "nfits", 0);
+  b.ntau_max = (int32_t)get_prop_num(env, argv[1], "ntauMax", 0);
+  b.knodes = (int32_t)get_prop_num(env, argv[1], "knodes", 1);
+  b.model = (int32_t)get_prop_num(env, argv[1], "model", DDM_MODEL_SWIMMER_3D);
+  b.seed_mode = (int32_t)get_prop_num(env, argv[1], "seedMode", DDM_SEED_GRID);
+  b.nstarts = (int32_t)get_prop_num(env, argv[1], "nstarts", 3);
+  b.max_iter = (int32_t)get_prop_num(env, argv[1], "maxIter", 100);
+  void *ntau, *tau, *data, *w, *qn, *bd, *sd;
+  size_t n0, n1, n2, n3, n4, n5, n6;
+  if (!get_typed(env, argv[1], "ntau", napi_int32_array, &ntau, &n0, 1) ||
+      !get_typed(env, argv[1], "tau", napi_float64_array, &tau, &n1, 1) ||
+      !get_typed(env, argv[1], "data", napi_float64_array, &data, &n2, 1) ||
+      !get_typed(env, argv[1], "weight", napi_float64_array, &w, &n3, 1) ||
+      !get_typed(env, argv[1], "qnodes", napi_float64_array, &qn, &n4, 1) ||
+      !get_typed(env, argv[1], "bounds", napi_float64_array, &bd, &n5, 1) ||
+      !get_typed(env, argv[1], "seeds", napi_float64_array, &sd, &n6, 0)) {
+    napi_throw_type_error(env, NULL, "fit: batch arrays must be typed (Int32Array ntau, Float64Array others)");
+    return NULL;
+  }
+  size_t n = (size_t)(b.nfits > 0 ? b.nfits : 0);
+  if (n0 < n || n1 < n * b.ntau_max || n2 < n * b.ntau_max || n3 < n || n4 < n * b.knodes || n5 < 4 * n ||
+      (sd && n6 < 6 * n)) {
+    napi_throw_range_error(env, NULL, "fit: array shorter than nfits x its row size");
+    return NULL;
+  }
+  b.ntau = (const int32_t *)ntau;
+  b.tau = (const double *)tau;
+  b.data = (const double *)data;
+  b.weight = (const double *)w;
+  b.qnodes = (const double *)qn;
+  b.bounds = (const double *)bd;
+  b.seeds = (const double *)sd;
+  void *th, *chi, *it;
+  napi_value vth = make_typed(env, napi_float64_array, 6 * (n ? n : 1), 8, &th);
+  napi_value vchi = make_typed(env, napi_float64_array, n ? n : 1, 8, &chi);
+  napi_value vit = make_typed(env, napi_int32_array, n ? n : 1, 4, &it);
+  if (!vth || !vchi || !vit) {
+    napi_throw_error(env, NULL, "fit: allocation failed");
+    return NULL;
+  }
+  double ms = 0;
+  int rc = ddm_fit(f, &b, (double *)th, (double *)chi, (int32_t *)it, &ms);
+  if (rc != DDM_OK) {
+    napi_throw_error(env, NULL, ddm_fitter_last_error(f));
+    return NULL;
+  }
+  NAPI_CALL(env, napi_create_object(env, &out));
+  set_prop(env, out, "theta", vth);
+  set_prop(env, out, "chi2", vchi);
+  set_prop(env, out, "iters", vit);
+  set_prop(env, out, "kernelMs", num(env, ms));
+  return out;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
@@ -320,6 +428,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readRadial", NULL, js_read_radial, NULL, NULL, NULL, napi_default, NULL},
       {"readHistory", NULL, js_read_history, NULL, NULL, NULL, napi_default, NULL},
       {"idle", NULL, js_idle, NULL, NULL, NULL, napi_default, NULL},
+      {"fitterCreate", NULL, js_fitter_create, NULL, NULL, NULL, napi_default, NULL},
+      {"fit", NULL, js_fit, NULL, NULL, NULL, napi_default, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

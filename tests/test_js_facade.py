@@ -47,3 +47,45 @@ def test_facade_parity(gpu, tmp_path):
     assert np.max(np.abs(D[ok] - refD[ok]) / scale) < 1e-5
     assert np.max(np.abs(P[ok] - refP[ok]) / refP[ok]) < 1e-5
     assert out["nModes"] == [int(x) for x in bins["nModes"]]
+
+
+def test_js_lag_grid_matches_python():
+    """lagGrid of the TS-facing module against the Python host restatement (no GPU work)."""
+    code = ("import('" + os.path.join(ROOT, "hackathon_amd", "js", "motility-fit.mjs") +
+            "').then(m => console.log('RESULT ' + JSON.stringify([2, 11, 64, 128, 1024, 4096].map(L => m.lagGrid(L)))))")
+    if not os.path.exists(ADDON):
+        pytest.fail("N-API addon not built (python -m hackathon_amd.napi.build_addon)")
+    r = subprocess.run(["node", "--input-type=module", "-e", code], capture_output=True, text=True, timeout=60)
+    line = [x for x in r.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, r.stdout + r.stderr
+    from hackathon_amd.fit import lag_grid
+    for L, lags in zip([2, 11, 64, 128, 1024, 4096], json.loads(line[0][7:])):
+        assert lags == list(lag_grid(L)), L
+
+
+@pytest.mark.gpu
+def test_facade_frames_to_fits(gpu, tmp_path):
+    """Frame stack in, fitted per-q parameters out, through the TS-facing API only; identical to the
+    Python path on the same radial."""
+    from hackathon_amd import synth
+    from hackathon_amd.engine import Radial
+    from hackathon_amd import fit as F
+    fr = synth.config1()
+    T = L = 64
+    (tmp_path / "frames.bin").write_bytes(fr.tobytes())
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "fit_run.mjs"), str(tmp_path), "128", str(T),
+                        str(L), "64", "0", "64", "30"], capture_output=True, text=True, timeout=180)
+    line = [x for x in r.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, r.stdout + r.stderr
+    out = json.loads(line[-1][7:])
+    assert "error" not in out, out
+    th = np.fromfile(tmp_path / "fit.f64").reshape(-1, 3)
+    D = np.fromfile(tmp_path / "radD.f64")
+    from hackathon_amd.engine import DdmEngine
+    with DdmEngine.create(N=128, T=T, L=L, NB=64, kmin=0, kmax=64) as e:   # the same geometry's bin tables
+        bins = e.bins
+    rad = Radial(128, 64, L, bins.kCenter, bins.kNodes, bins.nModes, D, np.zeros(64), np.zeros(1))
+    pr, res = F.analyzeMotilityLM(rad, 1.0, 30.0, model=F.BROWNIAN)
+    assert [b.b for b in pr.bins] == out["bins"] and len(out["bins"]) == out["nPrepared"] > 0
+    assert np.allclose(res.chi2, out["chi2"], rtol=1e-12, atol=0)
+    assert np.allclose(res.theta[:, 0], th[:, 0], rtol=1e-12) and np.allclose(res.D, th[:, 2], rtol=1e-12)
