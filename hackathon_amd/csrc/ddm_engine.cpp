@@ -261,6 +261,11 @@ struct ddm_engine {
   // no host waits, the pinned metadata staging then belongs to the graph
   bool capturing = false;
   ddm_tuning tune{};
+  // the last computeTemporal's metadata, for readTemporalFull
+  bool meta_valid = false, last_identity = true;
+  float last_n0 = 0.f;
+  float *fullD = nullptr, *fullP = nullptr;   // per-mode outputs of one chunk of modes
+  float2* fullC = nullptr;
   // pinned staging of the per-analysis metadata (order, 0/1 mask, N(tau));
   // meta_ev marks when the last upload from it finished
   uint32_t* meta_order = nullptr;
@@ -329,7 +334,8 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
-                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part};
+                  e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part,
+                  e->fullD, e->fullC, e->fullP};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -632,6 +638,9 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
     np[tau] = (float)c;
   }
   const float n0 = np[0];
+  e->meta_valid = true;
+  e->last_identity = identity;
+  e->last_n0 = n0;
   int rc = DDM_OK;
   if ((he = hipMemcpyAsync(e->order, e->meta_order, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
       (he = hipMemcpyAsync(e->mask, m01, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
@@ -679,6 +688,37 @@ int ddm_read_history(ddm_engine* e, float* out) {
   if (!out) return fail(e, DDM_EINVAL, "readHistory: null output");
   HIPCHK(e, hipStreamSynchronize(e->stream), "readHistory: stream");
   HIPCHK(e, hipMemcpy(out, e->hist, (size_t)e->modes * e->g.T * 8, hipMemcpyDeviceToHost), "readHistory: copy");
+  return DDM_OK;
+}
+
+int ddm_read_temporal_full(ddm_engine* e, float* D, float* C, float* P) {
+  if (!e) return DDM_EINVAL;
+  if (!e->meta_valid) return fail(e, DDM_ESTATE, "readTemporalFull: computeTemporal has not run");
+  const int T = e->g.T, L = e->g.L;
+  const long MC = 8192;   // modes per launch: bounds the device scratch to MC * L * 12 bytes
+  hipError_t he;
+  if (!e->fullD) {
+    if ((he = dalloc(&e->fullD, (size_t)MC * L)) != hipSuccess || (he = dalloc(&e->fullC, (size_t)MC * L)) != hipSuccess ||
+        (he = dalloc(&e->fullP, (size_t)MC)) != hipSuccess)
+      return hip_fail(e, he, "readTemporalFull: device scratch");
+  }
+  const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
+  for (long m0 = 0; m0 < e->modes; m0 += MC) {
+    const int nm = (int)std::min<long>(MC, e->modes - m0);
+    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, L, m0, nm, e->order, e->mask, e->last_identity ? 1 : 0,
+                                        inv_n0, e->npairs, e->tw_t, e->KT, e->fullD, e->fullC, e->fullP, e->stream),
+           "readTemporalFull: launch");
+    if (D)
+      HIPCHK(e, hipMemcpyAsync(D + m0 * L, e->fullD, (size_t)nm * L * 4, hipMemcpyDeviceToHost, e->stream),
+             "readTemporalFull: D");
+    if (C)
+      HIPCHK(e, hipMemcpyAsync(C + m0 * L * 2, e->fullC, (size_t)nm * L * 8, hipMemcpyDeviceToHost, e->stream),
+             "readTemporalFull: C");
+    if (P)
+      HIPCHK(e, hipMemcpyAsync(P + m0, e->fullP, (size_t)nm * 4, hipMemcpyDeviceToHost, e->stream),
+             "readTemporalFull: P");
+    HIPCHK(e, hipStreamSynchronize(e->stream), "readTemporalFull: stream");
+  }
   return DDM_OK;
 }
 

@@ -20,6 +20,10 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st);
+// per-mode D, C, P of modes m0 .. m0 + nm - 1 (readTemporalFull)
+hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+                                const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
+                                int KT, float* outD, float2* outC, float* outP, hipStream_t st);
 // KT: the temporal kernels sum Re G of the last KT lags (T-KT..T-1) directly
 // into G_part[item][KT]; lm: log2 of the r-major interleave of S_part rows
 // (0 = frequency order); full != 0: every frame valid (prefix-sum form of the

@@ -696,6 +696,221 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   }
 }
 
+// ----------------------------------------------------------------- k_temporal_full
+//
+// Per-mode outputs of readTemporalFull (opt-in, not part of computeTemporal):
+// for modes m0 .. m0 + gridDim.x - 1 (every mode, binned or not), one
+// workgroup per mode:
+//   C(tau) = G(tau) / N(tau), G the masked autocorrelation of the gathered
+//            series y = m x (no mean removal: C is not shift-invariant);
+//   D(tau) = (A'(tau) + B'(tau) - 2 Re G'(tau)) / N(tau) on the mean-removed
+//            series y' = m (y - mean) (D is shift-invariant; the removal keeps
+//            the cancellation small), A'(tau) = sum_t m(t) p'(t + tau),
+//            B'(tau) = A'(-tau), p' = m |y'|^2;
+//   P = sum_t m |x|^2 / N(0).
+// Three zero-padded TPAD transforms pairs: y -> |Y|^2 -> G; y' -> |Y'|^2 -> G';
+// z = m + i s p' -> conj(M) P -> A' (s = T / sum p' brings p' to the mask's
+// scale so the packed transform does not bury it).  The last KT lags take G,
+// G' and A' + B' from direct sums (the transforms' error there is ~ulp x T /
+// N(tau) of the value).
+template <int LOGT>
+__global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__ hist, int T, int L, long m0,
+                                                      const uint32_t* __restrict__ order,
+                                                      const float* __restrict__ mask, int identity, float inv_n0,
+                                                      const float* __restrict__ npairs, const float2* __restrict__ tw_g,
+                                                      int KT, float* __restrict__ outD, float2* __restrict__ outC,
+                                                      float* __restrict__ outP) {
+  constexpr int n = 1 << LOGT;
+  constexpr int TJ = (n / 2 + NT - 1) / NT;   // time samples per thread (T <= n/2)
+  constexpr int LJ = (n / 2 + NT - 1) / NT;   // lags per thread (L <= T <= n/2)
+  extern __shared__ float2 lds[];
+  float2* buf = lds;                           // n
+  float2* tw = lds + n;                        // n
+  float2* tailC = tw + n;                      // KT (complex G of the tail lags)
+  float* tailG = reinterpret_cast<float*>(tailC + 64);   // KT
+  float* tailAB = tailG + 64;                  // KT
+  float* red = tailAB + 64;                    // 2 * NT / 64
+  const int tid = threadIdx.x;
+  const long mode = m0 + blockIdx.x;
+  for (int i = tid; i < n; i += NT) tw[i] = tw_g[i];
+  const float2* x = hist + mode * T;
+  float mk[TJ];
+  float2 y[TJ];
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    const int j = tid + i * NT;
+    mk[i] = (j < T && mask[j] != 0.f) ? 1.f : 0.f;
+    y[i] = j < T ? x[j] : make_float2(0.f, 0.f);
+  }
+  if (!identity) {
+#pragma unroll
+    for (int i = 0; i < TJ; i++) {
+      const int j = tid + i * NT;
+      if (j < T) buf[j] = y[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TJ; i++) {
+      const int j = tid + i * NT;
+      y[i] = j < T ? buf[order[j]] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  }
+  float aP = 0.f, cP = 0.f;
+  float2 sy = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    y[i] = make_float2(y[i].x * mk[i], y[i].y * mk[i]);
+    sy = cadd(sy, y[i]);
+    two_sum_add(aP, cP, y[i].x * y[i].x + y[i].y * y[i].y);
+  }
+  const float2 Ptot = block_sum2(make_float2(aP + cP, 0.f), red, tid);
+  float2 mean = block_sum2(sy, red, tid);
+  mean.x *= inv_n0;
+  mean.y *= inv_n0;
+  if (tid == 0) outP[blockIdx.x] = Ptot.x * inv_n0;
+  const int j4 = tid >> 2, s4 = tid & 3;      // 4 threads per tail lag
+
+  // ---- pass 1: G of y -> C
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    const int j = tid + i * NT;
+    if (j < n / 2) buf[j] = y[i];
+  }
+  for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
+  __syncthreads();
+  {
+    float gr = 0.f, gi = 0.f;
+    if (j4 < KT)
+      for (int t = s4; t < KT - j4; t += 4) {
+        const float2 a = buf[t], b = buf[T - KT + j4 + t];   // conj(y(t)) y(t + tau), as IFFT(|Y|^2)
+        gr += a.x * b.x + a.y * b.y;
+        gi += a.x * b.y - a.y * b.x;
+      }
+    gr += __shfl_xor(gr, 1, 64);
+    gr += __shfl_xor(gr, 2, 64);
+    gi += __shfl_xor(gi, 1, 64);
+    gi += __shfl_xor(gi, 2, 64);
+    if (s4 == 0 && j4 < KT) tailC[j4] = make_float2(gr, gi);
+  }
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
+  for (int i = tid; i < n; i += NT) {
+    const float2 v = buf[i];
+    buf[i] = make_float2(v.x * v.x + v.y * v.y, 0.f);
+  }
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
+  const float invn = 1.0f / n;
+#pragma unroll
+  for (int k = 0; k < LJ; k++) {
+    const int tau = tid + k * NT;
+    if (tau < L) {
+      float2 g = make_float2(buf[tau].x * invn, buf[tau].y * invn);
+      if (tau >= T - KT) g = tailC[tau - (T - KT)];
+      const float np = npairs[tau];
+      outC[(long)blockIdx.x * L + tau] = np > 0.5f ? make_float2(g.x / np, g.y / np) : make_float2(0.f, 0.f);
+    }
+  }
+  __syncthreads();
+
+  // ---- pass 2: G' of the mean-removed series
+  float psum = 0.f;
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    y[i] = make_float2((y[i].x - mean.x) * mk[i], (y[i].y - mean.y) * mk[i]);
+    psum += y[i].x * y[i].x + y[i].y * y[i].y;
+    const int j = tid + i * NT;
+    if (j < n / 2) buf[j] = y[i];
+  }
+  for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
+  const float2 pt = block_sum2(make_float2(psum, 0.f), red, tid);   // barriers inside
+  {
+    float g = 0.f, ab = 0.f;
+    if (j4 < KT)
+      for (int t = s4; t < KT - j4; t += 4) {
+        const float2 a = buf[t], b = buf[T - KT + j4 + t];
+        g += a.x * b.x + a.y * b.y;
+        // A' + B' pairs (t, t + tau): m(t) p'(t + tau) + p'(t) m(t + tau); y' = 0 where m = 0
+        const float ma = mask[t] != 0.f ? 1.f : 0.f, mb = mask[T - KT + j4 + t] != 0.f ? 1.f : 0.f;
+        ab += ma * (b.x * b.x + b.y * b.y) + mb * (a.x * a.x + a.y * a.y);
+      }
+    g += __shfl_xor(g, 1, 64);
+    g += __shfl_xor(g, 2, 64);
+    ab += __shfl_xor(ab, 1, 64);
+    ab += __shfl_xor(ab, 2, 64);
+    if (s4 == 0 && j4 < KT) {
+      tailG[j4] = g;
+      tailAB[j4] = ab;
+    }
+  }
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
+  for (int i = tid; i < n; i += NT) {
+    const float2 v = buf[i];
+    buf[i] = make_float2(v.x * v.x + v.y * v.y, 0.f);
+  }
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
+  float gp[LJ];
+#pragma unroll
+  for (int k = 0; k < LJ; k++) {
+    const int tau = tid + k * NT;
+    gp[k] = tau < L ? (tau >= T - KT ? tailG[tau - (T - KT)] : buf[tau].x * invn) : 0.f;
+  }
+  __syncthreads();
+
+  // ---- pass 3: A' + B' from z = m + i s p'
+  const float sc = pt.x > 0.f ? (float)T / pt.x : 1.f;
+#pragma unroll
+  for (int i = 0; i < TJ; i++) {
+    const int j = tid + i * NT;
+    if (j < n / 2) buf[j] = make_float2(mk[i], sc * mk[i] * (y[i].x * y[i].x + y[i].y * y[i].y));
+  }
+  for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, false>(buf, tw, tid);
+  // S(f) = conj(M(f)) P(f) with M = (Z(f) + conj Z(-f)) / 2, P = (Z(f) - conj Z(-f)) / 2i,
+  // pairs (f, n - f) by one thread
+  float2 sf[(n / 2 + 1 + NT - 1) / NT], sg[(n / 2 + 1 + NT - 1) / NT];
+#pragma unroll
+  for (int k = 0; k < (n / 2 + 1 + NT - 1) / NT; k++) {
+    const int f = tid + k * NT;
+    if (f <= n / 2) {
+      const int g = (n - f) & (n - 1);
+      const float2 zf = buf[f], zg = buf[g];
+      // at f: M = (zf + conj zg)/2, P = (zf - conj zg)/(2i); at g: M' = (zg + conj zf)/2 = conj M, P' = conj P
+      const float2 M = make_float2(0.5f * (zf.x + zg.x), 0.5f * (zf.y - zg.y));
+      const float2 Pp = make_float2(0.5f * (zf.y + zg.y), -0.5f * (zf.x - zg.x));
+      // conj(M) P at f and conj(M') P' = M conj(P) at g
+      sf[k] = make_float2(M.x * Pp.x + M.y * Pp.y, M.x * Pp.y - M.y * Pp.x);
+      sg[k] = make_float2(sf[k].x, -sf[k].y);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < (n / 2 + 1 + NT - 1) / NT; k++) {
+    const int f = tid + k * NT;
+    if (f <= n / 2) {
+      buf[f] = sf[k];
+      if (f != 0 && f != n / 2) buf[n - f] = sg[k];
+    }
+  }
+  __syncthreads();
+  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
+  const float inv_sn = invn / sc;
+#pragma unroll
+  for (int k = 0; k < LJ; k++) {
+    const int tau = tid + k * NT;
+    if (tau < L) {
+      const float ab = tau >= T - KT ? tailAB[tau - (T - KT)]
+                                     : (buf[tau].x + buf[(n - tau) & (n - 1)].x) * inv_sn;
+      const float np = npairs[tau];
+      outD[(long)blockIdx.x * L + tau] = np > 0.5f ? (ab - 2.f * gp[k]) / np : 0.f;
+    }
+  }
+}
+
 // ----------------------------------------------------------------- radial finalisation
 //
 // Three launches.  k_bin_sum adds the work items' partials of each bin
@@ -1241,6 +1456,36 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
     return launch_temporal_w_t<LG_, LM_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, tw_f, \
                                          S_part, p_part, P_part, KT, G_part, st);
     CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int LOGT>
+static hipError_t launch_temporal_full_t(const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+                                         const float* mask, int identity, float inv_n0, const float* npairs,
+                                         const float2* tw, int KT, float* outD, float2* outC, float* outP,
+                                         hipStream_t st) {
+  constexpr int n = 1 << LOGT;
+  size_t lds = sizeof(float2) * (2 * n + 64) + sizeof(float) * (2 * 64 + 2 * (NT / 64));
+  auto kern = k_temporal_full<LOGT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(nm), dim3(NT), lds, st, hist, T, L, m0, order, mask, identity, inv_n0, npairs, tw, KT,
+                     outD, outC, outP);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+                                const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
+                                int KT, float* outD, float2* outC, float* outP, hipStream_t st) {
+  if (KT < 1 || KT > 64 || KT > T || nm < 1) return hipErrorInvalidValue;
+  switch (logt) {
+#define CASE(LG_)                                                                                                   \
+  case LG_:                                                                                                       \
+    return launch_temporal_full_t<LG_>(hist, T, L, m0, nm, order, mask, identity, inv_n0, npairs, tw, KT, outD,   \
+                                       outC, outP, st);
+    CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
   }
   return hipErrorInvalidValue;

@@ -35,7 +35,7 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step",
            "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
            "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
-           "ddm_batch_last_error", "ddm_batch_destroy")
+           "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full")
 
 
 class DdmError(RuntimeError):
@@ -86,6 +86,7 @@ def load_library(path: str = LIB_PATH):
     L.ddm_compute_temporal.argtypes = [vp, vp, vp]
     L.ddm_read_radial.argtypes = [vp, vp, vp, vp]
     L.ddm_read_history.argtypes = [vp, vp]
+    L.ddm_read_temporal_full.argtypes = [vp, vp, vp, vp]
     L.ddm_idle.argtypes = [vp]
     L.ddm_history_device.argtypes = [vp, C.POINTER(vp)]
     L.ddm_set_profiling.argtypes = [vp, C.c_int32]
@@ -95,7 +96,7 @@ def load_library(path: str = LIB_PATH):
     for name in EXPORTS:
         if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
                         "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error",
-                        "ddm_batch_last_error", "ddm_batch_destroy"):
+                        "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -266,6 +267,15 @@ class DdmEngine:
         out = np.zeros(self.modes * self.T * 2, dtype=np.float32)
         self._check(self._lib.ddm_read_history(self._h, _ptr(out)))
         return out.view(np.complex64).reshape(self.modes, self.T)
+
+    def readTemporalFull(self):
+        """readTemporalFull (src/ddm-engine.ts:403-408): per-mode D (modes, L), C complex (modes, L), P (modes,)."""
+        self._alive()
+        D = np.zeros(self.modes * self.L, dtype=np.float32)
+        Cc = np.zeros(self.modes * self.L * 2, dtype=np.float32)
+        P = np.zeros(self.modes, dtype=np.float32)
+        self._check(self._lib.ddm_read_temporal_full(self._h, _ptr(D), _ptr(Cc), _ptr(P)))
+        return D.reshape(self.modes, self.L), Cc.view(np.complex64).reshape(self.modes, self.L), P
 
     def idle(self) -> None:
         self._alive()

@@ -66,8 +66,12 @@ export declare class DdmEngine {
   computeTemporal(order: Uint32Array, mask: Float32Array, opts?: { yieldBetweenChunks?: boolean }): Promise<void>;
   readRadial(): Promise<Radial>;
   readHistory(): Promise<Float32Array>;
+  /** per-mode D[m*L+tau], C[(m*L+tau)*2+{re,im}], P[m] for every mode (computed on demand) */
+  readTemporalFull(): Promise<{ D: Float32Array; C: Float32Array; P: Float32Array }>;
   idle(): Promise<void>;
   destroy(): void;
 }
 
 export declare function linearOrder(valid: ArrayLike<boolean | number>): { order: Uint32Array; mask: Float32Array };
+export declare function shuffleOrder(o: { order: Uint32Array; mask: Float32Array }, rand: () => number): { order: Uint32Array; mask: Float32Array };
+export declare function mulberry32(seed: number): () => number;

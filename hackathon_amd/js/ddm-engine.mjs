@@ -97,6 +97,9 @@ export class DdmEngine {
     };
   }
 
+  /** readTemporalFull (src/ddm-engine.ts:403-408): per-mode {D, C, P} of every mode (opt-in, computed on demand). */
+  async readTemporalFull() { return addon.readTemporalFull(this._h); }
+
   /** readHistory (src/ddm-engine.ts:397-400): hist[(mode*T + slot)*2 + {0,1}]. */
   async readHistory() { return addon.readHistory(this._h); }
 
@@ -107,6 +110,30 @@ export class DdmEngine {
     this._destroyed = true;
     addon.destroy(this._h);
   }
+}
+
+/** mulberry32 seeded uniform RNG on [0, 1) (the reference's test/sim.mjs:14-23 generator). */
+export function mulberry32(seed) {
+  let a = seed >>> 0;
+  return function () {
+    a = (a + 0x6d2b79f5) >>> 0;
+    let t = a;
+    t = Math.imul(t ^ (t >>> 15), t | 1);
+    t ^= t + Math.imul(t ^ (t >>> 7), t | 61);
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  };
+}
+
+/** shuffleOrder (src/ddm_ref.js:504-519): seeded Fisher-Yates over the (slot, mask) pairs. */
+export function shuffleOrder(o, rand) {
+  const order = Uint32Array.from(o.order);
+  const mask = Float32Array.from(o.mask);
+  for (let i = order.length - 1; i > 0; i--) {
+    const j = Math.floor(rand() * (i + 1));
+    const to = order[i]; order[i] = order[j]; order[j] = to;
+    const tm = mask[i]; mask[i] = mask[j]; mask[j] = tm;
+  }
+  return { order, mask };
 }
 
 /** linearOrder (src/ddm_ref.js:493-502). */

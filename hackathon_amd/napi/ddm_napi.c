@@ -309,6 +309,32 @@ static napi_value js_idle(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* readTemporalFull(h) -> {D: Float32Array(modes*L), C: Float32Array(modes*L*2), P: Float32Array(modes)} */
+static napi_value js_read_temporal_full(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  ddm_sizes s;
+  ddm_get_sizes(h->e, &s);
+  void *D, *C, *P;
+  napi_value vD = make_typed(env, napi_float32_array, (size_t)s.modes * s.L, 4, &D);
+  napi_value vC = make_typed(env, napi_float32_array, (size_t)s.modes * s.L * 2, 4, &C);
+  napi_value vP = make_typed(env, napi_float32_array, (size_t)s.modes, 4, &P);
+  if (!vD || !vC || !vP) {
+    napi_throw_error(env, NULL, "readTemporalFull: allocation failed");
+    return NULL;
+  }
+  int rc = ddm_read_temporal_full(h->e, (float *)D, (float *)C, (float *)P);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  NAPI_CALL(env, napi_create_object(env, &out));
+  set_prop(env, out, "D", vD);
+  set_prop(env, out, "C", vC);
+  set_prop(env, out, "P", vP);
+  return out;
+}
+
 /* ------------------------------------------------------------------ per-q fits */
 
 static void finalize_fitter(napi_env env, void *data, void *hint) {
@@ -428,6 +454,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readRadial", NULL, js_read_radial, NULL, NULL, NULL, napi_default, NULL},
       {"readHistory", NULL, js_read_history, NULL, NULL, NULL, napi_default, NULL},
       {"idle", NULL, js_idle, NULL, NULL, NULL, napi_default, NULL},
+      {"readTemporalFull", NULL, js_read_temporal_full, NULL, NULL, NULL, napi_default, NULL},
       {"fitterCreate", NULL, js_fitter_create, NULL, NULL, NULL, napi_default, NULL},
       {"fit", NULL, js_fit, NULL, NULL, NULL, napi_default, NULL},
   };

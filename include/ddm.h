@@ -136,6 +136,14 @@ int ddm_read_radial(ddm_engine *e, float *D, float *P, float *drift);
  * modes*T*2 floats. */
 int ddm_read_history(ddm_engine *e, float *out);
 
+/* readTemporalFull (src/ddm-engine.ts:403-408): per-mode results for EVERY
+ * mode m (binned or not) of the current history under the last
+ * ddm_compute_temporal's order and mask: D[m*L + tau], C[(m*L + tau)*2 +
+ * {re,im}] = G(tau)/N(tau) (masked autocorrelation, no mean removal) and
+ * P[m] = sum |x|^2 / N(0).  Opt-in: computed on demand in chunks of 8192
+ * modes (modes*L*12 bytes of host output).  Any pointer may be NULL. */
+int ddm_read_temporal_full(ddm_engine *e, float *D, float *C, float *P);
+
 /* idle (src/ddm-engine.ts:417-419): wait for all work on the engine's stream. */
 int ddm_idle(ddm_engine *e);
 

@@ -89,3 +89,22 @@ def test_facade_frames_to_fits(gpu, tmp_path):
     assert [b.b for b in pr.bins] == out["bins"] and len(out["bins"]) == out["nPrepared"] > 0
     assert np.allclose(res.chi2, out["chi2"], rtol=1e-12, atol=0)
     assert np.allclose(res.theta[:, 0], th[:, 0], rtol=1e-12) and np.allclose(res.D, th[:, 2], rtol=1e-12)
+
+
+def test_js_shuffle_order_matches_oracle():
+    """shuffleOrder + mulberry32 of the TS-facing facade against the oracle's restatement (no GPU work)."""
+    import oracle as O
+    if not os.path.exists(ADDON):
+        pytest.fail("N-API addon not built (python -m hackathon_amd.napi.build_addon)")
+    code = ("import('" + os.path.join(ROOT, "hackathon_amd", "js", "ddm-engine.mjs") + "').then(m => {"
+            " const T = 50; const o = m.linearOrder(Array.from({length: T}, (_, i) => i % 7 !== 3));"
+            " const s = m.shuffleOrder(o, m.mulberry32(4242));"
+            " console.log('RESULT ' + JSON.stringify({order: Array.from(s.order), mask: Array.from(s.mask)})); })")
+    r = subprocess.run(["node", "--input-type=module", "-e", code], capture_output=True, text=True, timeout=60)
+    line = [x for x in r.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, r.stdout + r.stderr
+    out = json.loads(line[0][7:])
+    T = 50
+    order, mask = O.shuffle_order(np.arange(T), np.array([i % 7 != 3 for i in range(T)], np.float32),
+                                  O.mulberry32(4242))
+    assert out["order"] == order.tolist() and out["mask"] == mask.tolist()
