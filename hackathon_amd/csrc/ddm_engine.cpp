@@ -266,6 +266,9 @@ struct ddm_engine {
   float last_n0 = 0.f;
   float *fullD = nullptr, *fullP = nullptr;   // per-mode outputs of one chunk of modes
   float2* fullC = nullptr;
+  double* drift_acc = nullptr;   // NB*L*8 phase-drift sums (FP64)
+  int2* kxy = nullptr;           // (kx, ky) per binned mode
+  int* bin_off = nullptr;        // NB+1 binned-mode offsets
   // pinned staging of the per-analysis metadata (order, 0/1 mask, N(tau));
   // meta_ev marks when the last upload from it finished
   uint32_t* meta_order = nullptr;
@@ -335,7 +338,7 @@ void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
                   e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part,
-                  e->fullD, e->fullC, e->fullP};
+                  e->fullD, e->fullC, e->fullP, e->drift_acc, e->kxy, e->bin_off};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -674,12 +677,60 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
   return rc;
 }
 
+// Phase-drift sums (opt-in: only when the caller asks for them): C(tau) of
+// every binned mode from the per-mode kernel, in chunks of the binned-mode
+// list, reduced per (bin, lag) in FP64 in stream order.
+static int compute_drift(ddm_engine* e, float* drift) {
+  const int NB = e->g.NB, L = e->g.L, T = e->g.T;
+  const size_t nout = (size_t)NB * L * 8;
+  if (!e->meta_valid) {   // nothing analysed yet: the reference's record holds zeros
+    std::memset(drift, 0, nout * sizeof(float));
+    return DDM_OK;
+  }
+  const long MC = 8192;
+  hipError_t he;
+  if (!e->fullC && (he = dalloc(&e->fullC, (size_t)MC * L)) != hipSuccess) return hip_fail(e, he, "drift: scratch");
+  if (!e->drift_acc) {
+    const long binned = e->bins.offsets[NB];
+    std::vector<int2> kxy(std::max<long>(binned, 1));
+    for (long i = 0; i < binned; i++) kxy[i] = make_int2(e->bins.modes4[4 * i + 1], e->bins.modes4[4 * i + 2]);
+    std::vector<int> off(e->bins.offsets.begin(), e->bins.offsets.end());
+    if ((he = dalloc(&e->drift_acc, nout)) != hipSuccess || (he = dalloc(&e->kxy, kxy.size())) != hipSuccess ||
+        (he = dalloc(&e->bin_off, off.size())) != hipSuccess ||
+        (he = hipMemcpy(e->kxy, kxy.data(), kxy.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess ||
+        (he = hipMemcpy(e->bin_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, he, "drift: tables");
+  }
+  HIPCHK(e, hipMemsetAsync(e->drift_acc, 0, nout * sizeof(double), e->stream), "drift: clear");
+  const long binned = e->bins.offsets[NB];
+  const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
+  const auto& off = e->bins.offsets;
+  for (long c0 = 0; c0 < binned; c0 += MC) {
+    const int nc = (int)std::min<long>(MC, binned - c0);
+    // bins whose mode ranges meet [c0, c0 + nc)
+    const int b0 = (int)(std::upper_bound(off.begin(), off.end(), (uint32_t)c0) - off.begin()) - 1;
+    const int b1 = (int)(std::upper_bound(off.begin(), off.end(), (uint32_t)(c0 + nc - 1)) - off.begin()) - 1;
+    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, L, c0, nc, e->order, e->mask, e->last_identity ? 1 : 0,
+                                        inv_n0, e->npairs, e->tw_t, e->KT, e->bin_modes, nullptr, e->fullC, nullptr,
+                                        e->stream),
+           "drift: per-mode C");
+    HIPCHK(e, ddm::launch_drift(e->fullC, L, (int)c0, nc, e->bin_off, e->kxy, b0, b1 - b0 + 1, e->drift_acc, e->stream),
+           "drift: sums");
+  }
+  std::vector<double> acc(nout);
+  HIPCHK(e, hipMemcpyAsync(acc.data(), e->drift_acc, nout * sizeof(double), hipMemcpyDeviceToHost, e->stream),
+         "drift: copy");
+  HIPCHK(e, hipStreamSynchronize(e->stream), "drift: stream");
+  for (size_t i = 0; i < nout; i++) drift[i] = (float)acc[i];
+  return DDM_OK;
+}
+
 int ddm_read_radial(ddm_engine* e, float* D, float* P, float* drift) {
   if (!e) return DDM_EINVAL;
   HIPCHK(e, hipStreamSynchronize(e->stream), "readRadial: stream");
   if (D) HIPCHK(e, hipMemcpy(D, e->radD, (size_t)e->g.NB * e->g.L * 4, hipMemcpyDeviceToHost), "readRadial: D");
   if (P) HIPCHK(e, hipMemcpy(P, e->radP, (size_t)e->g.NB * 4, hipMemcpyDeviceToHost), "readRadial: P");
-  if (drift) std::memset(drift, 0, (size_t)e->g.NB * e->g.L * 8 * sizeof(float));
+  if (drift) return compute_drift(e, drift);
   return DDM_OK;
 }
 
@@ -697,16 +748,16 @@ int ddm_read_temporal_full(ddm_engine* e, float* D, float* C, float* P) {
   const int T = e->g.T, L = e->g.L;
   const long MC = 8192;   // modes per launch: bounds the device scratch to MC * L * 12 bytes
   hipError_t he;
-  if (!e->fullD) {
-    if ((he = dalloc(&e->fullD, (size_t)MC * L)) != hipSuccess || (he = dalloc(&e->fullC, (size_t)MC * L)) != hipSuccess ||
-        (he = dalloc(&e->fullP, (size_t)MC)) != hipSuccess)
-      return hip_fail(e, he, "readTemporalFull: device scratch");
-  }
+  if ((!e->fullD && (he = dalloc(&e->fullD, (size_t)MC * L)) != hipSuccess) ||
+      (!e->fullC && (he = dalloc(&e->fullC, (size_t)MC * L)) != hipSuccess) ||
+      (!e->fullP && (he = dalloc(&e->fullP, (size_t)MC)) != hipSuccess))
+    return hip_fail(e, he, "readTemporalFull: device scratch");
   const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
   for (long m0 = 0; m0 < e->modes; m0 += MC) {
     const int nm = (int)std::min<long>(MC, e->modes - m0);
     HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, L, m0, nm, e->order, e->mask, e->last_identity ? 1 : 0,
-                                        inv_n0, e->npairs, e->tw_t, e->KT, e->fullD, e->fullC, e->fullP, e->stream),
+                                        inv_n0, e->npairs, e->tw_t, e->KT, nullptr, e->fullD, e->fullC, e->fullP,
+                                        e->stream),
            "readTemporalFull: launch");
     if (D)
       HIPCHK(e, hipMemcpyAsync(D + m0 * L, e->fullD, (size_t)nm * L * 4, hipMemcpyDeviceToHost, e->stream),

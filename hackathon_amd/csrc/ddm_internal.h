@@ -21,9 +21,13 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st);
 // per-mode D, C, P of modes m0 .. m0 + nm - 1 (readTemporalFull)
+// (mode_list: entries m0 .. m0 + nm - 1 of a mode list instead; outD/outP may be null)
 hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
                                 const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
-                                int KT, float* outD, float2* outC, float* outP, hipStream_t st);
+                                int KT, const int* mode_list, float* outD, float2* outC, float* outP, hipStream_t st);
+// phase-drift sums of bins b0 .. b0 + nb - 1 over binned modes c0 .. c0 + nc - 1 (C rows of those modes)
+hipError_t launch_drift(const float2* C, int L, int c0, int nc, const int* bin_offsets, const int2* kxy, int b0,
+                        int nb, double* acc, hipStream_t st);
 // KT: the temporal kernels sum Re G of the last KT lags (T-KT..T-1) directly
 // into G_part[item][KT]; lm: log2 of the r-major interleave of S_part rows
 // (0 = frequency order); full != 0: every frame valid (prefix-sum form of the

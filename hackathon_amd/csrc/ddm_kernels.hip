@@ -718,7 +718,8 @@ __global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__
                                                       const uint32_t* __restrict__ order,
                                                       const float* __restrict__ mask, int identity, float inv_n0,
                                                       const float* __restrict__ npairs, const float2* __restrict__ tw_g,
-                                                      int KT, float* __restrict__ outD, float2* __restrict__ outC,
+                                                      int KT, const int* __restrict__ mode_list,
+                                                      float* __restrict__ outD, float2* __restrict__ outC,
                                                       float* __restrict__ outP) {
   constexpr int n = 1 << LOGT;
   constexpr int TJ = (n / 2 + NT - 1) / NT;   // time samples per thread (T <= n/2)
@@ -731,7 +732,8 @@ __global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__
   float* tailAB = tailG + 64;                  // KT
   float* red = tailAB + 64;                    // 2 * NT / 64
   const int tid = threadIdx.x;
-  const long mode = m0 + blockIdx.x;
+  // modes m0 + blockIdx.x, or entries m0 + blockIdx.x of mode_list
+  const long mode = mode_list ? (long)mode_list[m0 + blockIdx.x] : m0 + blockIdx.x;
   for (int i = tid; i < n; i += NT) tw[i] = tw_g[i];
   const float2* x = hist + mode * T;
   float mk[TJ];
@@ -768,7 +770,7 @@ __global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__
   float2 mean = block_sum2(sy, red, tid);
   mean.x *= inv_n0;
   mean.y *= inv_n0;
-  if (tid == 0) outP[blockIdx.x] = Ptot.x * inv_n0;
+  if (tid == 0 && outP) outP[blockIdx.x] = Ptot.x * inv_n0;
   const int j4 = tid >> 2, s4 = tid & 3;      // 4 threads per tail lag
 
   // ---- pass 1: G of y -> C
@@ -906,9 +908,46 @@ __global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__
       const float ab = tau >= T - KT ? tailAB[tau - (T - KT)]
                                      : (buf[tau].x + buf[(n - tau) & (n - 1)].x) * inv_sn;
       const float np = npairs[tau];
-      outD[(long)blockIdx.x * L + tau] = np > 0.5f ? (ab - 2.f * gp[k]) / np : 0.f;
+      if (outD) outD[(long)blockIdx.x * L + tau] = np > 0.5f ? (ab - 2.f * gp[k]) / np : 0.f;
     }
   }
+}
+
+// ----------------------------------------------------------------- k_drift
+//
+// Phase-drift least-squares sums of the radial record (accumulateMode,
+// src/ddm_ref.js:360-381): for every bin b and lag tau < L-1, over the bin's
+// modes (kx, ky), z = C(tau+1) conj C(tau), w = |z|, phi = arg z:
+//   Sw, Sxx = sum w kx^2, Sxy, Syy, Rx = sum w kx phi, Ry, Rpp = sum w phi^2, count(w > 0).
+// One thread per (bin, lag) over the bin's modes that lie in the current chunk
+// [c0, c0 + nc) of the binned-mode list; FP64 accumulation in acc, chunk after
+// chunk in stream order (deterministic).  C rows are the chunk's.
+__global__ __launch_bounds__(NT) void k_drift(const float2* __restrict__ C, int L, int c0, int nc,
+                                              const int* __restrict__ bin_offsets, const int2* __restrict__ kxy,
+                                              int b0, double* __restrict__ acc) {
+  const int b = b0 + blockIdx.y;
+  const int tau = blockIdx.x * NT + threadIdx.x;
+  if (tau >= L - 1) return;
+  const int i0 = max(bin_offsets[b], c0), i1 = min(bin_offsets[b + 1], c0 + nc);
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = i0; i < i1; i++) {
+    const float2 a = C[(long)(i - c0) * L + tau], bq = C[(long)(i - c0) * L + tau + 1];
+    const double zr = (double)bq.x * a.x + (double)bq.y * a.y;
+    const double zi = (double)bq.y * a.x - (double)bq.x * a.y;
+    const double w = sqrt(zr * zr + zi * zi), phi = atan2(zi, zr);
+    const double kx = kxy[i].x, ky = kxy[i].y;
+    s[0] += w;
+    s[1] += w * kx * kx;
+    s[2] += w * kx * ky;
+    s[3] += w * ky * ky;
+    s[4] += w * kx * phi;
+    s[5] += w * ky * phi;
+    s[6] += w * phi * phi;
+    s[7] += w > 0 ? 1.0 : 0.0;
+  }
+  double* o = acc + ((long)b * L + tau) * 8;
+#pragma unroll
+  for (int c = 0; c < 8; c++) o[c] += s[c];
 }
 
 // ----------------------------------------------------------------- radial finalisation
@@ -1464,31 +1503,38 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
 template <int LOGT>
 static hipError_t launch_temporal_full_t(const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
                                          const float* mask, int identity, float inv_n0, const float* npairs,
-                                         const float2* tw, int KT, float* outD, float2* outC, float* outP,
-                                         hipStream_t st) {
+                                         const float2* tw, int KT, const int* mode_list, float* outD, float2* outC,
+                                         float* outP, hipStream_t st) {
   constexpr int n = 1 << LOGT;
   size_t lds = sizeof(float2) * (2 * n + 64) + sizeof(float) * (2 * 64 + 2 * (NT / 64));
   auto kern = k_temporal_full<LOGT>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nm), dim3(NT), lds, st, hist, T, L, m0, order, mask, identity, inv_n0, npairs, tw, KT,
-                     outD, outC, outP);
+                     mode_list, outD, outC, outP);
   return hipGetLastError();
 }
 
 hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
                                 const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
-                                int KT, float* outD, float2* outC, float* outP, hipStream_t st) {
+                                int KT, const int* mode_list, float* outD, float2* outC, float* outP, hipStream_t st) {
   if (KT < 1 || KT > 64 || KT > T || nm < 1) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                   \
   case LG_:                                                                                                       \
-    return launch_temporal_full_t<LG_>(hist, T, L, m0, nm, order, mask, identity, inv_n0, npairs, tw, KT, outD,   \
-                                       outC, outP, st);
+    return launch_temporal_full_t<LG_>(hist, T, L, m0, nm, order, mask, identity, inv_n0, npairs, tw, KT,         \
+                                       mode_list, outD, outC, outP, st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_drift(const float2* C, int L, int c0, int nc, const int* bin_offsets, const int2* kxy, int b0,
+                        int nb, double* acc, hipStream_t st) {
+  if (L < 2 || nb < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_drift, dim3((L - 1 + NT - 1) / NT, nb), dim3(NT), 0, st, C, L, c0, nc, bin_offsets, kxy, b0, acc);
+  return hipGetLastError();
 }
 
 template <int LOGT>

@@ -251,13 +251,16 @@ class DdmEngine:
             raise DdmError(f"temporalMeta: order/mask must have length {self.T}")
         self._check(self._lib.ddm_compute_temporal(self._h, _ptr(order), _ptr(mask)))
 
-    def readRadial(self) -> Radial:
-        """readRadial (src/ddm-engine.ts:391-395): the Radial record, FP64 arrays of the f32 results."""
+    def readRadial(self, drift: bool = False) -> Radial:
+        """readRadial (src/ddm-engine.ts:391-395): the Radial record, FP64 arrays of the f32 results.
+        drift=True also computes the phase-drift sums (a per-mode pass over the binned modes);
+        otherwise the record's drift array is zeros."""
         self._alive()
         D = np.zeros(self.NB * self.L, dtype=np.float32)
         P = np.zeros(self.NB, dtype=np.float32)
-        drift = np.zeros(self.NB * self.L * 8, dtype=np.float32)
-        self._check(self._lib.ddm_read_radial(self._h, _ptr(D), _ptr(P), _ptr(drift)))
+        dr = np.zeros(self.NB * self.L * 8, dtype=np.float32)
+        self._check(self._lib.ddm_read_radial(self._h, _ptr(D), _ptr(P), _ptr(dr) if drift else None))
+        drift = dr
         return Radial(self.N, self.NB, self.L, self.bins.kCenter.copy(), self.bins.kNodes.copy(),
                       self.bins.nModes.copy(), D.astype(np.float64), P.astype(np.float64), drift.astype(np.float64))
 

@@ -16,7 +16,7 @@ export interface Radial {
   /** D[b*L + tau] */
   D: Float64Array;
   P: Float64Array;
-  /** NB*L*8 phase-drift sums (zero-filled by this engine) */
+  /** NB*L*8 phase-drift sums: (b*L + tau)*8 + {Sw, Sxx, Sxy, Syy, Rx, Ry, Rpp, count} */
   drift: Float64Array;
 }
 
@@ -64,7 +64,7 @@ export declare class DdmEngine {
   ingestLuma(luma: Float32Array, slot: number): boolean;
   ingestFrames(frames: Uint8Array | Uint16Array | Float32Array, nframes: number, firstSlot?: number): boolean;
   computeTemporal(order: Uint32Array, mask: Float32Array, opts?: { yieldBetweenChunks?: boolean }): Promise<void>;
-  readRadial(): Promise<Radial>;
+  readRadial(opts?: { drift?: boolean }): Promise<Radial>;
   readHistory(): Promise<Float32Array>;
   /** per-mode D[m*L+tau], C[(m*L+tau)*2+{re,im}], P[m] for every mode (computed on demand) */
   readTemporalFull(): Promise<{ D: Float32Array; C: Float32Array; P: Float32Array }>;

@@ -83,9 +83,12 @@ export class DdmEngine {
     }
   }
 
-  /** readRadial (src/ddm-engine.ts:391-395): the Radial record of src/ddm_ref.js:336-348. */
-  async readRadial() {
-    const r = addon.readRadial(this._h);
+  /**
+   * readRadial (src/ddm-engine.ts:391-395): the Radial record of src/ddm_ref.js:336-348, phase-drift
+   * sums included as in the reference (opts.drift = false skips them: they take a per-mode pass).
+   */
+  async readRadial(opts = {}) {
+    const r = addon.readRadial(this._h, pick(opts.drift, true));
     return {
       N: this.N, NB: this.NB, L: this.L,
       kCenter: Float64Array.from(this.bins.kCenter),

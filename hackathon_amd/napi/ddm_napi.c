@@ -252,13 +252,15 @@ static napi_value js_compute_temporal(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* readRadial(h) -> {D: Float32Array, P: Float32Array, drift: Float32Array} */
+/* readRadial(h, withDrift) -> {D: Float32Array, P: Float32Array, drift: Float32Array} */
 static napi_value js_read_radial(napi_env env, napi_callback_info info) {
-  size_t argc = 1;
-  napi_value argv[1], out;
+  size_t argc = 2;
+  napi_value argv[2], out;
   handle_t *h;
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  bool with_drift = true;
+  if (argc >= 2) napi_get_value_bool(env, argv[1], &with_drift);
   ddm_sizes s;
   ddm_get_sizes(h->e, &s);
   void *D, *P, *dr;
@@ -269,7 +271,7 @@ static napi_value js_read_radial(napi_env env, napi_callback_info info) {
     napi_throw_error(env, NULL, "readRadial: allocation failed");
     return NULL;
   }
-  int rc = ddm_read_radial(h->e, (float *)D, (float *)P, (float *)dr);
+  int rc = ddm_read_radial(h->e, (float *)D, (float *)P, with_drift ? (float *)dr : NULL);
   if (rc != DDM_OK) return throw_status(env, h->e, rc);
   NAPI_CALL(env, napi_create_object(env, &out));
   set_prop(env, out, "D", vD);

@@ -128,8 +128,11 @@ int ddm_ingest_device(ddm_engine *e, const void *dframes, int32_t dtype, int32_t
 int ddm_compute_temporal(ddm_engine *e, const uint32_t *order, const float *mask);
 
 /* readRadial (src/ddm-engine.ts:391-395, radialFromGpu src/ddm_ref.js:584-590):
- * D[NB*L], P[NB]; `drift` (NB*L*8, may be NULL) is zero-filled: the phase-drift
- * sums are not computed by this engine. */
+ * D[NB*L], P[NB]; `drift` (NB*L*8, may be NULL): the phase-drift least-squares
+ * sums drift[(b*L + tau)*8 + c] (c = Sw, Sxx, Sxy, Syy, Rx, Ry, Rpp, count) of
+ * accumulateMode (src/ddm_ref.js:360-381), computed on demand from per-mode
+ * C(tau) only when the pointer is non-NULL (a per-mode pass over the binned
+ * modes; zeros before the first ddm_compute_temporal). */
 int ddm_read_radial(ddm_engine *e, float *D, float *P, float *drift);
 
 /* readHistory (src/ddm-engine.ts:397-400): hist[(m*T + slot)*2 + {re,im}],

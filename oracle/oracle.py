@@ -52,6 +52,7 @@ def lib():
         L.orc_make_bins.restype = C.c_long
         L.orc_analyze_radial.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
         L.orc_analyze_radial_split.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
+        L.orc_radial_drift.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp]
         L.orc_analyze_radial_direct.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp,
                                                 _dp]
         for name in ("orc_bessel_j0", "orc_lngamma"):
@@ -183,6 +184,18 @@ def analyze_radial(hist: np.ndarray, order, mask, bins: dict, L: int, direct: bo
     f(h, T, np.ascontiguousarray(order, dtype=np.uint32), np.ascontiguousarray(mask, dtype=np.float32),
       bins["binOffsets"], np.ascontiguousarray(bins["binModes"].ravel()), NB, L, D, P)
     return D.reshape(NB, L), P
+
+
+def radial_drift(hist: np.ndarray, order, mask, bins: dict, L: int) -> np.ndarray:
+    """Phase-drift sums drift[NB, L, 8] (src/ddm_ref.js:360-381), FP64; hist complex (modes, T)."""
+    h = np.ascontiguousarray(hist, dtype=np.complex128).view(np.float64).ravel()
+    T = hist.shape[1]
+    NB = bins["NB"]
+    out = np.zeros(NB * L * 8)
+    lib().orc_radial_drift(h, T, np.ascontiguousarray(order, dtype=np.uint32),
+                           np.ascontiguousarray(mask, dtype=np.float32), bins["binOffsets"],
+                           np.ascontiguousarray(bins["binModes"].ravel()), NB, L, out)
+    return out.reshape(NB, L, 8)
 
 
 # ------------------------------------------------------------------ special functions / fit blocks

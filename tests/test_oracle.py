@@ -338,3 +338,18 @@ def test_packed_transform_loses_p_precision_at_small_power():
     err_split = np.abs(Ds[0, -8:] - Dd[0, -8:]) / np.max(Dd[0])
     err_packed = np.abs(Dp[0, -8:] - Dd[0, -8:]) / np.max(Dd[0])
     assert err_split.max() < 1e-11 and err_packed.max() > 100 * err_split.max()
+
+
+def test_drift_sums_pure_phase_ramp():
+    """A mode whose series is exp(i (k.V) t) drifts by the same phase every step: z has phase k.V,
+    so Rx / Sxx recovers Vx when only kx modes are present (closed form of the least-squares sums)."""
+    T, L = 64, 16
+    Vx = 0.05
+    modes = [(0, kx, 0) for kx in (1, 2, 3, 4)]
+    h = np.array([np.exp(1j * kx * Vx * np.arange(T)) for _, kx, _ in modes])
+    bins = dict(NB=1, binOffsets=np.array([0, 4], np.uint32),
+                binModes=np.array([[i, kx, ky, 0] for i, (_, kx, ky) in enumerate(modes)], np.int32))
+    dr = O.radial_drift(h, np.arange(T, dtype=np.uint32), np.ones(T, np.float32), bins, L)
+    Sw, Sxx, Rx, count = dr[0, :L - 1, 0], dr[0, :L - 1, 1], dr[0, :L - 1, 4], dr[0, :L - 1, 7]
+    assert np.allclose(Rx / Sxx, Vx, rtol=1e-10) and np.all(count == 4) and np.all(Sw > 0)
+    assert np.all(dr[0, L - 1] == 0)
