@@ -439,7 +439,12 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->stage_bytes = (size_t)chunk * g.N * g.N * 4;
 
   const size_t nitems = std::max(e->nitems, 1);
-  if ((he = dalloc(&e->hist, (size_t)e->modes * g.T)) != hipSuccess ||
+  // TPAD/2 - T spare samples behind the last mode: the temporal kernel reads
+  // every series (and the frame order) as TPAD/2 entries without a guard; the
+  // entries past T belong to the next mode or to the pad and are masked off,
+  // the order's pad stays 0
+  const size_t hist_len = (size_t)e->modes * g.T + (size_t)(e->TPAD / 2 - g.T);
+  if ((he = dalloc(&e->hist, hist_len)) != hipSuccess ||
       (he = dalloc(&e->scratch, (size_t)chunk * e->NH * g.N)) != hipSuccess ||
       (he = dalloc(&e->rowinfo, (size_t)chunk * g.N)) != hipSuccess ||
       (he = hipMalloc(&e->stage, e->stage_bytes)) != hipSuccess || (he = dalloc(&e->win, g.N)) != hipSuccess ||
@@ -448,7 +453,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->wx, 4)) != hipSuccess || (he = dalloc(&e->items, nitems)) != hipSuccess ||
       (he = dalloc(&e->bin_modes, bin_modes_h.size())) != hipSuccess ||
       (he = dalloc(&e->bin_items, g.NB + 1)) != hipSuccess || (he = dalloc(&e->nmodes, g.NB)) != hipSuccess ||
-      (he = dalloc(&e->order, g.T)) != hipSuccess || (he = dalloc(&e->mask, g.T)) != hipSuccess ||
+      (he = dalloc(&e->order, (size_t)e->TPAD / 2)) != hipSuccess || (he = dalloc(&e->mask, g.T)) != hipSuccess ||
       (he = dalloc(&e->npairs, g.L)) != hipSuccess || (he = dalloc(&e->S_part, nitems * e->TPAD)) != hipSuccess ||
       (he = dalloc(&e->p_part, nitems * g.T)) != hipSuccess || (he = dalloc(&e->P_part, nitems)) != hipSuccess ||
       (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess ||
@@ -496,7 +501,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
           hipSuccess ||
       (he = hipMemcpy(e->bin_items, e->bin_items_h.data(), (g.NB + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->nmodes, e->bins.nmodes.data(), g.NB * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-      (he = hipMemset(e->hist, 0, (size_t)e->modes * g.T * 8)) != hipSuccess ||
+      (he = hipMemset(e->hist, 0, hist_len * 8)) != hipSuccess ||
+      (he = hipMemset(e->order, 0, (size_t)e->TPAD / 2 * 4)) != hipSuccess ||
       (he = hipMemset(e->radD, 0, (size_t)g.NB * g.L * 4)) != hipSuccess ||
       (he = hipMemset(e->radP, 0, (size_t)g.NB * 4)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: table upload");

@@ -508,19 +508,23 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   float aP = 0.f, cP = 0.f;
   float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / 4 (lanes 4 j, j < KT)
   const int rounds = (count + 3) / 4;
+  // Series loads without guards: a wave past the item's last mode re-reads
+  // that mode (its contributions are dropped through `active`); samples
+  // t in [T, n/2) read the next mode's head or the n/2 - T spare samples the
+  // history buffer carries behind its last mode (they exist only when the
+  // series is not dense, and the mask multiply zeroes them).  One base
+  // address with immediate offsets; guarded or clamped loads would cost a
+  // branch region or an address pair per sample.
+  auto load_series = [&](float2 (&dst)[PT], int slot) {
+    const float2* x = hist + (long)bin_modes[start + (slot < count ? slot : count - 1)] * T;
+#pragma unroll
+    for (int q = 0; q < PT; q++) dst[q] = x[lane + 64 * q];
+  };
   // next round's series, loaded while the current one is transformed (up to
   // n = 2048; above that the registers are needed by the split transforms)
   constexpr bool PREFETCH = LOGT <= 11;
   float2 nx[PREFETCH ? PT : 1];
-  if constexpr (PREFETCH) {
-    const int mi = wave;
-    const float2* x = hist + (long)bin_modes[start + (mi < count ? mi : 0)] * T;
-#pragma unroll
-    for (int q = 0; q < PT; q++) {
-      int t = lane + 64 * q;
-      nx[q] = (t < T && mi < count) ? x[t] : make_float2(0.f, 0.f);
-    }
-  }
+  if constexpr (PREFETCH) load_series(nx, wave);
   float* park = reinterpret_cast<float*>(xch);
   for (int rd = 0; rd < rounds; rd++) {
     const int mi = rd * 4 + wave;
@@ -529,33 +533,19 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     if constexpr (PREFETCH) {
 #pragma unroll
       for (int q = 0; q < PT; q++) y[q] = nx[q];
-      const int mn = mi + 4;
-      const float2* x = hist + (long)bin_modes[start + (mn < count ? mn : 0)] * T;
-#pragma unroll
-      for (int q = 0; q < PT; q++) {
-        int t = lane + 64 * q;
-        nx[q] = (t < T && mn < count) ? x[t] : make_float2(0.f, 0.f);
-      }
+      load_series(nx, mi + 4);
     } else {
-      const float2* x = hist + (long)bin_modes[start + (active ? mi : 0)] * T;
-#pragma unroll
-      for (int q = 0; q < PT; q++) {
-        int t = lane + 64 * q;
-        y[q] = (t < T && active) ? x[t] : make_float2(0.f, 0.f);
-      }
+      load_series(y, mi);
     }
     if (!identity) {
+      // every slot t < n/2 is written (xpad(n/2 - 1) lies inside the wave's
+      // buffer); order[] holds n/2 entries, those past T are 0 (their
+      // samples are masked off)
 #pragma unroll
-      for (int q = 0; q < PT; q++) {
-        int t = lane + 64 * q;
-        if (t < T) xch[xpad(t)] = y[q];
-      }
+      for (int q = 0; q < PT; q++) xch[xpad(lane + 64 * q)] = y[q];
       wave_sync();
 #pragma unroll
-      for (int q = 0; q < PT; q++) {
-        int t = lane + 64 * q;
-        y[q] = (t < T) ? xch[xpad((int)order[t])] : make_float2(0.f, 0.f);
-      }
+      for (int q = 0; q < PT; q++) y[q] = xch[xpad((int)order[lane + 64 * q])];
     }
     float sx = 0.f, sy = 0.f, ps = 0.f;
     if (dense) {   // every sample of every lane valid: no masking
@@ -577,7 +567,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     }
     // one compensated add per mode and lane: the lane's 16 positive terms are
     // summed plainly (relative error <= 15 ulp of this mode's share)
-    two_sum_add(aP, cP, ps);
+    two_sum_add(aP, cP, active ? ps : 0.f);
     const float mx = wave_sum(sx) * inv_n0, my = wave_sum(sy) * inv_n0;
     if (dense) {
 #pragma unroll
@@ -591,12 +581,19 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     }
     // Tail lags tau = T - KT + j (few valid pairs) directly: the FFT route's
     // error is ~ulp x sum_t |y|^2, far above sum over the N(tau) <= KT pairs
-    // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer.
+    // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer: the
+    // head is in register 0 (KT <= 16 < 64), the tail in the registers
+    // q0..q1 (wave-uniform, at most two).
+    if (lane < KT) xch[lane] = y[0];
+    {
+      const int q0 = (T - KT) >> 6, q1 = (T - 1) >> 6;
 #pragma unroll
-    for (int q = 0; q < PT; q++) {
-      const int t = lane + 64 * q;
-      if (t < KT) xch[t] = y[q];
-      if (t >= T - KT && t < T) xch[KT + t - (T - KT)] = y[q];
+      for (int q = 0; q < PT; q++) {
+        if (q >= q0 && q <= q1) {
+          const int t = lane + 64 * q;
+          if (t >= T - KT && t < T) xch[KT + t - (T - KT)] = y[q];
+        }
+      }
     }
     wave_sync();
     {
@@ -610,7 +607,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
         }
       g += __shfl_xor(g, 1, 64);
       g += __shfl_xor(g, 2, 64);
-      if (s4 == 0 && j < KT) two_sum_add(Gt, Gtc, g);
+      if (s4 == 0 && j < KT && active) two_sum_add(Gt, Gtc, g);
     }
     if constexpr (LM == 0) {
       float2 v[PF];
@@ -638,12 +635,23 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
 #pragma unroll
         for (int k = 1; k < m; k++)
           if (r == k) tb = tbase[k];
+        // the wave-uniform factors w_m^{r t2} and w_n^{64 r q} are entries of
+        // the sub-transform table (w_NF^k) read as LDS broadcasts; w_m^{r m/4}
+        // = (-i)^r is applied as a swap and a sign
 #pragma unroll
         for (int q = 0; q < PF; q++) {
           float2 u = y[q];
 #pragma unroll
-          for (int t2 = 1; t2 < m / 2; t2++) u = cadd(u, cmul(y[q + t2 * PF], tw_g[((r * t2) & (m - 1)) * NF]));
-          v[q] = r ? cmul(u, cmul(tb, tw_g[(64 * r * q) & (n - 1)])) : u;
+          for (int t2 = 1; t2 < m / 2; t2++) {
+            const float2 b = y[q + t2 * PF];
+            if (4 * t2 == m) {
+              const float2 rb = (r & 1) ? make_float2(b.y, -b.x) : b;
+              u = (r & 2) ? csub(u, rb) : cadd(u, rb);
+            } else {
+              u = cadd(u, cmul(b, twl[((r * t2) & (m - 1)) * (NF / m)]));
+            }
+          }
+          v[q] = r ? cmul(u, cmul(tb, twl[((64 / m) * r * q) & (NF - 1)])) : u;
         }
         wave_fft<LOGF, false, false>(v, xch, twl, lane);
         // |Y(m j + r)|^2 parked r-major (index r NF + j): consecutive lanes,
@@ -1494,7 +1502,7 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
   case LG_:                                                                                                         \
     return launch_temporal_w_t<LG_, LM_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, tw_f, \
                                          S_part, p_part, P_part, KT, G_part, st);
-    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
+    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2)
 #undef CASE
   }
   return hipErrorInvalidValue;
