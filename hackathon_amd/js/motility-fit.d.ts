@@ -17,3 +17,13 @@ export declare function analyzeMotilityLM(radial: Radial, opt?: {
   /** 6 per usable bin: A, B, alpha, ln vbar, ln Z, ln D */
   seeds?: Float64Array; nnlsAmplitudes?: boolean;
 }): { pr: Prepared; fits: BinFit[]; kernelMs: number };
+
+export declare const GATES: Readonly<{ FLOW_SIGMA: number; FLOW_MIN_UM_S: number; DRIFT_ISF_MAX: number }>;
+export declare function isf3d(x: number, Z: number): number;
+export declare function isf2d(x: number, Z: number): number;
+export interface DriftEstimate {
+  Vx: number; Vy: number; mag: number; sigma: number; angleDeg: number; significant: boolean; pairs: number; samples: number;
+}
+/** Uniform drift from Radial.drift (readRadial with drift on); swimmer turns on the ISF lag gating. */
+export declare function estimateDrift(radial: Radial, pr: Prepared, opt?: { dxUm?: number; geometry?: '3d' | '2d' },
+  swimmer?: { vbar: number; Z: number } | null): DriftEstimate;

@@ -6,7 +6,9 @@
 // q-nodes from kCenter / kNodes, weight w = 1 / max D.  analyzeMotilityLM runs
 // every usable bin of a Radial record as one batched Levenberg-Marquardt launch
 // on the MI355X (libddm.so ddm_fit through the N-API addon) and returns the
-// fitted per-q parameters.  Types are declared in motility-fit.d.ts.
+// fitted per-q parameters.  estimateDrift turns the radial record's drift sums
+// into a uniform drift velocity (src/motility_fit.js:671-725).  Types are
+// declared in motility-fit.d.ts.
 import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
@@ -102,4 +104,86 @@ export function analyzeMotilityLM(radial, opt = {}) {
       D: Math.exp(t[5]), chi2: r.chi2[i], iters: r.iters[i] };
   });
   return { pr, fits, kernelMs: r.kernelMs };
+}
+
+// ---------------------------------------------------------------- drift
+
+/** The drift gates of src/motility_fit.js:28-38 used here. */
+export const GATES = Object.freeze({ FLOW_SIGMA: 3, FLOW_MIN_UM_S: 2, DRIFT_ISF_MAX: 0.2 });
+
+/** 3D Schulz-averaged swimmer self-ISF (src/motility_fit.js:135-144). */
+export function isf3d(x, Z) {
+  const u = x / (Z + 1);
+  if (u < 1e-4) return 1 - (x * x * (Z + 2)) / (6 * (Z + 1));
+  return Math.sin(Z * Math.atan(u)) / (Z * u * Math.exp(0.5 * Z * Math.log1p(u * u)));
+}
+
+// P_nu(c), |c| <= 1: trapezoid rule on Laplace's integral (src/motility_fit.js:120-132)
+function legendreP(nu, c) {
+  const s = Math.sqrt(Math.max(0, 1 - c * c));
+  const n = 16 + Math.ceil(nu * Math.atan2(s, c));
+  let acc = 0;
+  for (let i = 0; i <= n; i++) {
+    const im = s * Math.cos((Math.PI * i) / n);
+    const v = Math.exp(0.5 * nu * Math.log(c * c + im * im)) * Math.cos(nu * Math.atan2(im, c));
+    acc += i === 0 || i === n ? 0.5 * v : v;
+  }
+  return acc / n;
+}
+
+/** In-plane Schulz-averaged swimmer self-ISF (src/motility_fit.js:146-159). */
+export function isf2d(x, Z) {
+  const u = x / (Z + 1);
+  if (u < 1e-4) return 1 - (x * x * (Z + 2)) / (4 * (Z + 1));
+  const w = 1 + u * u;
+  const lnPre = -0.5 * (Z + 1) * Math.log(w);
+  return lnPre < -25 ? 0 : legendreP(Z, 1 / Math.sqrt(w)) * Math.exp(lnPre);
+}
+
+/**
+ * estimateDrift(radial, pr, opt, swimmer) (src/motility_fit.js:671-725): uniform
+ * drift V from the radial record's phase sums (readRadial with drift on), by
+ * weighted least squares of phi = -q.V dt over the usable bins' lag pairs; with
+ * a swimmer component {vbar, Z}, only pairs where its ISF is below
+ * GATES.DRIFT_ISF_MAX.  opt: {dxUm = 1, geometry = '3d'}.
+ */
+export function estimateDrift(radial, pr, opt = {}, swimmer = null) {
+  const { L, N } = radial;
+  if (!radial.drift || radial.drift.length < radial.NB * L * 8) throw new Error('estimateDrift: Radial record has no drift sums');
+  const g = pick(opt.geometry, '3d') === '2d' ? isf2d : isf3d;
+  const S = new Float64Array(8);
+  let modesUsed = 0;
+  let pairs = 0;
+  for (const bin of pr.bins) {
+    let used = false;
+    for (let tau = 1; tau < L - 1; tau++) {
+      if (swimmer && !(Math.abs(g(bin.q * swimmer.vbar * tau * pr.dt, swimmer.Z)) < GATES.DRIFT_ISF_MAX)) continue;
+      const o = (bin.b * L + tau) * 8;
+      if (!(radial.drift[o] > 0)) continue;
+      for (let c = 0; c < 8; c++) S[c] += radial.drift[o + c];
+      used = true;
+      pairs++;
+    }
+    if (used) modesUsed += radial.nModes[bin.b];
+  }
+  const [Sw, Sxx, Sxy, Syy, Rx, Ry, Rpp, n] = S;
+  const det = Sxx * Syy - Sxy * Sxy;
+  if (n < 10 || !(det > 1e-12 * Sxx * Syy) || !(Sw > 0)) {
+    return { Vx: 0, Vy: 0, mag: 0, sigma: Infinity, angleDeg: 0, significant: false, pairs, samples: n };
+  }
+  const bx = (Syy * Rx - Sxy * Ry) / det;
+  const by = (Sxx * Ry - Sxy * Rx) / det;
+  const rss = Math.max(Rpp - bx * Rx - by * Ry, 0);
+  const s2 = (rss / (n - 2)) * Math.max(1, n / Math.max(modesUsed, 1));
+  const conv = -(N * pick(opt.dxUm, 1)) / (2 * Math.PI * pr.dt);
+  const Vx = conv * bx;
+  const Vy = conv * by;
+  const mag = Math.hypot(Vx, Vy);
+  // variance along V (isotropic mean when V = 0) of the covariance s2 [Sxx Sxy; Sxy Syy]^-1
+  const cxx = (s2 * Syy) / det, cyy = (s2 * Sxx) / det, cxy = (-s2 * Sxy) / det;
+  const ux = mag > 0 ? Vx / mag : 0, uy = mag > 0 ? Vy / mag : 0;
+  const varAlong = conv * conv * (mag > 0 ? ux * ux * cxx + 2 * ux * uy * cxy + uy * uy * cyy : (cxx + cyy) / 2);
+  const sigma = Math.sqrt(Math.max(varAlong, 0));
+  return { Vx, Vy, mag, sigma, angleDeg: (Math.atan2(Vy, Vx) * 180) / Math.PI,
+    significant: mag > GATES.FLOW_SIGMA * sigma, pairs, samples: n };
 }

@@ -606,6 +606,23 @@ void orc_analyze_radial(const double *hist, int T, const uint32_t *order, const 
  * z = C(tau+1) conj C(tau), w = |z|, phi = atan2(z); drift[(b*L+tau)*8 + c] =
  * sum of w, w kx^2, w kx ky, w ky^2, w kx phi, w ky phi, w phi^2, [w > 0]
  * (Math.hypot of the reference is the correctly rounded hypot here). */
+static void drift_accumulate(const double *c, int L, double kx, double ky, double *bin_drift) {
+  for (int tau = 0; tau < L - 1; tau++) {
+    double ar = c[2 * tau], ai = c[2 * tau + 1], br = c[2 * tau + 2], bi = c[2 * tau + 3];
+    double zr = br * ar + bi * ai, zi = bi * ar - br * ai;
+    double w = hypot(zr, zi), phi = atan2(zi, zr);
+    double *o = bin_drift + (long)tau * 8;
+    o[0] += w;
+    o[1] += w * kx * kx;
+    o[2] += w * kx * ky;
+    o[3] += w * ky * ky;
+    o[4] += w * kx * phi;
+    o[5] += w * ky * phi;
+    o[6] += w * phi * phi;
+    o[7] += w > 0 ? 1 : 0;
+  }
+}
+
 void orc_radial_drift(const double *hist, int T, const uint32_t *order, const float *mask, const uint32_t *binOffsets,
                       const int32_t *binModes, int NB, int L, double *drift) {
   double *d = (double *)malloc(sizeof(double) * L);
@@ -614,26 +631,23 @@ void orc_radial_drift(const double *hist, int T, const uint32_t *order, const fl
   for (int b = 0; b < NB; b++) {
     for (uint32_t i = binOffsets[b]; i < binOffsets[b + 1]; i++) {
       long m = binModes[4 * i];
-      double kx = binModes[4 * i + 1], ky = binModes[4 * i + 2];
       orc_temporal_mode(hist, m, T, order, mask, L, d, c);
-      for (int tau = 0; tau < L - 1; tau++) {
-        double ar = c[2 * tau], ai = c[2 * tau + 1], br = c[2 * tau + 2], bi = c[2 * tau + 3];
-        double zr = br * ar + bi * ai, zi = bi * ar - br * ai;
-        double w = hypot(zr, zi), phi = atan2(zi, zr);
-        double *o = drift + ((long)b * L + tau) * 8;
-        o[0] += w;
-        o[1] += w * kx * kx;
-        o[2] += w * kx * ky;
-        o[3] += w * ky * ky;
-        o[4] += w * kx * phi;
-        o[5] += w * ky * phi;
-        o[6] += w * phi * phi;
-        o[7] += w > 0 ? 1 : 0;
-      }
+      drift_accumulate(c, L, binModes[4 * i + 1], binModes[4 * i + 2], drift + (long)b * L * 8);
     }
   }
   free(d);
   free(c);
+}
+
+/* The same sums from given per-mode C(tau) (complex, C[(m*L + tau)*2 + {0,1}]),
+ * as radialFromFull (src/ddm_ref.js:474-487) does for per-mode arrays. */
+void orc_drift_from_modes(const double *C, const uint32_t *binOffsets, const int32_t *binModes, int NB, int L,
+                          double *drift) {
+  memset(drift, 0, sizeof(double) * NB * L * 8);
+  for (int b = 0; b < NB; b++)
+    for (uint32_t i = binOffsets[b]; i < binOffsets[b + 1]; i++)
+      drift_accumulate(C + (long)binModes[4 * i] * L * 2, L, binModes[4 * i + 1], binModes[4 * i + 2],
+                       drift + (long)b * L * 8);
 }
 
 /* Same reduction from direct O(T*L) sums (cancellation-free reference for the

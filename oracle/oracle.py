@@ -53,6 +53,7 @@ def lib():
         L.orc_analyze_radial.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
         L.orc_analyze_radial_split.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp, _dp]
         L.orc_radial_drift.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp]
+        L.orc_drift_from_modes.argtypes = [_dp, _u32p, _i32p, C.c_int, C.c_int, _dp]
         L.orc_analyze_radial_direct.argtypes = [_dp, C.c_int, _u32p, _fp, _u32p, _i32p, C.c_int, C.c_int, _dp,
                                                 _dp]
         for name in ("orc_bessel_j0", "orc_lngamma"):
@@ -196,6 +197,26 @@ def radial_drift(hist: np.ndarray, order, mask, bins: dict, L: int) -> np.ndarra
                            np.ascontiguousarray(mask, dtype=np.float32), bins["binOffsets"],
                            np.ascontiguousarray(bins["binModes"].ravel()), NB, L, out)
     return out.reshape(NB, L, 8)
+
+
+def radial_from_full(D: np.ndarray, Cm: np.ndarray, P: np.ndarray, bins: dict, L: int) -> dict:
+    """radialFromFull (src/ddm_ref.js:474-487, finalizeRadial :404-412) from per-mode arrays D[modes, L],
+    complex C[modes, L], P[modes]: bin means of D and P, drift sums [NB, L, 8]."""
+    NB = bins["NB"]
+    off = bins["binOffsets"]
+    bm = np.ascontiguousarray(bins["binModes"].ravel())
+    Dr = np.zeros((NB, L))
+    Pr = np.zeros(NB)
+    for b in range(NB):
+        ms = bins["binModes"][off[b]:off[b + 1], 0]
+        if len(ms):
+            Dr[b] = np.asarray(D)[ms].sum(axis=0) / len(ms)
+            Pr[b] = np.asarray(P)[ms].sum() / len(ms)
+    c = np.ascontiguousarray(Cm, dtype=np.complex128).view(np.float64).ravel()
+    drift = np.zeros(NB * L * 8)
+    lib().orc_drift_from_modes(c, off, bm, NB, L, drift)
+    return dict(N=bins["N"], NB=NB, L=L, kCenter=bins["kCenter"], kNodes=bins["kNodes"], nModes=bins["nModes"],
+                D=Dr.ravel(), P=Pr, drift=drift)
 
 
 # ------------------------------------------------------------------ special functions / fit blocks

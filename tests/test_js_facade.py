@@ -108,3 +108,49 @@ def test_js_shuffle_order_matches_oracle():
     order, mask = O.shuffle_order(np.arange(T), np.array([i % 7 != 3 for i in range(T)], np.float32),
                                   O.mulberry32(4242))
     assert out["order"] == order.tolist() and out["mask"] == mask.tolist()
+
+
+def test_js_estimate_drift_matches_python(tmp_path):
+    """estimateDrift of the TS-facing module against the Python host version on a synthetic radial
+    record (drift V plus a coherent swimmer term), open and ISF-gated (no GPU work)."""
+    import math
+    from tests.test_drift_estimate import DX, FPS, q_of, synth_radial
+    from hackathon_amd import drift as DR
+    from hackathon_amd.fit import prepare
+    if not os.path.exists(ADDON):
+        pytest.fail("N-API addon not built (python -m hackathon_amd.napi.build_addon)")
+    rng = np.random.default_rng(3)
+
+    def gen(kx, ky, t):
+        qx, qy = q_of(kx), q_of(ky)
+        g = np.exp(-0.05 * np.hypot(qx, qy) * 40 * t / FPS)
+        ph = -(qx * 3.0 - qy * 4.0) * t / FPS
+        noise = 0.02 * (rng.standard_normal(np.broadcast(ph, g).shape))
+        return np.exp(1j * ph) + g * np.exp(1j * 6 * ph) + noise
+
+    r = synth_radial(gen)
+    rec = dict(N=r.N, NB=r.NB, L=r.L, kCenter=r.kCenter.tolist(), kNodes=r.kNodes.tolist(),
+               nModes=[int(x) for x in r.nModes], D=r.D.tolist(), P=r.P.tolist(), drift=r.drift.tolist())
+    (tmp_path / "radial.json").write_text(json.dumps(rec))
+    mod = os.path.join(ROOT, "hackathon_amd", "js", "motility-fit.mjs")
+    code = ("Promise.all([import('" + mod + "'), import('fs')]).then(([m, fs]) => {"
+            " const r = JSON.parse(fs.readFileSync('" + str(tmp_path / "radial.json") + "', 'utf8'));"
+            " for (const k of ['kCenter', 'kNodes', 'D', 'P', 'drift']) r[k] = Float64Array.from(r[k]);"
+            " r.nModes = Uint32Array.from(r.nModes);"
+            f" const pr = m.prepare(r, {DX}, {FPS});"
+            f" const out = [m.estimateDrift(r, pr, {{dxUm: {DX}}}),"
+            f"  m.estimateDrift(r, pr, {{dxUm: {DX}}}, {{vbar: 40, Z: 4}}),"
+            f"  m.estimateDrift(r, pr, {{dxUm: {DX}, geometry: '2d'}}, {{vbar: 40, Z: 4}})];"
+            " console.log('RESULT ' + JSON.stringify(out)); })")
+    res = subprocess.run(["node", "--input-type=module", "-e", code], capture_output=True, text=True, timeout=60)
+    line = [x for x in res.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, res.stdout + res.stderr
+    out = json.loads(line[0][7:])
+    pr = prepare(r, DX, FPS)
+    want = [DR.estimate_drift(r, pr, DX), DR.estimate_drift(r, pr, DX, swimmer=(40, 4)),
+            DR.estimate_drift(r, pr, DX, geometry="2d", swimmer=(40, 4))]
+    for o, w in zip(out, want):
+        assert o["pairs"] == w.pairs and o["samples"] == w.samples and o["significant"] == w.significant
+        for k in ("Vx", "Vy", "mag", "sigma", "angleDeg"):
+            assert math.isclose(o[k], getattr(w, k), rel_tol=1e-12, abs_tol=1e-12), (k, o[k], getattr(w, k))
+    assert out[1]["pairs"] < out[0]["pairs"]
