@@ -75,3 +75,24 @@ export declare class DdmEngine {
 export declare function linearOrder(valid: ArrayLike<boolean | number>): { order: Uint32Array; mask: Float32Array };
 export declare function shuffleOrder(o: { order: Uint32Array; mask: Float32Array }, rand: () => number): { order: Uint32Array; mask: Float32Array };
 export declare function mulberry32(seed: number): () => number;
+
+export type TimeSource = 'captureTime' | 'mediaTime' | 'watchdog';
+export interface Snapshot { order: Uint32Array; mask: Float32Array; used: number; newestN: number }
+/** Live frame -> history-slot clock (the reference's FrameClock, ring length T). */
+export declare class FrameClock {
+  constructor(fps: number, T?: number);
+  readonly T: number;
+  readonly dtMs: number;
+  frames: number;
+  missed: number;
+  duplicates: number;
+  resets: number;
+  readonly timeSource: TimeSource | null;
+  readonly newestN: number;
+  readonly medianDtMs: number;
+  readonly validCount: number;
+  reset(): void;
+  place(tMs: number, source?: TimeSource): { slot: number; n: number } | null;
+  drop(n: number): void;
+  snapshot(opts?: { guard?: number; shuffle?: boolean; rand?: () => number }): Snapshot;
+}

@@ -127,7 +127,7 @@ export function mulberry32(seed) {
   };
 }
 
-/** shuffleOrder (src/ddm_ref.js:504-519): seeded Fisher-Yates over the (slot, mask) pairs. */
+/** shuffleOrder (src/ddm_ref.js:510-519): seeded Fisher-Yates over the (slot, mask) pairs. */
 export function shuffleOrder(o, rand) {
   const order = Uint32Array.from(o.order);
   const mask = Float32Array.from(o.mask);
@@ -146,4 +146,114 @@ export function linearOrder(valid) {
   const mask = new Float32Array(T);
   for (let j = 0; j < T; j++) { order[j] = j; mask[j] = valid[j] ? 1 : 0; }
   return { order, mask };
+}
+
+/**
+ * FrameClock (src/camera.ts:306-425): frame n = round((t - t0) / dt) goes to
+ * slot n mod T of the history ring; each slot remembers the frame index it
+ * holds, so snapshot() masks dropped, late, duplicated and overwritten frames.
+ * T is the engine's ring length (the reference fixes 256).  Time sources
+ * 'captureTime' and 'watchdog' share a timebase; switching to or from
+ * 'mediaTime' restarts the clock.
+ */
+export class FrameClock {
+  constructor(fps, T = DDM.T) {
+    if (!(fps > 0) || !(T >= 1)) throw new Error('FrameClock: fps must be positive and T >= 1');
+    this.T = T;
+    this.dtMs = 1000 / fps;
+    this.frames = 0;
+    this.missed = 0;
+    this.duplicates = 0;
+    this.resets = 0;
+    this._slotN = new Float64Array(T).fill(-1);
+    this._t0 = null;
+    this._lastN = -1;
+    this._src = null;
+    this._lastTs = NaN;
+    this._dts = [];
+  }
+
+  get timeSource() { return this._src; }
+
+  get newestN() { return this._lastN; }
+
+  reset() {
+    this._t0 = null;
+    this._lastN = -1;
+    this._slotN.fill(-1);
+    this._src = null;
+    this._lastTs = NaN;
+    this.resets++;
+  }
+
+  /** {slot, n} for a frame captured at tMs, or null for a duplicate / late frame. */
+  place(tMs, source = 'captureTime') {
+    if (this._src !== null && (source === 'mediaTime') !== (this._src === 'mediaTime')) this.reset();
+    this._src = source;
+    if (this._t0 === null) this._t0 = tMs;
+    let n = Math.round((tMs - this._t0) / this.dtMs);
+    if (n <= this._lastN) {
+      this.duplicates++;
+      return null;
+    }
+    if (this._lastN >= 0 && n - this._lastN > this.T) {   // gap longer than the ring: restart here
+      this.reset();
+      this._src = source;
+      this._t0 = tMs;
+      n = 0;
+    }
+    if (this._lastN >= 0) this.missed += n - this._lastN - 1;
+    if (Number.isFinite(this._lastTs)) {
+      this._dts.push(tMs - this._lastTs);
+      if (this._dts.length > 120) this._dts.shift();
+    }
+    this._lastTs = tMs;
+    const slot = n % this.T;
+    this._slotN[slot] = n;
+    this._lastN = n;
+    this.frames++;
+    return { slot, n };
+  }
+
+  /** Frame n was not ingested (e.g. back-pressure). */
+  drop(n) {
+    const slot = n % this.T;
+    if (this._slotN[slot] === n) this._slotN[slot] = -1;
+  }
+
+  get medianDtMs() {
+    if (!this._dts.length) return NaN;
+    const s = this._dts.slice().sort((a, b) => a - b);
+    return s[s.length >> 1];
+  }
+
+  get validCount() {
+    let c = 0;
+    for (let j = 0; j < this.T; j++) {
+      const n = this._lastN - this.T + 1 + j;
+      if (n >= 0 && this._slotN[n % this.T] === n) c++;
+    }
+    return c;
+  }
+
+  /** Time-ordered {order, mask, used, newestN}; the oldest `guard` indices masked; shuffle = null test. */
+  snapshot(opts = {}) {
+    const guard = pick(opts.guard, 0);
+    const order = new Uint32Array(this.T);
+    const mask = new Float32Array(this.T);
+    let used = 0;
+    for (let j = 0; j < this.T; j++) {
+      const n = this._lastN - this.T + 1 + j;
+      const slot = ((n % this.T) + this.T) % this.T;
+      const ok = n >= 0 && j >= guard && this._slotN[slot] === n;
+      order[j] = slot;
+      mask[j] = ok ? 1 : 0;
+      if (ok) used++;
+    }
+    if (opts.shuffle) {
+      const s = shuffleOrder({ order, mask }, opts.rand || Math.random);
+      return { order: s.order, mask: s.mask, used, newestN: this._lastN };
+    }
+    return { order, mask, used, newestN: this._lastN };
+  }
 }
