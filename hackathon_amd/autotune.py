@@ -28,7 +28,7 @@ PARITY_RTOL = 1e-5
 
 def candidates(N: int, quick: bool = False):
     items = (2048, 8192) if quick else (1024, 2048, 4096, 8192)
-    chunks = (128,) if quick else (64, 128, 256)
+    chunks = (128,) if quick else (64, 128, 256, 512, 1024)
     waves = (1, 2) if N in (512, 1024) else (1,)
     out = []
     for it, ch, wv in itertools.product(items, chunks, waves):
