@@ -518,7 +518,10 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   auto load_series = [&](float2 (&dst)[PT], int slot) {
     const float2* x = hist + (long)bin_modes[start + (slot < count ? slot : count - 1)] * T;
 #pragma unroll
-    for (int q = 0; q < PT; q++) dst[q] = x[lane + 64 * q];
+    for (int q = 0; q < PT; q++) {   // each series is read once: non-temporal loads
+      const float2* src = &x[lane + 64 * q];
+      dst[q] = make_float2(__builtin_nontemporal_load(&src->x), __builtin_nontemporal_load(&src->y));
+    }
   };
   // next round's series, loaded while the current one is transformed (up to
   // n = 2048; above that the registers are needed by the split transforms)
@@ -1193,8 +1196,9 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
     float xa[P], xb[P];
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      xa[q] = px_to_float(rowa[lane + 64 * q]);
-      xb[q] = px_to_float(rowb[lane + 64 * q]);
+      // frames are read once: non-temporal loads
+      xa[q] = px_to_float(__builtin_nontemporal_load(&rowa[lane + 64 * q]));
+      xb[q] = px_to_float(__builtin_nontemporal_load(&rowb[lane + 64 * q]));
     }
     float ha = 0.f, hb = 0.f, la = 0.f, lb = 0.f;
     int Sa = 0, Sb = 0;
@@ -1369,7 +1373,12 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
     const int fl = e % F, ky = e / F;
     if (f0 + fl >= nframes) continue;
     const int slot = (first_slot + f0 + fl) % T;
-    hist[((long)ky * NH + kx) * T + slot] = tile[ky * TS + fl];
+    // non-temporal: the history is read back only by the temporal pass, after
+    // the whole window; keeping it out of the caches leaves them to the row spectra
+    const float2 z = tile[ky * TS + fl];
+    float2* dst = &hist[((long)ky * NH + kx) * T + slot];
+    __builtin_nontemporal_store(z.x, &dst->x);
+    __builtin_nontemporal_store(z.y, &dst->y);
   }
 }
 

@@ -69,7 +69,7 @@ def test_autotune_candidates_pass_parity_then_time(gpu):
     fr = g.video(g.params(256, 512, 400, 2.0, 100, 200, D=0.3, alpha=0.6, vbar_px_frame=0.15, Z=3,
                           dtype="uint16", seed=3))
     geom = dict(N=512, T=256, L=256, NB=64, kmin=0, kmax=256)
-    best, table = AT.autotune(geom, fr.data_ptr(), np.uint16, 256, reps=2, quick=True)
+    best, table = AT.autotune(geom, fr.data_ptr(), np.uint16, 256, reps=2, quick=True, inflight=2)
     assert all(r["ok"] for r in table), [r for r in table if not r["ok"]]
     assert all(r["ms"] > 0 for r in table)
     timed = [r for r in table if r["ok"]]
