@@ -213,11 +213,19 @@ __host__ __device__ constexpr int wave_lr(int logn, int nslog) {
   return next_lr(logn - nslog) < (logn - 6) ? next_lr(logn - nslog) : (logn - 6);
 }
 
+// Twiddle-table slot of entry m in an LDS-staged table: one spare slot per 16
+// entries.  The radix-8 passes read entries k 2^s and 4 k 2^s for k = lane mod
+// Ns; in a plain table those land on a few bank pairs (up to 8-way conflicts),
+// padded they spread over all 16 pairs of ds_read_b64.  The slot is an
+// affine function of k within each pass, so the loads keep immediate offsets.
+__host__ __device__ constexpr int twpad(int m) { return m + (m >> 4); }
+
 // Twiddles w^r, r = 1..R-1, of one butterfly: w^1 and w^4 come from the
-// table, the others are at most two products away from them.
-template <int R, bool INV>
+// table, the others are at most two products away from them.  TWP: the table
+// is stored at twpad() slots.
+template <int R, bool INV, bool TWP = false>
 __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, float2* w) {
-  float2 w1 = tw[m1];
+  float2 w1 = tw[TWP ? twpad(m1) : m1];
   if (INV) w1.y = -w1.y;
   w[1] = w1;
   if (R >= 4) {
@@ -225,7 +233,7 @@ __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, flo
     w[3] = cmul(w[2], w1);
   }
   if (R >= 8) {
-    float2 w4 = tw[4 * m1];
+    float2 w4 = tw[TWP ? twpad(4 * m1) : 4 * m1];
     if (INV) w4.y = -w4.y;
     w[4] = w4;
     w[5] = cmul(w4, w1);
@@ -234,8 +242,9 @@ __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, flo
   }
 }
 
-template <int LOGN, int NSLOG, bool INV, bool HALF>
-__device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane) {
+template <int LOGN, int NSLOG, bool INV, bool HALF, bool TWP = false, bool WL = false>
+__device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
+                                          const float2* wl = nullptr) {
   constexpr int n = 1 << LOGN;
   constexpr int LR = wave_lr(LOGN, NSLOG);
   constexpr int R = 1 << LR;
@@ -256,7 +265,12 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
     for (int r = 0; r < R; r++) a[r] = v[b + r * QS];
     if (NSLOG > 0) {
       float2 w[R];
-      butterfly_twiddles<R, INV>(tw, k << (LOGN - NSLOG - LR), w);
+      if constexpr (WL && LAST && R == 8 && NBL == 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) w[r] = wl[r];
+      } else {
+        butterfly_twiddles<R, INV, TWP>(tw, k << (LOGN - NSLOG - LR), w);
+      }
 #pragma unroll
       for (int r = 1; r < R; r++) a[r] = cmul(a[r], w[r]);
     }
@@ -278,15 +292,24 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
     wave_sync();
 #pragma unroll
     for (int q = 0; q < n / 64; q++) v[q] = xch[xpad(lane + 64 * q)];
-    wave_pass<LOGN, NSLOG + LR, INV, false>(v, xch, tw, lane);
+    wave_pass<LOGN, NSLOG + LR, INV, false, TWP, WL>(v, xch, tw, lane, wl);
   }
 }
 
 // In-place canonical-distribution FFT of one wavefront.  HALF: inputs with
-// index >= n/2 are known to be zero (v[q] for q >= n/128 are ignored).
-template <int LOGN, bool INV, bool HALF>
+// index >= n/2 are known to be zero (v[q] for q >= n/128 are ignored).  TWP:
+// tw is an LDS table stored at twpad() slots.
+template <int LOGN, bool INV, bool HALF, bool TWP = false>
 __device__ __forceinline__ void wave_fft(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane) {
-  wave_pass<LOGN, 0, INV, HALF>(v, xch, tw, lane);
+  wave_pass<LOGN, 0, INV, HALF, TWP>(v, xch, tw, lane);
+}
+// The same with the last pass's twiddles supplied by the caller (wl[1..7]; a
+// radix-8 last pass with one butterfly per lane, k = lane): a caller that runs
+// many transforms keeps them in registers instead of rebuilding them each time.
+template <int LOGN, bool INV, bool HALF, bool TWP>
+__device__ __forceinline__ void wave_fft_wl(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
+                                            const float2* wl) {
+  wave_pass<LOGN, 0, INV, HALF, TWP, true>(v, xch, tw, lane, wl);
 }
 
 // 2Sum accumulate (branch-free): (s, c) += x, value s + c.

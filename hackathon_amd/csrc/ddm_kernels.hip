@@ -479,16 +479,20 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * WS;
   float* red = reinterpret_cast<float*>(lds + 4 * WS);  // 4
-  // the sub-transform's twiddle table, staged once per workgroup: the FFT
-  // passes read it from LDS instead of waiting on global loads every pass
+  // the sub-transform's twiddle table, staged once per workgroup at twpad()
+  // slots: the FFT passes read it from LDS instead of waiting on global loads
   float2* twl = lds + 4 * WS + 2;
-  for (int i = tid; i < NF; i += NT) twl[i] = tw_f[i];
+  for (int i = tid; i < NF; i += NT) twl[twpad(i)] = tw_f[i];
   // DIF pre-twiddles w_n^{r (lane + 64 q)} = w_n^{r lane} w_n^{64 r q}: the
   // per-lane factor is loaded once here, the second is wave-uniform
   float2 tbase[m > 1 ? m : 1];
 #pragma unroll
   for (int r = 0; r < (m > 1 ? m : 1); r++) tbase[r] = tw_g[r * lane];
   __syncthreads();
+  // last-pass twiddles w_NF^{r lane} of the 512-point sub-transforms, the same
+  // for every mode: kept in registers
+  float2 wlast[8];
+  if constexpr (LOGF == 9) butterfly_twiddles<8, false, true>(twl, lane, wlast);
   const int4 it = items[blockIdx.x];
   const int start = it.y, count = it.z;
   uint32_t mbits = 0;
@@ -618,7 +622,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
       for (int q = 0; q < PT; q++) v[q] = y[q];
 #pragma unroll
       for (int q = PT; q < PF; q++) v[q] = make_float2(0.f, 0.f);
-      wave_fft<LOGF, false, true>(v, xch, twl, lane);
+      wave_fft<LOGF, false, true, true>(v, xch, twl, lane);
       wave_sync();
 #pragma unroll
       for (int q = 0; q < PF; q++) park[lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
@@ -651,12 +655,13 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
               const float2 rb = (r & 1) ? make_float2(b.y, -b.x) : b;
               u = (r & 2) ? csub(u, rb) : cadd(u, rb);
             } else {
-              u = cadd(u, cmul(b, twl[((r * t2) & (m - 1)) * (NF / m)]));
+              u = cadd(u, cmul(b, twl[twpad(((r * t2) & (m - 1)) * (NF / m))]));
             }
           }
-          v[q] = r ? cmul(u, cmul(tb, twl[((64 / m) * r * q) & (NF - 1)])) : u;
+          v[q] = r ? cmul(u, cmul(tb, twl[twpad(((64 / m) * r * q) & (NF - 1))])) : u;
         }
-        wave_fft<LOGF, false, false>(v, xch, twl, lane);
+        if constexpr (LOGF == 9) wave_fft_wl<LOGF, false, false, true>(v, xch, twl, lane, wlast);
+        else wave_fft<LOGF, false, false, true>(v, xch, twl, lane);
         // |Y(m j + r)|^2 parked r-major (index r NF + j): consecutive lanes,
         // consecutive words, no bank conflicts; k_bin_sum undoes the order
 #pragma unroll
@@ -1301,6 +1306,15 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
   float2* tile = lds;                          // N x TS, reused after every wave's FFTs
   const int kx = blockIdx.x;
   const int f0 = blockIdx.y * F;
+  // N <= 512: the column FFT's twiddle table in LDS (twpad slots, behind the
+  // tile/exchange area), read there instead of waiting on global loads; at
+  // N = 1024 the extra 8.5 KB would cost a resident workgroup
+  constexpr bool TWL = LOGN <= 9;
+  float2* twl = lds + (N * TS > 4 * XS ? N * TS : 4 * XS);
+  if constexpr (TWL) {
+    for (int i = tid; i < N; i += NT) twl[twpad(i)] = tw[i];
+    __syncthreads();
+  }
   float2 res[FW][P];
 #pragma unroll
   for (int i = 0; i < FW; i++) {
@@ -1354,7 +1368,7 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
           }
         }
       }
-      wave_fft<LOGN, false, false>(res[i], xch, tw, lane);
+      wave_fft<LOGN, false, false, TWL>(res[i], xch, TWL ? twl : tw, lane);
     }
   }
   __syncthreads();   // every wave is done with its exchange buffer: the tile reuses the space
@@ -1489,7 +1503,7 @@ static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* ite
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
                                       const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                                       float* P_part, int KT, float* G_part, hipStream_t st) {
-  size_t lds = sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + ((1 << LOGT) >> LM));
+  size_t lds = sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + twpad((1 << LOGT) >> LM));
   auto kern = k_temporal_w<LOGT, LM>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -1623,7 +1637,7 @@ static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, in
                                   int first_slot, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
   size_t tile = sizeof(float2) * N * (F + 1), xch = sizeof(float2) * 4 * xch_size(N);
-  size_t lds = tile > xch ? tile : xch;
+  size_t lds = (tile > xch ? tile : xch) + (LOGN <= 9 ? sizeof(float2) * twpad(N) : 0);
   auto kern = k_cols_w<LOGN, F>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
