@@ -35,7 +35,8 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_mlp_set_params", "ddm_mlp_get_params", "ddm_mlp_forward", "ddm_mlp_loss_grad", "ddm_mlp_train_step",
            "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
            "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
-           "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full")
+           "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full", "ddm_analyze_motility",
+           "ddm_verdict_last_error")
 
 
 class DdmError(RuntimeError):
@@ -96,7 +97,8 @@ def load_library(path: str = LIB_PATH):
     for name in EXPORTS:
         if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
                         "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error",
-                        "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full"):
+                        "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full",
+                        "ddm_verdict_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -304,6 +304,72 @@ int ddm_synth_video(ddm_synth *s, const ddm_video_params *p, void *dframes);
 /* Particle positions only: dpos[(t * n_spots + i) * 2 + {x, y}] (device, f32). */
 int ddm_synth_tracks(ddm_synth *s, const ddm_video_params *p, float *dpos);
 
+/* ---------------------------------------------------------------- verdict layer
+ * analyzeMotility (src/motility_fit.js:808-990) on the host, FP64: global
+ * VarPro fits of the Brownian and Schulz-swimmer models (grid + golden
+ * section, multi-start Nelder-Mead), the drift estimate and a drift-aware
+ * second pass, BIC on the effective sample size, TSI, motile share, tau_1/2
+ * scaling, gates G1-G5 and the verdict.  Host code only: no GPU work. */
+enum { DDM_GEOMETRY_3D = 0, DDM_GEOMETRY_2D = 1 };
+enum {
+  DDM_VERDICT_MOTILE = 0,
+  DDM_VERDICT_NOT_MOTILE = 1,
+  DDM_VERDICT_DIRECTED_TRANSPORT = 2,
+  DDM_VERDICT_NO_DYNAMICS = 3,
+  DDM_VERDICT_UNLOCKED_EXPOSURE = 4
+};
+
+/* The Radial record (src/ddm_ref.js:336-348) as readRadial returns it, FP64:
+ * kNodes[NB * knodes] (knodes = 0: q-nodes are the bin centres), D[NB * L],
+ * drift[NB * L * 8] (the phase sums, ddm_read_radial with drift). */
+typedef struct ddm_radial_record {
+  int32_t N, NB, L, knodes;
+  const double *kCenter, *kNodes, *D, *drift;
+  const uint32_t *nModes;
+} ddm_radial_record;
+
+typedef struct ddm_motility_opts {
+  double dx_um, fps;        /* AnalysisOptions.dxUm, .fps */
+  int32_t geometry;         /* DDM_GEOMETRY_3D (default of the reference) or _2D */
+  int32_t exposure_locked;  /* AnalysisOptions.exposureLocked */
+} ddm_motility_opts;
+
+/* estimateDrift's result (src/motility_fit.js:671-725) */
+typedef struct ddm_drift_estimate {
+  double Vx, Vy, mag, sigma, angle_deg, samples;
+  int32_t significant, pairs;
+} ddm_drift_estimate;
+
+/* Gate: id 0 (usable q-bins), 1 (TSI), 2 (uniform flow), 3 (delta BIC),
+ * 4 (motile share), 5 (exposure locked), in the reference's order. */
+typedef struct ddm_gate {
+  int32_t id, pass;
+  double value, sigma;
+  int32_t resolved_bins, pad_;
+} ddm_gate;
+
+typedef struct ddm_motility_result {
+  int32_t verdict, physics_verdict, insufficient, ngates;
+  ddm_gate gates[6];
+  double TSI, alpha, vbar, sigma_v, Z, Ddiff, Ddiff_brownian, dBIC, mu, nData, nEff, residual_rho;
+  double swim_rss, swim_Vmag, brown_rss, brown_Vmag;
+  int32_t alpha_resolved, resolved_bins, nbins, nscaling;
+  ddm_drift_estimate drift;
+} ddm_motility_result;
+
+/* Optional per-bin detail (any pointer may be NULL; arrays hold cap entries):
+ * usable bins' index and q, both models' amplitudes (A, B) per bin, and the
+ * (q, tau_1/2) points of the scaling fit. */
+typedef struct ddm_motility_detail {
+  int32_t cap, pad_;
+  int32_t *bin;
+  double *q, *swim_amps, *brown_amps, *scaling_q, *scaling_tau_half;
+} ddm_motility_detail;
+
+int ddm_analyze_motility(const ddm_radial_record *r, const ddm_motility_opts *opt, ddm_motility_result *res,
+                         ddm_motility_detail *detail);
+const char *ddm_verdict_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1458,3 +1458,160 @@ double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int 
   if (iters) *iters = tot;
   return best;
 }
+
+/* ---------------------------------------------------------------- scenario simulator
+ * test/sim.mjs:13-205 (simulate, makeRng, mulberry32): Gaussian blobs on a
+ * periodic field, Schulz-speed swimmers (3D-isotropic headings seen in
+ * projection, or in-plane), Brownian and fixed particles, a uniform drift,
+ * additive Gaussian camera noise, 8-bit quantisation, dropped frames.  The
+ * draws are taken in the reference's order from the same generators, so the
+ * movies are the reference's up to libm's last-ulp differences in exp/log. */
+typedef struct {
+  uint32_t a;
+  int have_spare;
+  double spare;
+} orc_rng;
+
+static double rng_u(orc_rng *r) {   /* mulberry32 */
+  r->a += 0x6D2B79F5u;
+  uint32_t t = r->a;
+  t = (t ^ (t >> 15)) * (t | 1u);
+  t ^= t + (t ^ (t >> 7)) * (t | 61u);
+  return (double)(t ^ (t >> 14)) / 4294967296.0;
+}
+
+static double rng_normal(orc_rng *r) {   /* polar Box-Muller with one spare */
+  if (r->have_spare) {
+    r->have_spare = 0;
+    return r->spare;
+  }
+  double x, y, s;
+  do {
+    x = 2 * rng_u(r) - 1;
+    y = 2 * rng_u(r) - 1;
+    s = x * x + y * y;
+  } while (s >= 1 || s == 0);
+  const double f = sqrt(-2 * log(s) / s);
+  r->spare = y * f;
+  r->have_spare = 1;
+  return x * f;
+}
+
+static double rng_gamma(orc_rng *r, double k, double theta) {   /* Marsaglia-Tsang */
+  if (k < 1) return rng_gamma(r, k + 1, theta) * pow(rng_u(r), 1 / k);
+  const double d = k - 1.0 / 3, c = 1 / sqrt(9 * d);
+  for (;;) {
+    double x, v;
+    do {
+      x = rng_normal(r);
+      v = 1 + c * x;
+    } while (v <= 0);
+    v = v * v * v;
+    const double uu = rng_u(r);
+    if (uu < 1 - 0.0331 * x * x * x * x) return d * v * theta;
+    if (log(uu) < 0.5 * x * x + d * (1 - v + log(v))) return d * v * theta;
+  }
+}
+
+typedef struct {
+  int N, T, seed, n_swim, swim_2d, n_passive, n_fixed, drop_exact;
+  double fps, dx_um, vbar, sigma_v, D_um2s, drift_speed, drift_angle_deg, drop_frac, blob_sigma, amp, background,
+      noise;
+} orc_sim_params;
+
+/* frames[T * N * N] u8, valid[T]; returns the mean swimmer speed */
+double orc_simulate(const orc_sim_params *p, uint8_t *frames, uint8_t *valid) {
+  const int N = p->N, T = p->T;
+  const double dt = 1 / p->fps, Lf = N * p->dx_um, sig = p->blob_sigma;
+  orc_rng rng = {(uint32_t)p->seed, 0, 0};
+  const double Vx = p->drift_speed * cos(p->drift_angle_deg * ORC_PI / 180);
+  const double Vy = p->drift_speed * sin(p->drift_angle_deg * ORC_PI / 180);
+  const int np_ = p->n_swim + p->n_passive + p->n_fixed;
+  double *px = malloc(sizeof(double) * np_), *py = malloc(sizeof(double) * np_);
+  double *vx = calloc(np_, sizeof(double)), *vy = calloc(np_, sizeof(double));
+  int *mob = malloc(sizeof(int) * np_);
+  double speed_sum = 0;
+  int k = 0;
+  if (p->n_swim > 0) {
+    const double Z = (p->vbar / p->sigma_v) * (p->vbar / p->sigma_v) - 1;
+    for (int i = 0; i < p->n_swim; i++, k++) {
+      const double v = rng_gamma(&rng, Z + 1, p->vbar / (Z + 1));
+      speed_sum += v;
+      const double phi = 2 * ORC_PI * rng_u(&rng);
+      double s = 1;
+      if (!p->swim_2d) {
+        const double cz = 2 * rng_u(&rng) - 1;
+        s = sqrt(1 - cz * cz);
+      }
+      px[k] = rng_u(&rng) * Lf;
+      py[k] = rng_u(&rng) * Lf;
+      vx[k] = v * s * cos(phi);
+      vy[k] = v * s * sin(phi);
+      mob[k] = 1;
+    }
+  }
+  for (int i = 0; i < p->n_passive; i++, k++) {
+    px[k] = rng_u(&rng) * Lf;
+    py[k] = rng_u(&rng) * Lf;
+    mob[k] = 1;
+  }
+  for (int i = 0; i < p->n_fixed; i++, k++) {
+    px[k] = rng_u(&rng) * Lf;
+    py[k] = rng_u(&rng) * Lf;
+    mob[k] = 0;
+  }
+  const int R = (int)ceil(4 * sig), W = 2 * R + 2;
+  double *gx = malloc(sizeof(double) * W), *gy = malloc(sizeof(double) * W);
+  double *img = malloc(sizeof(double) * N * N);
+  const double step = sqrt(2 * p->D_um2s * dt);
+  for (int t = 0; t < T; t++) {
+    for (long i = 0; i < (long)N * N; i++) img[i] = p->background;
+    for (int q = 0; q < np_; q++) {
+      const double cx = px[q] / p->dx_um, cy = py[q] / p->dx_um;
+      const int x0 = (int)floor(cx) - R, y0 = (int)floor(cy) - R;
+      for (int i = 0; i < W; i++) {
+        const double ddx = x0 + i - cx, ddy = y0 + i - cy;
+        gx[i] = exp(-ddx * ddx / (2 * sig * sig));
+        gy[i] = p->amp * exp(-ddy * ddy / (2 * sig * sig));
+      }
+      for (int j = 0; j < W; j++) {
+        const int yy = (((y0 + j) % N) + N) % N;
+        for (int i = 0; i < W; i++) {
+          const int xx = (((x0 + i) % N) + N) % N;
+          img[(long)yy * N + xx] += gx[i] * gy[j];
+        }
+      }
+    }
+    uint8_t *f = frames + (long)t * N * N;
+    for (long i = 0; i < (long)N * N; i++) {
+      const double v = js_round((img[i] + p->noise * rng_normal(&rng)) * 255);
+      f[i] = v < 0 ? 0 : v > 255 ? 255 : (uint8_t)v;
+    }
+    for (int q = 0; q < np_; q++) {
+      if (!mob[q]) continue;
+      px[q] += (vx[q] + Vx) * dt + step * rng_normal(&rng);
+      py[q] += (vy[q] + Vy) * dt + step * rng_normal(&rng);
+      px[q] = fmod(fmod(px[q], Lf) + Lf, Lf);
+      py[q] = fmod(fmod(py[q], Lf) + Lf, Lf);
+    }
+  }
+  for (int t = 0; t < T; t++) valid[t] = 1;
+  orc_rng drng = {(uint32_t)(p->seed * 7919 + 13), 0, 0};
+  if (p->drop_exact) {
+    int *idx = malloc(sizeof(int) * (T - 1));
+    for (int i = 0; i < T - 1; i++) idx[i] = i + 1;
+    for (int i = T - 2; i > 0; i--) {
+      const int j = (int)floor(rng_u(&drng) * (i + 1));
+      const int tmp = idx[i];
+      idx[i] = idx[j];
+      idx[j] = tmp;
+    }
+    for (int i = 0; i < p->drop_exact; i++) valid[idx[i]] = 0;
+    free(idx);
+  } else if (p->drop_frac > 0) {
+    for (int t = 1; t < T; t++)
+      if (rng_u(&drng) < p->drop_frac) valid[t] = 0;
+  }
+  free(px), free(py), free(vx), free(vy), free(mob), free(gx), free(gy), free(img);
+  return p->n_swim ? speed_sum / p->n_swim : 0;
+}

@@ -407,3 +407,33 @@ def lm_plain(tau, d, w, qn, bounds4, swim=True, geometry="3d", nstarts=3, maxit=
                          np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], nstarts, maxit,
                          th, C.byref(it))
     return th, chi, it.value
+
+
+# ------------------------------------------------------------------ scenario simulator
+
+class SimParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("N", "T", "seed", "n_swim", "swim_2d", "n_passive", "n_fixed", "drop_exact")] + \
+               [(n, C.c_double) for n in ("fps", "dx_um", "vbar", "sigma_v", "D_um2s", "drift_speed", "drift_angle_deg",
+                                          "drop_frac", "blob_sigma", "amp", "background", "noise")]
+
+
+def simulate(sim: dict):
+    """simulate (test/sim.mjs:80-205) -> (frames u8 [T, N, N], valid bool [T], mean swimmer speed).
+    sim: the reference's SimParams (N, T, fps, dxUm, seed, swimmers {n, vbar, sigma, geometry}, passive {n},
+    Dum2s, fixed {n}, drift {speed, angleDeg}, dropFrac, dropExact, blobSigmaPx, amp, background, noise)."""
+    sw = sim.get("swimmers") or {}
+    p = SimParams(N=sim.get("N", 256), T=sim.get("T", 256), seed=sim.get("seed", 1), n_swim=sw.get("n", 0),
+                  swim_2d=1 if sw.get("geometry") == "2d" else 0, n_passive=(sim.get("passive") or {}).get("n", 0),
+                  n_fixed=(sim.get("fixed") or {}).get("n", 0), drop_exact=sim.get("dropExact", 0) or 0,
+                  fps=sim.get("fps", 30), dx_um=sim.get("dxUm", 1), vbar=sw.get("vbar", 0), sigma_v=sw.get("sigma", 1),
+                  D_um2s=sim.get("Dum2s", 0), drift_speed=(sim.get("drift") or {}).get("speed", 0),
+                  drift_angle_deg=(sim.get("drift") or {}).get("angleDeg", 0), drop_frac=sim.get("dropFrac", 0) or 0,
+                  blob_sigma=sim.get("blobSigmaPx", 1.5), amp=sim.get("amp", 0.15),
+                  background=sim.get("background", 0.25), noise=sim.get("noise", 0.01))
+    L = lib()
+    L.orc_simulate.argtypes = [C.POINTER(SimParams), C.c_void_p, C.c_void_p]
+    L.orc_simulate.restype = C.c_double
+    frames = np.zeros((p.T, p.N, p.N), dtype=np.uint8)
+    valid = np.zeros(p.T, dtype=np.uint8)
+    ms = L.orc_simulate(C.byref(p), frames.ctypes.data, valid.ctypes.data)
+    return frames, valid.astype(bool), ms
