@@ -27,3 +27,20 @@ export interface DriftEstimate {
 /** Uniform drift from Radial.drift (readRadial with drift on); swimmer turns on the ISF lag gating. */
 export declare function estimateDrift(radial: Radial, pr: Prepared, opt?: { dxUm?: number; geometry?: '3d' | '2d' },
   swimmer?: { vbar: number; Z: number } | null): DriftEstimate;
+
+export type Verdict = 'MOTILE' | 'NOT_MOTILE' | 'DIRECTED_TRANSPORT' | 'NO_DYNAMICS' | 'UNLOCKED_EXPOSURE';
+export declare const VERDICTS: readonly Verdict[];
+export interface Gate { id: string; name: string; value: number; threshold: string; pass: boolean; sigma?: number; resolvedBins?: number }
+export interface ModelFit { vbar: number; Z: number; Dd: number; Vmag: number; alpha: number; rss: number;
+  amps: { A: number; B: number; W: number; P: number }[] }
+export interface MotilityResult {
+  verdict: Verdict; physicsVerdict: Verdict; gates: Gate[]; TSI: number;
+  alpha?: number; alphaResolved?: boolean; resolvedBins?: number; vbar?: number; sigmaV?: number; Z?: number;
+  Ddiff?: number; DdiffBrownian?: number; dBIC?: number; drift?: DriftEstimate; mu?: number;
+  scaling?: { q: number[]; tauHalf: number[]; mu: number }; geometry?: '3d' | '2d';
+  fit?: { swimmer: ModelFit; brownian: ModelFit }; q?: number[]; bins?: number[]; lags?: number[]; tau?: number[];
+  nData?: number; nEff?: number; residualRho?: number;
+}
+/** The reference's analyzeMotility on a Radial record with drift sums (fits, drift, gates, verdict). */
+export declare function analyzeMotility(radial: Radial, opt?: { dxUm?: number; fps?: number; geometry?: '3d' | '2d';
+  exposureLocked?: boolean }): MotilityResult;

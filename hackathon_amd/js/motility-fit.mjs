@@ -7,8 +7,9 @@
 // every usable bin of a Radial record as one batched Levenberg-Marquardt launch
 // on the MI355X (libddm.so ddm_fit through the N-API addon) and returns the
 // fitted per-q parameters.  estimateDrift turns the radial record's drift sums
-// into a uniform drift velocity (src/motility_fit.js:671-725).  Types are
-// declared in motility-fit.d.ts.
+// into a uniform drift velocity (src/motility_fit.js:671-725); analyzeMotility
+// is the reference's verdict layer (:808-990) run by the library's host code.
+// Types are declared in motility-fit.d.ts.
 import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
@@ -186,4 +187,66 @@ export function estimateDrift(radial, pr, opt = {}, swimmer = null) {
   const sigma = Math.sqrt(Math.max(varAlong, 0));
   return { Vx, Vy, mag, sigma, angleDeg: (Math.atan2(Vy, Vx) * 180) / Math.PI,
     significant: mag > GATES.FLOW_SIGMA * sigma, pairs, samples: n };
+}
+
+// ---------------------------------------------------------------- verdict
+
+export const VERDICTS = Object.freeze(['MOTILE', 'NOT_MOTILE', 'DIRECTED_TRANSPORT', 'NO_DYNAMICS', 'UNLOCKED_EXPOSURE']);
+const GATE_TEXT = {
+  0: ['G0', 'Usable q-bins', '>= 4'],
+  1: ['G1', 'Temporal structure index', '>= 0.15'],
+  2: ['G2', 'Uniform flow', 'not (|V| > 3 sigma and |V| > 2 um/s)'],
+  3: ['G3', 'Swimmer vs Brownian Delta BIC', '> 10'],
+  4: ['G4', 'Motile contrast share alpha', '> 0.05, swimmer decay resolved in >= 1 bin'],
+  5: ['G5', 'Exposure locked', 'manual exposure readback'],
+};
+const f64 = (a) => (a instanceof Float64Array ? a : Float64Array.from(a));
+
+/**
+ * analyzeMotility(radial, opt) (src/motility_fit.js:808-990): fits, drift,
+ * gates and verdict of one Radial record (readRadial with its drift sums), in
+ * the library's host code.  opt: {dxUm, fps, geometry = '3d', exposureLocked = true}.
+ * Returns the reference's MotilityResult fields.
+ */
+export function analyzeMotility(radial, opt = {}) {
+  const geometry = pick(opt.geometry, '3d');
+  if (geometry !== '3d' && geometry !== '2d') throw new Error(`analyzeMotility: geometry must be '3d' or '2d'`);
+  const r = addon.analyzeMotility({
+    N: radial.N, NB: radial.NB, L: radial.L, kCenter: f64(radial.kCenter),
+    kNodes: radial.kNodes && radial.kNodes.length ? f64(radial.kNodes) : undefined,
+    nModes: radial.nModes instanceof Uint32Array ? radial.nModes : Uint32Array.from(radial.nModes),
+    D: f64(radial.D), drift: f64(radial.drift || []),
+  }, { dxUm: pick(opt.dxUm, 1), fps: pick(opt.fps, 30), geometry: geometry === '2d' ? 1 : 0,
+    exposureLocked: pick(opt.exposureLocked, true) ? 1 : 0 });
+  const gates = [];
+  for (let i = 0; i < r.ngates; i++) {
+    const g = r.gates.subarray(5 * i, 5 * i + 5);
+    const [id, name, threshold] = GATE_TEXT[g[0]];
+    const o = { id, name, value: g[2], threshold, pass: g[1] !== 0 };
+    if (g[0] === 2) o.sigma = g[3];
+    if (g[0] === 4) o.resolvedBins = g[4];
+    gates.push(o);
+  }
+  const out = { verdict: VERDICTS[r.verdict], physicsVerdict: VERDICTS[r.physicsVerdict], gates, TSI: r.TSI };
+  if (r.insufficient) return out;
+  const nb = r.nbins, ns = r.nscaling;
+  const amps = (a, alpha) => Array.from({ length: nb }, (_, i) =>
+    ({ A: a[2 * i], B: a[2 * i + 1], W: alpha * a[2 * i], P: (1 - alpha) * a[2 * i] }));
+  const lags = lagGrid(radial.L);
+  const dt = 1 / pick(opt.fps, 30);
+  return Object.assign(out, {
+    alpha: r.alpha, alphaResolved: r.alphaResolved !== 0, resolvedBins: r.resolvedBins, vbar: r.vbar,
+    sigmaV: r.sigmaV, Z: r.Z, Ddiff: r.Ddiff, DdiffBrownian: r.DdiffBrownian, dBIC: r.dBIC,
+    drift: { Vx: r.driftVx, Vy: r.driftVy, mag: r.driftMag, sigma: r.driftSigma, angleDeg: r.driftAngleDeg,
+      significant: r.driftSignificant !== 0, pairs: r.driftPairs, samples: r.driftSamples },
+    mu: r.mu,
+    scaling: { q: Array.from(r.scalingQ.subarray(0, ns)), tauHalf: Array.from(r.scalingTauHalf.subarray(0, ns)), mu: r.mu },
+    geometry,
+    fit: {
+      swimmer: { vbar: r.vbar, Z: r.Z, Dd: r.Ddiff, Vmag: r.swimVmag, alpha: r.alpha, rss: r.swimRss, amps: amps(r.swimAmps, r.alpha) },
+      brownian: { vbar: 0, Z: 1, Dd: r.DdiffBrownian, Vmag: r.brownVmag, alpha: 0, rss: r.brownRss, amps: amps(r.brownAmps, 0) },
+    },
+    q: Array.from(r.q.subarray(0, nb)), bins: Array.from(r.bins.subarray(0, nb)), lags, tau: lags.map((l) => l * dt),
+    nData: r.nData, nEff: r.nEff, residualRho: r.residualRho,
+  });
 }

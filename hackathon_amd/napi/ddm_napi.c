@@ -445,6 +445,95 @@ static napi_value js_fit(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* analyzeMotility(radial, opt): radial {N, NB, L, kCenter, kNodes, nModes, D, drift} as
+ * Float64Array / Uint32Array, opt {dxUm, fps, geometry (0 = 3d, 1 = 2d), exposureLocked (0/1)}.
+ * Returns the flat result; motility-fit.mjs shapes it like the reference's MotilityResult. */
+static napi_value js_analyze_motility(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2], out;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 2) {
+    napi_throw_type_error(env, NULL, "analyzeMotility(radial, opt)");
+    return NULL;
+  }
+  ddm_radial_record r;
+  memset(&r, 0, sizeof r);
+  r.N = (int32_t)get_prop_num(env, argv[0], "N", 0);
+  r.NB = (int32_t)get_prop_num(env, argv[0], "NB", 0);
+  r.L = (int32_t)get_prop_num(env, argv[0], "L", 0);
+  void *kc, *kn, *nm, *D, *dr;
+  size_t nkc, nkn = 0, nnm, nD, ndr;
+  if (!get_typed(env, argv[0], "kCenter", napi_float64_array, &kc, &nkc, 1) ||
+      !get_typed(env, argv[0], "kNodes", napi_float64_array, &kn, &nkn, 0) ||
+      !get_typed(env, argv[0], "nModes", napi_uint32_array, &nm, &nnm, 1) ||
+      !get_typed(env, argv[0], "D", napi_float64_array, &D, &nD, 1) ||
+      !get_typed(env, argv[0], "drift", napi_float64_array, &dr, &ndr, 1)) {
+    napi_throw_type_error(env, NULL, "analyzeMotility: radial arrays must be Float64Array (nModes Uint32Array)");
+    return NULL;
+  }
+  const size_t NB = r.NB > 0 ? (size_t)r.NB : 0, L = r.L > 0 ? (size_t)r.L : 0;
+  if (nkc < NB || nnm < NB || nD < NB * L || ndr < NB * L * 8) {
+    napi_throw_range_error(env, NULL, "analyzeMotility: radial arrays shorter than NB, NB x L, NB x L x 8");
+    return NULL;
+  }
+  r.knodes = (kn && NB) ? (int32_t)(nkn / NB) : 0;
+  r.kCenter = (const double *)kc;
+  r.kNodes = r.knodes ? (const double *)kn : NULL;
+  r.nModes = (const uint32_t *)nm;
+  r.D = (const double *)D;
+  r.drift = (const double *)dr;
+  ddm_motility_opts o;
+  o.dx_um = get_prop_num(env, argv[1], "dxUm", 1);
+  o.fps = get_prop_num(env, argv[1], "fps", 30);
+  o.geometry = (int32_t)get_prop_num(env, argv[1], "geometry", DDM_GEOMETRY_3D);
+  o.exposure_locked = (int32_t)get_prop_num(env, argv[1], "exposureLocked", 1);
+  void *bins, *q, *sa, *ba, *sq, *st, *gt;
+  const size_t cap = NB ? NB : 1;
+  napi_value vb = make_typed(env, napi_int32_array, cap, 4, &bins), vq = make_typed(env, napi_float64_array, cap, 8, &q);
+  napi_value vsa = make_typed(env, napi_float64_array, 2 * cap, 8, &sa), vba = make_typed(env, napi_float64_array, 2 * cap, 8, &ba);
+  napi_value vsq = make_typed(env, napi_float64_array, cap, 8, &sq), vst = make_typed(env, napi_float64_array, cap, 8, &st);
+  napi_value vg = make_typed(env, napi_float64_array, 6 * 5, 8, &gt);
+  if (!vb || !vq || !vsa || !vba || !vsq || !vst || !vg) {
+    napi_throw_error(env, NULL, "analyzeMotility: allocation failed");
+    return NULL;
+  }
+  ddm_motility_detail det = {(int32_t)cap, 0, (int32_t *)bins, (double *)q, (double *)sa, (double *)ba, (double *)sq,
+                             (double *)st};
+  ddm_motility_result res;
+  if (ddm_analyze_motility(&r, &o, &res, &det) != DDM_OK) {
+    napi_throw_error(env, NULL, ddm_verdict_last_error());
+    return NULL;
+  }
+  double *g = (double *)gt;
+  for (int i = 0; i < res.ngates; i++) {
+    g[5 * i] = res.gates[i].id;
+    g[5 * i + 1] = res.gates[i].pass;
+    g[5 * i + 2] = res.gates[i].value;
+    g[5 * i + 3] = res.gates[i].sigma;
+    g[5 * i + 4] = res.gates[i].resolved_bins;
+  }
+  NAPI_CALL(env, napi_create_object(env, &out));
+  const struct { const char *k; double v; } f[] = {
+      {"verdict", res.verdict}, {"physicsVerdict", res.physics_verdict}, {"insufficient", res.insufficient},
+      {"ngates", res.ngates}, {"TSI", res.TSI}, {"alpha", res.alpha}, {"vbar", res.vbar}, {"sigmaV", res.sigma_v},
+      {"Z", res.Z}, {"Ddiff", res.Ddiff}, {"DdiffBrownian", res.Ddiff_brownian}, {"dBIC", res.dBIC}, {"mu", res.mu},
+      {"nData", res.nData}, {"nEff", res.nEff}, {"residualRho", res.residual_rho}, {"swimRss", res.swim_rss},
+      {"swimVmag", res.swim_Vmag}, {"brownRss", res.brown_rss}, {"brownVmag", res.brown_Vmag},
+      {"alphaResolved", res.alpha_resolved}, {"resolvedBins", res.resolved_bins}, {"nbins", res.nbins},
+      {"nscaling", res.nscaling}, {"driftVx", res.drift.Vx}, {"driftVy", res.drift.Vy}, {"driftMag", res.drift.mag},
+      {"driftSigma", res.drift.sigma}, {"driftAngleDeg", res.drift.angle_deg}, {"driftSamples", res.drift.samples},
+      {"driftSignificant", res.drift.significant}, {"driftPairs", res.drift.pairs}};
+  for (size_t i = 0; i < sizeof f / sizeof f[0]; i++) set_prop(env, out, f[i].k, num(env, f[i].v));
+  set_prop(env, out, "gates", vg);
+  set_prop(env, out, "bins", vb);
+  set_prop(env, out, "q", vq);
+  set_prop(env, out, "swimAmps", vsa);
+  set_prop(env, out, "brownAmps", vba);
+  set_prop(env, out, "scalingQ", vsq);
+  set_prop(env, out, "scalingTauHalf", vst);
+  return out;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
@@ -459,6 +548,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readTemporalFull", NULL, js_read_temporal_full, NULL, NULL, NULL, napi_default, NULL},
       {"fitterCreate", NULL, js_fitter_create, NULL, NULL, NULL, napi_default, NULL},
       {"fit", NULL, js_fit, NULL, NULL, NULL, napi_default, NULL},
+      {"analyzeMotility", NULL, js_analyze_motility, NULL, NULL, NULL, napi_default, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

@@ -154,3 +154,36 @@ def test_js_estimate_drift_matches_python(tmp_path):
         for k in ("Vx", "Vy", "mag", "sigma", "angleDeg"):
             assert math.isclose(o[k], getattr(w, k), rel_tol=1e-12, abs_tol=1e-12), (k, o[k], getattr(w, k))
     assert out[1]["pairs"] < out[0]["pairs"]
+
+
+def test_js_analyze_motility_matches_python(tmp_path):
+    """analyzeMotility of the TS-facing module (N-API -> library host code) against the Python wrapper on the
+    oracle pipeline's record of a motile scenario, 3d and 2d, with and without the exposure lock (no GPU work)."""
+    from tests.test_verdict import oracle_radial
+    from hackathon_amd.verdict import analyze_motility
+    if not os.path.exists(ADDON):
+        pytest.fail("N-API addon not built (python -m hackathon_amd.napi.build_addon)")
+    r = oracle_radial("motile_3d")
+    rec = dict(N=r.N, NB=r.NB, L=r.L, kCenter=list(map(float, r.kCenter)), kNodes=list(map(float, r.kNodes)),
+               nModes=[int(x) for x in r.nModes], D=list(map(float, r.D)), drift=list(map(float, r.drift)))
+    (tmp_path / "radial.json").write_text(json.dumps(rec))
+    mod = os.path.join(ROOT, "hackathon_amd", "js", "motility-fit.mjs")
+    code = ("Promise.all([import('" + mod + "'), import('fs')]).then(([m, fs]) => {"
+            " const r = JSON.parse(fs.readFileSync('" + str(tmp_path / "radial.json") + "', 'utf8'));"
+            " const out = [m.analyzeMotility(r, {dxUm: 1, fps: 30}), m.analyzeMotility(r, {dxUm: 1, fps: 30, geometry: '2d'}),"
+            "  m.analyzeMotility(r, {dxUm: 1, fps: 30, exposureLocked: false})];"
+            " console.log('RESULT ' + JSON.stringify(out)); })")
+    res = subprocess.run(["node", "--input-type=module", "-e", code], capture_output=True, text=True, timeout=120)
+    line = [x for x in res.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, res.stdout + res.stderr
+    out = json.loads(line[0][7:])
+    want = [analyze_motility(r, 1.0, 30.0), analyze_motility(r, 1.0, 30.0, "2d"),
+            analyze_motility(r, 1.0, 30.0, exposureLocked=False)]
+    for o, w in zip(out, want):
+        assert o["verdict"] == w["verdict"] and o["physicsVerdict"] == w["physicsVerdict"]
+        assert [(g["id"], g["pass"]) for g in o["gates"]] == [(g["id"], g["pass"]) for g in w["gates"]]
+        for k in ("TSI", "alpha", "vbar", "Z", "Ddiff", "DdiffBrownian", "dBIC", "nEff"):
+            assert o[k] == w[k], k
+        assert o["drift"]["mag"] == w["drift"]["mag"] and o["bins"] == w["bins"] and o["lags"] == w["lags"]
+        assert o["fit"]["swimmer"]["amps"] == w["fit"]["swimmer"]["amps"]
+    assert out[0]["verdict"] == "MOTILE" and out[2]["verdict"] == "UNLOCKED_EXPOSURE"
