@@ -44,6 +44,24 @@ def test_spectrum_parity(gpu, N, dtype):
     assert err < 1e-5, err
 
 
+@pytest.mark.parametrize("N,tuning", [(1024, dict(wave_spatial=2, cols_f=8)), (1024, dict(wave_spatial=2, cols_f=16)),
+                                      (1024, dict(wave_spatial=1)), (512, dict(wave_spatial=1)), (2048, None)])
+def test_spectrum_parity_large(gpu, N, tuning):
+    """The wavefront spatial kernels at N = 1024 (radix 8, 8, 4, 4; 8- and 16-frame column tiles), the
+    workgroup-level kernels at 512 and 1024, and N = 2048."""
+    T = 4
+    fr = _frames(T, N, np.uint16, seed=N + 7)
+    print(f"spectrum N={N} tuning={tuning}", flush=True)
+    with DdmEngine.create(N=N, T=T, L=2, NB=4, kmin=0, kmax=N // 2, tuning=tuning) as e:
+        assert e.ingestFrames(fr, 0)
+        h = e.readHistory()
+        if tuning:
+            assert all(e.tuning()[k] == v for k, v in tuning.items())
+    ref = O.history(fr, T)
+    err = _spec_err(h, ref)
+    assert err < 1e-5, err
+
+
 def test_spectrum_config1(gpu):
     fr = synth.config1()
     with DdmEngine.create(N=128, T=64, L=64, NB=64, kmin=0, kmax=64) as e:
