@@ -5,7 +5,8 @@ the batch records every video's ingest + temporal + radial work into one HIP
 graph (or enqueues it eagerly) with `chains` concurrent branches.  Per-video
 results are read through `engine(i)`, a non-owning `DdmEngine` view, and the
 per-q fits of all bins of all videos go to one `Fitter.fit` launch
-(`fit_all`).
+(`fit_all`); `Batch.verdicts` runs the verdict layer of every video on host
+threads.
 """
 from __future__ import annotations
 
@@ -88,8 +89,13 @@ class Batch:
             self._views[i] = DdmEngine(e, self._lib, owned=False)
         return self._views[i]
 
-    def radials(self):
-        return [self.engine(i).readRadial() for i in range(self.n)]
+    def radials(self, drift: bool = False):
+        return [self.engine(i).readRadial(drift=drift) for i in range(self.n)]
+
+    def verdicts(self, dxUm, fps, geometry="3d", exposureLocked=True, threads: int = 0):
+        """analyzeMotility of every video (radial records with drift sums), on the library's host threads."""
+        from .verdict import analyze_motility_batch
+        return analyze_motility_batch(self.radials(drift=True), dxUm, fps, geometry, exposureLocked, threads)
 
     def close(self):
         if getattr(self, "_h", None):

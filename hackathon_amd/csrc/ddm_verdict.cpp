@@ -15,11 +15,13 @@
 // (they decide which local optimum is reported); the arithmetic is C++ double
 // with libm, so values agree with the reference's to rounding, not bit for bit.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ddm.h"
@@ -752,6 +754,40 @@ int ddm_analyze_motility(const ddm_radial_record* rad, const ddm_motility_opts* 
   res->physics_verdict = verdict;
   res->verdict = opt->exposure_locked ? verdict : DDM_VERDICT_UNLOCKED_EXPOSURE;
   return DDM_OK;
+}
+
+int ddm_analyze_motility_batch(const ddm_radial_record *r, const ddm_motility_opts *opt, int32_t n,
+                               ddm_motility_result *res, ddm_motility_detail *detail, int32_t nthreads) {
+  if (n < 0 || (n > 0 && (!r || !opt || !res))) {
+    g_err = "analyzeMotility batch: null argument or negative count";
+    return DDM_EINVAL;
+  }
+  int workers = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+  workers = std::min(workers, std::max(1, (int)n));
+  std::atomic<int> next(0);
+  std::mutex mu;
+  int first_rc = DDM_OK;
+  std::string first_err;
+  auto work = [&] {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= n) return;
+      const int rc = ddm_analyze_motility(r + i, opt + i, res + i, detail ? detail + i : nullptr);
+      if (rc != DDM_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first_rc == DDM_OK) {
+          first_rc = rc;
+          first_err = "record " + std::to_string(i) + ": " + g_err;
+        }
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < workers; w++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  if (first_rc != DDM_OK) g_err = first_err;
+  return first_rc;
 }
 
 }  // extern "C"

@@ -74,6 +74,9 @@ def _declare(L):
     L.ddm_analyze_motility.restype = C.c_int
     L.ddm_verdict_last_error.argtypes = []
     L.ddm_verdict_last_error.restype = C.c_char_p
+    L.ddm_analyze_motility_batch.argtypes = [C.POINTER(RadialRecord), C.POINTER(MotilityOpts), C.c_int32,
+                                             C.POINTER(MotilityResult), C.POINTER(MotilityDetail), C.c_int32]
+    L.ddm_analyze_motility_batch.restype = C.c_int
     L._verdict_declared = True
     return L
 
@@ -82,69 +85,100 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+class _Prepared:
+    """ctypes inputs and output buffers of one record (kept alive for the call)."""
+
+    def __init__(self, radial: Radial, dxUm: float, fps: float, geometry: str, exposureLocked: bool):
+        if geometry not in GEOMETRY:
+            raise DdmError(f"analyzeMotility: geometry must be '3d' or '2d', got {geometry!r}")
+        NB, Lg = int(radial.NB), int(radial.L)
+        self.kc = _f64(radial.kCenter)
+        self.kn = _f64(radial.kNodes) if radial.kNodes is not None and len(radial.kNodes) else np.zeros(0)
+        self.D = _f64(radial.D)
+        self.dr = _f64(radial.drift) if radial.drift is not None else np.zeros(0)
+        self.nm = np.ascontiguousarray(radial.nModes, dtype=np.uint32)
+        if len(self.kc) != NB or len(self.nm) != NB or self.D.size != NB * Lg:
+            raise DdmError("analyzeMotility: Radial record arrays do not match NB and L")
+        if self.dr.size != NB * Lg * 8:
+            raise DdmError("analyzeMotility: the Radial record has no drift sums (readRadial(drift=True))")
+        self.rec = RadialRecord(int(radial.N), NB, Lg, len(self.kn) // NB if len(self.kn) else 0,
+                                self.kc.ctypes.data, self.kn.ctypes.data if len(self.kn) else None,
+                                self.D.ctypes.data, self.dr.ctypes.data, self.nm.ctypes.data)
+        self.opt = MotilityOpts(float(dxUm), float(fps), GEOMETRY[geometry], 1 if exposureLocked else 0)
+        self.L, self.fps, self.geometry = Lg, float(fps), geometry
+        self.bins = np.zeros(NB, np.int32)
+        self.q = np.zeros(NB)
+        self.sa = np.zeros(2 * NB)
+        self.ba = np.zeros(2 * NB)
+        self.sq = np.zeros(NB)
+        self.st = np.zeros(NB)
+        self.det = MotilityDetail(NB, 0, self.bins.ctypes.data, self.q.ctypes.data, self.sa.ctypes.data,
+                                  self.ba.ctypes.data, self.sq.ctypes.data, self.st.ctypes.data)
+
+    def result(self, res: MotilityResult) -> dict:
+        gates = []
+        for g in res.gates[:res.ngates]:
+            gid, name, thr = _GATE_TEXT[g.id]
+            d = dict(id=gid, name=name, value=float(g.value), threshold=thr, **{"pass": bool(g.passed)})
+            if g.id == 2:
+                d["sigma"] = float(g.sigma)
+            if g.id == 4:
+                d["resolvedBins"] = int(g.resolved_bins)
+            gates.append(d)
+        out = dict(verdict=VERDICTS[res.verdict], physicsVerdict=VERDICTS[res.physics_verdict], gates=gates,
+                   TSI=float(res.TSI))
+        if res.insufficient:
+            return out
+        nb, ns = int(res.nbins), int(res.nscaling)
+        drift = res.drift
+
+        def amps(a, alpha):
+            return [dict(A=float(a[2 * i]), B=float(a[2 * i + 1]), W=alpha * float(a[2 * i]),
+                         P=(1 - alpha) * float(a[2 * i])) for i in range(nb)]
+        dt = 1.0 / self.fps
+        lags = [int(x) for x in lag_grid(self.L)]
+        out.update(
+            alpha=float(res.alpha), alphaResolved=bool(res.alpha_resolved), resolvedBins=int(res.resolved_bins),
+            vbar=float(res.vbar), sigmaV=float(res.sigma_v), Z=float(res.Z), Ddiff=float(res.Ddiff),
+            DdiffBrownian=float(res.Ddiff_brownian), dBIC=float(res.dBIC),
+            drift=dict(Vx=drift.Vx, Vy=drift.Vy, mag=drift.mag, sigma=drift.sigma, angleDeg=drift.angle_deg,
+                       significant=bool(drift.significant), pairs=int(drift.pairs), samples=drift.samples),
+            mu=float(res.mu),
+            scaling=dict(q=self.sq[:ns].tolist(), tauHalf=self.st[:ns].tolist(), mu=float(res.mu)),
+            geometry=self.geometry,
+            fit=dict(swimmer=dict(vbar=float(res.vbar), Z=float(res.Z), Dd=float(res.Ddiff),
+                                  Vmag=float(res.swim_Vmag), alpha=float(res.alpha), rss=float(res.swim_rss),
+                                  amps=amps(self.sa, float(res.alpha))),
+                     brownian=dict(vbar=0.0, Z=1.0, Dd=float(res.Ddiff_brownian), Vmag=float(res.brown_Vmag),
+                                   alpha=0.0, rss=float(res.brown_rss), amps=amps(self.ba, 0.0))),
+            q=self.q[:nb].tolist(), bins=self.bins[:nb].tolist(), lags=lags, tau=[l * dt for l in lags],
+            nData=float(res.nData), nEff=float(res.nEff), residualRho=float(res.residual_rho))
+        return out
+
+
 def analyze_motility(radial: Radial, dxUm: float, fps: float, geometry: str = "3d",
                      exposureLocked: bool = True) -> dict:
     """analyzeMotility(radial, {dxUm, fps, geometry, exposureLocked}) -> the reference's MotilityResult."""
-    if geometry not in GEOMETRY:
-        raise DdmError(f"analyzeMotility: geometry must be '3d' or '2d', got {geometry!r}")
     L = _declare(load_library())
-    NB, Lg = int(radial.NB), int(radial.L)
-    kc = _f64(radial.kCenter)
-    kn = _f64(radial.kNodes) if radial.kNodes is not None and len(radial.kNodes) else np.zeros(0)
-    D = _f64(radial.D)
-    dr = _f64(radial.drift) if radial.drift is not None else np.zeros(0)
-    nm = np.ascontiguousarray(radial.nModes, dtype=np.uint32)
-    if len(kc) != NB or len(nm) != NB or D.size != NB * Lg:
-        raise DdmError("analyzeMotility: Radial record arrays do not match NB and L")
-    if dr.size != NB * Lg * 8:
-        raise DdmError("analyzeMotility: the Radial record has no drift sums (readRadial(drift=True))")
-    rec = RadialRecord(int(radial.N), NB, Lg, len(kn) // NB if len(kn) else 0, kc.ctypes.data,
-                       kn.ctypes.data if len(kn) else None, D.ctypes.data, dr.ctypes.data, nm.ctypes.data)
-    opt = MotilityOpts(float(dxUm), float(fps), GEOMETRY[geometry], 1 if exposureLocked else 0)
+    p = _Prepared(radial, dxUm, fps, geometry, exposureLocked)
     res = MotilityResult()
-    bins = np.zeros(NB, np.int32)
-    q = np.zeros(NB)
-    sa = np.zeros(2 * NB)
-    ba = np.zeros(2 * NB)
-    sq = np.zeros(NB)
-    st = np.zeros(NB)
-    det = MotilityDetail(NB, 0, bins.ctypes.data, q.ctypes.data, sa.ctypes.data, ba.ctypes.data, sq.ctypes.data,
-                         st.ctypes.data)
-    rc = L.ddm_analyze_motility(C.byref(rec), C.byref(opt), C.byref(res), C.byref(det))
-    if rc != 0:
+    if L.ddm_analyze_motility(C.byref(p.rec), C.byref(p.opt), C.byref(res), C.byref(p.det)) != 0:
         raise DdmError(L.ddm_verdict_last_error().decode())
-    gates = []
-    for g in res.gates[:res.ngates]:
-        gid, name, thr = _GATE_TEXT[g.id]
-        d = dict(id=gid, name=name, value=float(g.value), threshold=thr, **{"pass": bool(g.passed)})
-        if g.id == 2:
-            d["sigma"] = float(g.sigma)
-        if g.id == 4:
-            d["resolvedBins"] = int(g.resolved_bins)
-        gates.append(d)
-    out = dict(verdict=VERDICTS[res.verdict], physicsVerdict=VERDICTS[res.physics_verdict], gates=gates,
-               TSI=float(res.TSI))
-    if res.insufficient:
-        return out
-    nb, ns = int(res.nbins), int(res.nscaling)
-    drift = res.drift
-    amps = lambda a, alpha: [dict(A=float(a[2 * i]), B=float(a[2 * i + 1]), W=alpha * float(a[2 * i]),
-                                  P=(1 - alpha) * float(a[2 * i])) for i in range(nb)]
-    dt = 1.0 / fps
-    lags = [int(x) for x in lag_grid(Lg)]
-    out.update(
-        alpha=float(res.alpha), alphaResolved=bool(res.alpha_resolved), resolvedBins=int(res.resolved_bins),
-        vbar=float(res.vbar), sigmaV=float(res.sigma_v), Z=float(res.Z), Ddiff=float(res.Ddiff),
-        DdiffBrownian=float(res.Ddiff_brownian), dBIC=float(res.dBIC),
-        drift=dict(Vx=drift.Vx, Vy=drift.Vy, mag=drift.mag, sigma=drift.sigma, angleDeg=drift.angle_deg,
-                   significant=bool(drift.significant), pairs=int(drift.pairs), samples=drift.samples),
-        mu=float(res.mu),
-        scaling=dict(q=sq[:ns].tolist(), tauHalf=st[:ns].tolist(), mu=float(res.mu)),
-        geometry=geometry,
-        fit=dict(swimmer=dict(vbar=float(res.vbar), Z=float(res.Z), Dd=float(res.Ddiff), Vmag=float(res.swim_Vmag),
-                              alpha=float(res.alpha), rss=float(res.swim_rss), amps=amps(sa, float(res.alpha))),
-                 brownian=dict(vbar=0.0, Z=1.0, Dd=float(res.Ddiff_brownian), Vmag=float(res.brown_Vmag), alpha=0.0,
-                               rss=float(res.brown_rss), amps=amps(ba, 0.0))),
-        q=q[:nb].tolist(), bins=bins[:nb].tolist(), lags=lags, tau=[l * dt for l in lags],
-        nData=float(res.nData), nEff=float(res.nEff), residualRho=float(res.residual_rho))
-    return out
+    return p.result(res)
+
+
+def analyze_motility_batch(radials, dxUm, fps, geometry="3d", exposureLocked=True, threads: int = 0) -> list:
+    """analyzeMotility of many records at once on the library's host threads (threads <= 0: one per
+    hardware thread).  dxUm, fps, geometry and exposureLocked are scalars or one per record."""
+    L = _declare(load_library())
+    n = len(radials)
+    per = lambda v: list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
+    ps = [_Prepared(r, d, f, g, x) for r, d, f, g, x in
+          zip(radials, per(dxUm), per(fps), per(geometry), per(exposureLocked))]
+    recs = (RadialRecord * max(n, 1))(*[p.rec for p in ps])
+    opts = (MotilityOpts * max(n, 1))(*[p.opt for p in ps])
+    dets = (MotilityDetail * max(n, 1))(*[p.det for p in ps])
+    res = (MotilityResult * max(n, 1))()
+    if L.ddm_analyze_motility_batch(recs, opts, n, res, dets, int(threads)) != 0:
+        raise DdmError(L.ddm_verdict_last_error().decode())
+    return [p.result(res[i]) for i, p in enumerate(ps)]

@@ -368,6 +368,11 @@ typedef struct ddm_motility_detail {
 
 int ddm_analyze_motility(const ddm_radial_record *r, const ddm_motility_opts *opt, ddm_motility_result *res,
                          ddm_motility_detail *detail);
+/* n records at once on host threads (nthreads <= 0: one per hardware thread):
+ * r, opt, res (and detail, when not NULL) are arrays of n.  Returns the first
+ * failing record's status; the message names the record. */
+int ddm_analyze_motility_batch(const ddm_radial_record *r, const ddm_motility_opts *opt, int32_t n,
+                               ddm_motility_result *res, ddm_motility_detail *detail, int32_t nthreads);
 const char *ddm_verdict_last_error(void);
 
 #ifdef __cplusplus

@@ -61,6 +61,13 @@ def test_batch_fits_all_bins_in_one_launch(gpu):
     prs, res = BA.fit_all(rads, 1.0, 100.0)
     assert len(res.chi2) == sum(len(p.bins) for p in prs) > 0
     assert np.all(np.isfinite(res.chi2))
+    # the verdict layer of every video on host threads = one record at a time
+    from hackathon_amd.verdict import VERDICTS, analyze_motility
+    vs = b.verdicts(1.0, 100.0, threads=3)
+    assert len(vs) == 3 and all(v["verdict"] in VERDICTS for v in vs)
+    for i, v in enumerate(vs):
+        one = analyze_motility(b.engine(i).readRadial(drift=True), 1.0, 100.0)
+        assert v["verdict"] == one["verdict"] and v["TSI"] == one["TSI"] and v.get("dBIC") == one.get("dBIC")
     b.close()
 
 

@@ -138,3 +138,28 @@ def test_matrix_through_the_gpu_engine(gpu):
                 if not (u["verdict"] == "UNLOCKED_EXPOSURE" and u["physicsVerdict"] == "MOTILE"):
                     failures.append(("exposure gate", u["verdict"]))
     assert failures == []
+
+
+def test_batch_equals_single_records():
+    """The threaded batch entry gives each record the single-record result (records, options per record)."""
+    from hackathon_amd.verdict import analyze_motility_batch
+    names = ["motile_3d", "killed_brownian", "flow_0deg_10ums", "static_debris"]
+    rads = [oracle_radial(n) for n in names]
+    geo = ["3d", "3d", "3d", "2d"]
+    got = analyze_motility_batch(rads, 1.0, 30.0, geo, [True, True, False, True], threads=3)
+    def same(a, b):   # equal, NaN == NaN (mu of a record without a tau_1/2 fit)
+        if isinstance(a, dict):
+            return a.keys() == b.keys() and all(same(a[k], b[k]) for k in a)
+        if isinstance(a, list):
+            return len(a) == len(b) and all(same(x, y) for x, y in zip(a, b))
+        if isinstance(a, float) and math.isnan(a):
+            return isinstance(b, float) and math.isnan(b)
+        return a == b
+    for r, g, x, out in zip(rads, geo, [True, True, False, True], got):
+        assert same(out, analyze_motility(r, 1.0, 30.0, g, x))
+    assert [o["verdict"] for o in got] == ["MOTILE", "NOT_MOTILE", "UNLOCKED_EXPOSURE", "NO_DYNAMICS"]
+    assert analyze_motility_batch([], 1.0, 30.0) == []
+    bad = Radial(rads[0].N, rads[0].NB, rads[0].L, rads[0].kCenter, rads[0].kNodes, rads[0].nModes, rads[0].D,
+                 rads[0].P, rads[0].drift)
+    with pytest.raises(DdmError, match="record 1"):
+        analyze_motility_batch([rads[0], bad], [1.0, -1.0], 30.0)
