@@ -5,7 +5,8 @@
 // src/motility_fit.js:417-432) with explicit amplitudes:
 //   f(tau) = A c(tau) + B,   c = (1/K) sum_i [ -expm1(-D q_i^2 tau) + E_i alpha (1 - g(q_i vbar tau; Z)) ]
 // (= 1 - <E (1 - alpha + alpha g)> without the 1 - (~1) cancellation), g the
-// Schulz-averaged 3D swimmer ISF (isf3d, :140-144).  chi^2 = w^2 sum (d - f)^2
+// Schulz-averaged 3D swimmer ISF (isf3d, :140-144) or the in-plane one
+// (the isf2dFast table, :168-219).  chi^2 = w^2 sum (d - f)^2
 // with w = 1/max d (:392-398, 469).  Parameters theta = (A, B, alpha, ln vbar,
 // ln Z, ln D) (swimmer) or (A, B, ln D) (Brownian), box-constrained by
 // projection with the reference's bounds (:480-491).
@@ -89,11 +90,63 @@ struct FitIn {
   const float* bounds;  // [nfits * 4] vmin, vmax, Dmin, Dmax
   const float* seeds;   // [nfits * 6] or null
   int K;
+  Isf2dDev t2d;         // in-plane swimmer model only
 };
 
-// Column value (and derivatives w.r.t. alpha, ln vbar, ln Z, ln D) at one lag.
+// 1 - g of the in-plane swimmer ISF from the reference's table (isf2dFast,
+// src/motility_fit.js:168-219), FP64, with d(1-g)/d ln x and d(1-g)/d ln Z at
+// fixed x: g = sum_k l_k(t) R_k(ln x - ln(Z_k+1)) over the three rows around
+// Z (quadratic Lagrange weights l_k in t = (ln Z - ln Z_0)/dlz - i0), each row
+// a Catmull-Rom spline R_k along ln Lambda (derivative: the spline's own), the
+// series 1 - x^2 (Z_k+2)/(4(Z_k+1)) below Lambda = 1e-4 and 0 past the row.
+// 1 - g = sum_k l_k (1 - R_k) (the weights sum to 1), so it does not cancel.
+__device__ __noinline__ double one_minus_isf2d(double x, double Z, const Isf2dDev& T, double& dlx, double& dlz) {
+  const double zc = fmin(100.0, fmax(0.5, Z));
+  const double zf = (log(zc) - T.lz0) / T.dlz;
+  const int i0 = min(max((int)floor(zf + 0.5) - 1, 0), 72 - 3);
+  const double t = zf - i0;
+  const double l[3] = {0.5 * (t - 1) * (t - 2), -t * (t - 2), 0.5 * t * (t - 1)};
+  const double dl[3] = {t - 1.5, 2 - 2 * t, t - 0.5};
+  // the interpolant's own slope also at and past the table ends: the LM box
+  // holds Z in [0.5, 100], and a zero slope on a bound would pin Z there
+  const double dtdz = 1 / T.dlz;
+  double om = 0;
+  dlx = dlz = 0;
+  for (int k = 0; k < 3; k++) {
+    const int i = i0 + k;
+    const double Zi = T.Z[i], Lm = x / (Zi + 1);
+    double omr, dR;   // 1 - R_k, dR_k / d ln x
+    if (Lm < 1e-4) {
+      omr = x * x * (Zi + 2) / (4 * (Zi + 1));
+      dR = -2 * omr;
+    } else {
+      const double f = (log(Lm) - T.u0[i]) / T.du[i];
+      if (f >= T.n[i] - 1) {
+        omr = 1;
+        dR = 0;
+      } else {
+        const int j = (int)floor(f);
+        const double u = f - j;
+        const double* v = T.v + T.off[i];
+        const double p1 = v[j], p2 = v[j + 1], p3 = v[j + 2], p0 = j > 0 ? v[j - 1] : 2 * p1 - p2;
+        const double c1 = p2 - p0, c2 = 2 * p0 - 5 * p1 + 4 * p2 - p3, c3 = 3 * (p1 - p2) + p3 - p0;
+        omr = 1 - (p1 + 0.5 * u * (c1 + u * (c2 + u * c3)));
+        dR = (0.5 * c1 + u * c2 + 1.5 * u * u * c3) / T.du[i];
+      }
+    }
+    om += l[k] * omr;
+    dlx -= l[k] * dR;
+    dlz += dl[k] * dtdz * omr;
+  }
+  return om;
+}
+
+// Column value (and derivatives w.r.t. alpha, ln vbar, ln Z, ln D) at one lag;
+// GEO 0: the 3D swimmer ISF, 1: the in-plane one (table T).
+template <int GEO>
 __device__ __forceinline__ void column(float t, const float* q, int K, float alpha, float vbar, float Z, float Dd,
-                                       bool swim, bool deriv, float& c, float& da, float& dlv, float& dlz,
+                                       const Isf2dDev& T, bool swim, bool deriv, float& c, float& da, float& dlv,
+                                       float& dlz,
                                        float& dld) {
   c = da = dlv = dlz = dld = 0.f;
 #pragma unroll
@@ -103,7 +156,16 @@ __device__ __forceinline__ void column(float t, const float* q, int K, float alp
     const float s = Dd * qi * qi * t;
     const float E = expf(-s), omE = -expm1f(-s);
     Omg og = {0.f, 0.f, 0.f};
-    if (swim) og = one_minus_isf3d(qi * vbar * t, Z);
+    if (swim) {
+      if constexpr (GEO == 1) {
+        double gx, gz;
+        og.v = (float)one_minus_isf2d((double)qi * vbar * t, Z, T, gx, gz);
+        og.dlx = (float)gx;
+        og.dlz = (float)gz;
+      } else {
+        og = one_minus_isf3d(qi * vbar * t, Z);
+      }
+    }
     c += omE + E * alpha * og.v;
     if (deriv) {
       da += E * og.v;
@@ -129,14 +191,17 @@ __device__ __noinline__ double one_minus_isf3d_d(double x, double Z) {
 }
 
 // Column value at one lag in FP64 (the objective; the Jacobian stays FP32).
-__device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd, bool swim) {
+template <int GEO>
+__device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd,
+                           const Isf2dDev& T, bool swim) {
   double c = 0;
 #pragma unroll
   for (int i = 0; i < MAXK; i++) {
     if (i >= K) break;
     const double qi = q[i];
     const double s = Dd * qi * qi * t;
-    const double og = swim ? one_minus_isf3d_d(qi * vbar * t, Z) : 0.0;
+    double og = 0.0, gx, gz;
+    if (swim) og = GEO == 1 ? one_minus_isf2d(qi * vbar * t, Z, T, gx, gz) : one_minus_isf3d_d(qi * vbar * t, Z);
     c += -expm1(-s) + exp(-s) * alpha * og;
   }
   return c / K;
@@ -146,8 +211,9 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
 // objective is resolved far below the 1e-6 relative level at which fits are
 // compared, so LM accept/reject and termination decisions are not driven by
 // FP32 rounding of the model.
+template <int GEO>
 __device__ __noinline__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q, int K,
-                          bool swim, int lane) {
+                          const Isf2dDev& T, bool swim, int lane) {
   double s = 0;
   const double alpha = swim ? th[2] : 0.0, vbar = exp((double)th[3]), Z = exp((double)th[4]),
                Dd = exp((double)th[5]);
@@ -155,7 +221,7 @@ __device__ __noinline__ double chi2_at(const float* th, const float* tv, const f
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
-      const double c = column_d(tv[r], q, K, alpha, vbar, Z, Dd, swim);
+      const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim);
       const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
       s += res * res;
     }
@@ -200,15 +266,16 @@ __device__ double nnls2(double g11, double g12, double n, double h1, double h2, 
   return best > 0 ? best : 0;
 }
 
+template <int GEO>
 __device__ double grid_point(const float* tv, const float* dv, int nt, const float* q, int K, float alpha, float vbar,
-                             float Z, float Dd, bool swim, int lane, float& A, float& B) {
+                             float Z, float Dd, const Isf2dDev& T, bool swim, int lane, float& A, float& B) {
   double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
       float c, a, b, e, f;
-      column(tv[r], q, K, alpha, vbar, Z, Dd, swim, false, c, a, b, e, f);
+      column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim, false, c, a, b, e, f);
       g11 += (double)c * c;
       g12 += c;
       h1 += (double)c * dv[r];
@@ -227,14 +294,14 @@ __device__ double grid_point(const float* tv, const float* dv, int nt, const flo
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
 
 // LM from theta (in/out); returns chi^2, adds the steps taken to *steps.
-template <bool SWIM>
+template <bool SWIM, int GEO>
 __device__ double lm_run(float* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
-                         const float* q, int K, int maxit, int lane, int* steps) {
+                         const float* q, int K, const Isf2dDev& T, int maxit, int lane, int* steps) {
   constexpr bool swim = SWIM;
   constexpr int p = SWIM ? 6 : 3;
   constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
   for (int k = 0; k < 6; k++) th[k] = clampf(th[k], lo[k], hi[k]);
-  double chi = chi2_at(th, tv, dv, nt, w, q, K, swim, lane);
+  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, T, swim, lane);
   double lam = 1e-3, nu = 2;
   int it = 0;
   bool done = false;
@@ -250,7 +317,7 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
       const int j = lane + 64 * r;
       if (j < nt) {
         float c, da, dlv, dlz, dld;
-        column(tv[r], q, K, alpha, vbar, Z, Dd, swim, true, c, da, dlv, dlz, dld);
+        column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim, true, c, da, dlv, dlz, dld);
         double F[6];
         F[0] = (double)w * c;
         F[1] = w;
@@ -351,7 +418,7 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
           const double dk = (double)nth[k] - th[k];
           pred += dk * (lam * diag[a] * dk + g[a]);
         }
-        const double nchi = chi2_at(nth, tv, dv, nt, w, q, K, swim, lane);
+        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, T, swim, lane);
         const double rho = pred > 0 ? (chi - nchi) / pred : -1;
         if (nchi < chi && rho > 0) {
           const double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
@@ -383,7 +450,7 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
 
 // mode 0: LM from the seeds; 1: plain grid-seeded LM from the best `nstarts`
 // starts; 2: nonlinear parameters from the seeds, A and B by NNLS there.
-template <bool SWIM>
+template <bool SWIM, int GEO>
 __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
                                                 float* __restrict__ theta_out, double* __restrict__ chi_out,
                                                 int* __restrict__ steps_out) {
@@ -417,8 +484,8 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
 #pragma unroll
       for (int k = 3; k < 6; k++) st[0][k] = fminf(hi[k], fmaxf(lo[k], st[0][k]));
       float A, B;
-      grid_point(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), SWIM, lane,
-                 A, B);
+      grid_point<GEO>(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), in.t2d,
+                      SWIM, lane, A, B);
       st[0][0] = A;
       st[0][1] = B;
     }
@@ -429,7 +496,7 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
     for (int i = 0; i <= 48; i++) {
       const float lD = lo[5] + ((hi[5] - lo[5]) * i) / 48.f;
       float A, B;
-      const double r = grid_point(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), false, lane, A, B);
+      const double r = grid_point<GEO>(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), in.t2d, false, lane, A, B);
       if (r < bchi) {
         bchi = r;
         bA = A;
@@ -455,7 +522,8 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
           for (int a = 0; a < 2; a++) {
             float A, B;
             const double r =
-                grid_point(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), true, lane, A, B) * w * w;
+                grid_point<GEO>(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), in.t2d, true, lane, A, B) *
+                w * w;
             // keep the best 3 (stable: earlier grid points win ties); an
             // insertion step with static register indices
             float cand[6] = {A, B, as[a], lv, lZs[z], blD};
@@ -485,7 +553,7 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
     float th[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
-    const double chi = lm_run<SWIM>(th, lo, hi, tv, dv, nt, w, q, K, maxit, lane, &steps);
+    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, in.t2d, maxit, lane, &steps);
     if (chi < bestchi) {
       bestchi = chi;
 #pragma unroll
@@ -500,19 +568,22 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
 }
 
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int swim, int seed_mode, int nstarts,
-                         int maxit, float* theta, double* chi, int* steps, hipStream_t st) {
+                         const float* bounds, const float* seeds, int nfits, int model, const Isf2dDev* t2d,
+                         int seed_mode, int nstarts, int maxit, float* theta, double* chi, int* steps, hipStream_t st) {
   // seed_mode: 0 grid, 1 given, 2 nonlinear given (the ABI's DDM_SEED_*) -> kernel mode
   const int mode = seed_mode == 0 ? 1 : (seed_mode == 2 ? 2 : 0);
-  if (K > MAXK) return hipErrorInvalidValue;
-  FitIn in{tau, d, ntau, w, qn, bounds, seeds, K};
+  if (K > MAXK || model < 0 || model > 2 || (model == 2 && !t2d)) return hipErrorInvalidValue;
+  FitIn in{tau, d, ntau, w, qn, bounds, seeds, K, model == 2 ? *t2d : Isf2dDev{}};
   const int blocks = (nfits + FNT / 64 - 1) / (FNT / 64);
-  if (swim)
-    hipLaunchKernelGGL(k_lm_fit<true>, dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta, chi,
-                       steps);
+  if (model == 2)
+    hipLaunchKernelGGL((k_lm_fit<true, 1>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
+                       chi, steps);
+  else if (model == 1)
+    hipLaunchKernelGGL((k_lm_fit<true, 0>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
+                       chi, steps);
   else
-    hipLaunchKernelGGL(k_lm_fit<false>, dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta, chi,
-                       steps);
+    hipLaunchKernelGGL((k_lm_fit<false, 0>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
+                       chi, steps);
   return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 // fit), launches k_lm_fit on the fitter's stream and times it with HIP events.
 #include "../../include/ddm.h"
 #include "ddm_internal.h"
+#include "ddm_isf2d.h"
 
 #include <hip/hip_runtime.h>
 
@@ -20,6 +21,10 @@ struct ddm_fitter {
   float* theta = nullptr;
   double* chi = nullptr;
   int* steps = nullptr;
+  // in-plane ISF table (uploaded on the first in-plane fit)
+  double *t2v = nullptr, *t2z = nullptr, *t2u0 = nullptr, *t2du = nullptr;
+  int *t2off = nullptr, *t2n = nullptr;
+  ddm::Isf2dDev t2d{};
 };
 
 namespace {
@@ -34,6 +39,36 @@ void release(ddm_fitter* f) {
   f->ntau = f->steps = nullptr;
   f->chi = nullptr;
   f->cap = 0;
+}
+
+void free_isf2d(ddm_fitter* f) {
+  for (void** p : {(void**)&f->t2v, (void**)&f->t2z, (void**)&f->t2u0, (void**)&f->t2du, (void**)&f->t2off,
+                   (void**)&f->t2n}) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// The in-plane ISF table of ddm_isf2d.h, copied to device memory once (a
+// failed upload leaves nothing behind, so the next in-plane fit retries).
+hipError_t upload_isf2d(ddm_fitter* f) {
+  const ddm::Isf2dTable& T = ddm::isf2d_table();
+  const size_t nz = T.Z.size();
+  hipError_t e;
+  if ((e = hipMalloc((void**)&f->t2v, T.v.size() * 8)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->t2z, nz * 8)) != hipSuccess || (e = hipMalloc((void**)&f->t2u0, nz * 8)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->t2du, nz * 8)) != hipSuccess || (e = hipMalloc((void**)&f->t2off, nz * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->t2n, nz * 4)) != hipSuccess)
+    return free_isf2d(f), e;
+  if ((e = hipMemcpy(f->t2v, T.v.data(), T.v.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(f->t2z, T.Z.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(f->t2u0, T.u0.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(f->t2du, T.du.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(f->t2off, T.off.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(f->t2n, T.n.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return free_isf2d(f), e;
+  f->t2d = ddm::Isf2dDev{f->t2v, f->t2z, f->t2u0, f->t2du, f->t2off, f->t2n, T.lz0, T.dlz};
+  return hipSuccess;
 }
 
 int ffail(ddm_fitter* f, int code, const std::string& m) {
@@ -78,6 +113,7 @@ void ddm_fitter_destroy(ddm_fitter* f) {
   if (!f) return;
   hipStreamSynchronize(f->stream);
   release(f);
+  free_isf2d(f);
   hipEventDestroy(f->ev0);
   hipEventDestroy(f->ev1);
   hipStreamDestroy(f->stream);
@@ -92,7 +128,8 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
   if (b->nfits == 0) return DDM_OK;
   if (b->ntau_max < 1 || b->ntau_max > kMaxT) return ffail(f, DDM_EINVAL, "fit: ntau_max must be in [1, 128]");
   if (b->knodes < 1 || b->knodes > kMaxK) return ffail(f, DDM_EINVAL, "fit: knodes must be in [1, 8]");
-  if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D) return ffail(f, DDM_EINVAL, "fit: model");
+  if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D && b->model != DDM_MODEL_SWIMMER_2D)
+    return ffail(f, DDM_EINVAL, "fit: model");
   if (b->seed_mode != DDM_SEED_GRID && b->seed_mode != DDM_SEED_GIVEN && b->seed_mode != DDM_SEED_NONLINEAR)
     return ffail(f, DDM_EINVAL, "fit: seed_mode");
   const bool seeded = b->seed_mode != DDM_SEED_GRID;
@@ -128,9 +165,11 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
       (e = up(f->bounds, bd.data(), n * 16)) != hipSuccess || (e = up(f->seeds, sd.data(), n * 24)) != hipSuccess ||
       (e = up(f->ntau, nt.data(), n * 4)) != hipSuccess)
     return ffail(f, DDM_EHIP, std::string("fit: upload: ") + hipGetErrorString(e));
+  if (b->model == DDM_MODEL_SWIMMER_2D && !f->t2v && (e = upload_isf2d(f)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("fit: in-plane ISF table: ") + hipGetErrorString(e));
   hipEventRecord(f->ev0, f->stream);
-  e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds, b->nfits,
-                         b->model == DDM_MODEL_SWIMMER_3D, b->seed_mode,
+  e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds, b->nfits, b->model,
+                         &f->t2d, b->seed_mode,
                          std::max(1, std::min(3, b->nstarts)), std::max(0, b->max_iter), f->theta, f->chi, f->steps,
                          f->stream);
   if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: launch: ") + hipGetErrorString(e));

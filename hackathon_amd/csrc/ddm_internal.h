@@ -44,9 +44,18 @@ hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* row
                          float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                          int first_slot, hipStream_t st);
 
+// Device copy of the in-plane ISF table (ddm_isf2d.h): row i's values at
+// v[off[i] .. off[i] + n[i] + 1], sampled at ln Lambda = u0[i] + k du[i].
+struct Isf2dDev {
+  const double *v, *Z, *u0, *du;
+  const int *off, *n;
+  double lz0, dlz;
+};
+
+// model: 0 Brownian, 1 3D swimmer, 2 in-plane swimmer (t2d required)
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int swim, int seed_mode, int nstarts,
-                         int maxit, float* theta, double* chi, int* steps, hipStream_t st);
+                         const float* bounds, const float* seeds, int nfits, int model, const Isf2dDev* t2d,
+                         int seed_mode, int nstarts, int maxit, float* theta, double* chi, int* steps, hipStream_t st);
 
 hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
                             float* C, long ldc, int M, int N, int K, int acc, hipStream_t st);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/ddm.h"
+#include "ddm_isf2d.h"
 
 namespace {
 
@@ -57,77 +58,29 @@ double isf3d(double x, double Z) {
   return std::sin(Z * std::atan(u)) / (Z * u * std::exp(0.5 * Z * std::log1p(u * u)));
 }
 
-// P_nu(c) by the trapezoid rule on Laplace's first integral (:120-132)
-double legendre_p(double nu, double c) {
-  const double s = std::sqrt(std::max(0.0, 1 - c * c));
-  const int n = 16 + (int)std::ceil(nu * std::atan2(s, c));
-  double acc = 0;
-  for (int i = 0; i <= n; i++) {
-    const double im = s * std::cos(M_PI * i / n);
-    const double v = std::exp(0.5 * nu * std::log(c * c + im * im)) * std::cos(nu * std::atan2(im, c));
-    acc += (i == 0 || i == n) ? 0.5 * v : v;
-  }
-  return acc / n;
-}
-
-double isf2d_lambda(double Lm, double Z) {   // P_Z(1/sqrt(1+L^2)) (1+L^2)^(-(Z+1)/2)
-  const double u = 1 + Lm * Lm;
-  const double ln_pre = -0.5 * (Z + 1) * std::log(u);
-  if (ln_pre < -25) return 0;
-  return legendre_p(Z, 1 / std::sqrt(u)) * std::exp(ln_pre);
-}
-
-// The fitter's tabulated in-plane ISF (isf2dFast, :162-214): rows on a uniform
-// ln Z grid (0.5..100, 72 rows), Catmull-Rom along ln Lambda, quadratic
-// Lagrange across rows.
-struct Row2d {
-  double Z, u0, du;
-  int n;
-  std::vector<double> v;
-};
-struct Table2d {
-  static constexpr int nz = 72;
-  double lz0 = std::log(0.5), lz1 = std::log(100.0), dlz = 0;
-  std::vector<Row2d> rows;
-};
-const Table2d& table2d() {
-  static Table2d t;
-  static std::once_flag once;
-  std::call_once(once, [] {
-    t.dlz = (t.lz1 - t.lz0) / (Table2d::nz - 1);
-    for (int i = 0; i < Table2d::nz; i++) {
-      Row2d r;
-      r.Z = std::exp(t.lz0 + i * t.dlz);
-      const double Lc = std::min(1e4, std::sqrt(std::expm1(50 / (r.Z + 1))));
-      r.u0 = std::log(1e-4);
-      const double u1 = std::log(Lc);
-      r.du = std::min(0.04, 0.1 / ((r.Z + 0.5) / 2));
-      r.n = (int)std::ceil((u1 - r.u0) / r.du) + 1;
-      r.v.resize(r.n + 2);
-      for (int k = 0; k < r.n + 2; k++) r.v[k] = isf2d_lambda(std::exp(r.u0 + k * r.du), r.Z);
-      t.rows.push_back(std::move(r));
-    }
-  });
-  return t;
-}
-double row_eval(const Row2d& r, double x) {
-  const double Lm = x / (r.Z + 1);
-  if (Lm < 1e-4) return 1 - x * x * (r.Z + 2) / (4 * (r.Z + 1));
-  const double f = (std::log(Lm) - r.u0) / r.du;
-  if (f >= r.n - 1) return 0;
-  const int i = (int)std::floor(f);
-  const double t = f - i;
-  const double p1 = r.v[i], p2 = r.v[i + 1], p3 = r.v[i + 2];
-  const double p0 = i > 0 ? r.v[i - 1] : 2 * p1 - p2;
+// The fitter's tabulated in-plane ISF (isf2dFast, :168-219) on the shared
+// table (ddm_isf2d.h): Catmull-Rom along ln Lambda, quadratic Lagrange across
+// rows.
+double row_eval(const ddm::Isf2dTable& T, int i, double x) {
+  const double Zi = T.Z[i];
+  const double Lm = x / (Zi + 1);
+  if (Lm < ddm::Isf2dTable::lser) return 1 - x * x * (Zi + 2) / (4 * (Zi + 1));
+  const double f = (std::log(Lm) - T.u0[i]) / T.du[i];
+  if (f >= T.n[i] - 1) return 0;
+  const int k = (int)std::floor(f);
+  const double t = f - k;
+  const double* v = T.v.data() + T.off[i];
+  const double p1 = v[k], p2 = v[k + 1], p3 = v[k + 2];
+  const double p0 = k > 0 ? v[k - 1] : 2 * p1 - p2;
   return p1 + 0.5 * t * (p2 - p0 + t * (2 * p0 - 5 * p1 + 4 * p2 - p3 + t * (3 * (p1 - p2) + p3 - p0)));
 }
 double isf2d_fast(double x, double Z) {
-  const Table2d& T = table2d();
+  const ddm::Isf2dTable& T = ddm::isf2d_table();
   const double zc = std::min(100.0, std::max(0.5, Z));
   const double zf = (std::log(zc) - T.lz0) / T.dlz;
-  const int i0 = std::min(std::max((int)std::floor(zf + 0.5) - 1, 0), Table2d::nz - 3);
+  const int i0 = std::min(std::max((int)std::floor(zf + 0.5) - 1, 0), ddm::Isf2dTable::nz - 3);
   const double t = zf - i0;
-  const double g0 = row_eval(T.rows[i0], x), g1 = row_eval(T.rows[i0 + 1], x), g2 = row_eval(T.rows[i0 + 2], x);
+  const double g0 = row_eval(T, i0, x), g1 = row_eval(T, i0 + 1, x), g2 = row_eval(T, i0 + 2, x);
   return 0.5 * (t - 1) * (t - 2) * g0 - t * (t - 2) * g1 + 0.5 * t * (t - 1) * g2;
 }
 

@@ -19,7 +19,7 @@ from .engine import DdmError, Radial, load_library
 
 MAXT = 128
 MAXK = 8
-BROWNIAN, SWIMMER_3D = 0, 1
+BROWNIAN, SWIMMER_3D, SWIMMER_2D = 0, 1, 2   # SWIMMER_2D: the in-plane ISF (isf2dFast table)
 SEED_GRID, SEED_GIVEN, SEED_NONLINEAR = 0, 1, 2
 SWIM_MIN_UM_S = 2.0   # GATES.SWIM_MIN_UM_S (src/motility_fit.js:37)
 

@@ -1,7 +1,7 @@
 // Type declarations of the per-q motility fit module (motility-fit.mjs).
 import { Radial } from './ddm-engine';
 
-export declare const MODEL: Readonly<{ BROWNIAN: 0; SWIMMER_3D: 1 }>;
+export declare const MODEL: Readonly<{ BROWNIAN: 0; SWIMMER_3D: 1; SWIMMER_2D: 2 }>;
 export declare const SWIM_MIN_UM_S: number;
 
 export interface PreparedBin { b: number; q: number; qn: Float64Array; d: Float64Array; w: number }

@@ -18,7 +18,7 @@ const here = path.dirname(fileURLToPath(import.meta.url));
 const require = createRequire(import.meta.url);
 const addon = require(path.join(here, '..', 'napi', 'ddm_napi.node'));
 
-export const MODEL = Object.freeze({ BROWNIAN: 0, SWIMMER_3D: 1 });
+export const MODEL = Object.freeze({ BROWNIAN: 0, SWIMMER_3D: 1, SWIMMER_2D: 2 });
 const SEED = Object.freeze({ GRID: 0, GIVEN: 1, NONLINEAR: 2 });
 /** GATES.SWIM_MIN_UM_S (src/motility_fit.js:37) */
 export const SWIM_MIN_UM_S = 2;

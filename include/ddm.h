@@ -173,10 +173,13 @@ int ddm_kernel_times(ddm_engine *e, double *ms, int32_t *count, int32_t n);
  * alpha, vbar, Z, D globally with Nelder-Mead over VarPro
  * (fitSwimmer/fitBrownian :500-611); this entry point is the per-q LM of the
  * MI355X design (no reference counterpart for the LM itself).
- * theta = (A, B, alpha, ln vbar, ln Z, ln D); chi^2 = w^2 sum (d - f)^2. */
+ * theta = (A, B, alpha, ln vbar, ln Z, ln D); chi^2 = w^2 sum (d - f)^2.
+ * g: DDM_MODEL_SWIMMER_3D the Schulz swimmer ISF sin(Z atan L)/(Z L (1+L^2)^(Z/2))
+ * (isf3d :140-144), DDM_MODEL_SWIMMER_2D the in-plane one from the reference's
+ * tabulated kernel (isf2dFast :168-219, the same table as the verdict layer). */
 typedef struct ddm_fitter ddm_fitter;
 
-enum { DDM_MODEL_BROWNIAN = 0, DDM_MODEL_SWIMMER_3D = 1 };
+enum { DDM_MODEL_BROWNIAN = 0, DDM_MODEL_SWIMMER_3D = 1, DDM_MODEL_SWIMMER_2D = 2 };
 enum { DDM_SEED_GRID = 0, DDM_SEED_GIVEN = 1, DDM_SEED_NONLINEAR = 2 };
 
 typedef struct ddm_fit_batch {

@@ -1,4 +1,4 @@
-"""Batched per-q LM on the GPU against the FP64 oracle LM."""
+"""Batched per-q LM on the GPU against the FP64 oracle LM (3D swimmer ISF and the in-plane table)."""
 import numpy as np
 import pytest
 
@@ -7,6 +7,7 @@ from hackathon_amd import fit as F
 from tests.test_fit_oracle import truth_radial
 
 pytestmark = pytest.mark.gpu
+MODEL = {"3d": F.SWIMMER_3D, "2d": F.SWIMMER_2D}
 
 
 def _problems(pr):
@@ -33,14 +34,15 @@ def test_gpu_brownian_closed_form(gpu):
     assert np.max(np.abs(res.theta[:, 0] / A - 1)) < 1e-4
 
 
-def test_gpu_swimmer_noise_free(gpu):
-    pr = O.prepare(truth_radial("3d"), 1, 30)
+@pytest.mark.parametrize("geometry", ["3d", "2d"])
+def test_gpu_swimmer_noise_free(gpu, geometry):
+    pr = O.prepare(truth_radial(geometry), 1, 30)
     tau, data, w, qn, bd, b4 = _problems(pr)
-    res = F.Fitter().fit(tau, data, w, qn, bd, model=F.SWIMMER_3D)
+    res = F.Fitter().fit(tau, data, w, qn, bd, model=MODEL[geometry])
     print("GPU kernel ms", res.kernel_ms, "iters", res.iters)
     good = 0
     for i, bn in enumerate(pr["bins"]):
-        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, "3d")
+        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, geometry)
         if chi < 1e-20:   # the oracle pins this bin exactly: the GPU must find the truth too
             good += 1
             assert res.chi2[i] < 1e-9, (i, res.chi2[i])
@@ -49,17 +51,18 @@ def test_gpu_swimmer_noise_free(gpu):
     assert good >= 24
 
 
-def test_gpu_swimmer_noisy_matches_oracle(gpu):
-    r = truth_radial("3d")
+@pytest.mark.parametrize("geometry", ["3d", "2d"])
+def test_gpu_swimmer_noisy_matches_oracle(gpu, geometry):
+    r = truth_radial(geometry)
     rng = np.random.default_rng(7)
     r["D"] = r["D"] * (1 + 0.003 * rng.standard_normal(r["D"].shape))
     pr = O.prepare(r, 1, 30)
     tau, data, w, qn, bd, b4 = _problems(pr)
-    res = F.Fitter().fit(tau, data, w, qn, bd, model=F.SWIMMER_3D)
+    res = F.Fitter().fit(tau, data, w, qn, bd, model=MODEL[geometry])
     agree = 0
     rows = []
     for i, bn in enumerate(pr["bins"]):
-        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, "3d")
+        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, geometry)
         rel_chi = (res.chi2[i] - chi) / chi
         p_gpu = np.array([res.alpha[i], res.vbar[i], res.Z[i], res.D[i]])
         p_orc = np.array([th[2], np.exp(th[3]), np.exp(th[4]), np.exp(th[5])])
@@ -74,16 +77,17 @@ def test_gpu_swimmer_noisy_matches_oracle(gpu):
     assert agree >= 0.75 * len(rows)
 
 
-def test_gpu_seeded_from_oracle_optimum(gpu):
-    pr = O.prepare(truth_radial("3d"), 1, 30)
+@pytest.mark.parametrize("geometry", ["3d", "2d"])
+def test_gpu_seeded_from_oracle_optimum(gpu, geometry):
+    pr = O.prepare(truth_radial(geometry), 1, 30)
     tau, data, w, qn, bd, b4 = _problems(pr)
     seeds = []
     chis = []
     for bn in pr["bins"]:
-        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, "3d")
+        th, chi, it = O.lm_plain(pr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, geometry)
         seeds.append(th)
         chis.append(chi)
-    res = F.Fitter().fit(tau, data, w, qn, bd, model=F.SWIMMER_3D, seeds=np.array(seeds))
+    res = F.Fitter().fit(tau, data, w, qn, bd, model=MODEL[geometry], seeds=np.array(seeds))
     chis = np.array(chis)
     assert np.all(res.chi2 <= np.maximum(chis * (1 + 1e-3), 1e-9))
     assert np.median(res.iters) < 20
