@@ -36,8 +36,7 @@ __host__ __device__ constexpr int rows_pairs_per_wg(int logn) {
 // n/2 floats of p) behind the sub-transform exchange buffer when the
 // transform is split, else whichever of the two is larger (they alias).
 __host__ __device__ constexpr int temporal_wave_lds(int logt, int lm) {
-  return lm ? xch_size(1 << (logt - lm)) + (3 << logt) / 4
-            : (xch_size(1 << logt) > (3 << logt) / 4 ? xch_size(1 << logt) : (3 << logt) / 4);
+  return xch_size(1 << (logt - lm)) + (3 << logt) / 4;
 }
 
 // ----------------------------------------------------------------- helpers
@@ -452,10 +451,13 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 //   Y(m k + r) = FFT_{n/m}[ u_r ](k),  u_r(t1) = w_n^{r t1} sum_{t2} y(t1 + t2 n/m) w_m^{r t2}
 // (only t2 < m/2 contribute), so a wave never holds more than n/m points of a
 // transform (m = 1 for n <= 512, whose first pass skips the zero half
-// instead; 512-point sub-transforms above).  Each sub-transform uses wave-private LDS exchanges; |Y|^2 and p
-// are parked in the wave's buffer and, after a workgroup barrier, every thread
-// adds its slice of the 4 parked spectra into 2Sum accumulators.  The next
-// round's series is loaded while the current one is transformed.
+// instead; 512-point sub-transforms above).  Each sub-transform uses
+// wave-private LDS exchanges.  |Y|^2 and p are added into accumulators
+// private to the wave (plain LDS read-modify-write: one lane owns each
+// address, sums in mode order), so the waves never wait for each other inside
+// the mode loop; the 4 waves' sums are added pairwise once, at the end.  (The
+// same adds as LDS float atomics made the kernel 3.7x slower on gfx950.)  The next round's
+// series is loaded while the current one is transformed.
 template <int LOGT, int LM>
 __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__ hist, int T,
                                                       const int4* __restrict__ items,
@@ -504,11 +506,13 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   constexpr uint32_t all_bits = PT == 32 ? 0xffffffffu : ((1u << PT) - 1u);
   // wave-uniform: every lane's PT samples valid (full mask and T = n/2)
   const bool dense = __all(mbits == all_bits);
-  float S[KS], Sc[KS], pa[TS], pc[TS];
-#pragma unroll
-  for (int i = 0; i < KS; i++) S[i] = Sc[i] = 0.f;
-#pragma unroll
-  for (int i = 0; i < TS; i++) pa[i] = pc[i] = 0.f;
+  // the wave's accumulators behind its exchange buffer: sum |Y(k)|^2 at
+  // [0, n) (r-major for LM > 0: index r n/m + j holds k = m j + r; k_bin_sum
+  // undoes the order) and sum p(t) at [n, 3n/2).  A wave sums at most 16
+  // modes of an item plainly (positive terms: relative error <= 15 ulp); the
+  // items' partials are added with compensation in k_bin_sum.
+  float* acc = reinterpret_cast<float*>(xch + xch_size(NF));
+  for (int i = lane; i < n + n / 2; i += 64) acc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
   float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / 4 (lanes 4 j, j < KT)
   const int rounds = (count + 3) / 4;
@@ -519,12 +523,26 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   // series is not dense, and the mask multiply zeroes them).  One base
   // address with immediate offsets; guarded or clamped loads would cost a
   // branch region or an address pair per sample.
+  // A non-identity order is applied by the loads themselves: sample t is
+  // gathered from slot order[t] (the n/2 entries of order[] past T are 0,
+  // and their samples are masked off).
+  uint32_t ord[PT];
+#pragma unroll
+  for (int q = 0; q < PT; q++) ord[q] = identity ? (uint32_t)(lane + 64 * q) : order[lane + 64 * q];
   auto load_series = [&](float2 (&dst)[PT], int slot) {
     const float2* x = hist + (long)bin_modes[start + (slot < count ? slot : count - 1)] * T;
+    if (identity) {
 #pragma unroll
-    for (int q = 0; q < PT; q++) {   // each series is read once: non-temporal loads
-      const float2* src = &x[lane + 64 * q];
-      dst[q] = make_float2(__builtin_nontemporal_load(&src->x), __builtin_nontemporal_load(&src->y));
+      for (int q = 0; q < PT; q++) {   // each series is read once: non-temporal loads
+        const float2* src = &x[lane + 64 * q];
+        dst[q] = make_float2(__builtin_nontemporal_load(&src->x), __builtin_nontemporal_load(&src->y));
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        const float2* src = &x[ord[q]];
+        dst[q] = make_float2(__builtin_nontemporal_load(&src->x), __builtin_nontemporal_load(&src->y));
+      }
     }
   };
   // next round's series, loaded while the current one is transformed (up to
@@ -532,7 +550,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   constexpr bool PREFETCH = LOGT <= 11;
   float2 nx[PREFETCH ? PT : 1];
   if constexpr (PREFETCH) load_series(nx, wave);
-  float* park = reinterpret_cast<float*>(xch);
+  wave_sync();
   for (int rd = 0; rd < rounds; rd++) {
     const int mi = rd * 4 + wave;
     const bool active = mi < count;
@@ -543,16 +561,6 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
       load_series(nx, mi + 4);
     } else {
       load_series(y, mi);
-    }
-    if (!identity) {
-      // every slot t < n/2 is written (xpad(n/2 - 1) lies inside the wave's
-      // buffer); order[] holds n/2 entries, those past T are 0 (their
-      // samples are masked off)
-#pragma unroll
-      for (int q = 0; q < PT; q++) xch[xpad(lane + 64 * q)] = y[q];
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < PT; q++) y[q] = xch[xpad((int)order[lane + 64 * q])];
     }
     float sx = 0.f, sy = 0.f, ps = 0.f;
     if (dense) {   // every sample of every lane valid: no masking
@@ -590,7 +598,9 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     // error is ~ulp x sum_t |y|^2, far above sum over the N(tau) <= KT pairs
     // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer: the
     // head is in register 0 (KT <= 16 < 64), the tail in the registers
-    // q0..q1 (wave-uniform, at most two).
+    // q0..q1 (wave-uniform, at most two).  The previous mode's last exchange
+    // reads are done first.
+    wave_sync();
     if (lane < KT) xch[lane] = y[0];
     {
       const int q0 = (T - KT) >> 6, q1 = (T - 1) >> 6;
@@ -623,17 +633,17 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
 #pragma unroll
       for (int q = PT; q < PF; q++) v[q] = make_float2(0.f, 0.f);
       wave_fft<LOGF, false, true, true>(v, xch, twl, lane);
-      wave_sync();
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < PF; q++) park[lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
+        for (int q = 0; q < PF; q++) acc[lane + 64 * q] += v[q].x * v[q].x + v[q].y * v[q].y;
 #pragma unroll
-      for (int q = 0; q < PT; q++) park[n + lane + 64 * q] = active ? y[q].x * y[q].x + y[q].y * y[q].y : 0.f;
+        for (int q = 0; q < PT; q++) acc[n + lane + 64 * q] += y[q].x * y[q].x + y[q].y * y[q].y;
+      }
     } else {
       // u_r(t1) = w_n^{r t1} sum_{t2 < m/2} y(t1 + t2 n/m) w_m^{r t2}: for
       // t1 = lane + 64 q the samples t1 + t2 n/m = lane + 64 (q + t2 PF) are
-      // in this lane's own registers.  |Y(m k + r)|^2 is parked after each
-      // sub-transform in the area behind the exchange buffer.
-      float* parky = reinterpret_cast<float*>(xch + xch_size(NF));
+      // in this lane's own registers.  |Y(m k + r)|^2 is added after each
+      // sub-transform (consecutive lanes, consecutive words: no bank conflicts).
 #pragma unroll 1
       for (int r = 0; r < m; r++) {
         float2 v[PF];
@@ -662,42 +672,34 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
         }
         if constexpr (LOGF == 9) wave_fft_wl<LOGF, false, false, true>(v, xch, twl, lane, wlast);
         else wave_fft<LOGF, false, false, true>(v, xch, twl, lane);
-        // |Y(m j + r)|^2 parked r-major (index r NF + j): consecutive lanes,
-        // consecutive words, no bank conflicts; k_bin_sum undoes the order
+        if (active) {
 #pragma unroll
-        for (int q = 0; q < PF; q++) parky[r * NF + lane + 64 * q] = active ? v[q].x * v[q].x + v[q].y * v[q].y : 0.f;
+          for (int q = 0; q < PF; q++) acc[r * NF + lane + 64 * q] += v[q].x * v[q].x + v[q].y * v[q].y;
+        }
       }
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < PT; q++) parky[n + lane + 64 * q] = active ? y[q].x * y[q].x + y[q].y * y[q].y : 0.f;
-    }
-    __syncthreads();
-    {
-      // the 4 waves' values are added pairwise (positive terms: relative error
-      // <= 2 ulp), then compensated into the running sums
-      const int po = LM ? 2 * xch_size(NF) : 0;   // parked area, in floats
-      const float* p0 = reinterpret_cast<const float*>(lds) + po;
-      constexpr int WF = 2 * WS;                   // wave stride, in floats
-#pragma unroll
-      for (int i = 0; i < KS; i++) {
-        const int k = tid + NT * i;
-        two_sum_add(S[i], Sc[i], (p0[k] + p0[WF + k]) + (p0[2 * WF + k] + p0[3 * WF + k]));
-      }
-#pragma unroll
-      for (int i = 0; i < TS; i++) {
-        const int t = n + tid + NT * i;
-        if (tid + NT * i < n / 2) two_sum_add(pa[i], pc[i], (p0[t] + p0[WF + t]) + (p0[2 * WF + t] + p0[3 * WF + t]));
+        for (int q = 0; q < PT; q++) acc[n + lane + 64 * q] += y[q].x * y[q].x + y[q].y * y[q].y;
       }
     }
-    __syncthreads();
   }
-  float* Sp = S_part + (long)blockIdx.x * n;
+  __syncthreads();
+  {
+    // the 4 waves' sums, added pairwise (positive terms: relative error <= 2 ulp)
+    const float* p0 = reinterpret_cast<const float*>(lds) + 2 * xch_size(NF);
+    constexpr int WF = 2 * WS;                     // wave stride, in floats
+    float* Sp = S_part + (long)blockIdx.x * n;
 #pragma unroll
-  for (int i = 0; i < KS; i++) Sp[tid + NT * i] = S[i] + Sc[i];
-  float* pp = p_part + (long)blockIdx.x * T;
+    for (int i = 0; i < KS; i++) {
+      const int k = tid + NT * i;
+      Sp[k] = (p0[k] + p0[WF + k]) + (p0[2 * WF + k] + p0[3 * WF + k]);
+    }
+    float* pp = p_part + (long)blockIdx.x * T;
 #pragma unroll
-  for (int i = 0; i < TS; i++) {
-    const int t = tid + NT * i;
-    if (t < T) pp[t] = pa[i] + pc[i];
+    for (int i = 0; i < TS; i++) {
+      const int t = tid + NT * i;
+      if (t < T) pp[t] = (p0[n + t] + p0[WF + n + t]) + (p0[2 * WF + n + t] + p0[3 * WF + n + t]);
+    }
   }
   const float pw = wave_sum(aP + cP);
   if (lane == 0) red[wave] = pw;
