@@ -89,6 +89,7 @@ struct FitIn {
   const float* qn;      // [nfits * K]
   const float* bounds;  // [nfits * 4] vmin, vmax, Dmin, Dmax
   const float* seeds;   // [nfits * 6] or null
+  const float* vmag;    // [nfits] uniform drift speed |V|, or null (none)
   int K;
   Isf2dDev t2d;         // in-plane swimmer model only
 };
@@ -141,20 +142,47 @@ __device__ __noinline__ double one_minus_isf2d(double x, double Z, const Isf2dDe
   return om;
 }
 
+// J0 by the reference's rational approximation (besselJ0,
+// src/motility_fit.js:54-68: a ratio of two degree-5 polynomials in x^2 below
+// |x| = 8, the two-term asymptotic form above; absolute error < 1e-8), FP64.
+// The coefficients are the reference's: the drift factor must be the same
+// function.
+__device__ __noinline__ double bessel_j0(double x) {
+  const double ax = fabs(x);
+  if (ax < 8) {
+    const double y = x * x;
+    const double p = 57568490574.0 +
+                     y * (-13362590354.0 + y * (651619640.7 + y * (-11214424.18 + y * (77392.33017 + y * -184.9052456))));
+    const double r = 57568490411.0 + y * (1029532985.0 + y * (9494680.718 + y * (59272.64853 + y * (267.8532712 + y))));
+    return p / r;
+  }
+  const double z = 8 / ax, y = z * z, xx = ax - 0.785398164;
+  const double p = 1 + y * (-0.1098628627e-2 + y * (0.2734510407e-4 + y * (-0.2073370639e-5 + y * 0.2093887211e-6)));
+  const double r =
+      -0.1562499995e-1 + y * (0.1430488765e-3 + y * (-0.6911147651e-5 + y * (0.7621095161e-6 - y * 0.934935152e-7)));
+  return sqrt(0.636619772 / ax) * (cos(xx) * p - z * sin(xx) * r);
+}
+
 // Column value (and derivatives w.r.t. alpha, ln vbar, ln Z, ln D) at one lag;
-// GEO 0: the 3D swimmer ISF, 1: the in-plane one (table T).
+// GEO 0: the 3D swimmer ISF, 1: the in-plane one (table T).  V: a uniform
+// drift speed, E = exp(-D q^2 tau) J0(q V tau) (modelColumn), with
+// 1 - E = -expm1(-s) + exp(-s) (1 - J0) so the small-lag values do not cancel.
 template <int GEO>
 __device__ __forceinline__ void column(float t, const float* q, int K, float alpha, float vbar, float Z, float Dd,
-                                       const Isf2dDev& T, bool swim, bool deriv, float& c, float& da, float& dlv,
-                                       float& dlz,
-                                       float& dld) {
+                                       float V, const Isf2dDev& T, bool swim, bool deriv, float& c, float& da,
+                                       float& dlv, float& dlz, float& dld) {
   c = da = dlv = dlz = dld = 0.f;
 #pragma unroll
   for (int i = 0; i < MAXK; i++) {
     if (i >= K) break;
     const float qi = q[i];
     const float s = Dd * qi * qi * t;
-    const float E = expf(-s), omE = -expm1f(-s);
+    float E = expf(-s), omE = -expm1f(-s);
+    if (V > 0.f) {
+      const double J = bessel_j0((double)qi * V * t);
+      omE += (float)((double)E * (1.0 - J));
+      E = (float)(E * J);
+    }
     Omg og = {0.f, 0.f, 0.f};
     if (swim) {
       if constexpr (GEO == 1) {
@@ -192,7 +220,7 @@ __device__ __noinline__ double one_minus_isf3d_d(double x, double Z) {
 
 // Column value at one lag in FP64 (the objective; the Jacobian stays FP32).
 template <int GEO>
-__device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd,
+__device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd, double V,
                            const Isf2dDev& T, bool swim) {
   double c = 0;
 #pragma unroll
@@ -202,7 +230,13 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
     const double s = Dd * qi * qi * t;
     double og = 0.0, gx, gz;
     if (swim) og = GEO == 1 ? one_minus_isf2d(qi * vbar * t, Z, T, gx, gz) : one_minus_isf3d_d(qi * vbar * t, Z);
-    c += -expm1(-s) + exp(-s) * alpha * og;
+    double E = exp(-s), omE = -expm1(-s);
+    if (V > 0) {
+      const double J = bessel_j0(qi * V * t);
+      omE += E * (1 - J);
+      E *= J;
+    }
+    c += omE + E * alpha * og;
   }
   return c / K;
 }
@@ -212,8 +246,8 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
 // compared, so LM accept/reject and termination decisions are not driven by
 // FP32 rounding of the model.
 template <int GEO>
-__device__ __noinline__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q, int K,
-                          const Isf2dDev& T, bool swim, int lane) {
+__device__ __noinline__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q,
+                                       int K, float V, const Isf2dDev& T, bool swim, int lane) {
   double s = 0;
   const double alpha = swim ? th[2] : 0.0, vbar = exp((double)th[3]), Z = exp((double)th[4]),
                Dd = exp((double)th[5]);
@@ -221,7 +255,7 @@ __device__ __noinline__ double chi2_at(const float* th, const float* tv, const f
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
-      const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim);
+      const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim);
       const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
       s += res * res;
     }
@@ -268,14 +302,14 @@ __device__ double nnls2(double g11, double g12, double n, double h1, double h2, 
 
 template <int GEO>
 __device__ double grid_point(const float* tv, const float* dv, int nt, const float* q, int K, float alpha, float vbar,
-                             float Z, float Dd, const Isf2dDev& T, bool swim, int lane, float& A, float& B) {
+                             float Z, float Dd, float V, const Isf2dDev& T, bool swim, int lane, float& A, float& B) {
   double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
       float c, a, b, e, f;
-      column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim, false, c, a, b, e, f);
+      column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, false, c, a, b, e, f);
       g11 += (double)c * c;
       g12 += c;
       h1 += (double)c * dv[r];
@@ -296,12 +330,12 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fm
 // LM from theta (in/out); returns chi^2, adds the steps taken to *steps.
 template <bool SWIM, int GEO>
 __device__ double lm_run(float* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
-                         const float* q, int K, const Isf2dDev& T, int maxit, int lane, int* steps) {
+                         const float* q, int K, float V, const Isf2dDev& T, int maxit, int lane, int* steps) {
   constexpr bool swim = SWIM;
   constexpr int p = SWIM ? 6 : 3;
   constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
   for (int k = 0; k < 6; k++) th[k] = clampf(th[k], lo[k], hi[k]);
-  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, T, swim, lane);
+  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane);
   double lam = 1e-3, nu = 2;
   int it = 0;
   bool done = false;
@@ -317,7 +351,7 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
       const int j = lane + 64 * r;
       if (j < nt) {
         float c, da, dlv, dlz, dld;
-        column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, T, swim, true, c, da, dlv, dlz, dld);
+        column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, true, c, da, dlv, dlz, dld);
         double F[6];
         F[0] = (double)w * c;
         F[1] = w;
@@ -418,7 +452,7 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
           const double dk = (double)nth[k] - th[k];
           pred += dk * (lam * diag[a] * dk + g[a]);
         }
-        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, T, swim, lane);
+        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane);
         const double rho = pred > 0 ? (chi - nchi) / pred : -1;
         if (nchi < chi && rho > 0) {
           const double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
@@ -469,6 +503,7 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
 #pragma unroll
   for (int i = 0; i < MAXK; i++) q[i] = i < K ? in.qn[(long)f * in.K + i] : 0.f;
   const float w = in.w[f];
+  const float V = in.vmag ? in.vmag[f] : 0.f;
   const float vmin = in.bounds[4 * f], vmax = in.bounds[4 * f + 1], Dmin = in.bounds[4 * f + 2],
               Dmax = in.bounds[4 * f + 3];
   const float lo[6] = {0.f, 0.f, 0.f, logf(vmin), logf(0.5f), logf(Dmin)};
@@ -484,8 +519,8 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
 #pragma unroll
       for (int k = 3; k < 6; k++) st[0][k] = fminf(hi[k], fmaxf(lo[k], st[0][k]));
       float A, B;
-      grid_point<GEO>(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), in.t2d,
-                      SWIM, lane, A, B);
+      grid_point<GEO>(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), V,
+                      in.t2d, SWIM, lane, A, B);
       st[0][0] = A;
       st[0][1] = B;
     }
@@ -496,7 +531,7 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
     for (int i = 0; i <= 48; i++) {
       const float lD = lo[5] + ((hi[5] - lo[5]) * i) / 48.f;
       float A, B;
-      const double r = grid_point<GEO>(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), in.t2d, false, lane, A, B);
+      const double r = grid_point<GEO>(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), V, in.t2d, false, lane, A, B);
       if (r < bchi) {
         bchi = r;
         bA = A;
@@ -522,7 +557,8 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
           for (int a = 0; a < 2; a++) {
             float A, B;
             const double r =
-                grid_point<GEO>(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), in.t2d, true, lane, A, B) *
+                grid_point<GEO>(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), V, in.t2d, true, lane, A,
+                                B) *
                 w * w;
             // keep the best 3 (stable: earlier grid points win ties); an
             // insertion step with static register indices
@@ -553,7 +589,7 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
     float th[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
-    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, in.t2d, maxit, lane, &steps);
+    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps);
     if (chi < bestchi) {
       bestchi = chi;
 #pragma unroll
@@ -568,12 +604,13 @@ __global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, i
 }
 
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int model, const Isf2dDev* t2d,
-                         int seed_mode, int nstarts, int maxit, float* theta, double* chi, int* steps, hipStream_t st) {
+                         const float* bounds, const float* seeds, const float* vmag, int nfits, int model,
+                         const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
+                         int* steps, hipStream_t st) {
   // seed_mode: 0 grid, 1 given, 2 nonlinear given (the ABI's DDM_SEED_*) -> kernel mode
   const int mode = seed_mode == 0 ? 1 : (seed_mode == 2 ? 2 : 0);
   if (K > MAXK || model < 0 || model > 2 || (model == 2 && !t2d)) return hipErrorInvalidValue;
-  FitIn in{tau, d, ntau, w, qn, bounds, seeds, K, model == 2 ? *t2d : Isf2dDev{}};
+  FitIn in{tau, d, ntau, w, qn, bounds, seeds, vmag, K, model == 2 ? *t2d : Isf2dDev{}};
   const int blocks = (nfits + FNT / 64 - 1) / (FNT / 64);
   if (model == 2)
     hipLaunchKernelGGL((k_lm_fit<true, 1>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -16,7 +17,8 @@ struct ddm_fitter {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   size_t cap = 0;  // fits the buffers hold
-  float *tau = nullptr, *d = nullptr, *w = nullptr, *qn = nullptr, *bounds = nullptr, *seeds = nullptr;
+  float *tau = nullptr, *d = nullptr, *w = nullptr, *qn = nullptr, *bounds = nullptr, *seeds = nullptr,
+        *vmag = nullptr;
   int* ntau = nullptr;
   float* theta = nullptr;
   double* chi = nullptr;
@@ -32,10 +34,10 @@ constexpr int kMaxT = 128;
 constexpr int kMaxK = 8;
 
 void release(ddm_fitter* f) {
-  void* ptrs[] = {f->tau, f->d, f->w, f->qn, f->bounds, f->seeds, f->ntau, f->theta, f->chi, f->steps};
+  void* ptrs[] = {f->tau, f->d, f->w, f->qn, f->bounds, f->seeds, f->vmag, f->ntau, f->theta, f->chi, f->steps};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  f->tau = f->d = f->w = f->qn = f->bounds = f->seeds = f->theta = nullptr;
+  f->tau = f->d = f->w = f->qn = f->bounds = f->seeds = f->vmag = f->theta = nullptr;
   f->ntau = f->steps = nullptr;
   f->chi = nullptr;
   f->cap = 0;
@@ -86,6 +88,7 @@ hipError_t grow(ddm_fitter* f, size_t n) {
   if ((e = hipMalloc((void**)&f->qn, n * kMaxK * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->bounds, n * 16)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->seeds, n * 24)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&f->vmag, n * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->ntau, n * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->theta, n * 24)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->chi, n * 8)) != hipSuccess) return e;
@@ -137,7 +140,8 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
   if (!b->ntau || !b->tau || !b->data || !b->weight || !b->qnodes || !b->bounds || !theta || !chi2 || !iters)
     return ffail(f, DDM_EINVAL, "fit: null array");
   const size_t n = (size_t)b->nfits;
-  std::vector<float> tau(n * kMaxT, 0.f), d(n * kMaxT, 0.f), w(n), qn(n * b->knodes), bd(n * 4), sd(n * 6, 0.f);
+  std::vector<float> tau(n * kMaxT, 0.f), d(n * kMaxT, 0.f), w(n), qn(n * b->knodes), bd(n * 4), sd(n * 6, 0.f),
+      vm(b->vmag ? n : 0);
   std::vector<int> nt(n);
   for (size_t i = 0; i < n; i++) {
     const int m = b->ntau[i];
@@ -154,6 +158,10 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
       return ffail(f, DDM_EINVAL, "fit: bounds must satisfy 0 < vmin < vmax, 0 < Dmin < Dmax");
     if (seeded)
       for (int k = 0; k < 6; k++) sd[i * 6 + k] = (float)b->seeds[i * 6 + k];
+    if (b->vmag) {
+      if (!(b->vmag[i] >= 0 && std::isfinite(b->vmag[i]))) return ffail(f, DDM_EINVAL, "fit: vmag must be >= 0");
+      vm[i] = (float)b->vmag[i];
+    }
   }
   hipError_t e = grow(f, n);
   if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: allocation: ") + hipGetErrorString(e));
@@ -163,13 +171,13 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
   if ((e = up(f->tau, tau.data(), tau.size() * 4)) != hipSuccess || (e = up(f->d, d.data(), d.size() * 4)) != hipSuccess ||
       (e = up(f->w, w.data(), n * 4)) != hipSuccess || (e = up(f->qn, qn.data(), qn.size() * 4)) != hipSuccess ||
       (e = up(f->bounds, bd.data(), n * 16)) != hipSuccess || (e = up(f->seeds, sd.data(), n * 24)) != hipSuccess ||
-      (e = up(f->ntau, nt.data(), n * 4)) != hipSuccess)
+      (e = up(f->ntau, nt.data(), n * 4)) != hipSuccess || (b->vmag && (e = up(f->vmag, vm.data(), n * 4)) != hipSuccess))
     return ffail(f, DDM_EHIP, std::string("fit: upload: ") + hipGetErrorString(e));
   if (b->model == DDM_MODEL_SWIMMER_2D && !f->t2v && (e = upload_isf2d(f)) != hipSuccess)
     return ffail(f, DDM_EHIP, std::string("fit: in-plane ISF table: ") + hipGetErrorString(e));
   hipEventRecord(f->ev0, f->stream);
-  e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds, b->nfits, b->model,
-                         &f->t2d, b->seed_mode,
+  e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds,
+                         b->vmag ? f->vmag : nullptr, b->nfits, b->model, &f->t2d, b->seed_mode,
                          std::max(1, std::min(3, b->nstarts)), std::max(0, b->max_iter), f->theta, f->chi, f->steps,
                          f->stream);
   if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: launch: ") + hipGetErrorString(e));

@@ -52,10 +52,12 @@ struct Isf2dDev {
   double lz0, dlz;
 };
 
-// model: 0 Brownian, 1 3D swimmer, 2 in-plane swimmer (t2d required)
+// model: 0 Brownian, 1 3D swimmer, 2 in-plane swimmer (t2d required); vmag:
+// per-fit uniform drift speed or null
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
-                         const float* bounds, const float* seeds, int nfits, int model, const Isf2dDev* t2d,
-                         int seed_mode, int nstarts, int maxit, float* theta, double* chi, int* steps, hipStream_t st);
+                         const float* bounds, const float* seeds, const float* vmag, int nfits, int model,
+                         const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
+                         int* steps, hipStream_t st);
 
 hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
                             float* C, long ldc, int M, int N, int K, int acc, hipStream_t st);

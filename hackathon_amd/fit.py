@@ -28,7 +28,7 @@ class FitBatch(C.Structure):
     _fields_ = [("nfits", C.c_int32), ("ntau_max", C.c_int32), ("knodes", C.c_int32), ("model", C.c_int32),
                 ("seed_mode", C.c_int32), ("nstarts", C.c_int32), ("max_iter", C.c_int32),
                 ("ntau", C.c_void_p), ("tau", C.c_void_p), ("data", C.c_void_p), ("weight", C.c_void_p),
-                ("qnodes", C.c_void_p), ("bounds", C.c_void_p), ("seeds", C.c_void_p)]
+                ("qnodes", C.c_void_p), ("bounds", C.c_void_p), ("seeds", C.c_void_p), ("vmag", C.c_void_p)]
 
 
 def _declare(L):
@@ -157,10 +157,11 @@ class Fitter:
             pass
 
     def fit(self, tau, data, weight, qnodes, bnds, model=SWIMMER_3D, seeds=None, nstarts=3,
-            max_iter=100, seed_ab=True) -> FitResult:
+            max_iter=100, seed_ab=True, vmag=None) -> FitResult:
         """tau/data: lists (or 2D arrays) of per-fit lag times and D values; weight (nfits,);
         qnodes (nfits, K); bnds (nfits, 4); seeds (nfits, 6) for seeded LM, else grid seeding.
-        seed_ab=False takes only the nonlinear parameters from the seeds and solves A, B by NNLS."""
+        seed_ab=False takes only the nonlinear parameters from the seeds and solves A, B by NNLS.
+        vmag: a fixed uniform drift speed per fit (scalar or (nfits,)), E = exp(-D q^2 tau) J0(q vmag tau)."""
         n = len(data)
         ntau = np.array([len(x) for x in data], dtype=np.int32)
         if n == 0:
@@ -177,10 +178,11 @@ class Fitter:
         qn = np.ascontiguousarray(qnodes, dtype=np.float64).reshape(n, -1)
         bd = np.ascontiguousarray(bnds, dtype=np.float64).reshape(n, 4)
         sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float64).reshape(n, 6)
+        vm = None if vmag is None else np.ascontiguousarray(np.broadcast_to(np.asarray(vmag, dtype=np.float64), (n,)))
         mode = SEED_GRID if sd is None else (SEED_GIVEN if seed_ab else SEED_NONLINEAR)
         b = FitBatch(n, tmax, qn.shape[1], model, mode, nstarts, max_iter,
                      ntau.ctypes.data, T.ctypes.data, Dd.ctypes.data, w.ctypes.data, qn.ctypes.data, bd.ctypes.data,
-                     None if sd is None else sd.ctypes.data)
+                     None if sd is None else sd.ctypes.data, None if vm is None else vm.ctypes.data)
         th = np.zeros((n, 6))
         chi = np.zeros(n)
         it = np.zeros(n, dtype=np.int32)
@@ -203,9 +205,11 @@ def problems_from(pr: Prepared):
 
 
 def analyzeMotilityLM(radial: Radial, dxUm: float, fps: float, model=SWIMMER_3D, fitter: Fitter | None = None,
-                      seeds=None, nstarts=3, max_iter=100):
-    """Per-q LM fits of every usable bin of one Radial record -> (Prepared, FitResult)."""
+                      seeds=None, nstarts=3, max_iter=100, vmag=None):
+    """Per-q LM fits of every usable bin of one Radial record -> (Prepared, FitResult).  vmag: a uniform
+    drift speed (um/s, e.g. estimate_drift's mag) held fixed in every bin's model."""
     pr = prepare(radial, dxUm, fps)
     f = fitter or Fitter()
-    res = f.fit(*problems_from(pr), model=model, seeds=seeds, nstarts=nstarts, max_iter=max_iter)
+    res = f.fit(*problems_from(pr), model=model, seeds=seeds, nstarts=nstarts, max_iter=max_iter,
+                vmag=None if not vmag else vmag)
     return pr, res

@@ -16,6 +16,8 @@ export declare function analyzeMotilityLM(radial: Radial, opt?: {
   dxUm?: number; fps?: number; model?: number; nstarts?: number; maxIter?: number;
   /** 6 per usable bin: A, B, alpha, ln vbar, ln Z, ln D */
   seeds?: Float64Array; nnlsAmplitudes?: boolean;
+  /** uniform drift speed (um/s) held fixed in every bin's model: E = exp(-D q^2 tau) J0(q vmag tau) */
+  vmag?: number;
 }): { pr: Prepared; fits: BinFit[]; kernelMs: number };
 
 export declare const GATES: Readonly<{ FLOW_SIGMA: number; FLOW_MIN_UM_S: number; DRIFT_ISF_MAX: number }>;

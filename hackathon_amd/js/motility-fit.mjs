@@ -72,7 +72,8 @@ let fitter = null;
 
 /**
  * Per-q LM fits of every usable bin of one Radial record.
- * opt: {dxUm, fps, model (MODEL.*), nstarts, maxIter, seeds (Float64Array 6 per usable bin), nnlsAmplitudes}
+ * opt: {dxUm, fps, model (MODEL.*), nstarts, maxIter, seeds (Float64Array 6 per usable bin), nnlsAmplitudes,
+ *      vmag (a uniform drift speed in um/s, e.g. estimateDrift's mag: E = exp(-D q^2 tau) J0(q vmag tau))}
  * -> {pr, fits: [{b, q, A, B, alpha, vbar, Z, D, chi2, iters}], kernelMs}
  */
 export function analyzeMotilityLM(radial, opt = {}) {
@@ -89,6 +90,7 @@ export function analyzeMotilityLM(radial, opt = {}) {
     ntau: new Int32Array(n).fill(ntauMax), tau: new Float64Array(n * ntauMax), data: new Float64Array(n * ntauMax),
     weight: new Float64Array(n), qnodes: new Float64Array(n * K), bounds: new Float64Array(4 * n),
     seeds: opt.seeds || null,
+    vmag: opt.vmag > 0 ? new Float64Array(n).fill(opt.vmag) : null,
   };
   pr.bins.forEach((bin, i) => {
     batch.tau.set(pr.tau, i * ntauMax);

@@ -377,7 +377,7 @@ static int get_typed(napi_env env, napi_value obj, const char *name, napi_typeda
 }
 
 /* fit(fitter, {nfits, ntauMax, knodes, model, seedMode, nstarts, maxIter, ntau: Int32Array,
- *              tau, data, weight, qnodes, bounds: Float64Array, seeds?: Float64Array})
+ *              tau, data, weight, qnodes, bounds: Float64Array, seeds?, vmag?: Float64Array})
  *   -> {theta: Float64Array(6 nfits), chi2: Float64Array, iters: Int32Array, kernelMs} */
 static napi_value js_fit(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -398,21 +398,22 @@ static napi_value js_fit(napi_env env, napi_callback_info info) {
   b.seed_mode = (int32_t)get_prop_num(env, argv[1], "seedMode", DDM_SEED_GRID);
   b.nstarts = (int32_t)get_prop_num(env, argv[1], "nstarts", 3);
   b.max_iter = (int32_t)get_prop_num(env, argv[1], "maxIter", 100);
-  void *ntau, *tau, *data, *w, *qn, *bd, *sd;
-  size_t n0, n1, n2, n3, n4, n5, n6;
+  void *ntau, *tau, *data, *w, *qn, *bd, *sd, *vm;
+  size_t n0, n1, n2, n3, n4, n5, n6, n7;
   if (!get_typed(env, argv[1], "ntau", napi_int32_array, &ntau, &n0, 1) ||
       !get_typed(env, argv[1], "tau", napi_float64_array, &tau, &n1, 1) ||
       !get_typed(env, argv[1], "data", napi_float64_array, &data, &n2, 1) ||
       !get_typed(env, argv[1], "weight", napi_float64_array, &w, &n3, 1) ||
       !get_typed(env, argv[1], "qnodes", napi_float64_array, &qn, &n4, 1) ||
       !get_typed(env, argv[1], "bounds", napi_float64_array, &bd, &n5, 1) ||
-      !get_typed(env, argv[1], "seeds", napi_float64_array, &sd, &n6, 0)) {
+      !get_typed(env, argv[1], "seeds", napi_float64_array, &sd, &n6, 0) ||
+      !get_typed(env, argv[1], "vmag", napi_float64_array, &vm, &n7, 0)) {
     napi_throw_type_error(env, NULL, "fit: batch arrays must be typed (Int32Array ntau, Float64Array others)");
     return NULL;
   }
   size_t n = (size_t)(b.nfits > 0 ? b.nfits : 0);
   if (n0 < n || n1 < n * b.ntau_max || n2 < n * b.ntau_max || n3 < n || n4 < n * b.knodes || n5 < 4 * n ||
-      (sd && n6 < 6 * n)) {
+      (sd && n6 < 6 * n) || (vm && n7 < n)) {
     napi_throw_range_error(env, NULL, "fit: array shorter than nfits x its row size");
     return NULL;
   }
@@ -423,6 +424,7 @@ static napi_value js_fit(napi_env env, napi_callback_info info) {
   b.qnodes = (const double *)qn;
   b.bounds = (const double *)bd;
   b.seeds = (const double *)sd;
+  b.vmag = (const double *)vm;
   void *th, *chi, *it;
   napi_value vth = make_typed(env, napi_float64_array, 6 * (n ? n : 1), 8, &th);
   napi_value vchi = make_typed(env, napi_float64_array, n ? n : 1, 8, &chi);

@@ -200,6 +200,9 @@ typedef struct ddm_fit_batch {
   const double *qnodes;  /* [nfits * knodes] effective |q| nodes (rad/um) */
   const double *bounds;  /* [nfits * 4] vmin, vmax, Dmin, Dmax */
   const double *seeds;   /* [nfits * 6] theta seeds (DDM_SEED_GIVEN / _NONLINEAR), else NULL */
+  const double *vmag;    /* [nfits] a fixed uniform drift speed |V| >= 0 (um/s) per fit, or NULL for none:
+                            E = exp(-D q^2 tau) J0(q |V| tau) in the column, as modelColumn takes the
+                            drift estimate (src/motility_fit.js:417-432, J0 :54-68) */
 } ddm_fit_batch;
 
 int ddm_fitter_create(ddm_fitter **out);

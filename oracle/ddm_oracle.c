@@ -28,7 +28,8 @@
  *                               values: parity unpinned (motility_fit.js does not parse
  *                               on this Node, so it cannot be imported)
  *   orc_nnls3                   KKT conditions (t_kernels.mjs:107-134)
- *   orc_lm_fit                  noise-free recovery, closed-form Brownian data
+ *   orc_lm_fit                  noise-free recovery, closed-form Brownian data; with a
+ *                               drift speed, recovery on orc_model_column data
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off; contraction would change
  * V8-hypot bin edges).
@@ -1170,12 +1171,17 @@ static double one_minus_isf(double x, double Z, int geometry) {
 /* model column and analytic/numeric derivatives; dc[k][j] for k over the
  * nonlinear parameters (alpha, ln vbar, ln Z, ln D) */
 static void lm_column(int ntau, const double *tau, int K, const double *qn, double alpha, double vbar, double Z,
-                      double Dd, int swim, int geometry, double *c, double (*dc)[128]) {
+                      double Dd, int swim, int geometry, double vmag, double *c, double (*dc)[128]) {
   for (int j = 0; j < ntau; j++) {
     double t = tau[j], cj = 0, da = 0, dlv = 0, dlz = 0, dld = 0;
     for (int i = 0; i < K; i++) {
       double q = qn[i], s = Dd * q * q * t;
       double E = exp(-s), omE = -expm1(-s);
+      if (vmag > 0) {   /* uniform drift: E = exp(-D q^2 tau) J0(q |V| tau) (modelColumn :417-432) */
+        double J = orc_bessel_j0(q * vmag * t);
+        omE += E * (1 - J);
+        E *= J;
+      }
       double x = q * vbar * t;
       double omg = swim ? one_minus_isf(x, Z, geometry) : 0;
       cj += omE + E * alpha * omg;
@@ -1230,8 +1236,8 @@ static int cholesky_solve(int p, double *H, const double *g, double *x) {
 /* theta layout (6): A, B, alpha, ln vbar, ln Z, ln D.  Brownian uses A, B, ln D
  * (alpha = 0; vbar, Z unused).  lo/hi: box (6). */
 static double lm_chi2(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
-                      const double *th, int swim, int geometry, double *c) {
-  lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, c, NULL);
+                      const double *th, int swim, int geometry, double vmag, double *c) {
+  lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, vmag, c, NULL);
   double s = 0;
   for (int j = 0; j < ntau; j++) {
     double r = w * (d[j] - (th[0] * c[j] + th[1]));
@@ -1261,13 +1267,13 @@ static double nnls_ab(int ntau, const double *c, const double *d, double w, doub
  * log-D grid, then (swimmer) 16 vbar x Z{1.5, 8} x alpha{0.3, 0.8} at that D;
  * A, B by NNLS at every grid point. */
 void orc_lm_seed(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
-                 const double *bounds4, int swim, int geometry, double *th) {
+                 const double *bounds4, int swim, int geometry, double vmag, double *th) {
   double c[128];
   double lo = log(bounds4[2]), hi = log(bounds4[3]);
   double best = INFINITY;
   for (int i = 0; i <= 48; i++) {
     double lD = lo + ((hi - lo) * i) / 48, A, B;
-    lm_column(ntau, tau, K, qn, 0, 1, 1, exp(lD), 0, geometry, c, NULL);
+    lm_column(ntau, tau, K, qn, 0, 1, 1, exp(lD), 0, geometry, vmag, c, NULL);
     double r = nnls_ab(ntau, c, d, w, &A, &B);
     if (r < best) {
       best = r;
@@ -1288,7 +1294,7 @@ void orc_lm_seed(int ntau, const double *tau, const double *d, double w, int K, 
     for (int z = 0; z < 2; z++)
       for (int q = 0; q < 2; q++) {
         double A, B;
-        lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(lD), 1, geometry, c, NULL);
+        lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(lD), 1, geometry, vmag, c, NULL);
         double r = nnls_ab(ntau, c, d, w, &A, &B);
         if (r < best) {
           best = r;
@@ -1307,7 +1313,7 @@ void orc_lm_seed(int ntau, const double *tau, const double *d, double w, int K, 
  * th: in = seed, out = fit.  Marquardt scaling D = diag(F^T F), Nielsen's
  * gain-ratio damping update.  Returns chi^2; *iters = LM steps taken. */
 double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
-                  const double *bounds4, int swim, int geometry, int maxit, double *th, int *iters) {
+                  const double *bounds4, int swim, int geometry, double vmag, int maxit, double *th, int *iters) {
   double lo[6] = {0, 0, 0, log(bounds4[0]), log(0.5), log(bounds4[2])};
   double hi[6] = {INFINITY, INFINITY, 1, log(bounds4[1]), log(100.0), log(bounds4[3])};
   int idx[6] = {0, 1, 2, 3, 4, 5}, p = 6;
@@ -1317,11 +1323,11 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
   }
   for (int k = 0; k < 6; k++) th[k] = clampd(th[k], lo[k], hi[k]);
   double c[128], dc[4][128], cn[128];
-  double chi = lm_chi2(ntau, tau, d, w, K, qn, th, swim, geometry, c);
+  double chi = lm_chi2(ntau, tau, d, w, K, qn, th, swim, geometry, vmag, c);
   double lam = 1e-3, nu = 2;
   int it = 0, done = 0;
   while (it < maxit && !done) {
-    lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, c, dc);
+    lm_column(ntau, tau, K, qn, swim ? th[2] : 0, exp(th[3]), exp(th[4]), exp(th[5]), swim, geometry, vmag, c, dc);
     double H[36] = {0}, g[6] = {0};
     for (int j = 0; j < ntau; j++) {
       double F[6];
@@ -1374,7 +1380,7 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
           double dk = ntheta[k] - th[k];
           pred += dk * (lam * diag[a] * dk + gf[a]);
         }
-        double nchi = lm_chi2(ntau, tau, d, w, K, qn, ntheta, swim, geometry, cn);
+        double nchi = lm_chi2(ntau, tau, d, w, K, qn, ntheta, swim, geometry, vmag, cn);
         double rho = pred > 0 ? (chi - nchi) / pred : -1;
         if (nchi < chi && rho > 0) {
           double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
@@ -1405,13 +1411,13 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
  * chi^2 at NNLS amplitudes, LM runs from the best `nstarts`, the lowest final
  * chi^2 wins.  out6 = theta; returns chi^2; *iters = total LM steps. */
 double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
-                    const double *bounds4, int swim, int geometry, int nstarts, int maxit, double *out6,
-                    int *iters) {
+                    const double *bounds4, int swim, int geometry, double vmag, int nstarts, int maxit,
+                    double *out6, int *iters) {
   double c[128];
   double starts[64][6], sf[64];
   int ns = 0;
   double th0[6];
-  orc_lm_seed(ntau, tau, d, w, K, qn, bounds4, 0, geometry, th0);   /* Brownian seed: D, A, B */
+  orc_lm_seed(ntau, tau, d, w, K, qn, bounds4, 0, geometry, vmag, th0);   /* Brownian seed: D, A, B */
   if (!swim) {
     starts[0][0] = th0[0];
     for (int k = 0; k < 6; k++) starts[0][k] = th0[k];
@@ -1424,7 +1430,7 @@ double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int 
       for (int z = 0; z < 2; z++)
         for (int q = 0; q < 2; q++) {
           double A, B;
-          lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(th0[5]), 1, geometry, c, NULL);
+          lm_column(ntau, tau, K, qn, as[q], exp(lv), exp(lZs[z]), exp(th0[5]), 1, geometry, vmag, c, NULL);
           sf[ns] = nnls_ab(ntau, c, d, w, &A, &B);
           double s[6] = {A, B, as[q], lv, lZs[z], th0[5]};
           memcpy(starts[ns], s, sizeof(s));
@@ -1448,7 +1454,7 @@ double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int 
     double th[6];
     int it;
     memcpy(th, starts[idx[s]], sizeof(th));
-    double chi = orc_lm_fit(ntau, tau, d, w, K, qn, bounds4, swim, geometry, maxit, th, &it);
+    double chi = orc_lm_fit(ntau, tau, d, w, K, qn, bounds4, swim, geometry, vmag, maxit, th, &it);
     tot += it;
     if (chi < best) {
       best = chi;

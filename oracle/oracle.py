@@ -302,9 +302,9 @@ def _fit_decls():
     L.orc_fit_swimmer.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
                                   _dp]
     L.orc_fit_swimmer.restype = C.c_double
-    L.orc_lm_seed.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
-    L.orc_lm_fit.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
-                             C.POINTER(C.c_int)]
+    L.orc_lm_seed.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp]
+    L.orc_lm_fit.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                             _dp, C.POINTER(C.c_int)]
     L.orc_lm_fit.restype = C.c_double
     L._fit_declared = True
     return L
@@ -371,41 +371,42 @@ def fit_swimmer(pr, geometry="3d", Vmag=0.0, Dstart=0.0):
 GEOMETRY = {"3d": 0, "2d-exact": 1, "2d": 2}
 
 
-def lm_seed(tau, d, w, qn, bounds4, swim=True, geometry="3d"):
+def lm_seed(tau, d, w, qn, bounds4, swim=True, geometry="3d", vmag=0.0):
     th = np.zeros(6)
     _fit_decls().orc_lm_seed(len(tau), np.ascontiguousarray(tau, dtype=np.float64),
                              np.ascontiguousarray(d, dtype=np.float64), w, len(qn),
                              np.ascontiguousarray(qn, dtype=np.float64), np.ascontiguousarray(bounds4, dtype=np.float64),
-                             int(swim), GEOMETRY[geometry], th)
+                             int(swim), GEOMETRY[geometry], float(vmag), th)
     return th
 
 
-def lm_fit(tau, d, w, qn, bounds4, seed, swim=True, geometry="3d", maxit=200):
-    """Per-q LM (FP64) from theta = (A, B, alpha, ln vbar, ln Z, ln D) -> (theta, chi2, iterations)."""
+def lm_fit(tau, d, w, qn, bounds4, seed, swim=True, geometry="3d", maxit=200, vmag=0.0):
+    """Per-q LM (FP64) from theta = (A, B, alpha, ln vbar, ln Z, ln D) -> (theta, chi2, iterations).
+    vmag: a fixed uniform drift speed (E = exp(-D q^2 tau) J0(q vmag tau))."""
     th = np.ascontiguousarray(seed, dtype=np.float64).copy()
     it = C.c_int()
     chi = _fit_decls().orc_lm_fit(len(tau), np.ascontiguousarray(tau, dtype=np.float64),
                                   np.ascontiguousarray(d, dtype=np.float64), w, len(qn),
                                   np.ascontiguousarray(qn, dtype=np.float64),
-                                  np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], maxit,
-                                  th, C.byref(it))
+                                  np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry],
+                                  float(vmag), maxit, th, C.byref(it))
     return th, chi, it.value
 
 
-def lm_plain(tau, d, w, qn, bounds4, swim=True, geometry="3d", nstarts=3, maxit=100):
+def lm_plain(tau, d, w, qn, bounds4, swim=True, geometry="3d", nstarts=3, maxit=100, vmag=0.0):
     """Grid-seeded per-q LM from the best `nstarts` grid starts -> (theta, chi2, total iterations)."""
     L = _fit_decls()
     if not hasattr(L, "_plain"):
-        L.orc_lm_plain.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
-                                   C.c_int, _dp, C.POINTER(C.c_int)]
+        L.orc_lm_plain.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                   C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
         L.orc_lm_plain.restype = C.c_double
         L._plain = True
     th = np.zeros(6)
     it = C.c_int()
     chi = L.orc_lm_plain(len(tau), np.ascontiguousarray(tau, dtype=np.float64), np.ascontiguousarray(d, dtype=np.float64),
                          w, len(qn), np.ascontiguousarray(qn, dtype=np.float64),
-                         np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], nstarts, maxit,
-                         th, C.byref(it))
+                         np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], float(vmag),
+                         nstarts, maxit, th, C.byref(it))
     return th, chi, it.value
 
 

@@ -20,8 +20,12 @@ async function run(dir, N, T, L, NB, kmin, kmax, fps) {
   const th = new Float64Array(res.fits.length * 3);
   res.fits.forEach((f, i) => { th[3 * i] = f.A; th[3 * i + 1] = f.B; th[3 * i + 2] = f.D; });
   fs.writeFileSync(path.join(dir, 'fit.f64'), Buffer.from(th.buffer));
+  // the same fits with a fixed drift speed in every bin's model
+  const resV = analyzeMotilityLM(radial, { dxUm: 1, fps, model: MODEL.BROWNIAN, vmag: 2.5 });
+  fs.writeFileSync(path.join(dir, 'fitV.f64'), Buffer.from(Float64Array.from(resV.fits.map((f) => f.D)).buffer));
   fs.writeFileSync(path.join(dir, 'radD.f64'), Buffer.from(radial.D.buffer));
-  return { bins: res.fits.map((f) => f.b), chi2: res.fits.map((f) => f.chi2), lags: lagGrid(L),
+  return { bins: res.fits.map((f) => f.b), chi2: res.fits.map((f) => f.chi2), chi2V: resV.fits.map((f) => f.chi2),
+    lags: lagGrid(L),
     nPrepared: prepare(radial, 1, fps).bins.length, kernelMs: res.kernelMs };
 }
 

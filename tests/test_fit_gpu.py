@@ -4,7 +4,7 @@ import pytest
 
 import oracle as O
 from hackathon_amd import fit as F
-from tests.test_fit_oracle import truth_radial
+from tests.test_fit_oracle import drift_problem, truth_radial
 
 pytestmark = pytest.mark.gpu
 MODEL = {"3d": F.SWIMMER_3D, "2d": F.SWIMMER_2D}
@@ -91,3 +91,51 @@ def test_gpu_seeded_from_oracle_optimum(gpu, geometry):
     chis = np.array(chis)
     assert np.all(res.chi2 <= np.maximum(chis * (1 + 1e-3), 1e-9))
     assert np.median(res.iters) < 20
+
+
+DRIFT_CASES = [(False, "3d", F.BROWNIAN), (True, "3d", F.SWIMMER_3D), (True, "2d", F.SWIMMER_2D)]
+
+
+@pytest.mark.parametrize("swim,geometry,model", DRIFT_CASES)
+def test_gpu_drift_noise_free(gpu, swim, geometry, model):
+    """A fixed drift speed per fit (E = exp(-D q^2 tau) J0(q |V| tau)): noise-free curves written by the
+    oracle's model column are recovered, one q per fit, |V| = 4 um/s."""
+    qs = [0.3, 0.5, 0.8, 1.0, 1.3, 1.6]
+    probs = [drift_problem(q, swim, geometry) for q in qs]
+    res = F.Fitter().fit([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs],
+                         np.array(qs)[:, None], np.array([p[3] for p in probs]), model=model, vmag=4.0)
+    for i, q in enumerate(qs):
+        tau, d, w, bd = probs[i]
+        th, chi, _ = O.lm_plain(tau, d, w, [q], bd, swim, geometry, vmag=4.0)
+        print(f"q {q}: GPU chi2 {res.chi2[i]:.2e} D {res.D[i]:.6f} | oracle chi2 {chi:.2e}")
+        assert res.chi2[i] < 1e-9, (q, res.chi2[i])
+        assert abs(res.D[i] / 0.3 - 1) < 1e-3
+        if swim:
+            assert abs(res.alpha[i] - 0.6) < 1e-3 and abs(res.vbar[i] / 30 - 1) < 1e-3 and abs(res.Z[i] / 4 - 1) < 1e-2
+    # the same data without the drift factor: clearly worse fits
+    res0 = F.Fitter().fit([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs],
+                          np.array(qs)[:, None], np.array([p[3] for p in probs]), model=model)
+    assert np.all(res0.chi2 > 1e3 * np.maximum(res.chi2, 1e-12))
+
+
+@pytest.mark.parametrize("swim,geometry,model", DRIFT_CASES)
+def test_gpu_drift_noisy_matches_oracle(gpu, swim, geometry, model):
+    """Noisy curves with drift: the GPU LM (given |V|) reaches the FP64 oracle LM's chi2 from the same grid."""
+    rng = np.random.default_rng(11)
+    qs = np.linspace(0.3, 1.8, 16)
+    probs = []
+    for q in qs:
+        tau, d, w, bd = drift_problem(q, swim, geometry)
+        d = d * (1 + 0.003 * rng.standard_normal(d.shape))
+        probs.append((tau, d, 1 / d.max(), bd))
+    res = F.Fitter().fit([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], qs[:, None],
+                         np.array([p[3] for p in probs]), model=model, vmag=4.0)
+    worse = []
+    for i, q in enumerate(qs):
+        tau, d, w, bd = probs[i]
+        th, chi, _ = O.lm_plain(tau, d, w, [q], bd, swim, geometry, vmag=4.0)
+        rel = (res.chi2[i] - chi) / chi
+        print(f"q {q:.2f}: GPU chi2 {res.chi2[i]:.4e} oracle {chi:.4e} rel {rel:+.1e}")
+        if rel > 1e-3:
+            worse.append((q, rel))
+    assert worse == []

@@ -62,6 +62,32 @@ def test_lm_recovers_truth_on_conditioned_bins():
     assert good >= 24
 
 
+def drift_problem(q, swim, geometry, V=4.0, truth=dict(vbar=30, Z=4, Dd=0.3, alpha=0.6), L=128, fps=30):
+    """One bin's noise-free curve with a uniform drift |V|, written by the oracle's model column
+    (modelColumn, src/motility_fit.js:417-432: E = exp(-D q^2 tau) J0(q |V| tau)); A = 1000, B = 2."""
+    tau = O.lag_grid(L) / fps
+    c = O.model_column(tau, [q], truth["vbar"], truth["Z"], truth["Dd"], V, truth["alpha"] if swim else 0.0, swim,
+                       O.GEOMETRY[geometry])
+    d = 1000 * c + 2
+    bd = np.array([max(2, 0.3 / (q * tau[-1])), 3 / (q / fps), 0.01 / (q * q * tau[-1]), 10 / (q * q / fps)])
+    return tau, d, 1 / d.max(), bd
+
+
+@pytest.mark.parametrize("swim,geometry", [(False, "3d"), (True, "3d"), (True, "2d")])
+def test_lm_with_drift_recovers_truth(swim, geometry):
+    """Given the drift speed, the per-q LM recovers the model column's parameters exactly (its own
+    column is a separate evaluation of the same formula); without it the same data cannot be fitted."""
+    for q in (0.5, 1.0, 1.6):
+        tau, d, w, bd = drift_problem(q, swim, geometry)
+        th, chi, it = O.lm_plain(tau, d, w, [q], bd, swim, geometry, vmag=4.0)
+        assert chi < 1e-20, (q, chi)
+        assert abs(np.exp(th[5]) / 0.3 - 1) < 1e-6 and abs(th[0] / 1000 - 1) < 1e-6 and abs(th[1] - 2) < 1e-5
+        if swim:
+            assert abs(th[2] - 0.6) < 1e-6 and abs(np.exp(th[3]) / 30 - 1) < 1e-6 and abs(np.exp(th[4]) / 4 - 1) < 1e-5
+        th0, chi0, _ = O.lm_plain(tau, d, w, [q], bd, swim, geometry)
+        assert chi0 > 1e6 * max(chi, 1e-30)
+
+
 def test_prepare_and_lag_grid_match_oracle():
     r = truth_radial("3d", kn=np.repeat(4 + 3 * (np.arange(32) + 0.5), 4))
     po = O.prepare(r, 1, 30)

@@ -89,6 +89,11 @@ def test_facade_frames_to_fits(gpu, tmp_path):
     assert [b.b for b in pr.bins] == out["bins"] and len(out["bins"]) == out["nPrepared"] > 0
     assert np.allclose(res.chi2, out["chi2"], rtol=1e-12, atol=0)
     assert np.allclose(res.theta[:, 0], th[:, 0], rtol=1e-12) and np.allclose(res.D, th[:, 2], rtol=1e-12)
+    # a fixed drift speed (opt.vmag) reaches the library as the Python path passes it
+    _, resV = F.analyzeMotilityLM(rad, 1.0, 30.0, model=F.BROWNIAN, vmag=2.5)
+    assert np.allclose(resV.chi2, out["chi2V"], rtol=1e-12, atol=0)
+    assert np.allclose(resV.D, np.fromfile(tmp_path / "fitV.f64"), rtol=1e-12)
+    assert not np.allclose(resV.chi2, res.chi2, rtol=1e-6)
 
 
 def test_js_shuffle_order_matches_oracle():
