@@ -222,6 +222,8 @@ struct ddm_engine {
   int chunk = 0;
   int cols_f = 16;        // frames per column workgroup (wavefront kernels)
   bool wave_spatial = false;
+  int mean_frame = -1;    // rowinfo frame of the last frame ingested (-1: none yet)
+  int mean_dtype = 0;
   hipStream_t stream = nullptr;
   std::string err;
   Bins bins;
@@ -581,6 +583,8 @@ static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframe
                             e->hist, e->g.T, slot % e->g.T, e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial cols launch");
+  e->mean_frame = nframes - 1;   // its row sums stay in rowinfo until the next ingest
+  e->mean_dtype = dtype;
   return DDM_OK;
 }
 
@@ -737,6 +741,35 @@ int ddm_read_radial(ddm_engine* e, float* D, float* P, float* drift) {
   if (D) HIPCHK(e, hipMemcpy(D, e->radD, (size_t)e->g.NB * e->g.L * 4, hipMemcpyDeviceToHost), "readRadial: D");
   if (P) HIPCHK(e, hipMemcpy(P, e->radP, (size_t)e->g.NB * 4, hipMemcpyDeviceToHost), "readRadial: P");
   if (drift) return compute_drift(e, drift);
+  return DDM_OK;
+}
+
+int ddm_read_mean(ddm_engine* e, double* mean) {
+  if (!e) return DDM_EINVAL;
+  if (!mean) return fail(e, DDM_EINVAL, "readMean: null output");
+  HIPCHK(e, hipStreamSynchronize(e->stream), "readMean: stream");
+  *mean = 0;
+  if (e->mean_frame < 0) return DDM_OK;
+  const int N = e->g.N;
+  std::vector<int32_t> ri(2 * (size_t)N);
+  HIPCHK(e, hipMemcpy(ri.data(), e->rowinfo + (size_t)e->mean_frame * N, (size_t)N * 8, hipMemcpyDeviceToHost),
+         "readMean: copy");
+  if (e->mean_dtype == DDM_F32) {
+    // per row the mean as an unevaluated (hi, lo) pair
+    double s = 0;
+    for (int y = 0; y < N; y++) {
+      float hi, lo;
+      std::memcpy(&hi, &ri[2 * y], 4);
+      std::memcpy(&lo, &ri[2 * y + 1], 4);
+      s += (double)hi + (double)lo;
+    }
+    *mean = s / N;
+  } else {
+    // exact integer row sums, scaled to luma as the spectra are
+    int64_t s = 0;
+    for (int y = 0; y < N; y++) s += ri[2 * y];
+    *mean = (double)s / ((double)N * N) / (e->mean_dtype == DDM_U8 ? 255.0 : 65535.0);
+  }
   return DDM_OK;
 }
 

@@ -36,7 +36,7 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
            "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
            "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full", "ddm_analyze_motility",
-           "ddm_verdict_last_error", "ddm_analyze_motility_batch")
+           "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean")
 
 
 class DdmError(RuntimeError):
@@ -265,6 +265,13 @@ class DdmEngine:
         drift = dr
         return Radial(self.N, self.NB, self.L, self.bins.kCenter.copy(), self.bins.kNodes.copy(),
                       self.bins.nModes.copy(), D.astype(np.float64), P.astype(np.float64), drift.astype(np.float64))
+
+    def readMean(self) -> float:
+        """readMean (src/ddm-engine.ts:410-414): mean luma of the last frame ingested (u8/255, u16/65535)."""
+        self._alive()
+        m = C.c_double()
+        self._check(self._lib.ddm_read_mean(self._h, C.byref(m)))
+        return m.value
 
     def readHistory(self) -> np.ndarray:
         """readHistory (src/ddm-engine.ts:397-400): complex64 (modes, T)."""

@@ -66,6 +66,8 @@ export declare class DdmEngine {
   computeTemporal(order: Uint32Array, mask: Float32Array, opts?: { yieldBetweenChunks?: boolean }): Promise<void>;
   readRadial(opts?: { drift?: boolean }): Promise<Radial>;
   readHistory(): Promise<Float32Array>;
+  /** mean luma of the last frame ingested (u8/255, u16/65535, f32 as given) */
+  readMean(): Promise<number>;
   /** per-mode D[m*L+tau], C[(m*L+tau)*2+{re,im}], P[m] for every mode (computed on demand) */
   readTemporalFull(): Promise<{ D: Float32Array; C: Float32Array; P: Float32Array }>;
   idle(): Promise<void>;

@@ -103,6 +103,9 @@ export class DdmEngine {
   /** readTemporalFull (src/ddm-engine.ts:403-408): per-mode {D, C, P} of every mode (opt-in, computed on demand). */
   async readTemporalFull() { return addon.readTemporalFull(this._h); }
 
+  /** readMean (src/ddm-engine.ts:410-414): mean luma of the last frame ingested (u8/255, u16/65535). */
+  async readMean() { return addon.readMean(this._h); }
+
   /** readHistory (src/ddm-engine.ts:397-400): hist[(mode*T + slot)*2 + {0,1}]. */
   async readHistory() { return addon.readHistory(this._h); }
 

@@ -300,6 +300,19 @@ static napi_value js_read_history(napi_env env, napi_callback_info info) {
   return v;
 }
 
+static napi_value js_read_mean(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  handle_t *h;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1 || !get_handle(env, argv[0], &h)) return NULL;
+  double m = 0;
+  int rc = ddm_read_mean(h->e, &m);
+  if (rc != DDM_OK) return throw_status(env, h->e, rc);
+  NAPI_CALL(env, napi_create_double(env, m, &out));
+  return out;
+}
+
 static napi_value js_idle(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -546,6 +559,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"computeTemporal", NULL, js_compute_temporal, NULL, NULL, NULL, napi_default, NULL},
       {"readRadial", NULL, js_read_radial, NULL, NULL, NULL, napi_default, NULL},
       {"readHistory", NULL, js_read_history, NULL, NULL, NULL, napi_default, NULL},
+      {"readMean", NULL, js_read_mean, NULL, NULL, NULL, napi_default, NULL},
       {"idle", NULL, js_idle, NULL, NULL, NULL, napi_default, NULL},
       {"readTemporalFull", NULL, js_read_temporal_full, NULL, NULL, NULL, napi_default, NULL},
       {"fitterCreate", NULL, js_fitter_create, NULL, NULL, NULL, napi_default, NULL},

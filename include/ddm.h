@@ -139,6 +139,13 @@ int ddm_read_radial(ddm_engine *e, float *D, float *P, float *drift);
  * modes*T*2 floats. */
 int ddm_read_history(ddm_engine *e, float *out);
 
+/* readMean (src/ddm-engine.ts:410-414): the mean luma of the last frame
+ * ingested (the reference's hi + lo of reduce_mean, src/ddm.wgsl:135-150), in
+ * the units the spectra use (u8 / 255, u16 / 65535, f32 as given), from the
+ * row sums the row transform keeps (exact integers for u8/u16); 0 before the
+ * first frame. */
+int ddm_read_mean(ddm_engine *e, double *mean);
+
 /* readTemporalFull (src/ddm-engine.ts:403-408): per-mode results for EVERY
  * mode m (binned or not) of the current history under the last
  * ddm_compute_temporal's order and mask: D[m*L + tau], C[(m*L + tau)*2 +

@@ -25,13 +25,14 @@ async function run(dir, N, T, L, NB, kmin, kmax, dtype) {
   await e.computeTemporal(order, mask);
   const r = await e.readRadial();
   const h = await e.readHistory();
+  const mean = await e.readMean();
   let err = '';
   try { await e.computeTemporal(new Uint32Array(T - 1), new Float32Array(T - 1)); } catch (x) { err = x.message; }
   e.destroy();
   fs.writeFileSync(path.join(dir, 'D.f64'), Buffer.from(r.D.buffer));
   fs.writeFileSync(path.join(dir, 'P.f64'), Buffer.from(r.P.buffer));
   fs.writeFileSync(path.join(dir, 'hist.f32'), Buffer.from(h.buffer));
-  return { ok, lenErr: err, kCenter: Array.from(r.kCenter), nModes: Array.from(r.nModes), L: r.L, NB: r.NB };
+  return { ok, lenErr: err, mean, kCenter: Array.from(r.kCenter), nModes: Array.from(r.nModes), L: r.L, NB: r.NB };
 }
 
 const a = process.argv.slice(2);

@@ -47,6 +47,7 @@ def test_facade_parity(gpu, tmp_path):
     assert np.max(np.abs(D[ok] - refD[ok]) / scale) < 1e-5
     assert np.max(np.abs(P[ok] - refP[ok]) / refP[ok]) < 1e-5
     assert out["nModes"] == [int(x) for x in bins["nModes"]]
+    assert out["mean"] == pytest.approx(float(np.mean(fr[T - 1].astype(np.float64))) / 255.0, rel=1e-15)
 
 
 def test_js_lag_grid_matches_python():

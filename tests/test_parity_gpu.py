@@ -100,6 +100,26 @@ def test_ingest_luma_single_frame(gpu):
     assert np.all(h[:, [0, 1, 3]] == 0)
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("wave", [1, 2])
+def test_read_mean(gpu, monkeypatch, dtype, wave):
+    """readMean: the last ingested frame's mean luma (the reference's reduce_mean hi + lo), in the spectra's
+    units, also when that frame is not the first of the last spatial chunk; 0 before any frame."""
+    N, T = 512, 40
+    fr = _frames(T, N, dtype, seed=21)
+    monkeypatch.setenv("DDM_CHUNK_MB", "8")   # 16-frame chunks: the last frame is frame 7 of the third
+    scale = O.SCALE[np.dtype(dtype)]
+    with DdmEngine.create(N=N, T=T, L=4, NB=4, kmin=0, kmax=256, tuning=dict(wave_spatial=wave)) as e:
+        assert e.readMean() == 0.0
+        e.ingestFrames(fr, 0)
+        m = e.readMean()
+        ref = float(np.mean(fr[-1].astype(np.float64))) * scale
+        # u8/u16: exact integer sums; f32: per-row (hi, lo) means
+        assert abs(m - ref) <= (1e-15 if dtype != np.float32 else 1e-7) * abs(ref), (m, ref)
+        e.ingestFrames(fr[3:4], 5)
+        assert abs(e.readMean() - float(np.mean(fr[3].astype(np.float64))) * scale) <= 1e-7 * abs(ref)
+
+
 # ------------------------------------------------------------------ bins
 
 @pytest.mark.parametrize("N,NB,kmin,kmax", [(256, 32, 4, 100), (128, 64, 0, 64), (256, 128, 0, 128),
