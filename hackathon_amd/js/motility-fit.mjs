@@ -13,6 +13,7 @@
 import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
+import { isMainThread, parentPort } from 'worker_threads';
 
 const here = path.dirname(fileURLToPath(import.meta.url));
 const require = createRequire(import.meta.url);
@@ -250,5 +251,23 @@ export function analyzeMotility(radial, opt = {}) {
     },
     q: Array.from(r.q.subarray(0, nb)), bins: Array.from(r.bins.subarray(0, nb)), lags, tau: lags.map((l) => l * dt),
     nData: r.nData, nEff: r.nEff, residualRho: r.residualRho,
+  });
+}
+
+// ---------------------------------------------------------------- worker entry
+// The reference runs analyzeMotility in a module Worker built from its fitting
+// module, so a fit never blocks frame delivery (src/motility_fit.js:992-1005,
+// caller src/main.ts:325-353): {id, radial, opt} in, {id, result} or
+// {id, error: stack text} out.  The same protocol in a Node worker thread
+// started on this file (new Worker(new URL('./motility-fit.mjs', ...)));
+// inert on the main thread.
+if (!isMainThread && parentPort) {
+  parentPort.on('message', (msg) => {
+    const { id, radial, opt } = msg || {};
+    try {
+      parentPort.postMessage({ id, result: analyzeMotility(radial, opt) });
+    } catch (e) {
+      parentPort.postMessage({ id, error: String((e && e.stack) || e) });
+    }
   });
 }
