@@ -1308,14 +1308,14 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * XS;
-  float2* tile = lds;                          // N x TS, reused after every wave's FFTs
+  float2* tile = lds;                          // (N/2) x TS, reused after every wave's FFTs
   const int kx = blockIdx.x;
   const int f0 = blockIdx.y * F;
   // N <= 512: the column FFT's twiddle table in LDS (twpad slots, behind the
   // tile/exchange area), read there instead of waiting on global loads; at
   // N = 1024 the extra 8.5 KB would cost a resident workgroup
   constexpr bool TWL = LOGN <= 9;
-  float2* twl = lds + (N * TS > 4 * XS ? N * TS : 4 * XS);
+  float2* twl = lds + ((N / 2) * TS > 4 * XS ? (N / 2) * TS : 4 * XS);
   if constexpr (TWL) {
     for (int i = tid; i < N; i += NT) twl[twpad(i)] = tw[i];
     __syncthreads();
@@ -1376,28 +1376,33 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
       wave_fft<LOGN, false, false, TWL>(res[i], xch, TWL ? twl : tw, lane);
     }
   }
-  __syncthreads();   // every wave is done with its exchange buffer: the tile reuses the space
+  // The corner turn in two halves of ky: a half-size tile lets four
+  // workgroups share a CU (the full N x (F+1) tile allowed two).
   const float sc = 1.0f / N;
 #pragma unroll
-  for (int i = 0; i < FW; i++) {
-    const int fl = wave * FW + i;
+  for (int h = 0; h < 2; h++) {
+    __syncthreads();   // h = 0: every wave is done with its exchange buffer; 1: the first half is stored
 #pragma unroll
-    for (int q = 0; q < P; q++) {
-      const float2 z = res[i][q];
-      tile[(lane + 64 * q) * TS + fl] = make_float2(z.x * sc, z.y * sc);
+    for (int i = 0; i < FW; i++) {
+      const int fl = wave * FW + i;
+#pragma unroll
+      for (int q = h * P / 2; q < (h + 1) * P / 2; q++) {
+        const float2 z = res[i][q];
+        tile[(lane + 64 * (q - h * P / 2)) * TS + fl] = make_float2(z.x * sc, z.y * sc);
+      }
     }
-  }
-  __syncthreads();
-  for (int e = tid; e < N * F; e += NT) {
-    const int fl = e % F, ky = e / F;
-    if (f0 + fl >= nframes) continue;
-    const int slot = (first_slot + f0 + fl) % T;
-    // non-temporal: the history is read back only by the temporal pass, after
-    // the whole window; keeping it out of the caches leaves them to the row spectra
-    const float2 z = tile[ky * TS + fl];
-    float2* dst = &hist[((long)ky * NH + kx) * T + slot];
-    __builtin_nontemporal_store(z.x, &dst->x);
-    __builtin_nontemporal_store(z.y, &dst->y);
+    __syncthreads();
+    for (int e = tid; e < (N / 2) * F; e += NT) {
+      const int fl = e % F, ky = h * (N / 2) + e / F;
+      if (f0 + fl >= nframes) continue;
+      const int slot = (first_slot + f0 + fl) % T;
+      // non-temporal: the history is read back only by the temporal pass, after
+      // the whole window; keeping it out of the caches leaves them to the row spectra
+      const float2 z = tile[(e / F) * TS + fl];
+      float2* dst = &hist[((long)ky * NH + kx) * T + slot];
+      __builtin_nontemporal_store(z.x, &dst->x);
+      __builtin_nontemporal_store(z.y, &dst->y);
+    }
   }
 }
 
@@ -1641,7 +1646,7 @@ static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, in
                                   const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                                   int first_slot, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
-  size_t tile = sizeof(float2) * N * (F + 1), xch = sizeof(float2) * 4 * xch_size(N);
+  size_t tile = sizeof(float2) * (N / 2) * (F + 1), xch = sizeof(float2) * 4 * xch_size(N);
   size_t lds = (tile > xch ? tile : xch) + (LOGN <= 9 ? sizeof(float2) * twpad(N) : 0);
   auto kern = k_cols_w<LOGN, F>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
