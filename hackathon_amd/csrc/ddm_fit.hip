@@ -485,7 +485,11 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
 // mode 0: LM from the seeds; 1: plain grid-seeded LM from the best `nstarts`
 // starts; 2: nonlinear parameters from the seeds, A and B by NNLS there.
 template <bool SWIM, int GEO>
-__global__ __launch_bounds__(FNT) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
+// Three waves per SIMD: the FP64 normal equations and Cholesky would keep
+// ~256 VGPRs live, and the kernel is latency-bound; capped at 168 VGPRs (the
+// rest spills) it is fastest (10,240 Schulz fits: 52 ms at one wave per SIMD,
+// 32 ms at two, 29 ms at three, 30 ms at four).
+__global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
                                                 float* __restrict__ theta_out, double* __restrict__ chi_out,
                                                 int* __restrict__ steps_out) {
   const int lane = threadIdx.x & 63;
