@@ -252,9 +252,9 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
   constexpr int NBL = n / (64 * R);          // butterflies per lane
   constexpr int QS = n / (64 * R);           // canonical index step between a butterfly's inputs
   constexpr bool LAST = (NSLOG + LR == LOGN);
-  // keep the scheduler from hoisting the next pass's twiddle loads and
-  // arithmetic across this pass: it would hold all of them live at once
-  __builtin_amdgcn_sched_barrier(0);
+  // No scheduling barrier here: the next pass's twiddle loads and products
+  // may be issued while this pass's exchange is in flight (measured: the
+  // 2048-point temporal transform 3.5 % faster, no VGPR growth).
   if constexpr (!LAST) wave_sync();          // the previous pass's reads of xch are done
 #pragma unroll
   for (int b = 0; b < NBL; b++) {
