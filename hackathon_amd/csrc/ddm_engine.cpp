@@ -503,10 +503,15 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
           hipSuccess ||
       (he = hipMemcpy(e->bin_items, e->bin_items_h.data(), (g.NB + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->nmodes, e->bins.nmodes.data(), g.NB * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-      (he = hipMemset(e->hist, 0, hist_len * 8)) != hipSuccess ||
-      (he = hipMemset(e->order, 0, (size_t)e->TPAD / 2 * 4)) != hipSuccess ||
-      (he = hipMemset(e->radD, 0, (size_t)g.NB * g.L * 4)) != hipSuccess ||
-      (he = hipMemset(e->radP, 0, (size_t)g.NB * 4)) != hipSuccess) {
+      // clears on the engine's own (non-blocking) stream, finished before
+      // create returns: a null-stream memset is not ordered before the
+      // engine's kernels, and a 69 GB history is still being cleared when
+      // the first frames' spectra arrive
+      (he = hipMemsetAsync(e->hist, 0, hist_len * 8, e->stream)) != hipSuccess ||
+      (he = hipMemsetAsync(e->order, 0, (size_t)e->TPAD / 2 * 4, e->stream)) != hipSuccess ||
+      (he = hipMemsetAsync(e->radD, 0, (size_t)g.NB * g.L * 4, e->stream)) != hipSuccess ||
+      (he = hipMemsetAsync(e->radP, 0, (size_t)g.NB * 4, e->stream)) != hipSuccess ||
+      (he = hipStreamSynchronize(e->stream)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: table upload");
     return bail(DDM_EHIP);
   }

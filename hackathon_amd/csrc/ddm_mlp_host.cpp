@@ -125,8 +125,12 @@ int ddm_mlp_create(int32_t max_batch, ddm_mlp** out) {
   for (int l = 0; l < NL && ok; l++)
     ok = hipMalloc((void**)&m->Z[l], B * DOUT[l] * 4) == hipSuccess &&
          hipMalloc((void**)&m->H[l], B * DOUT[l] * 2) == hipSuccess;
-  if (ok) ok = hipMemset(m->mo, 0, P * 4) == hipSuccess && hipMemset(m->vo, 0, P * 4) == hipSuccess &&
-               hipMemset(m->p, 0, P * 4) == hipSuccess && hipMemset(m->pb, 0, P * 2) == hipSuccess;
+  // on the network's own (non-blocking) stream, which a null-stream memset would not be ordered before
+  if (ok)
+    ok = hipMemsetAsync(m->mo, 0, P * 4, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->vo, 0, P * 4, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->p, 0, P * 4, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->pb, 0, P * 2, m->stream) == hipSuccess && hipStreamSynchronize(m->stream) == hipSuccess;
   if (!ok) {
     ddm_mlp_destroy(m);
     return DDM_ENOMEM;

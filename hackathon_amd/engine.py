@@ -266,6 +266,14 @@ class DdmEngine:
         return Radial(self.N, self.NB, self.L, self.bins.kCenter.copy(), self.bins.kNodes.copy(),
                       self.bins.nModes.copy(), D.astype(np.float64), P.astype(np.float64), drift.astype(np.float64))
 
+    def historyDevice(self) -> int:
+        """Device address of the spectral history (modes x T complex64, mode-major), for callers that keep
+        the data on the GPU (ddm_history_device)."""
+        self._alive()
+        p = C.c_void_p()
+        self._check(self._lib.ddm_history_device(self._h, C.byref(p)))
+        return p.value
+
     def readMean(self) -> float:
         """readMean (src/ddm-engine.ts:410-414): mean luma of the last frame ingested (u8/255, u16/65535)."""
         self._alive()
