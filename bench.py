@@ -11,8 +11,10 @@ GPU; every step is a complete pass.  Per-kernel times and the roofline come
 from serial steps on one engine (no overlap), reported beside.
 
 Prints one JSON line.  value = frame-pixels/s through to D(q, tau).
-`roofline` is for the dominant kernel, timed live with HIP events on the
-engine's stream; `cpu_baseline` is the FP64 oracle (one thread) on a bounded
+`roofline` is for the dominant kernel -- the longest launch, k_temporal_w
+(the row transform takes about as long per step, in ten short launches; every
+kernel's algorithmic GB/s is listed under pipeline.kernels_serial) -- timed
+live with HIP events on the engine's stream; `cpu_baseline` is the FP64 oracle (one thread) on a bounded
 slice of the same workload.  `fits` is the second half of the metric: per-q
 Schulz-model LM fits per second on held-out simulated videos (256x256x512,
 32 bins each), plain grid-seeded LM against surrogate-seeded LM, with the
@@ -219,7 +221,12 @@ def main():
     for name, (ms, cnt) in times.items():
         per_kernel[name] = {"ms_per_step": ms / prof_steps, "launches_per_step": cnt / prof_steps,
                             "avg_launch_ms": ms / max(cnt, 1)}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
+        if per_step_algo[name] and ms > 0:
+            gbs = per_step_algo[name] / (ms / prof_steps * 1e-3) / 1e9
+            per_kernel[name].update(algorithmic_GBs=gbs, frac=gbs / HBM_PEAK_GBS)
+    # the longest launch: per step the row transform is about as long, but its
+    # ten launches per step are a tenth of it each
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_launch_ms"])
     launches = max(per_kernel[dom]["launches_per_step"], 1)
     algo_launch = per_step_algo[dom] / launches
     achieved = algo_launch / (per_kernel[dom]["avg_launch_ms"] * 1e-3) / 1e9
