@@ -37,6 +37,44 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return __shfl(v, 0, 64);        // one lane's sum for everybody: identical branches
 }
 
+// Wave sums of 2^S values at once by recursive halving: at offset o = 32,
+// 16, ... each lane pair keeps one half of the current vector and sends the
+// other, so 2^S values cost 2^S - 1 + (6 - S) shuffles instead of 6 each.
+// Afterwards lane l holds the total of value k where bit 5 - j of l is bit
+// S - 1 - j of k (split_lane); readlane_d broadcasts it (a wave-uniform
+// result, the same in every lane).
+template <int S>
+__device__ __forceinline__ double wave_sum_split(double (&a)[1 << S], int lane) {
+#pragma unroll
+  for (int j = 0; j < S; j++) {
+    const int o = 32 >> j, h = (1 << S) >> (j + 1);
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < h; i++) {
+      const double send = up ? a[i] : a[i + h];
+      const double keep = up ? a[i + h] : a[i];
+      a[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  double v = a[0];
+#pragma unroll
+  for (int o = 32 >> S; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__host__ __device__ constexpr int split_lane(int k, int S) {
+  int l = 0;
+  for (int j = 0; j < S; j++) l |= ((k >> (S - 1 - j)) & 1) << (5 - j);
+  return l;
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // 1 - g(x; Z) for the 3D Schulz ISF and its derivatives with respect to ln x
 // and ln Z (x held fixed).  g = sin(Z atan u) / (Z u (1+u^2)^(Z/2)), u = x/(Z+1).
 struct Omg {
@@ -317,11 +355,13 @@ __device__ double grid_point(const float* tv, const float* dv, int nt, const flo
       dd += (double)dv[r] * dv[r];
     }
   }
-  g11 = wave_sum_d(g11);
-  g12 = wave_sum_d(g12);
-  h1 = wave_sum_d(h1);
-  h2 = wave_sum_d(h2);
-  dd = wave_sum_d(dd);
+  double a[8] = {g11, g12, h1, h2, dd, 0.0, 0.0, 0.0};
+  const double t = wave_sum_split<3>(a, lane);
+  g11 = readlane_d(t, split_lane(0, 3));
+  g12 = readlane_d(t, split_lane(1, 3));
+  h1 = readlane_d(t, split_lane(2, 3));
+  h2 = readlane_d(t, split_lane(3, 3));
+  dd = readlane_d(t, split_lane(4, 3));
   return nnls2(g11, g12, (double)nt, h1, h2, dd, A, B);
 }
 
@@ -371,10 +411,18 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
         }
       }
     }
+    // the p(p+1)/2 packed entries of H and the p of g in one split reduction
+    {
+      constexpr int NH = p * (p + 1) / 2, NV = NH + p, S = NV > 16 ? 5 : 4;
+      double a[1 << S];
 #pragma unroll
-    for (int k = 0; k < 21; k++) H[k] = wave_sum_d(H[k]);
+      for (int k = 0; k < (1 << S); k++) a[k] = k < NH ? H[k] : (k < NV ? g[k - NH] : 0.0);
+      const double t = wave_sum_split<S>(a, lane);
 #pragma unroll
-    for (int k = 0; k < 6; k++) g[k] = wave_sum_d(g[k]);
+      for (int k = 0; k < NH; k++) H[k] = readlane_d(t, split_lane(k, S));
+#pragma unroll
+      for (int k = 0; k < p; k++) g[k] = readlane_d(t, split_lane(NH + k, S));
+    }
     // Active set: a parameter on a bound whose gradient points out of the box
     // is held for this step (its row and column drop out of the system).
     // Marquardt scaling with a floor of 1e-12 x the largest diagonal, so a
