@@ -32,9 +32,8 @@ __host__ __device__ constexpr int rows_pairs_per_wg(int logn) {
                                                                 : (1 << (logn - 1)));
 }
 
-// float2 slots per wave of k_temporal_w: the park area (n floats of |Y|^2 +
-// n/2 floats of p) behind the sub-transform exchange buffer when the
-// transform is split, else whichever of the two is larger (they alias).
+// float2 slots per wave of k_temporal_w: the sub-transform exchange buffer,
+// then the wave's accumulators (n floats of sum |Y|^2, n/2 floats of sum p).
 __host__ __device__ constexpr int temporal_wave_lds(int logt, int lm) {
   return xch_size(1 << (logt - lm)) + (3 << logt) / 4;
 }
