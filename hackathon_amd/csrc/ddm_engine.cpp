@@ -668,7 +668,7 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
   } else if (e->nitems > 0 &&
              (he = timed(e, DDM_K_TEMPORAL, [&] {
                 const float inv_n0 = n0 > 0.5f ? 1.0f / n0 : 0.0f;
-                if (e->logt <= 11)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
+                if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
                   return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
                                                 e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->tw_f, e->S_part,
                                                 e->p_part, e->P_part, e->KT, e->G_part, e->stream);
@@ -680,7 +680,7 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
   } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
                 // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
                 // sub-transforms and stores their spectra r-major
-                const int lm = e->logt <= 11 && e->logt > 9 ? e->logt - 9 : 0;
+                const int lm = e->logt <= 12 && e->logt > 9 ? e->logt - 9 : 0;
                 return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, e->KT,
                                                 e->bin_items, e->g.NB,
                                                 e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t,

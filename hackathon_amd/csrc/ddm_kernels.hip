@@ -458,7 +458,9 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 // same adds as LDS float atomics made the kernel 3.7x slower on gfx950.)  The next round's
 // series is loaded while the current one is transformed.
 template <int LOGT, int LM>
-__global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__ hist, int T,
+// TPAD = 4096: one workgroup per CU by LDS (the per-wave accumulators), so
+// up to 512 registers per lane instead of spilling at 256
+__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const float2* __restrict__ hist, int T,
                                                       const int4* __restrict__ items,
                                                       const int* __restrict__ bin_modes,
                                                       const uint32_t* __restrict__ order,
@@ -513,7 +515,8 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   float* acc = reinterpret_cast<float*>(xch + xch_size(NF));
   for (int i = lane; i < n + n / 2; i += 64) acc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
-  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / 4 (lanes 4 j, j < KT)
+  const int lpl = KT <= 16 ? 4 : 2;   // lanes per tail lag (wave-uniform)
+  float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / lpl (lanes lpl j, j < KT)
   const int rounds = (count + 3) / 4;
   // Series loads without guards: a wave past the item's last mode re-reads
   // that mode (its contributions are dropped through `active`); samples
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     // Tail lags tau = T - KT + j (few valid pairs) directly: the FFT route's
     // error is ~ulp x sum_t |y|^2, far above sum over the N(tau) <= KT pairs
     // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer: the
-    // head is in register 0 (KT <= 16 < 64), the tail in the registers
+    // head is in register 0 (KT <= 32 < 64), the tail in the registers
     // q0..q1 (wave-uniform, at most two).  The previous mode's last exchange
     // reads are done first.
     wave_sync();
@@ -613,16 +616,17 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
     }
     wave_sync();
     {
-      // 4 lanes per lag (KT <= 16): lane 4 j + s sums pairs t = s, s + 4, ...
-      const int j = lane >> 2, s4 = lane & 3;
+      // lpl lanes per lag (4 for KT <= 16, else 2 for KT <= 32): lane lpl j + s
+      // sums pairs t = s, s + lpl, ...
+      const int j = lane / lpl, s4 = lane % lpl;
       float g = 0.f;
       if (j < KT)
-        for (int t = s4; t < KT - j; t += 4) {
+        for (int t = s4; t < KT - j; t += lpl) {
           const float2 a = xch[t], b = xch[KT + t + j];
           g += a.x * b.x + a.y * b.y;
         }
       g += __shfl_xor(g, 1, 64);
-      g += __shfl_xor(g, 2, 64);
+      if (lpl == 4) g += __shfl_xor(g, 2, 64);
       if (s4 == 0 && j < KT && active) two_sum_add(Gt, Gtc, g);
     }
     if constexpr (LM == 0) {
@@ -706,7 +710,7 @@ __global__ __launch_bounds__(NT, 2) void k_temporal_w(const float2* __restrict__
   const float pw = wave_sum(aP + cP);
   if (lane == 0) red[wave] = pw;
   // the 4 waves' tail lags meet in their (now idle) exchange buffers
-  if ((lane & 3) == 0 && (lane >> 2) < KT) reinterpret_cast<float*>(xch)[lane >> 2] = Gt + Gtc;
+  if (lane % lpl == 0 && lane / lpl < KT) reinterpret_cast<float*>(xch)[lane / lpl] = Gt + Gtc;
   __syncthreads();
   if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
   if (tid < KT) {
@@ -1531,13 +1535,13 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st) {
   // KT tail lags: four lanes each, and their samples must fit the exchange buffer
-  if (KT < 1 || KT > 16 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
+  if (KT < 1 || KT > 32 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_, LM_)                                                                                              \
   case LG_:                                                                                                         \
     return launch_temporal_w_t<LG_, LM_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, tw_f, \
                                          S_part, p_part, P_part, KT, G_part, st);
-    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2)
+    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
 #undef CASE
   }
   return hipErrorInvalidValue;
