@@ -143,8 +143,8 @@ def cpu_baseline(N, frames, nbins):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              f"(sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine runs on a single "
+              f"GPU; sharding one video over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     import torch
     from hackathon_amd import synth

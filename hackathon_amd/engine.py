@@ -2,10 +2,10 @@
 
 `DdmEngine` keeps the method names, argument meaning and error behaviour of
 the reference's `DdmEngine` (motility-ddm src/ddm-engine.ts:62-426): `create`,
-`ingestLuma`, `computeTemporal`, `readRadial`, `readHistory`, `idle`,
-`destroy`, `bins`; errors raise `DdmError` with the engine's message where the
-reference throws `Error`.  The frame-stack form `ingestFrames` is the batched
-ingest of the north-star API (a stack of u8/u16/f32 frames in one call), and
+`ingestLuma`, `computeTemporal`, `readRadial`, `readHistory`, `readMean`,
+`readTemporalFull`, `idle`, `destroy`, `bins`; errors raise `DdmError` with the
+engine's message where the reference throws `Error`.  `ingestFrames` is the
+batched form of `ingestLuma` (a stack of u8/u16/f32 frames in one call), and
 `ingestDevice` takes a stack already resident in GPU memory.
 
 All compute runs in libddm.so (HIP, gfx950).  If the library is missing this

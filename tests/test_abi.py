@@ -44,3 +44,12 @@ def test_create_rejects_bad_geometry_without_gpu_work():
         DdmEngine.create(N=64, T=8, L=9, NB=4, kmin=0, kmax=8)
     with pytest.raises(DdmError, match="kmin"):
         DdmEngine.create(N=64, T=8, L=4, NB=4, kmin=5, kmax=2)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message before touching a GPU (one-GPU engine)."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
