@@ -14,9 +14,10 @@ Pipeline (all GPU work in libddm.so):
      (w_b = 1 / max_tau D_b, the reference's weight, src/motility_fit.js:383-401).
   4. `train`: Adam on the GPU with the library's BF16 MFMA forward/backward.
   5. `Surrogate.seeds` turns a prediction into per-bin LM starts; `compare`
-     runs the plain grid-seeded LM (the fallback, and the yardstick seeded fits are accepted against) and the seeded
-     LM on the same problems: a seeded fit is accepted only if its chi^2 is at
-     most plain chi^2 (1 + 1e-6), rejected fits fall back to the plain result.
+     runs the plain grid-seeded LM (the fallback, and the yardstick the seeded
+     fits are accepted against) and the seeded LM on the same problems: a
+     seeded fit is accepted only if its chi^2 is at most plain chi^2 (1 + 1e-6),
+     rejected fits fall back to the plain result.
 
 Units: dx = 1 um per pixel and 100 frames/s, so vbar is in um/s (= px/s) and
 D in um^2/s (= 100 x px^2/frame).  `python -m hackathon_amd.surrogate train`
