@@ -16,6 +16,11 @@ struct ddm_batch {
   std::vector<std::vector<uint32_t>> order;
   std::vector<std::vector<float>> mask;
   std::vector<hipEvent_t> evs;
+  // graph mode: the pinned metadata staging the graph reads (detached from
+  // the engines) and each video's order/mask flags at capture
+  std::vector<void*> meta_owned;
+  std::vector<char> meta_identity;
+  std::vector<float> meta_n0;
   int mode = DDM_BATCH_EAGER, chains = 1;
   hipStream_t s0 = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -74,6 +79,8 @@ void release(ddm_batch* b) {
   if (b->t0) hipEventDestroy(b->t0);
   if (b->t1) hipEventDestroy(b->t1);
   if (b->s0) hipStreamDestroy(b->s0);
+  for (void* p : b->meta_owned)
+    if (p) hipHostFree(p);
 }
 }  // namespace
 
@@ -135,6 +142,19 @@ int ddm_batch_create(const ddm_batch_video* v, int32_t n, int32_t mode, int32_t 
     b->graph = g;
     if ((he = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0)) != hipSuccess)
       return (bfail(b, DDM_EHIP, std::string("batch: instantiate: ") + hipGetErrorString(he)), bail(DDM_EHIP));
+    for (ddm_engine* e : b->eng) {
+      bool id = true;
+      float n0 = 0.f;
+      ddm::engine_get_meta(e, &id, &n0);
+      b->meta_identity.push_back(id ? 1 : 0);
+      b->meta_n0.push_back(n0);
+      void* owned[3] = {nullptr, nullptr, nullptr};
+      he = ddm::engine_detach_meta(e, owned);
+      b->meta_owned.insert(b->meta_owned.end(), owned, owned + 3);
+      if (he != hipSuccess)
+        return (bfail(b, DDM_EHIP, std::string("batch: pinned metadata: ") + hipGetErrorString(he)),
+                bail(DDM_EHIP));
+    }
   }
   *out = b;
   return DDM_OK;
@@ -157,6 +177,8 @@ int ddm_batch_run(ddm_batch* b, double* gpu_ms) {
   if (hipEventRecord(b->t1, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: timing event");
   hipError_t he = hipEventSynchronize(b->t1);
   if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: run: ") + hipGetErrorString(he));
+  if (b->mode == DDM_BATCH_GRAPH)   // readers of the engines see the graph's order and mask again
+    for (size_t i = 0; i < b->eng.size(); i++) ddm::engine_set_meta(b->eng[i], b->meta_identity[i] != 0, b->meta_n0[i]);
   if (gpu_ms) {
     float ms = 0;
     hipEventElapsedTime(&ms, b->t0, b->t1);

@@ -65,6 +65,17 @@ std::vector<float> twiddles(int n) {
   return t;
 }
 
+// The same in FP64 (the bin-level inverse transforms).
+std::vector<double> twiddles64(int n) {
+  std::vector<double> t(2 * n);
+  for (int m = 0; m < n; m++) {
+    double a = -2 * kPi * m / n;
+    t[2 * m] = std::cos(a);
+    t[2 * m + 1] = std::sin(a);
+  }
+  return t;
+}
+
 // V8's two-argument Math.hypot: scale by the larger magnitude, Kahan-sum the
 // squared ratios in argument order, sqrt times the scale.  The reference bins
 // by it (src/ddm_ref.js:93,122,135); it is not correctly rounded, so a
@@ -252,11 +263,13 @@ struct ddm_engine {
   float* P_part = nullptr;
   float* radD = nullptr;
   float* radP = nullptr;
-  float* Sbin = nullptr;   // NB*TPAD summed |Y|^2 per bin (frequency order)
-  float* G_part = nullptr; // items*KT direct Re G sums of the tail lags
-  int KT = 1;              // tail lags T-KT..T-1 summed directly (ceil(T/32))
-  float* Gbin = nullptr;   // NB*L summed Re G per bin
-  float* pbin = nullptr;   // NB*T summed p per bin
+  double* Sbin = nullptr;  // NB*TPAD summed |Y|^2 per bin (frequency order)
+  float* G_part = nullptr; // items*KT direct D N(tau) sums of the tail lags
+  int KT = 1;              // tail lags T-KT..T-1 summed directly (ceil(T/64); computeTemporal may use more)
+  int KT_max = 1;          // capacity of G_part per work item
+  double* Gbin = nullptr;  // NB*L summed Re G(tau) - G(0) per bin, D N(tau) at the tail lags (FP64)
+  double2* tw_t64 = nullptr; // FP64 twiddles of the TPAD transform (bin-level inverse transforms)
+  double* pbin = nullptr;  // NB*T summed p per bin
   float* Pbin = nullptr;   // NB summed P per bin
   bool temporal_busy = false;
   // set while the engine's work is being recorded into a HIP graph (batch):
@@ -337,7 +350,7 @@ hipError_t timed(ddm_engine* e, int kind, F&& launch) {
 }
 
 void free_all(ddm_engine* e) {
-  void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_f, e->wx,
+  void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_t64, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
                   e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part,
                   e->fullD, e->fullC, e->fullP, e->drift_acc, e->kxy, e->bin_off};
@@ -386,11 +399,14 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->modes = (long)g.N * e->NH;
   e->logn = ilog2(g.N);
   e->TPAD = std::max(256, 1 << ilog2(2L * g.T));
-  // Lags whose valid-pair count is below T/64 take Re G from direct sums: the
-  // FFT route's error is ~ulp x sum_t |y|^2 whatever the lag, i.e. ~ulp x T/N(tau)
-  // relative to D(tau) there (measured at T = 1024: 3e-5 of the bin's largest D
-  // at the last lag without them, 3e-6 with the last T/64 lags summed directly)
+  // Lags with few valid pairs take Re G from direct sums: the FFT route's
+  // error is a few ulp x sum_t |y|^2 whatever the lag, i.e. ~ulp x T / N(tau)
+  // relative to D(tau) there.  KT (the default, for readTemporalFull) covers
+  // the last T/64 lags; computeTemporal extends the direct tail per mask up to
+  // KT_max lags so that every transform lag keeps N(tau) >= T/32 (measured at
+  // 1024 frames: the error at the last transform lag is ~2e-7 x T / N(tau)).
   e->KT = std::max(1, (g.T + 63) / 64);
+  e->KT_max = std::min(64, g.T);
   e->logt = ilog2(e->TPAD);
   auto bail = [&](int code) {
     g_create_error = e->err;
@@ -451,6 +467,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->rowinfo, (size_t)chunk * g.N)) != hipSuccess ||
       (he = hipMalloc(&e->stage, e->stage_bytes)) != hipSuccess || (he = dalloc(&e->win, g.N)) != hipSuccess ||
       (he = dalloc(&e->tw_n, g.N)) != hipSuccess || (he = dalloc(&e->tw_t, e->TPAD)) != hipSuccess ||
+      (he = dalloc(&e->tw_t64, e->TPAD)) != hipSuccess ||
       (he = dalloc(&e->tw_f, std::min(e->TPAD, 512))) != hipSuccess ||
       (he = dalloc(&e->wx, 4)) != hipSuccess || (he = dalloc(&e->items, nitems)) != hipSuccess ||
       (he = dalloc(&e->bin_modes, bin_modes_h.size())) != hipSuccess ||
@@ -461,7 +478,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->radD, (size_t)g.NB * g.L)) != hipSuccess || (he = dalloc(&e->radP, g.NB)) != hipSuccess ||
       (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess ||
       (he = dalloc(&e->Sbin, (size_t)g.NB * e->TPAD)) != hipSuccess ||
-      (he = dalloc(&e->G_part, nitems * (size_t)e->KT)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
+      (he = dalloc(&e->G_part, nitems * (size_t)e->KT_max)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
       (he = dalloc(&e->Pbin, g.NB)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: device allocation");
     return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
@@ -491,11 +508,13 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
     wx_h[2 * kx + 1] = (float)im;
   }
   std::vector<float> twn = twiddles(g.N), twt = twiddles(e->TPAD), twf = twiddles(std::min(e->TPAD, 512));
+  std::vector<double> twt64 = twiddles64(e->TPAD);
   std::vector<int4> items4(nitems, int4{0, 0, 0, 0});
   std::memcpy(items4.data(), items_h.data(), items_h.size() * sizeof(int));
   if ((he = hipMemcpy(e->win, w32.data(), g.N * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->tw_n, twn.data(), twn.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->tw_t, twt.data(), twt.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(e->tw_t64, twt64.data(), twt64.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->tw_f, twf.data(), twf.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->wx, wx_h.data(), 32, hipMemcpyHostToDevice)) != hipSuccess ||
       (he = hipMemcpy(e->items, items4.data(), items4.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -648,14 +667,39 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
     e->meta_order[j] = order[j];
     m01[j] = mask[j] != 0.f ? 1.f : 0.f;
   }
-  // exact valid-pair counts N(tau) (temporalMeta, src/ddm_ref.js:552-556)
-  float* np = e->meta_np;
-  for (int tau = 0; tau < L; tau++) {
+  // exact valid-pair counts N(tau) = sum_t m(t) m(t + tau) (temporalMeta,
+  // src/ddm_ref.js:552-556): T - tau when every frame is valid, else popcounts
+  // of the mask packed into 64-bit words (T/64 word operations per lag)
+  const int nw = (T + 63) / 64;
+  std::vector<uint64_t> bits(full ? 0 : nw, 0);
+  if (!full)
+    for (int j = 0; j < T; j++)
+      if (m01[j] != 0.f) bits[j >> 6] |= 1ull << (j & 63);
+  auto pairs = [&](int tau) -> int {
+    if (full) return T - tau;
+    const int ws = tau >> 6, bs = tau & 63;
     int c = 0;
-    for (int t = 0; t + tau < T; t++) c += (m01[t] != 0.f && m01[t + tau] != 0.f);
-    np[tau] = (float)c;
-  }
+    for (int w = 0; w + ws < nw; w++) {
+      uint64_t sh = bits[w + ws] >> bs;   // bit t of sh = m(64 w + t + tau)
+      if (bs && w + ws + 1 < nw) sh |= bits[w + ws + 1] << (64 - bs);
+      c += __builtin_popcountll(bits[w] & sh);
+    }
+    return c;
+  };
+  float* np = e->meta_np;
+  for (int tau = 0; tau < L; tau++) np[tau] = (float)pairs(tau);
   const float n0 = np[0];
+  // direct tail: the lags from the first one (within reach of KT_max) with
+  // N(tau) < need on, so that every transform lag keeps N(tau) >= need =
+  // max(T/32, min(32, T/4)) (with dropped frames N(tau) is not monotone: a look
+  // at the boundary lag alone would leave single-pair lags to the transform)
+  int KT = e->KT;
+  const int need = std::max((T + 31) / 32, std::min(32, T / 4));
+  for (int tau = std::max(1, T - e->KT_max); tau < T - KT; tau++)
+    if (pairs(tau) < need) {
+      KT = T - tau;
+      break;
+    }
   e->meta_valid = true;
   e->last_identity = identity;
   e->last_n0 = n0;
@@ -671,19 +715,19 @@ int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask
                 if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
                   return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
                                                 e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->tw_f, e->S_part,
-                                                e->p_part, e->P_part, e->KT, e->G_part, e->stream);
+                                                e->p_part, e->P_part, KT, e->G_part, e->stream);
                 return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
                                             identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part,
-                                            e->KT, e->G_part, e->stream);
+                                            KT, e->G_part, e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "temporal launch");
   } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
                 // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
                 // sub-transforms and stores their spectra r-major
                 const int lm = e->logt <= 12 && e->logt > 9 ? e->logt - 9 : 0;
-                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, e->KT,
+                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, KT,
                                                 e->bin_items, e->g.NB,
-                                                e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t,
+                                                e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t64,
                                                 e->Sbin, e->Gbin, e->pbin, e->Pbin, e->radD, e->radP, e->stream);
               })) != hipSuccess) {
     rc = hip_fail(e, he, "radial finalize launch");
@@ -864,4 +908,33 @@ int ddm_kernel_times(ddm_engine* e, double* ms, int32_t* count, int32_t n) {
 namespace ddm {
 hipStream_t engine_stream(ddm_engine* e) { return e->stream; }
 void engine_set_capturing(ddm_engine* e, bool on) { e->capturing = on; }
+
+// A recorded graph keeps reading the pinned order / mask / N(tau) staging it
+// was captured with: hand those buffers to the graph's owner and give the
+// engine fresh ones, so a later direct computeTemporal on the engine cannot
+// change what the graph uploads.
+hipError_t engine_detach_meta(ddm_engine* e, void** owned3) {
+  owned3[0] = e->meta_order;
+  owned3[1] = e->meta_mask;
+  owned3[2] = e->meta_np;
+  e->meta_order = nullptr;
+  e->meta_mask = nullptr;
+  e->meta_np = nullptr;
+  hipError_t he;
+  if ((he = hipHostMalloc((void**)&e->meta_order, e->g.T * 4)) != hipSuccess ||
+      (he = hipHostMalloc((void**)&e->meta_mask, e->g.T * 4)) != hipSuccess ||
+      (he = hipHostMalloc((void**)&e->meta_np, e->g.L * 4)) != hipSuccess)
+    return he;
+  return hipSuccess;
+}
+void engine_get_meta(const ddm_engine* e, bool* identity, float* n0) {
+  *identity = e->last_identity;
+  *n0 = e->last_n0;
+}
+// after a graph run: the engine's device metadata is the graph's again
+void engine_set_meta(ddm_engine* e, bool identity, float n0) {
+  e->meta_valid = true;
+  e->last_identity = identity;
+  e->last_n0 = n0;
+}
 }  // namespace ddm

@@ -131,6 +131,73 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) 
   }
 }
 
+// ----------------------------------------------------------------- FP64 Stockham
+//
+// The same autosort pass structure in FP64 for the few bin-level transforms
+// whose outputs are differenced (G(0) - G(tau)): radix 4, a last radix 2 when
+// LOGN is odd, one transform of length 2^LOGN in LDS, NT threads.  Twiddles
+// tw[m] = exp(-2 pi i m / n), m < n, come from an FP64 global table read
+// through the cache.
+__device__ __forceinline__ double2 dcadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 dcsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 dcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int LOGN, int NT, bool INV, int NSLOG = 0>
+__device__ __forceinline__ void lds_fft64(double2* buf, const double2* __restrict__ tw, int tid) {
+  if constexpr (NSLOG < LOGN) {
+    constexpr int LR = (LOGN - NSLOG) >= 2 ? 2 : 1;
+    constexpr int n = 1 << LOGN, R = 1 << LR, Ns = 1 << NSLOG;
+    constexpr int NBF = n / R;
+    constexpr int ITEMS = (NBF + NT - 1) / NT;
+    double2 a[ITEMS][R];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+      const int j = tid + it * NT;
+      if (NBF % NT == 0 || j < NBF) {
+        const int k = j & (Ns - 1);
+#pragma unroll
+        for (int r = 0; r < R; r++) a[it][r] = buf[j + r * NBF];
+        if (NSLOG > 0) {
+#pragma unroll
+          for (int r = 1; r < R; r++) {
+            double2 w = tw[(r * k) << (LOGN - NSLOG - LR)];
+            if (INV) w.y = -w.y;
+            a[it][r] = dcmul(a[it][r], w);
+          }
+        }
+        if constexpr (R == 2) {
+          const double2 t = a[it][0];
+          a[it][0] = dcadd(t, a[it][1]);
+          a[it][1] = dcsub(t, a[it][1]);
+        } else {
+          const double2 t0 = dcadd(a[it][0], a[it][2]), t1 = dcsub(a[it][0], a[it][2]);
+          const double2 t2 = dcadd(a[it][1], a[it][3]), d = dcsub(a[it][1], a[it][3]);
+          const double2 t3 = INV ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+          a[it][0] = dcadd(t0, t2);
+          a[it][2] = dcsub(t0, t2);
+          a[it][1] = dcadd(t1, t3);
+          a[it][3] = dcsub(t1, t3);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+      const int j = tid + it * NT;
+      if (NBF % NT == 0 || j < NBF) {
+        const int k = j & (Ns - 1);
+        double2* dst = buf + ((j - k) << LR) + k;
+#pragma unroll
+        for (int s = 0; s < R; s++) dst[s * Ns] = a[it][s];
+      }
+    }
+    __syncthreads();
+    lds_fft64<LOGN, NT, INV, NSLOG + LR>(buf, tw, tid);
+  }
+}
+
 // Error-free transformation accumulate: (s, c) += x with Neumaier's
 // compensation; the value is s + c.
 __device__ __forceinline__ void comp_add(float& s, float& c, float x) {

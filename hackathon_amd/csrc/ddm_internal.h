@@ -28,14 +28,16 @@ hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long
 // phase-drift sums of bins b0 .. b0 + nb - 1 over binned modes c0 .. c0 + nc - 1 (C rows of those modes)
 hipError_t launch_drift(const float2* C, int L, int c0, int nc, const int* bin_offsets, const int2* kxy, int b0,
                         int nb, double* acc, hipStream_t st);
-// KT: the temporal kernels sum Re G of the last KT lags (T-KT..T-1) directly
-// into G_part[item][KT]; lm: log2 of the r-major interleave of S_part rows
+// KT: the temporal kernels sum D N(tau) of the last KT lags (T-KT..T-1)
+// directly into G_part[item][KT]; lm: log2 of the r-major interleave of S_part rows
 // (0 = frequency order); full != 0: every frame valid (prefix-sum form of the
-// masked correlations)
+// masked correlations); tw64: FP64 twiddles of the TPAD transform; Sbin,
+// pbin: FP64 bin sums; Gbin: NB*L doubles, Re G(tau) - G(0) per bin (D N(tau)
+// at the tail lags)
 hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const float* p_part, const float* P_part,
                                const float* G_part, int KT, const int* bin_items, int NB, const uint32_t* nmodes,
                                const float* mask, int full, const float* npairs, int T, int L, float n0,
-                               const float2* tw, float* Sbin, float* Gbin, float* pbin, float* Pbin, float* radD,
+                               const double2* tw64, double* Sbin, double* Gbin, double* pbin, float* Pbin, float* radD,
                                float* radP, hipStream_t st);
 
 hipError_t launch_rows_w(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
@@ -77,5 +79,8 @@ hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp,
 // engine internals used by the batch runner (ddm_batch.cpp)
 hipStream_t engine_stream(::ddm_engine* e);
 void engine_set_capturing(::ddm_engine* e, bool on);
+hipError_t engine_detach_meta(::ddm_engine* e, void** owned3);
+void engine_get_meta(const ::ddm_engine* e, bool* identity, float* n0);
+void engine_set_meta(::ddm_engine* e, bool identity, float n0);
 
 }  // namespace ddm

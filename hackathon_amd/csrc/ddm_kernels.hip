@@ -12,9 +12,11 @@
 //              gather (order, mask), masked-mean removal, p = m|y|^2, zero pad
 //              to TPAD, forward FFT, |Y|^2; p and |Y|^2 are summed over the
 //              item's modes (compensated) and written as a per-item partial.
-//  k_finalize  one workgroup per bin: sum the bin's partials, inverse FFT of the
-//              summed |Y|^2 (= sum of the modes' Re G by linearity), direct
-//              masked sums A, B of the summed p, D = (A + B - 2 Re G)/N(tau)/n.
+//  k_bin_*     per bin: sum the items' partials, FP64 inverse FFT of the
+//              summed |Y|^2 (= sum of the modes' Re G by linearity) taken
+//              relative to its zero lag, FP64 masked sums A, B of the summed
+//              p, D = (A + B - 2 sum p - 2 (G(tau) - G(0)))/N(tau)/n; the
+//              tail lags from the temporal kernels' direct sums.
 //
 // Reference: src/ddm.wgsl (reduce_mean/fft_rows/fft_cols :115-192, temporal
 // :202-259, radial :265-322) and its FP64 twin src/ddm_ref.js:229-288,360-412.
@@ -402,13 +404,17 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
     for (int j = tid; j < n / 2; j += NT) buf[n / 2 + j] = make_float2(0.f, 0.f);
     __syncthreads();
     {
-      // 4 threads per lag (KT <= 64): thread 4 j + s sums pairs t = s, s + 4, ...
+      // tail lags in the difference form sum m m |y(t + tau) - y(t)|^2 (see
+      // k_temporal_w); 4 threads per lag (KT <= 64): thread 4 j + s sums
+      // pairs t = s, s + 4, ...
       const int j = tid >> 2, s4 = tid & 3;
       float g = 0.f;
       if (j < KT)
         for (int t = s4; t < KT - j; t += 4) {
           const float2 a = buf[t], b = buf[T - KT + j + t];
-          g += a.x * b.x + a.y * b.y;
+          const float w = (mask[t] != 0.f && mask[T - KT + j + t] != 0.f) ? 1.f : 0.f;
+          const float dx = b.x - a.x, dy = b.y - a.y;
+          g += w * (dx * dx + dy * dy);
         }
       g += __shfl_xor(g, 1, 64);
       g += __shfl_xor(g, 2, 64);
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
   float* acc = reinterpret_cast<float*>(xch + xch_size(NF));
   for (int i = lane; i < n + n / 2; i += 64) acc[i] = 0.f;
   float aP = 0.f, cP = 0.f;
-  const int lpl = KT <= 16 ? 4 : 2;   // lanes per tail lag (wave-uniform)
+  const int lpl = KT <= 16 ? 4 : (KT <= 32 ? 2 : 1);   // lanes per tail lag (wave-uniform; KT <= 64)
   float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / lpl (lanes lpl j, j < KT)
   const int rounds = (count + 3) / 4;
   // Series loads without guards: a wave past the item's last mode re-reads
@@ -596,36 +602,46 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
         y[q] = make_float2((y[q].x - mx) * mq, (y[q].y - my) * mq);
       }
     }
-    // Tail lags tau = T - KT + j (few valid pairs) directly: the FFT route's
-    // error is ~ulp x sum_t |y|^2, far above sum over the N(tau) <= KT pairs
-    // there.  y(t < KT) and y(t >= T - KT) meet in the exchange buffer: the
-    // head is in register 0 (KT <= 32 < 64), the tail in the registers
-    // q0..q1 (wave-uniform, at most two).  The previous mode's last exchange
-    // reads are done first.
+    // Tail lags tau = T - KT + j (few valid pairs) directly, in the
+    // difference form sum_t m(t) m(t + tau) |y(t + tau) - y(t)|^2: the FFT
+    // route's error is ~ulp x sum_t |y|^2, far above D N(tau) there, and
+    // products y*(t) y(t + tau) would still cancel in slow modes.  y(t < KT)
+    // and y(t >= T - KT) meet in the exchange buffer with their mask values
+    // (xm, behind the first 128 slots): the head is in register 0 (KT <= 64),
+    // the tail in the registers q0..q1 (wave-uniform, at most two).  The
+    // previous mode's last exchange reads are done first.
+    float* xm = reinterpret_cast<float*>(xch + 128);
     wave_sync();
-    if (lane < KT) xch[lane] = y[0];
+    if (lane < KT) {
+      xch[lane] = y[0];
+      xm[lane] = (mbits & 1u) ? 1.f : 0.f;
+    }
     {
       const int q0 = (T - KT) >> 6, q1 = (T - 1) >> 6;
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         if (q >= q0 && q <= q1) {
           const int t = lane + 64 * q;
-          if (t >= T - KT && t < T) xch[KT + t - (T - KT)] = y[q];
+          if (t >= T - KT && t < T) {
+            xch[KT + t - (T - KT)] = y[q];
+            xm[KT + t - (T - KT)] = ((mbits >> q) & 1u) ? 1.f : 0.f;
+          }
         }
       }
     }
     wave_sync();
     {
-      // lpl lanes per lag (4 for KT <= 16, else 2 for KT <= 32): lane lpl j + s
+      // lpl lanes per lag (4 for KT <= 16, 2 for KT <= 32, else 1): lane lpl j + s
       // sums pairs t = s, s + lpl, ...
       const int j = lane / lpl, s4 = lane % lpl;
       float g = 0.f;
       if (j < KT)
         for (int t = s4; t < KT - j; t += lpl) {
           const float2 a = xch[t], b = xch[KT + t + j];
-          g += a.x * b.x + a.y * b.y;
+          const float dx = b.x - a.x, dy = b.y - a.y;
+          g += (xm[t] * xm[KT + t + j]) * (dx * dx + dy * dy);
         }
-      g += __shfl_xor(g, 1, 64);
+      if (lpl >= 2) g += __shfl_xor(g, 1, 64);
       if (lpl == 4) g += __shfl_xor(g, 2, 64);
       if (s4 == 0 && j < KT && active) two_sum_add(Gt, Gtc, g);
     }
@@ -990,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_drift(const float2* __restrict__ C, int 
 template <int LOGT>
 __global__ __launch_bounds__(NT) void k_bin_sum(const float* __restrict__ S_part, const float* __restrict__ p_part,
                                                 const float* __restrict__ P_part, const int* __restrict__ bin_items,
-                                                int T, int lm, float* __restrict__ Sbin, float* __restrict__ pbin,
+                                                int T, int lm, double* __restrict__ Sbin, double* __restrict__ pbin,
                                                 float* __restrict__ Pbin) {
   constexpr int n = 1 << LOGT;
   extern __shared__ float2 lds[];
@@ -1026,9 +1042,9 @@ __global__ __launch_bounds__(NT) void k_bin_sum(const float* __restrict__ S_part
     for (; it < i1; it++) two_sum_add(s, c, src[(long)it * stride]);
     if ((int)blockIdx.y < fb) {
       const int k = ((idx & ((n >> lm) - 1)) << lm) | (idx >> (LOGT - lm));
-      Sbin[(long)b * n + k] = s + c;
+      Sbin[(long)b * n + k] = (double)s + (double)c;
     } else {
-      pbin[(long)b * T + idx] = s + c;
+      pbin[(long)b * T + idx] = (double)s + (double)c;
     }
   }
   if (blockIdx.y == 0) {
@@ -1039,44 +1055,64 @@ __global__ __launch_bounds__(NT) void k_bin_sum(const float* __restrict__ S_part
   }
 }
 
-// k_bin_ifft: one workgroup per bin, Re G(tau) = IFFT(sum |Y|^2)(tau) / n.
+// FP64 block sum (NT threads; every thread gets the result; red: NT/64 doubles).
+__device__ __forceinline__ double block_sum_d(double v, double* red, int tid) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; i++) r += red[i];
+  __syncthreads();
+  return r;
+}
+
+// k_bin_ifft: one workgroup per bin, in FP64.
+//   transform lags tau < T - KT: Gd(tau) = Re G(tau) - G(0) =
+//       [IFFT(S)(tau) - IFFT(S)(0)] / n = -(1/n) sum_f S(f) (1 - cos(2 pi f tau / n)),
+//       S = sum |Y|^2 over the bin's modes: a positively weighted sum of S,
+//       whose FP32 rounding stays relative to D itself instead of to 2 G(0)
+//       (slow bins at short lags have D << 2 P, where an FP32 transform of S
+//       loses the digits).  The lag kernels form
+//       D N(tau) = (A + B - 2 sum p) - 2 Gd, whose first term holds only the
+//       samples a lag leaves unpaired (FP64 sums of the bin's p);
+//   tail lags tau >= T - KT: D N(tau) itself, the items' direct sums.
 template <int LOGT>
-__global__ __launch_bounds__(NT) void k_bin_ifft(const float* __restrict__ Sbin, int L, const float2* __restrict__ tw_g,
-                                                 const float* __restrict__ G_part, const int* __restrict__ bin_items,
-                                                 int T, int KT, float* __restrict__ Gbin) {
+__global__ __launch_bounds__(NT) void k_bin_ifft(const double* __restrict__ Sbin, int L,
+                                                 const double2* __restrict__ tw64, const float* __restrict__ G_part,
+                                                 const int* __restrict__ bin_items, int T, int KT,
+                                                 double* __restrict__ Gbin) {
   constexpr int n = 1 << LOGT;
   extern __shared__ float2 lds[];
-  float2* buf = lds;
-  float2* tw = lds + n;
+  double2* buf = reinterpret_cast<double2*>(lds);
   const int tid = threadIdx.x, b = blockIdx.x;
-  for (int i = tid; i < n; i += NT) {
-    tw[i] = tw_g[i];
-    buf[i] = make_float2(Sbin[(long)b * n + i], 0.f);
-  }
+  for (int i = tid; i < n; i += NT) buf[i] = make_double2(Sbin[(long)b * n + i], 0.0);
   __syncthreads();
-  lds_fft<LOGT, 1, NT, true>(buf, tw, tid);
-  const float invn = 1.0f / n;
+  lds_fft64<LOGT, NT, true>(buf, tw64, tid);
+  const double g0 = buf[0].x, invn = 1.0 / n;
   const int i0 = bin_items[b], i1 = bin_items[b + 1];
   for (int tau = tid; tau < L; tau += NT) {
-    float g = buf[tau].x * invn;
-    if (tau >= T - KT) {   // tail lags: the direct sums of the temporal kernel
-      float s = 0.f, c = 0.f;
-      for (int it = i0; it < i1; it++) two_sum_add(s, c, G_part[(long)it * KT + tau - (T - KT)]);
-      g = s + c;
+    double v;
+    if (tau >= T - KT) {   // tail lags: D N(tau) from the temporal kernel's direct sums
+      v = 0.0;
+      for (int it = i0; it < i1; it++) v += (double)G_part[(long)it * KT + tau - (T - KT)];
+    } else {
+      v = tau == 0 ? 0.0 : (buf[tau].x - g0) * invn;
     }
-    Gbin[(long)b * L + tau] = g;
+    Gbin[(long)b * L + tau] = v;
   }
 }
 
 // k_bin_lags_full: every frame valid (m = 1 on [0, T)).  Then
-// A(tau) + B(tau) = sum_{t >= tau} p(t) + sum_{t < T - tau} p(t)
-//                 = (S(T) - S(tau)) + S(T - tau)
+// A(tau) + B(tau) - 2 sum_t p(t) = -S(tau) - (S(T) - S(T - tau))
 // with S the exclusive prefix sum of the bin's p, scanned in FP64 (one
-// workgroup per bin); D = (A + B - 2G) / N(tau) / nModes.
-__global__ __launch_bounds__(NT) void k_bin_lags_full(const float* __restrict__ Gbin, const float* __restrict__ pbin,
+// workgroup per bin); D = (A + B - 2 sum p - 2 Gd) / N(tau) / nModes, and
+// D = DN / N(tau) / nModes at the directly summed tail lags tau >= T - KT.
+__global__ __launch_bounds__(NT) void k_bin_lags_full(const double* __restrict__ Gbin, const double* __restrict__ pbin,
                                                       const float* __restrict__ Pbin,
                                                       const uint32_t* __restrict__ nmodes,
-                                                      const float* __restrict__ npairs, int T, int L, float n0,
+                                                      const float* __restrict__ npairs, int T, int L, int KT, float n0,
                                                       float* __restrict__ radD, float* __restrict__ radP) {
   extern __shared__ float2 lds[];
   double* S = reinterpret_cast<double*>(lds);   // T + 1
@@ -1086,7 +1122,7 @@ __global__ __launch_bounds__(NT) void k_bin_lags_full(const float* __restrict__ 
   const int per = (T + NT - 1) / NT;
   const int c0 = min(T, tid * per), c1 = min(T, c0 + per);
   double loc = 0;
-  for (int t = c0; t < c1; t++) loc += (double)pbin[(long)b * T + t];
+  for (int t = c0; t < c1; t++) loc += pbin[(long)b * T + t];
   // inclusive scan of the per-thread totals across the block
   double v = loc;
 #pragma unroll
@@ -1101,69 +1137,65 @@ __global__ __launch_bounds__(NT) void k_bin_lags_full(const float* __restrict__ 
   double run = base + v - loc;                   // exclusive prefix at c0
   for (int t = c0; t < c1; t++) {
     S[t] = run;
-    run += (double)pbin[(long)b * T + t];
+    run += pbin[(long)b * T + t];
   }
   if (tid == NT - 1) S[T] = run;
   __syncthreads();
   const uint32_t nm = nmodes[b];
   for (int tau = tid; tau < L; tau += NT) {
-    const double ab = (S[T] - S[tau]) + S[T - tau];
+    const double gd = Gbin[(long)b * L + tau];
+    const double dn = tau >= T - KT ? gd : (-S[tau] - (S[T] - S[T - tau])) - 2.0 * gd;
     const float np = npairs[tau];
-    const double g = Gbin[(long)b * L + tau];
     float d = 0.f;
-    if (np > 0.5f && nm > 0) d = (float)((ab - 2.0 * g) / (double)np / (double)nm);
+    if (np > 0.5f && nm > 0) d = (float)(dn / (double)np / (double)nm);
     radD[(long)b * L + tau] = d;
   }
   if (tid == 0) radP[b] = (nm > 0 && n0 > 0.5f) ? Pbin[b] / n0 / (float)nm : 0.f;
 }
 
-__global__ __launch_bounds__(NT) void k_bin_lags(const float* __restrict__ Gbin, const float* __restrict__ pbin,
+// k_bin_lags: any mask.  One workgroup per (bin, 64 lags): lane = lag, the
+// four waves split t.  A(tau) + B(tau) = sum_t m(t) p(t + tau) + p(t) m(t + tau)
+// in FP64 (m is 0/1), less 2 sum_t p(t); D = (A + B - 2 sum p - 2 Gd) /
+// N(tau) / nModes, and D = DN / N(tau) / nModes at the directly summed tail
+// lags tau >= T - KT.
+__global__ __launch_bounds__(NT) void k_bin_lags(const double* __restrict__ Gbin, const double* __restrict__ pbin,
                                                  const float* __restrict__ Pbin, const uint32_t* __restrict__ nmodes,
                                                  const float* __restrict__ mask01, const float* __restrict__ npairs,
-                                                 int T, int L, float n0, float* __restrict__ radD,
+                                                 int T, int L, int KT, float n0, float* __restrict__ radD,
                                                  float* __restrict__ radP) {
   extern __shared__ float2 lds[];
-  float* p = reinterpret_cast<float*>(lds);      // T
-  float* m = p + T;                              // T
-  float* part = m + T;                           // 4 waves x 64 lanes x 4 (sa, ca, sb, cb)
+  double* part = reinterpret_cast<double*>(lds);  // 4 waves x 64 lanes, then NT / 64 for the p sum
+  double* red = part + NT;
+  double* p = red + NT / 64;                             // T
+  float* m = reinterpret_cast<float*>(p + T);            // T
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int b = blockIdx.x;
   const int tau = blockIdx.y * 64 + lane;
+  double ps = 0.0;
   for (int t = tid; t < T; t += NT) {
-    p[t] = pbin[(long)b * T + t];
+    const double pv = pbin[(long)b * T + t];
+    p[t] = pv;
     m[t] = mask01[t];
+    ps += pv;
   }
-  __syncthreads();
-  float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
+  const double psum = block_sum_d(ps, red, tid);   // barriers inside (p, m staged)
+  double s = 0.0;
   if (tau < L) {
     const int tend = T - tau;                 // t + tau < T
     const int q = (tend + 3) / 4;
     const int t0 = wave * q, t1 = min(tend, t0 + q);
-    for (int t = t0; t < t1; t++) {
-      two_sum_add(sa, ca, m[t] * p[t + tau]);
-      two_sum_add(sb, cb, p[t] * m[t + tau]);
-    }
+    for (int t = t0; t < t1; t++) s += (double)m[t] * p[t + tau] + p[t] * (double)m[t + tau];
   }
-  float* pw = part + 4 * (wave * 64 + lane);
-  pw[0] = sa;
-  pw[1] = ca;
-  pw[2] = sb;
-  pw[3] = cb;
+  part[wave * 64 + lane] = s;
   __syncthreads();
   if (wave == 0 && tau < L) {
-    float s = 0.f, c = 0.f;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-      const float* q4 = part + 4 * (w * 64 + lane);
-      two_sum_add(s, c, q4[0]);
-      two_sum_add(s, c, q4[2]);
-      c += q4[1] + q4[3];
-    }
+    const double ab = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
     const float np = npairs[tau];
     const uint32_t nm = nmodes[b];
-    const float g = Gbin[(long)b * L + tau];
+    const double gd = Gbin[(long)b * L + tau];
+    const double dn = tau >= T - KT ? gd : (ab - 2.0 * psum) - 2.0 * gd;
     float d = 0.f;
-    if (np > 0.5f && nm > 0) d = ((s + c) - 2.0f * g) / np / (float)nm;
+    if (np > 0.5f && nm > 0) d = (float)(dn / (double)np / (double)nm);
     radD[(long)b * L + tau] = d;
   }
   if (tid == 0 && blockIdx.y == 0) {
@@ -1535,7 +1567,7 @@ hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* it
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st) {
   // KT tail lags: four lanes each, and their samples must fit the exchange buffer
-  if (KT < 1 || KT > 32 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
+  if (KT < 1 || KT > 64 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_, LM_)                                                                                              \
   case LG_:                                                                                                         \
@@ -1586,32 +1618,33 @@ hipError_t launch_drift(const float2* C, int L, int c0, int nc, const int* bin_o
 
 template <int LOGT>
 static hipError_t launch_bin_sum_ifft_t(const float* S_part, const float* p_part, const float* P_part,
-                                       const int* bin_items, int NB, int T, int L, int lm, const float2* tw,
-                                       const float* G_part, int KT, float* Sbin, float* Gbin, float* pbin,
+                                       const int* bin_items, int NB, int T, int L, int lm, const double2* tw64,
+                                       const float* G_part, int KT, double* Sbin, double* Gbin, double* pbin,
                                        float* Pbin, hipStream_t st) {
   constexpr int n = 1 << LOGT;
   hipLaunchKernelGGL(k_bin_sum<LOGT>, dim3(NB, n / NT + (T + NT - 1) / NT), dim3(NT), sizeof(float) * 2 * (NT / 64), st,
                      S_part, p_part, P_part, bin_items, T, lm, Sbin, pbin, Pbin);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t lds = sizeof(float2) * 2 * n;
+  size_t lds = sizeof(double2) * n;
   auto kern = k_bin_ifft<LOGT>;
   e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, (const float*)Sbin, L, tw, G_part, bin_items, T, KT, Gbin);
+  hipLaunchKernelGGL(kern, dim3(NB), dim3(NT), lds, st, (const double*)Sbin, L, tw64, G_part, bin_items, T, KT,
+                     Gbin);
   return hipGetLastError();
 }
 
 hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const float* p_part, const float* P_part,
                                const float* G_part, int KT, const int* bin_items, int NB, const uint32_t* nmodes,
                                const float* mask, int full, const float* npairs, int T, int L, float n0,
-                               const float2* tw, float* Sbin, float* Gbin, float* pbin, float* Pbin, float* radD,
+                               const double2* tw64, double* Sbin, double* Gbin, double* pbin, float* Pbin, float* radD,
                                float* radP, hipStream_t st) {
   hipError_t e = hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                \
   case LG_:                                                                                                    \
-    e = launch_bin_sum_ifft_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, lm, tw, G_part, KT, Sbin, Gbin,   \
+    e = launch_bin_sum_ifft_t<LG_>(S_part, p_part, P_part, bin_items, NB, T, L, lm, tw64, G_part, KT, Sbin, Gbin, \
                                    pbin, Pbin, st);                                                            \
     break;
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
@@ -1622,15 +1655,15 @@ hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const floa
     size_t lds = sizeof(double) * (T + 1 + NT / 64);
     e = hipFuncSetAttribute((const void*)k_bin_lags_full, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_lags_full, dim3(NB), dim3(NT), lds, st, (const float*)Gbin, (const float*)pbin,
-                       (const float*)Pbin, nmodes, npairs, T, L, n0, radD, radP);
+    hipLaunchKernelGGL(k_bin_lags_full, dim3(NB), dim3(NT), lds, st, (const double*)Gbin, (const double*)pbin,
+                       (const float*)Pbin, nmodes, npairs, T, L, KT, n0, radD, radP);
     return hipGetLastError();
   }
-  size_t lds = sizeof(float) * (2 * T + 4 * NT);
+  size_t lds = sizeof(double) * (NT + NT / 64 + T) + sizeof(float) * T;
   e = hipFuncSetAttribute((const void*)k_bin_lags, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bin_lags, dim3(NB, (L + 63) / 64), dim3(NT), lds, st, Gbin, pbin, Pbin, nmodes, mask, npairs,
-                     T, L, n0, radD, radP);
+  hipLaunchKernelGGL(k_bin_lags, dim3(NB, (L + 63) / 64), dim3(NT), lds, st, (const double*)Gbin, (const double*)pbin,
+                     (const float*)Pbin, nmodes, mask, npairs, T, L, KT, n0, radD, radP);
   return hipGetLastError();
 }
 

@@ -53,6 +53,28 @@ def test_batch_bit_identical_to_single_engines(gpu, mode, chains):
     b.close()
 
 
+def test_graph_batch_survives_direct_calls_on_its_engines(gpu):
+    """A direct computeTemporal (shuffled order, dropped frames) on a graph batch's engine must not leak into
+    the next graph run: the graph keeps the order, mask and N(tau) it was recorded with."""
+    vids = _videos()[:3]
+    ref = [_alone(v) for v in vids]
+    b = BA.Batch(vids, mode=BA.GRAPH, chains=2)
+    b.run()
+    e0 = b.engine(0)
+    T = vids[0]["T"]
+    rng = np.random.default_rng(3)
+    order = rng.permutation(T).astype(np.uint32)
+    mask = (rng.uniform(size=T) > 0.3).astype(np.float32)
+    e0.computeTemporal(order, mask)
+    shuffled = e0.readRadial()
+    assert not np.array_equal(shuffled.D, ref[0].D)
+    for rep in range(2):
+        b.run()
+        for i, r in enumerate(b.radials(drift=(rep == 1))):
+            assert np.array_equal(r.D, ref[i].D) and np.array_equal(r.P, ref[i].P), (rep, i)
+    b.close()
+
+
 def test_batch_fits_all_bins_in_one_launch(gpu):
     vids = _videos()[:3]
     b = BA.Batch(vids, mode=BA.GRAPH, chains=3)

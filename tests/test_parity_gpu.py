@@ -137,13 +137,22 @@ def test_engine_bins_match_oracle(gpu, N, NB, kmin, kmax):
 
 # ------------------------------------------------------------------ temporal + radial (c)
 
-def _radial_errors(gD, gP, refD, refP, nmodes):
+def _radial_errors(gD, gP, refD, refP, nmodes, mask=None):
     ok = nmodes > 0
     d = refD[ok][:, 1:]
     g = gD[ok][:, 1:]
     nz = d != 0          # lags with no valid frame pair have D = 0 on both sides
     assert np.all(g[~nz] == 0)
     rel = np.abs(g[nz] - d[nz]) / np.abs(d[nz])
+    if mask is not None and rel.size:   # where the largest error sits: bin, lag, valid pairs, D / bin max
+        r2 = np.zeros_like(d)
+        r2[nz] = rel
+        bi, li = np.unravel_index(int(np.argmax(r2)), r2.shape)
+        m = np.asarray(mask, np.float64) != 0
+        lag = li + 1
+        npairs = int(np.sum(m[:len(m) - lag] & m[lag:]))
+        print(f"   worst: bin {np.flatnonzero(ok)[bi]} lag {lag} N(tau) {npairs} "
+              f"D/maxD {d[bi, li] / np.max(np.abs(refD[ok][bi])):.3e} rel {r2[bi, li]:.2e}", flush=True)
     scale = np.max(np.abs(refD[ok]), axis=1, keepdims=True)
     harness = np.max(np.abs(gD[ok] - refD[ok]) / scale)
     prel = np.max(np.abs(gP[ok] - refP[ok]) / refP[ok])
@@ -179,18 +188,19 @@ def test_radial_parity_config1(gpu, case):
     D = r.D.reshape(NB, L)
     # (i) end to end against FP64 spectra + FP64 temporal
     refD, refP = O.analyze_radial(O.history(fr, T), order, mask, bins, L)
-    rel, harness, prel = _radial_errors(D, r.P, refD, refP, bins["nModes"])
+    rel, harness, prel = _radial_errors(D, r.P, refD, refP, bins["nModes"], mask)
     print(f"end-to-end: rel p50 {np.median(rel):.2e} p99 {np.quantile(rel, .99):.2e} max {rel.max():.2e} "
           f"harness {harness:.2e} P {prel:.2e}")
+    # every element of D within 1e-5 relative of FP64
     assert harness < 1e-5 and prel < 1e-5
-    assert np.quantile(rel, 0.99) < 1e-4
+    assert rel.max() < 1e-5, rel.max()
     # (ii) stage (c) alone: FP64 temporal on the GPU's own f32 spectra
     refD2, refP2 = O.analyze_radial(h.astype(np.complex128), order, mask, bins, L)
-    rel2, harness2, prel2 = _radial_errors(D, r.P, refD2, refP2, bins["nModes"])
+    rel2, harness2, prel2 = _radial_errors(D, r.P, refD2, refP2, bins["nModes"], mask)
     print(f"stage c: rel p50 {np.median(rel2):.2e} p99 {np.quantile(rel2, .99):.2e} max {rel2.max():.2e} "
           f"harness {harness2:.2e} P {prel2:.2e}")
     assert harness2 < 1e-5 and prel2 < 1e-6
-    assert np.quantile(rel2, 0.99) < 1e-4
+    assert rel2.max() < 1e-5, rel2.max()
     # empty bin 0 (only DC lies in [0, 1) and DC is excluded)
     assert bins["nModes"][0] == 0 and np.all(D[0] == 0) and r.P[0] == 0
 
@@ -207,9 +217,74 @@ def test_radial_parity_geometries(gpu, N, T, L, NB, kmin, kmax):
     print(f"radial N={N} T={T} L={L} NB={NB}", flush=True)
     r, h, bins = _run_radial(fr, T, L, NB, kmin, kmax, order, mask)
     refD, refP = O.analyze_radial(h.astype(np.complex128), order, mask, bins, L)
-    rel, harness, prel = _radial_errors(r.D.reshape(NB, L), r.P, refD, refP, bins["nModes"])
+    rel, harness, prel = _radial_errors(r.D.reshape(NB, L), r.P, refD, refP, bins["nModes"], mask)
+    print(f"rel max {rel.max():.2e} harness {harness:.2e} P {prel:.2e}")
     assert harness < 1e-5 and prel < 1e-6
-    assert np.quantile(rel, 0.99) < 1e-4
+    assert rel.max() < 1e-5, rel.max()
+
+
+def _ring_meta(T, case, seed):
+    """Order and mask of a live ring: the oldest slot first (rotated order), a few frames dropped and a guard
+    of the newest slots invalid; or the same mask under a seeded shuffle."""
+    rng = np.random.default_rng(seed)
+    head = int(rng.integers(1, T))
+    order = ((head + np.arange(T)) % T).astype(np.uint32)
+    mask = np.ones(T, np.float32)
+    mask[rng.uniform(size=T) < 0.05] = 0
+    mask[-3:] = 0          # guard
+    mask[0] = 1
+    if case == "shuffled":
+        order, mask = O.shuffle_order(order, mask, O.mulberry32(seed))
+    return order, mask
+
+
+def _gpu_video(T, N, seed):
+    from hackathon_amd.synth import GpuSynth
+    g = GpuSynth()
+    fr = g.video(g.params(T, N, 120, 1.5, 100.0, 20.0, D=0.4, alpha=0.5, vbar_px_frame=0.2, Z=4.0, dtype="uint8",
+                          seed=seed, heading3d=True, swim_diffuse=True))
+    return fr.cpu().numpy()
+
+
+@pytest.mark.parametrize("T", [300, 1000, 2000])
+@pytest.mark.parametrize("case", ["ring", "shuffled"])
+def test_radial_parity_ring_long(gpu, T, case):
+    """Gathered series (a non-identity order) with dropped frames and a guard at lengths that run the split
+    sub-transforms (TPAD 1024, 2048, 4096), against the FP64 oracle on the engine's own history.
+
+    Per element: 1e-5 for 99.9 % of the elements.  The exceptions are the last transform lags of slow bins
+    (few valid pairs, D << P); there the FP32 transform's error is ~1e-7 x (power of the samples a lag
+    leaves unpaired) / (D N), and with dropped frames that power stays large.  They are held to 1e-4."""
+    N, NB = 64, 32
+    fr = _gpu_video(T, N, seed=T + 1)
+    order, mask = _ring_meta(T, case, seed=T)
+    print(f"ring parity T={T} case={case}", flush=True)
+    r, h, bins = _run_radial(fr, T, T, NB, 0, N // 2, order, mask)
+    refD, refP = O.analyze_radial(h.astype(np.complex128), order, mask, bins, T, split=True)
+    rel, harness, prel = _radial_errors(r.D.reshape(NB, T), r.P, refD, refP, bins["nModes"], mask)
+    print(f"rel p99 {np.quantile(rel, .99):.2e} p99.9 {np.quantile(rel, .999):.2e} max {rel.max():.2e} "
+          f"harness {harness:.2e} P {prel:.2e}")
+    assert harness < 1e-5 and prel < 1e-6
+    assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-4, rel.max()
+
+
+@pytest.mark.parametrize("case", ["full", "shuffled"])
+def test_radial_parity_block_kernel(gpu, case):
+    """T = 2100 (TPAD 8192: the workgroup-per-mode temporal kernel), every bin and lag against the FP64 oracle
+    on the engine's own history; identity order with every frame valid, and a shuffled order with drops."""
+    N, NB, T = 64, 32, 2100
+    fr = _gpu_video(T, N, seed=77)
+    if case == "full":
+        order, mask = np.arange(T, dtype=np.uint32), np.ones(T, np.float32)
+    else:
+        order, mask = _ring_meta(T, "shuffled", seed=5)
+    print(f"block-kernel parity T={T} case={case}", flush=True)
+    r, h, bins = _run_radial(fr, T, T, NB, 0, N // 2, order, mask)
+    refD, refP = O.analyze_radial(h.astype(np.complex128), order, mask, bins, T, split=True)
+    rel, harness, prel = _radial_errors(r.D.reshape(NB, T), r.P, refD, refP, bins["nModes"], mask)
+    print(f"rel p99 {np.quantile(rel, .99):.2e} max {rel.max():.2e} harness {harness:.2e} P {prel:.2e}")
+    assert harness < 1e-5 and prel < 1e-6
+    assert rel.max() < 1e-5, rel.max()
 
 
 def test_temporal_errors(gpu):
