@@ -677,3 +677,164 @@ hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, cons
 }
 
 }  // namespace ddm
+
+// ----------------------------------------------------------------- allocation-free fits
+//
+// ddm_fit_device_noalloc (include/ddm.h): the LM above on device-resident
+// arrays with every buffer beyond the caller's inputs and outputs in one
+// fixed device array, `scratch`, so a call allocates nothing (for code under
+// HIP graph capture and serving loops that must not call hipMalloc).
+// Layout in bytes: the in-plane ISF table in [0, kNaTab) (uploaded once per
+// device at set-up), then per-fit arrays of kNaFits entries each.  Every
+// call on a GPU shares that GPU's copy: calls that overlap on one GPU race.
+#include "../../include/ddm.h"
+#include "ddm_isf2d.h"
+
+#include <mutex>
+
+namespace ddm {
+namespace {
+constexpr size_t kNaFits = DDM_FIT_NOALLOC_MAX;
+constexpr size_t kNaTabV = 98304;                 // table values (doubles)
+constexpr size_t kNaTabZ = kNaTabV * 8;           // Z, u0, du (doubles), off, n (ints): 72 rows each
+constexpr size_t kNaTab = 1 << 20;
+constexpr size_t kNaTau = kNaTab;
+constexpr size_t kNaD = kNaTau + kNaFits * MAXT * 4;
+constexpr size_t kNaW = kNaD + kNaFits * MAXT * 4;
+constexpr size_t kNaQn = kNaW + kNaFits * 4;
+constexpr size_t kNaBounds = kNaQn + kNaFits * MAXK * 4;
+constexpr size_t kNaSeeds = kNaBounds + kNaFits * 4 * 4;
+constexpr size_t kNaVmag = kNaSeeds + kNaFits * 6 * 4;
+constexpr size_t kNaNtau = kNaVmag + kNaFits * 4;
+constexpr size_t kNaValid = kNaNtau + kNaFits * 4;
+constexpr size_t kNaTheta = kNaValid + kNaFits * 4;
+constexpr size_t kNaChi = kNaTheta + kNaFits * 6 * 4;
+constexpr size_t kNaSteps = kNaChi + kNaFits * 8;
+constexpr size_t kNaBytes = kNaSteps + kNaFits * 4;
+static_assert(kNaChi % 8 == 0, "FP64 chi^2 slots");
+static_assert(kNaTabZ + Isf2dTable::nz * (3 * 8 + 2 * 4) <= kNaTab, "table region");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// per-device address of `scratch` (hipGetSymbolAddress; the array's name is a
+// host-side stand-in) and the device view of the table inside it
+constexpr int kNaMaxDev = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kNaMaxDev] = {};
+Isf2dDev g_na_t2d[kNaMaxDev];
+
+// One workgroup of MAXT threads per fit: the caller's FP64 rows -> the f32
+// rows k_lm_fit reads, with the checks ddm_fit makes on the host (a failing
+// fit gets a safe placeholder problem and valid = 0).
+__global__ __launch_bounds__(MAXT) void k_fit_pack(ddm_fit_batch b, unsigned char* __restrict__ s) {
+  const int i = blockIdx.x, j = threadIdx.x;
+  const int K = b.knodes;
+  const bool seeded = b.seed_mode != DDM_SEED_GRID;
+  const int m = b.ntau[i];
+  const bool mok = m >= 1 && m <= b.ntau_max;
+  const int mm = mok ? m : 1;
+  float* tau = reinterpret_cast<float*>(s + kNaTau) + (size_t)i * MAXT;
+  float* d = reinterpret_cast<float*>(s + kNaD) + (size_t)i * MAXT;
+  tau[j] = j < mm ? (float)b.tau[(size_t)i * b.ntau_max + j] : 0.f;
+  d[j] = j < mm ? (float)b.data[(size_t)i * b.ntau_max + j] : 0.f;
+  if (j < K) reinterpret_cast<float*>(s + kNaQn)[(size_t)i * K + j] = (float)b.qnodes[(size_t)i * K + j];
+  if (j < 6)
+    reinterpret_cast<float*>(s + kNaSeeds)[(size_t)i * 6 + j] = seeded ? (float)b.seeds[(size_t)i * 6 + j] : 0.f;
+  if (j == 0) {
+    float bd[4];
+    for (int k = 0; k < 4; k++) bd[k] = (float)b.bounds[(size_t)i * 4 + k];
+    const bool bok = bd[0] > 0 && bd[1] > bd[0] && bd[2] > 0 && bd[3] > bd[2];
+    float vm = 0.f;
+    bool vok = true;
+    if (b.vmag) {
+      const double v = b.vmag[i];
+      vok = v >= 0 && isfinite(v);
+      vm = vok ? (float)v : 0.f;
+    }
+    float* bo = reinterpret_cast<float*>(s + kNaBounds) + (size_t)i * 4;
+    for (int k = 0; k < 4; k++) bo[k] = bok ? bd[k] : (k & 1 ? 2.f : 1.f);
+    reinterpret_cast<float*>(s + kNaVmag)[i] = vm;
+    reinterpret_cast<float*>(s + kNaW)[i] = (float)b.weight[i];
+    reinterpret_cast<int*>(s + kNaNtau)[i] = mm;
+    reinterpret_cast<int*>(s + kNaValid)[i] = (mok && bok && vok) ? 1 : 0;
+  }
+}
+
+// One thread per fit: FP64 outputs, NaN / -1 for the fits that failed a check.
+__global__ __launch_bounds__(256) void k_fit_unpack(const unsigned char* __restrict__ s, int nfits,
+                                                    double* __restrict__ theta, double* __restrict__ chi2,
+                                                    int* __restrict__ iters) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nfits) return;
+  const bool ok = reinterpret_cast<const int*>(s + kNaValid)[i] != 0;
+  const float* th = reinterpret_cast<const float*>(s + kNaTheta) + (size_t)i * 6;
+  for (int k = 0; k < 6; k++) theta[(size_t)i * 6 + k] = ok ? (double)th[k] : __builtin_nan("");
+  chi2[i] = ok ? reinterpret_cast<const double*>(s + kNaChi)[i] : __builtin_nan("");
+  iters[i] = ok ? reinterpret_cast<const int*>(s + kNaSteps)[i] : -1;
+}
+}  // namespace
+
+// Set-up on the current device, once: the address of `scratch` and the
+// in-plane ISF table copied into it.
+hipError_t noalloc_setup() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kNaMaxDev) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[dev]) return hipSuccess;
+  void* p = nullptr;
+  if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+  const Isf2dTable& T = isf2d_table();
+  const size_t nz = T.Z.size();
+  if (T.v.size() > kNaTabV || nz != (size_t)Isf2dTable::nz) return hipErrorInvalidValue;
+  unsigned char* b = static_cast<unsigned char*>(p);
+  double* v = reinterpret_cast<double*>(b);
+  double* z = reinterpret_cast<double*>(b + kNaTabZ);
+  double* u0 = z + nz;
+  double* du = u0 + nz;
+  int* off = reinterpret_cast<int*>(du + nz);
+  int* n = off + nz;
+  if ((e = hipMemcpy(v, T.v.data(), T.v.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(z, T.Z.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(u0, T.u0.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(du, T.du.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(off, T.off.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(n, T.n.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return e;
+  g_na_t2d[dev] = Isf2dDev{v, z, u0, du, off, n, T.lz0, T.dlz};
+  g_na_ptr[dev] = b;
+  return hipSuccess;
+}
+
+// Enqueues pack -> LM -> unpack on `st` (a stream of device `dev`); b's arrays
+// and the outputs are device pointers, b's sizes already checked.
+hipError_t launch_fit_noalloc(int dev, const ddm_fit_batch* b, double* theta, double* chi2, int* iters,
+                              hipStream_t st) {
+  if (dev < 0 || dev >= kNaMaxDev) return hipErrorInvalidDevice;
+  unsigned char* s = nullptr;
+  Isf2dDev t2d{};
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    s = g_na_ptr[dev];
+    t2d = g_na_t2d[dev];
+  }
+  if (!s) return hipErrorNotInitialized;
+  if (b->nfits < 1 || (size_t)b->nfits > kNaFits) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fit_pack, dim3(b->nfits), dim3(MAXT), 0, st, *b, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_lm_fit(reinterpret_cast<const float*>(s + kNaTau), reinterpret_cast<const float*>(s + kNaD),
+                    reinterpret_cast<const int*>(s + kNaNtau), reinterpret_cast<const float*>(s + kNaW),
+                    reinterpret_cast<const float*>(s + kNaQn), b->knodes,
+                    reinterpret_cast<const float*>(s + kNaBounds), reinterpret_cast<const float*>(s + kNaSeeds),
+                    b->vmag ? reinterpret_cast<const float*>(s + kNaVmag) : nullptr, b->nfits, b->model, &t2d,
+                    b->seed_mode, b->nstarts < 1 ? 1 : (b->nstarts > 3 ? 3 : b->nstarts),
+                    b->max_iter < 0 ? 0 : b->max_iter, reinterpret_cast<float*>(s + kNaTheta),
+                    reinterpret_cast<double*>(s + kNaChi), reinterpret_cast<int*>(s + kNaSteps), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fit_unpack, dim3((b->nfits + 255) / 256), dim3(256), 0, st, s, b->nfits, theta, chi2, iters);
+  return hipGetLastError();
+}
+
+}  // namespace ddm

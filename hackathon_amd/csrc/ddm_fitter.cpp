@@ -14,6 +14,8 @@
 
 struct ddm_fitter {
   hipStream_t stream = nullptr;
+  int device = 0;
+  hipError_t noalloc = hipErrorNotInitialized;   // set-up of the allocation-free path on `device`
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   size_t cap = 0;  // fits the buffers hold
@@ -103,11 +105,14 @@ extern "C" {
 int ddm_fitter_create(ddm_fitter** out) {
   if (!out) return DDM_EINVAL;
   ddm_fitter* f = new ddm_fitter();
-  if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipGetDevice(&f->device) != hipSuccess || hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&f->ev0) != hipSuccess || hipEventCreate(&f->ev1) != hipSuccess) {
     delete f;
     return DDM_EHIP;
   }
+  // the allocation-free path's fixed scratch on this device (its failure is
+  // reported by ddm_fit_device_noalloc; ddm_fit does not need it)
+  f->noalloc = ddm::noalloc_setup();
   *out = f;
   return DDM_OK;
 }
@@ -196,6 +201,31 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
     hipEventElapsedTime(&ms, f->ev0, f->ev1);
     *kernel_ms = ms;
   }
+  return DDM_OK;
+}
+
+int ddm_fit_device_noalloc(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, int32_t* iters,
+                           int32_t wait) {
+  if (!f || !b) return DDM_EINVAL;
+  if (b->nfits < 0 || b->nfits > DDM_FIT_NOALLOC_MAX)
+    return ffail(f, DDM_EINVAL, "fit (no-alloc): nfits must be in [0, " + std::to_string(DDM_FIT_NOALLOC_MAX) + "]");
+  if (b->nfits == 0) return DDM_OK;
+  if (b->ntau_max < 1 || b->ntau_max > kMaxT) return ffail(f, DDM_EINVAL, "fit (no-alloc): ntau_max must be in [1, 128]");
+  if (b->knodes < 1 || b->knodes > kMaxK) return ffail(f, DDM_EINVAL, "fit (no-alloc): knodes must be in [1, 8]");
+  if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D && b->model != DDM_MODEL_SWIMMER_2D)
+    return ffail(f, DDM_EINVAL, "fit (no-alloc): model");
+  if (b->seed_mode != DDM_SEED_GRID && b->seed_mode != DDM_SEED_GIVEN && b->seed_mode != DDM_SEED_NONLINEAR)
+    return ffail(f, DDM_EINVAL, "fit (no-alloc): seed_mode");
+  if (b->seed_mode != DDM_SEED_GRID && !b->seeds) return ffail(f, DDM_EINVAL, "fit (no-alloc): seeds required");
+  if (!b->ntau || !b->tau || !b->data || !b->weight || !b->qnodes || !b->bounds || !theta || !chi2 || !iters)
+    return ffail(f, DDM_EINVAL, "fit (no-alloc): null array");
+  if (f->noalloc != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("fit (no-alloc): scratch set-up failed on device ") +
+                                  std::to_string(f->device) + ": " + hipGetErrorString(f->noalloc));
+  hipError_t e = ddm::launch_fit_noalloc(f->device, b, theta, chi2, iters, f->stream);
+  if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit (no-alloc): launch: ") + hipGetErrorString(e));
+  if (wait && (e = hipStreamSynchronize(f->stream)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("fit (no-alloc): ") + hipGetErrorString(e));
   return DDM_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <cstdint>
 
 struct ddm_engine;
+struct ddm_fit_batch;
 
 namespace ddm {
 
@@ -60,6 +61,12 @@ hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, cons
                          const float* bounds, const float* seeds, const float* vmag, int nfits, int model,
                          const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
                          int* steps, hipStream_t st);
+
+// allocation-free fits (ddm_fit.hip): per-device set-up of the fixed scratch
+// array (current device), and pack -> LM -> unpack on device-resident arrays
+hipError_t noalloc_setup();
+hipError_t launch_fit_noalloc(int dev, const ::ddm_fit_batch* b, double* theta, double* chi2, int* iters,
+                              hipStream_t st);
 
 hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
                             float* C, long ldc, int M, int N, int K, int acc, hipStream_t st);

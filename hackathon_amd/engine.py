@@ -36,7 +36,7 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
            "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
            "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full", "ddm_analyze_motility",
-           "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean")
+           "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean", "ddm_fit_device_noalloc")
 
 
 class DdmError(RuntimeError):

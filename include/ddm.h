@@ -220,6 +220,24 @@ const char *ddm_fitter_last_error(const ddm_fitter *f);
  * NULL) is the fit kernel's duration from HIP events. */
 int ddm_fit(ddm_fitter *f, const ddm_fit_batch *b, double *theta, double *chi2, int32_t *iters, double *kernel_ms);
 
+/* ddm_fit on device-resident arrays, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every array of *b (ntau, tau, data, weight, qnodes, bounds,
+ * seeds, vmag) and theta (nfits*6), chi2 (nfits), iters (nfits) are DEVICE
+ * pointers; *b itself is read on the host.  The work is enqueued on the
+ * fitter's stream; wait != 0 blocks until it is done (leave it 0 under graph
+ * capture).  Results are bit-identical to ddm_fit's.  A fit whose ntau,
+ * bounds or vmag fails ddm_fit's checks gets chi2 = NaN, iters = -1 and NaN
+ * theta instead of an error status.  Buffers beyond the inputs and outputs
+ * live in one fixed device array of about 20 MiB per GPU (set up by
+ * ddm_fitter_create), so nfits <= DDM_FIT_NOALLOC_MAX; every call on a GPU
+ * shares that array, so calls that overlap on one GPU (several fitters'
+ * streams at once) race. */
+enum { DDM_FIT_NOALLOC_MAX = 16384 };
+int ddm_fit_device_noalloc(ddm_fitter *f, const ddm_fit_batch *b, double *theta, double *chi2, int32_t *iters,
+                           int32_t wait);
+
 /* ------------------------------------------------------------------ LM-seeding surrogate (MLP)
  *
  * 1536 -> 256 -> 256 -> 256 -> 68 with GELU after the hidden layers, BF16
