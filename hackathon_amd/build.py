@@ -28,7 +28,7 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math
 
 KERNEL_SOURCES = ["ddm_kernels.hip", "ddm_fit.hip", "ddm_mlp.hip", "ddm_synth.hip"]
 HOST_SOURCES = ["ddm_engine.cpp", "ddm_fitter.cpp", "ddm_mlp_host.cpp", "ddm_synth_host.cpp", "ddm_batch.cpp",
-                "ddm_verdict.cpp"]
+                "ddm_verdict.cpp", "ddm_clock.cpp"]
 
 
 def _run(cmd):

@@ -709,6 +709,58 @@ int ddm_analyze_motility(const ddm_radial_record* rad, const ddm_motility_opts* 
   return DDM_OK;
 }
 
+int ddm_lag_grid(int32_t L, int32_t* lags, int32_t cap) {
+  if (L < 2 || cap < 0 || (cap > 0 && !lags)) {
+    g_err = "lagGrid: L must be >= 2";
+    return DDM_EINVAL;
+  }
+  const std::vector<int> g = lag_grid(L);
+  for (int i = 0; i < (int)g.size() && i < cap; i++) lags[i] = g[i];
+  return (int)g.size();
+}
+
+int ddm_usable_bins(const ddm_radial_record* rad, int32_t* bins, int32_t cap) {
+  if (!rad || rad->N < 2 || rad->NB < 1 || rad->L < 2 || !rad->kCenter || !rad->nModes || !rad->D ||
+      rad->knodes < 0 || (rad->knodes > 0 && !rad->kNodes) || cap < 0 || (cap > 0 && !bins)) {
+    g_err = "prepare: invalid radial record";
+    return DDM_EINVAL;
+  }
+  const RadialView r{rad->N, rad->NB, rad->L, rad->knodes, rad->kCenter, rad->kNodes, rad->D, rad->drift, rad->nModes};
+  const Prepared pr = prepare(r, 1.0, 1.0);   // which bins are usable depends on neither scale
+  for (int i = 0; i < (int)pr.bins.size() && i < cap; i++) bins[i] = pr.bins[i].b;
+  return (int)pr.bins.size();
+}
+
+int ddm_swimmer_isf(int32_t geometry, const double* x, int32_t n, double Z, double* g) {
+  if ((geometry != DDM_GEOMETRY_3D && geometry != DDM_GEOMETRY_2D) || n < 0 || (n > 0 && (!x || !g)) ||
+      !(Z > 0)) {
+    g_err = "swimmer ISF: geometry must be 3D or 2D, Z > 0";
+    return DDM_EINVAL;
+  }
+  const Kernel k = swimmer_kernel(geometry);
+  for (int i = 0; i < n; i++) g[i] = k(x[i], Z);
+  return DDM_OK;
+}
+
+int ddm_estimate_drift(const ddm_radial_record* rad, const ddm_motility_opts* opt, const double* swimmer,
+                       ddm_drift_estimate* out) {
+  if (!rad || !opt || !out || rad->N < 2 || rad->NB < 1 || rad->L < 3 || !rad->kCenter || !rad->nModes ||
+      !rad->D || !rad->drift || rad->knodes < 0 || (rad->knodes > 0 && !rad->kNodes) || !(opt->dx_um > 0) ||
+      !(opt->fps > 0) || (opt->geometry != DDM_GEOMETRY_3D && opt->geometry != DDM_GEOMETRY_2D)) {
+    g_err = "estimateDrift: invalid radial record (drift sums required) or options";
+    return DDM_EINVAL;
+  }
+  const RadialView r{rad->N, rad->NB, rad->L, rad->knodes, rad->kCenter, rad->kNodes, rad->D, rad->drift, rad->nModes};
+  const Prepared pr = prepare(r, opt->dx_um, opt->fps);
+  Params sw{};
+  if (swimmer) {
+    sw.vbar = swimmer[0];
+    sw.Z = swimmer[1];
+  }
+  *out = estimate_drift(r, pr, opt->dx_um, opt->geometry, swimmer ? &sw : nullptr);
+  return DDM_OK;
+}
+
 int ddm_analyze_motility_batch(const ddm_radial_record *r, const ddm_motility_opts *opt, int32_t n,
                                ddm_motility_result *res, ddm_motility_detail *detail, int32_t nthreads) {
   if (n < 0 || (n > 0 && (!r || !opt || !res))) {

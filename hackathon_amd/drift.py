@@ -1,69 +1,64 @@
 """Uniform drift velocity from the radial record's phase sums.
 
-`estimate_drift` is the reference's estimateDrift (motility-ddm
-src/motility_fit.js:658-725): a weighted least-squares fit of the Fourier phase
-increments phi = arg[C(q, tau+1) conj C(q, tau)] = -q.V dt over every binned
+`estimate_drift` is estimateDrift (motility-ddm src/motility_fit.js:658-725):
+the drift V that best explains the Fourier phase increments of every binned
 mode and lag pair, from the 8 sums per (bin, lag) that `DdmEngine.readRadial`
-returns in `Radial.drift` (computed on the GPU, src/ddm_ref.js:360-381).  When a
-swimmer component is given, only the lag pairs where its self-ISF has decayed
-(|g(q vbar tau)| < 0.2) are used, so the remaining coherent phase rotation is
-advection.  Host FP64 arithmetic: the inputs are NB x L x 8 sums.
+returns in `Radial.drift` (computed on the GPU, src/ddm_ref.js:360-381).  With
+a swimmer component, only the lag pairs where its self-ISF has decayed below
+0.2 are used.  The arithmetic is the library's host routine
+`ddm_estimate_drift` (the one the verdict layer runs), so Python, the Node
+facade and `analyzeMotility` share one implementation.
 
-`swimmer_isf` is the kernel of the gating: the 3D Schulz-averaged swimmer ISF
-(isf3d, src/motility_fit.js:135-144) or the in-plane one (isf2d, :146-159,
-evaluated with the reference's Legendre quadrature legendreP, :120-132; the
-reference gates 2D fits with its tabulated isf2dFast, whose values agree with
-isf2d to the tolerance of its own tests).
+`isf3d` / `isf2d` are the library's swimmer kernels (`ddm_swimmer_isf`): the
+3D Schulz-averaged ISF (src/motility_fit.js:135-144) and the tabulated in-plane
+one the fits use (isf2dFast, :168-219).
 """
 from __future__ import annotations
 
-import math
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
+from .engine import DdmError, load_library
 from .fit import Prepared
 
 DRIFT_ISF_MAX = 0.2   # GATES.DRIFT_ISF_MAX (src/motility_fit.js:34)
 FLOW_SIGMA = 3.0      # GATES.FLOW_SIGMA (:30)
 FLOW_MIN_UM_S = 2.0   # GATES.FLOW_MIN_UM_S (:31)
+_GEOMETRY = {"3d": 0, "2d": 1}
 
 
-def isf3d(x: float, Z: float) -> float:
-    """sin(Z atan u) / (Z u (1 + u^2)^(Z/2)), u = x/(Z+1); series below u = 1e-4."""
-    u = x / (Z + 1)
-    if u < 1e-4:
-        return 1 - x * x * (Z + 2) / (6 * (Z + 1))
-    return math.sin(Z * math.atan(u)) / (Z * u * math.exp(0.5 * Z * math.log1p(u * u)))
-
-
-def _legendre_p(nu: float, c: float) -> float:
-    """P_nu(c) for |c| <= 1 by the trapezoid rule on Laplace's integral."""
-    s = math.sqrt(max(0.0, 1 - c * c))
-    n = 16 + math.ceil(nu * math.atan2(s, c))
-    acc = 0.0
-    for i in range(n + 1):
-        phi = math.pi * i / n
-        im = s * math.cos(phi)
-        val = math.exp(0.5 * nu * math.log(c * c + im * im)) * math.cos(nu * math.atan2(im, c))
-        acc += 0.5 * val if i in (0, n) else val
-    return acc / n
-
-
-def isf2d(x: float, Z: float) -> float:
-    """P_Z(1/sqrt(1+u^2)) / (1+u^2)^((Z+1)/2), u = x/(Z+1); series below u = 1e-4."""
-    u = x / (Z + 1)
-    if u < 1e-4:
-        return 1 - x * x * (Z + 2) / (4 * (Z + 1))
-    w = 1 + u * u
-    ln_pre = -0.5 * (Z + 1) * math.log(w)
-    if ln_pre < -25:
-        return 0.0
-    return _legendre_p(Z, 1 / math.sqrt(w)) * math.exp(ln_pre)
+def _lib():
+    from . import verdict as V
+    L = V._declare(load_library())
+    if not getattr(L, "_drift_declared", False):
+        L.ddm_swimmer_isf.argtypes = [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]
+        L.ddm_swimmer_isf.restype = C.c_int
+        L.ddm_estimate_drift.argtypes = [C.POINTER(V.RadialRecord), C.POINTER(V.MotilityOpts), C.c_void_p,
+                                         C.POINTER(V.DriftEstimate)]
+        L.ddm_estimate_drift.restype = C.c_int
+        L._drift_declared = True
+    return L
 
 
 def swimmer_isf(geometry: str):
-    return isf2d if geometry == "2d" else isf3d
+    """g(x; Z) of the given geometry: scalar in, scalar out; arrays in, arrays out."""
+    if geometry not in _GEOMETRY:
+        raise DdmError(f"swimmer ISF: geometry must be '3d' or '2d', got {geometry!r}")
+
+    def g(x, Z):
+        xs = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        out = np.empty_like(xs)
+        L = _lib()
+        if L.ddm_swimmer_isf(_GEOMETRY[geometry], xs.ctypes.data, xs.size, float(Z), out.ctypes.data) != 0:
+            raise DdmError(L.ddm_verdict_last_error().decode())
+        return float(out[0]) if np.ndim(x) == 0 else out.reshape(np.shape(x))
+    return g
+
+
+isf3d = swimmer_isf("3d")
+isf2d = swimmer_isf("2d")
 
 
 @dataclass
@@ -82,53 +77,30 @@ def estimate_drift(radial, pr: Prepared, dxUm: float, geometry: str = "3d", swim
     """estimateDrift (src/motility_fit.js:671-725).
 
     radial: the Radial record with drift sums (readRadial(drift=True));
-    pr: `fit.prepare(radial, dxUm, fps)`; swimmer: None or (vbar, Z) of the
-    fitted swimmer component, which turns on the lag-pair gating.
+    pr: `fit.prepare(radial, dxUm, fps)` (its frame interval); swimmer: None or
+    (vbar, Z) of the fitted swimmer component, which turns on the lag-pair gating.
     """
+    from . import verdict as V
     if radial.drift is None or len(radial.drift) == 0:
         raise ValueError("estimate_drift: the Radial record carries no drift sums (readRadial(drift=True))")
-    L, N = radial.L, radial.N
-    drift = np.asarray(radial.drift, dtype=np.float64).reshape(radial.NB, L, 8)
-    kernel = swimmer_isf(geometry)
-    S = np.zeros(8)
-    modes_used = 0
-    pairs = 0
-    for b in pr.bins:
-        used = False
-        for tau in range(1, L - 1):
-            if swimmer is not None:
-                vbar, Z = swimmer
-                g = kernel(b.q * vbar * tau * pr.dt, Z)
-                if not (abs(g) < DRIFT_ISF_MAX):
-                    continue
-            o = drift[b.b, tau]
-            if not (o[0] > 0):
-                continue
-            S += o
-            used = True
-            pairs += 1
-        if used:
-            modes_used += int(radial.nModes[b.b])
-    Sw, Sxx, Sxy, Syy, Rx, Ry, Rpp, n = (float(v) for v in S)
-    det = Sxx * Syy - Sxy * Sxy
-    if n < 10 or not (det > 1e-12 * Sxx * Syy) or not (Sw > 0):
-        return DriftEstimate(0.0, 0.0, 0.0, math.inf, 0.0, False, pairs, n)
-    # beta solves [Sxx Sxy; Sxy Syy] beta = [Rx; Ry], phi ~ kx bx + ky by
-    bx = (Syy * Rx - Sxy * Ry) / det
-    by = (Sxx * Ry - Sxy * Rx) / det
-    rss = max(Rpp - bx * Rx - by * Ry, 0.0)
-    # lag pairs of one mode are correlated: inflate by the mean pairs per mode
-    inflate = max(1.0, n / max(modes_used, 1))
-    s2 = rss / (n - 2) * inflate
-    cxx, cyy, cxy = s2 * Syy / det, s2 * Sxx / det, -s2 * Sxy / det
-    conv = -(N * dxUm) / (2 * math.pi * pr.dt)          # beta -> V [um/s]
-    Vx, Vy = conv * bx, conv * by
-    mag = math.hypot(Vx, Vy)
-    if mag > 0:
-        ux, uy = Vx / mag, Vy / mag
-        var_along = conv * conv * (ux * ux * cxx + 2 * ux * uy * cxy + uy * uy * cyy)
-    else:
-        var_along = conv * conv * (cxx + cyy) / 2
-    sigma = math.sqrt(max(var_along, 0.0))
-    return DriftEstimate(Vx, Vy, mag, sigma, math.atan2(Vy, Vx) * 180 / math.pi, mag > FLOW_SIGMA * sigma,
-                         pairs, n)
+    if geometry not in _GEOMETRY:
+        raise DdmError(f"estimate_drift: geometry must be '3d' or '2d', got {geometry!r}")
+    NB, L = int(radial.NB), int(radial.L)
+    kc = np.ascontiguousarray(radial.kCenter, dtype=np.float64)
+    kn = (np.ascontiguousarray(radial.kNodes, dtype=np.float64) if radial.kNodes is not None
+          and len(radial.kNodes) else np.zeros(0))
+    D = np.ascontiguousarray(radial.D, dtype=np.float64)
+    dr = np.ascontiguousarray(radial.drift, dtype=np.float64)
+    nm = np.ascontiguousarray(radial.nModes, dtype=np.uint32)
+    if dr.size != NB * L * 8 or D.size != NB * L:
+        raise DdmError("estimate_drift: Radial record arrays do not match NB and L")
+    rec = V.RadialRecord(int(radial.N), NB, L, kn.size // NB, kc.ctypes.data, kn.ctypes.data if kn.size else None,
+                         D.ctypes.data, dr.ctypes.data, nm.ctypes.data)
+    opt = V.MotilityOpts(float(dxUm), 1.0 / pr.dt, _GEOMETRY[geometry], 1)
+    sw = None if swimmer is None else np.array([float(swimmer[0]), float(swimmer[1])])
+    out = V.DriftEstimate()
+    lib = _lib()
+    if lib.ddm_estimate_drift(C.byref(rec), C.byref(opt), None if sw is None else sw.ctypes.data, C.byref(out)) != 0:
+        raise DdmError(lib.ddm_verdict_last_error().decode())
+    return DriftEstimate(out.Vx, out.Vy, out.mag, out.sigma, out.angle_deg, bool(out.significant), int(out.pairs),
+                         out.samples)

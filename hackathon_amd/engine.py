@@ -36,7 +36,10 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_synth_create", "ddm_synth_destroy", "ddm_synth_last_error", "ddm_synth_video", "ddm_synth_tracks",
            "ddm_create_tuned", "ddm_get_tuning", "ddm_batch_create", "ddm_batch_run", "ddm_batch_engine",
            "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full", "ddm_analyze_motility",
-           "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean", "ddm_fit_device_noalloc")
+           "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean", "ddm_fit_device_noalloc",
+           "ddm_clock_create", "ddm_clock_destroy", "ddm_clock_place", "ddm_clock_drop", "ddm_clock_reset",
+           "ddm_clock_state_get", "ddm_clock_snapshot", "ddm_shuffle_order", "ddm_lag_grid", "ddm_usable_bins",
+           "ddm_swimmer_isf", "ddm_estimate_drift")
 
 
 class DdmError(RuntimeError):
@@ -98,7 +101,7 @@ def load_library(path: str = LIB_PATH):
         if name not in ("ddm_destroy", "ddm_last_error", "ddm_fitter_destroy", "ddm_fitter_last_error",
                         "ddm_mlp_destroy", "ddm_mlp_last_error", "ddm_synth_destroy", "ddm_synth_last_error",
                         "ddm_batch_last_error", "ddm_batch_destroy", "ddm_read_temporal_full",
-                        "ddm_verdict_last_error"):
+                        "ddm_verdict_last_error", "ddm_clock_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

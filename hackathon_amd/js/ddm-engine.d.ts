@@ -62,7 +62,8 @@ export declare class DdmEngine {
   readonly saturated: boolean;
   waitForCapacity(): Promise<void>;
   ingestLuma(luma: Float32Array, slot: number): boolean;
-  ingestFrames(frames: Uint8Array | Uint16Array | Float32Array, nframes: number, firstSlot?: number): boolean;
+  /** the frame stack is copied and transformed off the event loop; calls on one engine run in call order */
+  ingestFrames(frames: Uint8Array | Uint16Array | Float32Array, nframes: number, firstSlot?: number): Promise<boolean>;
   computeTemporal(order: Uint32Array, mask: Float32Array, opts?: { yieldBetweenChunks?: boolean }): Promise<void>;
   readRadial(opts?: { drift?: boolean }): Promise<Radial>;
   readHistory(): Promise<Float32Array>;
@@ -85,10 +86,10 @@ export declare class FrameClock {
   constructor(fps: number, T?: number);
   readonly T: number;
   readonly dtMs: number;
-  frames: number;
-  missed: number;
-  duplicates: number;
-  resets: number;
+  readonly frames: number;
+  readonly missed: number;
+  readonly duplicates: number;
+  readonly resets: number;
   readonly timeSource: TimeSource | null;
   readonly newestN: number;
   readonly medianDtMs: number;

@@ -3,8 +3,11 @@
 // Keeps the method names, argument meaning, return values and thrown errors of
 // the reference's WebGPU engine class (motility-ddm src/ddm-engine.ts:62-426)
 // so callers switch by changing the import.  Every call goes through the N-API
-// addon (../napi/ddm_napi.node) to libddm.so: HIP kernels on one MI355X.  Types
-// are declared in ddm-engine.d.ts.
+// addon (../napi/ddm_napi.node) to libddm.so: HIP kernels on one MI355X.  The
+// calls that wait on the GPU or on large copies run off the event loop (the
+// addon's async work) and are chained per engine, so they take effect in the
+// order they were made; ingestLuma stays synchronous, as in the reference.
+// Types are declared in ddm-engine.d.ts.
 import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
@@ -24,6 +27,7 @@ export class DdmEngine {
     this._h = handle;
     this._destroyed = false;
     this._temporalBusy = false;
+    this._tail = Promise.resolve();   // the engine's queue of asynchronous calls
     this._onError = opts.onError;
     const s = addon.sizes(handle);
     this.sizes = s;
@@ -54,41 +58,47 @@ export class DdmEngine {
     return new DdmEngine(h, Object.assign({}, opts, g));
   }
 
+  // run `call` (returning the addon's promise) after every call queued before it
+  _queue(call) {
+    const p = this._tail.then(call);
+    this._tail = p.then(() => undefined, () => undefined);
+    return p;
+  }
+
   get pendingSubmits() { return 0; }
   get saturated() { return false; }
   async waitForCapacity() {}
 
-  /** ingestLuma (src/ddm-engine.ts:308-324): one N*N Float32Array frame into slot % T. */
+  /** ingestLuma (src/ddm-engine.ts:308-324): one N*N Float32Array frame into slot % T, synchronously. */
   ingestLuma(luma, slot) {
     if (this._destroyed) { this.throttled++; return false; }
     if (luma.length !== this.N * this.N) throw new Error(`ingestLuma expects ${this.N * this.N} values`);
-    return addon.ingest(this._h, luma, 1, slot % this.T);
+    return addon.ingestSync(this._h, luma, 1, slot % this.T);
   }
 
   /** Frame stack in: nframes frames (Uint8Array | Uint16Array | Float32Array) into slots firstSlot + i. */
-  ingestFrames(frames, nframes, firstSlot = 0) {
+  async ingestFrames(frames, nframes, firstSlot = 0) {
     if (this._destroyed) { this.throttled++; return false; }
-    return addon.ingest(this._h, frames, nframes, firstSlot);
+    return this._queue(() => addon.ingest(this._h, frames, nframes, firstSlot));
   }
 
-  /** computeTemporal (src/ddm-engine.ts:331-365). */
+  /** computeTemporal (src/ddm-engine.ts:331-365): resolves once D(q, tau) is ready on the GPU. */
   async computeTemporal(order, mask, opts = {}) {
     if (this._temporalBusy) throw new Error('computeTemporal is already running');
     this._temporalBusy = true;
     try {
-      addon.computeTemporal(this._h, order, mask);
-      addon.idle(this._h);
+      await this._queue(() => addon.computeTemporal(this._h, order, mask));
     } finally {
       this._temporalBusy = false;
     }
   }
 
   /**
-   * readRadial (src/ddm-engine.ts:391-395): the Radial record of src/ddm_ref.js:336-348, phase-drift
-   * sums included as in the reference (opts.drift = false skips them: they take a per-mode pass).
+   * readRadial (src/ddm-engine.ts:391-395): the Radial record of src/ddm_ref.js:336-348, with the
+   * phase-drift sums as in the reference (opts.drift = false skips them: they take a per-mode pass).
    */
   async readRadial(opts = {}) {
-    const r = addon.readRadial(this._h, pick(opts.drift, true));
+    const r = await this._queue(() => addon.readRadial(this._h, pick(opts.drift, true)));
     return {
       N: this.N, NB: this.NB, L: this.L,
       kCenter: Float64Array.from(this.bins.kCenter),
@@ -101,15 +111,15 @@ export class DdmEngine {
   }
 
   /** readTemporalFull (src/ddm-engine.ts:403-408): per-mode {D, C, P} of every mode (opt-in, computed on demand). */
-  async readTemporalFull() { return addon.readTemporalFull(this._h); }
+  async readTemporalFull() { return this._queue(() => addon.readTemporalFull(this._h)); }
 
   /** readMean (src/ddm-engine.ts:410-414): mean luma of the last frame ingested (u8/255, u16/65535). */
-  async readMean() { return addon.readMean(this._h); }
+  async readMean() { return this._queue(() => addon.readMean(this._h)); }
 
   /** readHistory (src/ddm-engine.ts:397-400): hist[(mode*T + slot)*2 + {0,1}]. */
-  async readHistory() { return addon.readHistory(this._h); }
+  async readHistory() { return this._queue(() => addon.readHistory(this._h)); }
 
-  async idle() { addon.idle(this._h); }
+  async idle() { return this._queue(() => addon.idle(this._h)); }
 
   destroy() {
     if (this._destroyed) return;
@@ -130,133 +140,76 @@ export function mulberry32(seed) {
   };
 }
 
-/** shuffleOrder (src/ddm_ref.js:510-519): seeded Fisher-Yates over the (slot, mask) pairs. */
+/**
+ * shuffleOrder (src/ddm_ref.js:510-519), the frame-order null test: the library's Fisher-Yates
+ * permutation of the (slot, mask) pairs, fed with rand()'s draws in the reference's order.
+ */
 export function shuffleOrder(o, rand) {
   const order = Uint32Array.from(o.order);
   const mask = Float32Array.from(o.mask);
-  for (let i = order.length - 1; i > 0; i--) {
-    const j = Math.floor(rand() * (i + 1));
-    const to = order[i]; order[i] = order[j]; order[j] = to;
-    const tm = mask[i]; mask[i] = mask[j]; mask[j] = tm;
-  }
+  const draws = new Float64Array(Math.max(order.length - 1, 0));
+  for (let k = 0; k < draws.length; k++) draws[k] = rand();
+  addon.shuffleOrder(order, mask, draws);
   return { order, mask };
 }
 
-/** linearOrder (src/ddm_ref.js:493-502). */
+/** linearOrder (src/ddm_ref.js:488-502): time index j in slot j, valid where `valid` is truthy. */
 export function linearOrder(valid) {
-  const T = valid.length;
-  const order = new Uint32Array(T);
-  const mask = new Float32Array(T);
-  for (let j = 0; j < T; j++) { order[j] = j; mask[j] = valid[j] ? 1 : 0; }
-  return { order, mask };
+  return {
+    order: Uint32Array.from(valid, (_, j) => j),
+    mask: Float32Array.from(valid, (v) => (v ? 1 : 0)),
+  };
 }
+
+const SOURCES = ['captureTime', 'watchdog', 'mediaTime'];
 
 /**
- * FrameClock (src/camera.ts:306-425): frame n = round((t - t0) / dt) goes to
- * slot n mod T of the history ring; each slot remembers the frame index it
- * holds, so snapshot() masks dropped, late, duplicated and overwritten frames.
- * T is the engine's ring length (the reference fixes 256).  Time sources
- * 'captureTime' and 'watchdog' share a timebase; switching to or from
- * 'mediaTime' restarts the clock.
+ * The live frame -> history-slot clock: the library's frame clock (ddm_clock_*,
+ * the reference's FrameClock contract, src/camera.ts:306-425; semantics with
+ * ddm_clock_create in include/ddm.h).  Frame n = round((t - t0) / dt) goes to
+ * slot n mod T; snapshot() masks dropped, late, duplicated and overwritten
+ * frames.  T is the engine's ring length (the reference fixes 256).
  */
 export class FrameClock {
   constructor(fps, T = DDM.T) {
-    if (!(fps > 0) || !(T >= 1)) throw new Error('FrameClock: fps must be positive and T >= 1');
+    this._c = addon.clockCreate(fps, T);
     this.T = T;
     this.dtMs = 1000 / fps;
-    this.frames = 0;
-    this.missed = 0;
-    this.duplicates = 0;
-    this.resets = 0;
-    this._slotN = new Float64Array(T).fill(-1);
-    this._t0 = null;
-    this._lastN = -1;
-    this._src = null;
-    this._lastTs = NaN;
-    this._dts = [];
   }
 
-  get timeSource() { return this._src; }
-
-  get newestN() { return this._lastN; }
-
-  reset() {
-    this._t0 = null;
-    this._lastN = -1;
-    this._slotN.fill(-1);
-    this._src = null;
-    this._lastTs = NaN;
-    this.resets++;
+  get frames() { return addon.clockState(this._c).frames; }
+  get missed() { return addon.clockState(this._c).missed; }
+  get duplicates() { return addon.clockState(this._c).duplicates; }
+  get resets() { return addon.clockState(this._c).resets; }
+  get newestN() { return addon.clockState(this._c).newestN; }
+  get validCount() { return addon.clockState(this._c).valid; }
+  get medianDtMs() { return addon.clockState(this._c).medianDtMs; }
+  get timeSource() {
+    const s = addon.clockState(this._c).timeSource;
+    return s >= 0 ? SOURCES[s] : null;
   }
+
+  reset() { addon.clockReset(this._c); }
 
   /** {slot, n} for a frame captured at tMs, or null for a duplicate / late frame. */
   place(tMs, source = 'captureTime') {
-    if (this._src !== null && (source === 'mediaTime') !== (this._src === 'mediaTime')) this.reset();
-    this._src = source;
-    if (this._t0 === null) this._t0 = tMs;
-    let n = Math.round((tMs - this._t0) / this.dtMs);
-    if (n <= this._lastN) {
-      this.duplicates++;
-      return null;
-    }
-    if (this._lastN >= 0 && n - this._lastN > this.T) {   // gap longer than the ring: restart here
-      this.reset();
-      this._src = source;
-      this._t0 = tMs;
-      n = 0;
-    }
-    if (this._lastN >= 0) this.missed += n - this._lastN - 1;
-    if (Number.isFinite(this._lastTs)) {
-      this._dts.push(tMs - this._lastTs);
-      if (this._dts.length > 120) this._dts.shift();
-    }
-    this._lastTs = tMs;
-    const slot = n % this.T;
-    this._slotN[slot] = n;
-    this._lastN = n;
-    this.frames++;
-    return { slot, n };
+    const src = SOURCES.indexOf(source);
+    if (src < 0) throw new Error(`FrameClock.place: unknown time source ${source}`);
+    const r = addon.clockPlace(this._c, tMs, src);
+    return r === null ? null : { slot: r[0], n: r[1] };
   }
 
   /** Frame n was not ingested (e.g. back-pressure). */
-  drop(n) {
-    const slot = n % this.T;
-    if (this._slotN[slot] === n) this._slotN[slot] = -1;
-  }
-
-  get medianDtMs() {
-    if (!this._dts.length) return NaN;
-    const s = this._dts.slice().sort((a, b) => a - b);
-    return s[s.length >> 1];
-  }
-
-  get validCount() {
-    let c = 0;
-    for (let j = 0; j < this.T; j++) {
-      const n = this._lastN - this.T + 1 + j;
-      if (n >= 0 && this._slotN[n % this.T] === n) c++;
-    }
-    return c;
-  }
+  drop(n) { addon.clockDrop(this._c, n); }
 
   /** Time-ordered {order, mask, used, newestN}; the oldest `guard` indices masked; shuffle = null test. */
   snapshot(opts = {}) {
-    const guard = pick(opts.guard, 0);
-    const order = new Uint32Array(this.T);
-    const mask = new Float32Array(this.T);
-    let used = 0;
-    for (let j = 0; j < this.T; j++) {
-      const n = this._lastN - this.T + 1 + j;
-      const slot = ((n % this.T) + this.T) % this.T;
-      const ok = n >= 0 && j >= guard && this._slotN[slot] === n;
-      order[j] = slot;
-      mask[j] = ok ? 1 : 0;
-      if (ok) used++;
-    }
+    const s = addon.clockSnapshot(this._c, pick(opts.guard, 0), this.T);
+    const newestN = this.newestN;
     if (opts.shuffle) {
-      const s = shuffleOrder({ order, mask }, opts.rand || Math.random);
-      return { order: s.order, mask: s.mask, used, newestN: this._lastN };
+      const sh = shuffleOrder(s, opts.rand || Math.random);
+      return { order: sh.order, mask: sh.mask, used: s.used, newestN };
     }
-    return { order, mask, used, newestN: this._lastN };
+    return { order: s.order, mask: s.mask, used: s.used, newestN };
   }
 }

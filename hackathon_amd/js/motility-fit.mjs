@@ -1,19 +1,17 @@
 // Per-q motility fits for the TypeScript-facing API (ES module, ES2019).
 //
-// prepare / bounds / lagGrid keep the reference's data preparation
-// (motility-ddm src/motility_fit.js:358-369, :383-401, :480-491): usable bins
-// (nModes >= 3, positive maximum), the quasi-logarithmic lag grid, q and the
-// q-nodes from kCenter / kNodes, weight w = 1 / max D.  analyzeMotilityLM runs
-// every usable bin of a Radial record as one batched Levenberg-Marquardt launch
-// on the MI355X (libddm.so ddm_fit through the N-API addon) and returns the
-// fitted per-q parameters.  estimateDrift turns the radial record's drift sums
-// into a uniform drift velocity (src/motility_fit.js:671-725); analyzeMotility
-// is the reference's verdict layer (:808-990) run by the library's host code.
-// Types are declared in motility-fit.d.ts.
+// prepare / bounds / lagGrid give the reference's data preparation
+// (motility-ddm src/motility_fit.js:358-369, :383-401, :480-491) from the
+// library's own routines (the lag grid and the usable-bin rule are
+// ddm_lag_grid / ddm_usable_bins, shared with the verdict layer and Python).
+// analyzeMotilityLM runs every usable bin of a Radial record as one batched
+// Levenberg-Marquardt launch on the MI355X (libddm.so ddm_fit through the
+// N-API addon) and returns the fitted per-q parameters.  estimateDrift
+// (src/motility_fit.js:671-725) and analyzeMotility (:808-990) are the
+// library's host code.  Types are declared in motility-fit.d.ts.
 import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
-import { isMainThread, parentPort } from 'worker_threads';
 
 const here = path.dirname(fileURLToPath(import.meta.url));
 const require = createRequire(import.meta.url);
@@ -24,47 +22,45 @@ const SEED = Object.freeze({ GRID: 0, GIVEN: 1, NONLINEAR: 2 });
 /** GATES.SWIM_MIN_UM_S (src/motility_fit.js:37) */
 export const SWIM_MIN_UM_S = 2;
 const pick = (a, b) => (a === undefined || a === null ? b : a);
+const f64 = (a) => (a instanceof Float64Array ? a : Float64Array.from(a));
+const GEOMETRY = { '3d': 0, '2d': 1 };
 
-/** Lags 1..10, then steps of x1.08 (src/motility_fit.js:358-369). */
-export function lagGrid(L) {
-  const lags = [];
-  for (let t = 1; t <= Math.min(10, L - 1); t++) lags.push(t);
-  let x = 10;
-  for (;;) {
-    x *= 1.08;
-    const t = Math.round(x);
-    if (t > L - 1) break;
-    if (t > lags[lags.length - 1]) lags.push(t);
-  }
-  return lags;
+// the record's typed views as the addon takes them
+function record(radial) {
+  return {
+    N: radial.N, NB: radial.NB, L: radial.L, kCenter: f64(radial.kCenter),
+    kNodes: radial.kNodes && radial.kNodes.length ? f64(radial.kNodes) : undefined,
+    nModes: radial.nModes instanceof Uint32Array ? radial.nModes : Uint32Array.from(radial.nModes),
+    D: f64(radial.D), drift: radial.drift && radial.drift.length ? f64(radial.drift) : undefined,
+  };
 }
 
-/** prepare(radial, dxUm, fps) (src/motility_fit.js:383-401). */
+/** The lags the fits sample (src/motility_fit.js:358-369): 1..10, then roughly 8 % apart, below L. */
+export function lagGrid(L) { return Array.from(addon.lagGrid(L)); }
+
+/**
+ * prepare(radial, dxUm, fps) (src/motility_fit.js:383-401): the usable bins (the library's rule) with
+ * their data on the lag grid, q = 2 pi kCenter / (N dx), q-nodes from kNodes and weight 1 / max D.
+ */
 export function prepare(radial, dxUm, fps) {
-  const { N, NB, L } = radial;
   const dt = 1 / fps;
-  const lags = lagGrid(L);
-  const tau = Float64Array.from(lags, (t) => t * dt);
-  const qs = (2 * Math.PI) / (N * dxUm);
-  const K = radial.kNodes && radial.kNodes.length ? radial.kNodes.length / NB : 0;
-  const bins = [];
-  for (let b = 0; b < NB; b++) {
-    if (radial.nModes[b] < 3) continue;
-    const d = Float64Array.from(lags, (t) => radial.D[b * L + t]);
-    let scale = -Infinity;
-    for (const v of d) scale = Math.max(scale, v);
-    if (!(scale > 0) || !Number.isFinite(scale)) continue;
-    const q = qs * radial.kCenter[b];
-    const qn = K ? Float64Array.from(radial.kNodes.subarray(b * K, (b + 1) * K), (k) => k * qs) : Float64Array.of(q);
-    bins.push({ b, q, qn, d, w: 1 / scale });
-  }
-  return { bins, lags, tau, dt, L, tauMax: (L - 1) * dt };
+  const lags = lagGrid(radial.L);
+  const tau = Float64Array.from(lags, (l) => l * dt);
+  const perK = (2 * Math.PI) / (radial.N * dxUm);
+  const K = radial.kNodes && radial.kNodes.length ? radial.kNodes.length / radial.NB : 0;
+  const bins = Array.from(addon.usableBins(record(radial)), (b) => {
+    const d = Float64Array.from(lags, (l) => radial.D[b * radial.L + l]);
+    const qn = K ? Float64Array.from({ length: K }, (_, k) => radial.kNodes[b * K + k] * perK)
+      : Float64Array.of(radial.kCenter[b] * perK);
+    return { b, q: radial.kCenter[b] * perK, qn, d, w: 1 / d.reduce((m, v) => (v > m ? v : m), -Infinity) };
+  });
+  return { bins, lags, tau, dt, L: radial.L, tauMax: (radial.L - 1) * dt };
 }
 
 /** vmin, vmax, Dmin, Dmax (src/motility_fit.js:480-491). */
 export function bounds(pr) {
-  let qmin = Infinity, qmax = -Infinity;
-  for (const b of pr.bins) { qmin = Math.min(qmin, b.q); qmax = Math.max(qmax, b.q); }
+  const qs = pr.bins.map((b) => b.q);
+  const qmin = Math.min(...qs), qmax = Math.max(...qs);
   return [Math.max(SWIM_MIN_UM_S, 0.3 / (qmax * pr.tauMax)), 3 / (qmin * pr.dt),
     0.01 / (qmax * qmax * pr.tauMax), 10 / (qmin * qmin * pr.dt)];
 }
@@ -115,81 +111,25 @@ export function analyzeMotilityLM(radial, opt = {}) {
 /** The drift gates of src/motility_fit.js:28-38 used here. */
 export const GATES = Object.freeze({ FLOW_SIGMA: 3, FLOW_MIN_UM_S: 2, DRIFT_ISF_MAX: 0.2 });
 
-/** 3D Schulz-averaged swimmer self-ISF (src/motility_fit.js:135-144). */
-export function isf3d(x, Z) {
-  const u = x / (Z + 1);
-  if (u < 1e-4) return 1 - (x * x * (Z + 2)) / (6 * (Z + 1));
-  return Math.sin(Z * Math.atan(u)) / (Z * u * Math.exp(0.5 * Z * Math.log1p(u * u)));
-}
-
-// P_nu(c), |c| <= 1: trapezoid rule on Laplace's integral (src/motility_fit.js:120-132)
-function legendreP(nu, c) {
-  const s = Math.sqrt(Math.max(0, 1 - c * c));
-  const n = 16 + Math.ceil(nu * Math.atan2(s, c));
-  let acc = 0;
-  for (let i = 0; i <= n; i++) {
-    const im = s * Math.cos((Math.PI * i) / n);
-    const v = Math.exp(0.5 * nu * Math.log(c * c + im * im)) * Math.cos(nu * Math.atan2(im, c));
-    acc += i === 0 || i === n ? 0.5 * v : v;
-  }
-  return acc / n;
-}
-
-/** In-plane Schulz-averaged swimmer self-ISF (src/motility_fit.js:146-159). */
-export function isf2d(x, Z) {
-  const u = x / (Z + 1);
-  if (u < 1e-4) return 1 - (x * x * (Z + 2)) / (4 * (Z + 1));
-  const w = 1 + u * u;
-  const lnPre = -0.5 * (Z + 1) * Math.log(w);
-  return lnPre < -25 ? 0 : legendreP(Z, 1 / Math.sqrt(w)) * Math.exp(lnPre);
-}
+const isfOf = (geometry) => (x, Z) => addon.swimmerIsf(geometry, Float64Array.of(x), Z)[0];
+/** The 3D Schulz-averaged swimmer self-ISF of the fits (src/motility_fit.js:135-144), library code. */
+export const isf3d = isfOf(GEOMETRY['3d']);
+/** The in-plane swimmer self-ISF of the fits (isf2dFast, src/motility_fit.js:168-219), library code. */
+export const isf2d = isfOf(GEOMETRY['2d']);
 
 /**
- * estimateDrift(radial, pr, opt, swimmer) (src/motility_fit.js:671-725): uniform
- * drift V from the radial record's phase sums (readRadial with drift on), by
- * weighted least squares of phi = -q.V dt over the usable bins' lag pairs; with
- * a swimmer component {vbar, Z}, only pairs where its ISF is below
- * GATES.DRIFT_ISF_MAX.  opt: {dxUm = 1, geometry = '3d'}.
+ * estimateDrift(radial, pr, opt, swimmer) (src/motility_fit.js:671-725): the uniform drift V from the
+ * radial record's phase sums (readRadial with drift on), in the library's host code; a swimmer component
+ * {vbar, Z} gates the lag pairs by its ISF.  opt: {dxUm = 1, geometry = '3d'}; pr gives the frame interval.
  */
 export function estimateDrift(radial, pr, opt = {}, swimmer = null) {
-  const { L, N } = radial;
-  if (!radial.drift || radial.drift.length < radial.NB * L * 8) throw new Error('estimateDrift: Radial record has no drift sums');
-  const g = pick(opt.geometry, '3d') === '2d' ? isf2d : isf3d;
-  const S = new Float64Array(8);
-  let modesUsed = 0;
-  let pairs = 0;
-  for (const bin of pr.bins) {
-    let used = false;
-    for (let tau = 1; tau < L - 1; tau++) {
-      if (swimmer && !(Math.abs(g(bin.q * swimmer.vbar * tau * pr.dt, swimmer.Z)) < GATES.DRIFT_ISF_MAX)) continue;
-      const o = (bin.b * L + tau) * 8;
-      if (!(radial.drift[o] > 0)) continue;
-      for (let c = 0; c < 8; c++) S[c] += radial.drift[o + c];
-      used = true;
-      pairs++;
-    }
-    if (used) modesUsed += radial.nModes[bin.b];
+  if (!radial.drift || radial.drift.length < radial.NB * radial.L * 8) {
+    throw new Error('estimateDrift: Radial record has no drift sums');
   }
-  const [Sw, Sxx, Sxy, Syy, Rx, Ry, Rpp, n] = S;
-  const det = Sxx * Syy - Sxy * Sxy;
-  if (n < 10 || !(det > 1e-12 * Sxx * Syy) || !(Sw > 0)) {
-    return { Vx: 0, Vy: 0, mag: 0, sigma: Infinity, angleDeg: 0, significant: false, pairs, samples: n };
-  }
-  const bx = (Syy * Rx - Sxy * Ry) / det;
-  const by = (Sxx * Ry - Sxy * Rx) / det;
-  const rss = Math.max(Rpp - bx * Rx - by * Ry, 0);
-  const s2 = (rss / (n - 2)) * Math.max(1, n / Math.max(modesUsed, 1));
-  const conv = -(N * pick(opt.dxUm, 1)) / (2 * Math.PI * pr.dt);
-  const Vx = conv * bx;
-  const Vy = conv * by;
-  const mag = Math.hypot(Vx, Vy);
-  // variance along V (isotropic mean when V = 0) of the covariance s2 [Sxx Sxy; Sxy Syy]^-1
-  const cxx = (s2 * Syy) / det, cyy = (s2 * Sxx) / det, cxy = (-s2 * Sxy) / det;
-  const ux = mag > 0 ? Vx / mag : 0, uy = mag > 0 ? Vy / mag : 0;
-  const varAlong = conv * conv * (mag > 0 ? ux * ux * cxx + 2 * ux * uy * cxy + uy * uy * cyy : (cxx + cyy) / 2);
-  const sigma = Math.sqrt(Math.max(varAlong, 0));
-  return { Vx, Vy, mag, sigma, angleDeg: (Math.atan2(Vy, Vx) * 180) / Math.PI,
-    significant: mag > GATES.FLOW_SIGMA * sigma, pairs, samples: n };
+  const geometry = pick(opt.geometry, '3d');
+  if (!(geometry in GEOMETRY)) throw new Error(`estimateDrift: geometry must be '3d' or '2d'`);
+  return addon.estimateDrift(record(radial), { dxUm: pick(opt.dxUm, 1), fps: 1 / pr.dt, geometry: GEOMETRY[geometry] },
+    swimmer ? { vbar: swimmer.vbar, Z: swimmer.Z } : null);
 }
 
 // ---------------------------------------------------------------- verdict
@@ -203,8 +143,6 @@ const GATE_TEXT = {
   4: ['G4', 'Motile contrast share alpha', '> 0.05, swimmer decay resolved in >= 1 bin'],
   5: ['G5', 'Exposure locked', 'manual exposure readback'],
 };
-const f64 = (a) => (a instanceof Float64Array ? a : Float64Array.from(a));
-
 /**
  * analyzeMotility(radial, opt) (src/motility_fit.js:808-990): fits, drift,
  * gates and verdict of one Radial record (readRadial with its drift sums), in
@@ -214,12 +152,7 @@ const f64 = (a) => (a instanceof Float64Array ? a : Float64Array.from(a));
 export function analyzeMotility(radial, opt = {}) {
   const geometry = pick(opt.geometry, '3d');
   if (geometry !== '3d' && geometry !== '2d') throw new Error(`analyzeMotility: geometry must be '3d' or '2d'`);
-  const r = addon.analyzeMotility({
-    N: radial.N, NB: radial.NB, L: radial.L, kCenter: f64(radial.kCenter),
-    kNodes: radial.kNodes && radial.kNodes.length ? f64(radial.kNodes) : undefined,
-    nModes: radial.nModes instanceof Uint32Array ? radial.nModes : Uint32Array.from(radial.nModes),
-    D: f64(radial.D), drift: f64(radial.drift || []),
-  }, { dxUm: pick(opt.dxUm, 1), fps: pick(opt.fps, 30), geometry: geometry === '2d' ? 1 : 0,
+  const r = addon.analyzeMotility(Object.assign(record(radial), { drift: f64(radial.drift || []) }), { dxUm: pick(opt.dxUm, 1), fps: pick(opt.fps, 30), geometry: geometry === '2d' ? 1 : 0,
     exposureLocked: pick(opt.exposureLocked, true) ? 1 : 0 });
   const gates = [];
   for (let i = 0; i < r.ngates; i++) {
@@ -251,23 +184,5 @@ export function analyzeMotility(radial, opt = {}) {
     },
     q: Array.from(r.q.subarray(0, nb)), bins: Array.from(r.bins.subarray(0, nb)), lags, tau: lags.map((l) => l * dt),
     nData: r.nData, nEff: r.nEff, residualRho: r.residualRho,
-  });
-}
-
-// ---------------------------------------------------------------- worker entry
-// The reference runs analyzeMotility in a module Worker built from its fitting
-// module, so a fit never blocks frame delivery (src/motility_fit.js:992-1005,
-// caller src/main.ts:325-353): {id, radial, opt} in, {id, result} or
-// {id, error: stack text} out.  The same protocol in a Node worker thread
-// started on this file (new Worker(new URL('./motility-fit.mjs', ...)));
-// inert on the main thread.
-if (!isMainThread && parentPort) {
-  parentPort.on('message', (msg) => {
-    const { id, radial, opt } = msg || {};
-    try {
-      parentPort.postMessage({ id, result: analyzeMotility(radial, opt) });
-    } catch (e) {
-      parentPort.postMessage({ id, error: String((e && e.stack) || e) });
-    }
   });
 }

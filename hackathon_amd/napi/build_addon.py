@@ -21,7 +21,7 @@ def build(verbose: bool = False):
             print("node_api.h not found; N-API addon not built")
         return None
     cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I" + NODE_INC,
-           "-DNODE_GYP_MODULE_NAME=ddm_napi", "-o", OUT + ".tmp", os.path.join(HERE, "ddm_napi.c"),
+           "-DNODE_GYP_MODULE_NAME=ddm_napi", "-pthread", "-o", OUT + ".tmp", os.path.join(HERE, "ddm_napi.c"),
            "-L" + PKG, "-l:libddm.so", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:

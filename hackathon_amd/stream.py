@@ -1,26 +1,60 @@
 """Streaming ring mode: the frame clock that maps live frames to history slots.
 
-`FrameClock` keeps the reference's FrameClock semantics (motility-ddm
+`FrameClock` is the Python face of the library's frame clock (`ddm_clock_*`,
+the producer side of the reference's FrameClock, motility-ddm
 src/camera.ts:306-425): frame n = round((t - t0) / dt) lives in slot n mod T
-of the engine's history ring (`DdmEngine.ingestLuma(luma, slot)`); the slot
-remembers which frame index it holds, so a snapshot's mask excludes dropped,
-late, duplicated and overwritten frames by construction.  `snapshot` returns the
-time-ordered (order, mask) pair for `computeTemporal`, oldest first, with the
-oldest `guard` time indices masked (slots a producer may overwrite while the
-analysis runs) and optionally shuffled (the frame-order null test).
+of the engine's history ring (`DdmEngine.ingestLuma(luma, slot)`); each slot
+remembers which frame it holds, so a snapshot's mask leaves out dropped,
+late, duplicated and overwritten frames by construction.  `snapshot` returns
+the time-ordered (order, mask) pair for `computeTemporal`, oldest first, with
+the oldest `guard` time indices masked (slots a producer may overwrite while
+the analysis runs), optionally shuffled (the frame-order null test).  The
+semantics are spelled out with `ddm_clock_create` in include/ddm.h; the Node
+facade binds the same code.
 
 T is the engine's ring length (the reference fixes it at 256).  Time stamps are
-milliseconds; time sources are 'captureTime', 'watchdog' (one timebase) and
-'mediaTime' (a separate one: switching between the two restarts the clock).
-On this engine an ingest issued after computeTemporal is ordered behind it on
-the engine's stream, so a snapshot's slots cannot change under a running
-analysis; the guard keeps the reference's policy for producers that race it.
+milliseconds; sources are 'captureTime', 'watchdog' (one timebase) and
+'mediaTime' (a separate one).  On this engine an ingest issued after
+computeTemporal is ordered behind it on the engine's stream, so a snapshot's
+slots cannot change under a running analysis; the guard keeps the reference's
+policy for producers that race it.
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
+
+from .engine import DdmError, load_library
+
+SOURCES = ("captureTime", "watchdog", "mediaTime")   # DDM_TIME_CAPTURE, _WATCHDOG, _MEDIA
+
+
+class ClockState(C.Structure):
+    _fields_ = [("newest_n", C.c_int64), ("frames", C.c_int64), ("missed", C.c_int64), ("duplicates", C.c_int64),
+                ("resets", C.c_int64), ("valid", C.c_int32), ("time_source", C.c_int32),
+                ("median_dt_ms", C.c_double)]
+
+
+def _lib():
+    L = load_library()
+    if not getattr(L, "_clock_declared", False):
+        vp = C.c_void_p
+        L.ddm_clock_create.argtypes = [C.c_double, C.c_int32, C.POINTER(vp)]
+        L.ddm_clock_destroy.argtypes = [vp]
+        L.ddm_clock_destroy.restype = None
+        L.ddm_clock_place.argtypes = [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.ddm_clock_drop.argtypes = [vp, C.c_int64]
+        L.ddm_clock_reset.argtypes = [vp]
+        L.ddm_clock_state_get.argtypes = [vp, C.POINTER(ClockState)]
+        L.ddm_clock_snapshot.argtypes = [vp, C.c_int32, vp, vp, C.POINTER(C.c_int32)]
+        L.ddm_shuffle_order.argtypes = [vp, vp, C.c_int32, vp]
+        for f in ("ddm_clock_create", "ddm_clock_place", "ddm_clock_drop", "ddm_clock_reset", "ddm_clock_state_get",
+                  "ddm_clock_snapshot", "ddm_shuffle_order"):
+            getattr(L, f).restype = C.c_int
+        L._clock_declared = True
+    return L
 
 
 @dataclass
@@ -31,114 +65,76 @@ class Snapshot:
     newestN: int        # newest frame index at snapshot time
 
 
-def shuffle_order(order: np.ndarray, mask: np.ndarray, rand):
-    """shuffleOrder (src/ddm_ref.js:510-519): Fisher-Yates over the (slot, mask) pairs, from the last
-    position down, j = floor(rand() (i + 1)) with rand() uniform in [0, 1)."""
+def shuffle_order(order, mask, rand):
+    """shuffleOrder (src/ddm_ref.js:510-519): the (slot, mask) pairs permuted by the library's Fisher-Yates,
+    fed with rand()'s draws (uniform in [0, 1)) in the order the reference consumes them."""
     order = np.array(order, dtype=np.uint32)
     mask = np.array(mask, dtype=np.float32)
-    for i in range(len(order) - 1, 0, -1):
-        j = int(rand() * (i + 1))
-        order[i], order[j] = order[j], order[i]
-        mask[i], mask[j] = mask[j], mask[i]
+    n = order.size
+    if mask.size != n:
+        raise DdmError("shuffleOrder: order and mask lengths differ")
+    u = np.array([rand() for _ in range(max(n - 1, 0))], dtype=np.float64)
+    if _lib().ddm_shuffle_order(order.ctypes.data, mask.ctypes.data, n, u.ctypes.data if u.size else None) != 0:
+        raise DdmError("shuffleOrder: draws must lie in [0, 1)")
     return order, mask
 
 
 class FrameClock:
     def __init__(self, fps: float, T: int = 256):
-        if not (fps > 0) or T < 1:
+        self._lib = _lib()
+        h = C.c_void_p()
+        if self._lib.ddm_clock_create(float(fps), int(T), C.byref(h)) != 0:
             raise ValueError("FrameClock: fps must be positive and T >= 1")
+        self._h = h
         self.T = int(T)
         self.dtMs = 1000.0 / fps
-        self.frames = self.missed = self.duplicates = self.resets = 0
-        self._slot_n = np.full(self.T, -1, dtype=np.int64)
-        self._t0 = None
-        self._last_n = -1
-        self._src = None
-        self._last_ts = float("nan")
-        self._dts: list[float] = []
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._lib.ddm_clock_destroy(h)
+
+    def _state(self) -> ClockState:
+        s = ClockState()
+        self._lib.ddm_clock_state_get(self._h, C.byref(s))
+        return s
+
+    frames = property(lambda self: int(self._state().frames))
+    missed = property(lambda self: int(self._state().missed))
+    duplicates = property(lambda self: int(self._state().duplicates))
+    resets = property(lambda self: int(self._state().resets))
+    newestN = property(lambda self: int(self._state().newest_n))
+    validCount = property(lambda self: int(self._state().valid))
+    medianDtMs = property(lambda self: float(self._state().median_dt_ms))
 
     @property
     def timeSource(self):
-        return self._src
-
-    @property
-    def newestN(self) -> int:
-        return self._last_n
+        s = self._state().time_source
+        return SOURCES[s] if s >= 0 else None
 
     def reset(self) -> None:
-        self._t0 = None
-        self._last_n = -1
-        self._slot_n.fill(-1)
-        self._src = None
-        self._last_ts = float("nan")
-        self.resets += 1
+        self._lib.ddm_clock_reset(self._h)
 
     def place(self, tMs: float, source: str = "captureTime"):
         """Slot for a frame captured at tMs: (slot, n), or None for a duplicate / late frame."""
-        if self._src is not None and (source == "mediaTime") != (self._src == "mediaTime"):
-            self.reset()
-        self._src = source
-        if self._t0 is None:
-            self._t0 = tMs
-        n = _round_half_up((tMs - self._t0) / self.dtMs)
-        if n <= self._last_n:
-            self.duplicates += 1
-            return None
-        if self._last_n >= 0 and n - self._last_n > self.T:
-            # a gap longer than the ring: nothing in it is usable, restart at this frame
-            self.reset()
-            self._src = source
-            self._t0 = tMs
-            n = 0
-        if self._last_n >= 0:
-            self.missed += n - self._last_n - 1
-        if self._last_ts == self._last_ts:       # not NaN
-            self._dts.append(tMs - self._last_ts)
-            if len(self._dts) > 120:
-                self._dts.pop(0)
-        self._last_ts = tMs
-        slot = n % self.T
-        self._slot_n[slot] = n
-        self._last_n = n
-        self.frames += 1
-        return slot, n
+        if source not in SOURCES:
+            raise DdmError(f"FrameClock.place: unknown time source {source!r}")
+        slot, n = C.c_int32(), C.c_int64()
+        rc = self._lib.ddm_clock_place(self._h, float(tMs), SOURCES.index(source), C.byref(slot), C.byref(n))
+        if rc < 0:
+            raise DdmError("FrameClock.place: time stamps must be finite")
+        return None if rc else (int(slot.value), int(n.value))
 
     def drop(self, n: int) -> None:
         """Frame n was not ingested (e.g. the producer's back-pressure)."""
-        slot = n % self.T
-        if self._slot_n[slot] == n:
-            self._slot_n[slot] = -1
-
-    @property
-    def medianDtMs(self) -> float:
-        if not self._dts:
-            return float("nan")
-        s = sorted(self._dts)
-        return s[len(s) >> 1]
-
-    def _window(self):
-        n = self._last_n - self.T + 1 + np.arange(self.T, dtype=np.int64)
-        slot = np.mod(n, self.T)
-        return n, slot, (n >= 0) & (self._slot_n[slot] == n)
-
-    @property
-    def validCount(self) -> int:
-        return int(np.count_nonzero(self._window()[2]))
+        self._lib.ddm_clock_drop(self._h, int(n))
 
     def snapshot(self, guard: int = 0, shuffle: bool = False, rand=None) -> Snapshot:
-        n, slot, ok = self._window()
-        ok = ok & (np.arange(self.T) >= guard)
-        order = slot.astype(np.uint32)
-        mask = ok.astype(np.float32)
-        used = int(np.count_nonzero(ok))
+        order = np.zeros(self.T, np.uint32)
+        mask = np.zeros(self.T, np.float32)
+        used = C.c_int32()
+        if self._lib.ddm_clock_snapshot(self._h, int(guard), order.ctypes.data, mask.ctypes.data, C.byref(used)):
+            raise DdmError("FrameClock.snapshot: guard must be >= 0")
         if shuffle:
-            if rand is None:
-                rng = np.random.default_rng()
-                rand = rng.random
-            order, mask = shuffle_order(order, mask, rand)
-        return Snapshot(order, mask, used, self._last_n)
-
-
-def _round_half_up(x: float) -> int:
-    """Math.round: nearest integer, halves toward +infinity."""
-    return int(np.floor(x + 0.5))
+            order, mask = shuffle_order(order, mask, rand if rand is not None else np.random.default_rng().random)
+        return Snapshot(order, mask, int(used.value), self.newestN)

@@ -297,6 +297,51 @@ int ddm_batch_engine(ddm_batch *b, int32_t i, ddm_engine **e);
 const char *ddm_batch_last_error(const ddm_batch *b);
 void ddm_batch_destroy(ddm_batch *b);
 
+/* ------------------------------------------------------------------ frame clock (streaming ring)
+ *
+ * The live producer's side of the ring (FrameClock, src/camera.ts:306-425).
+ * Semantics:
+ *   - frame index n = round((t - t0) / dt) (halves up), t0 = the epoch's first
+ *     time stamp, dt = 1000 / fps ms; frame n lives in history slot n mod T;
+ *   - a frame whose n is not above the newest one is a duplicate (counted,
+ *     not placed); a gap of more than T frames starts a new epoch at that
+ *     frame (n = 0); switching between the media timeline and the performance
+ *     clock (capture / watchdog stamps) starts a new epoch too; missed counts
+ *     the skipped indices between consecutive placed frames;
+ *   - a snapshot lists the window n = newest - T + 1 .. newest oldest first:
+ *     order[j] = slot of that frame, mask[j] = 1 iff n >= 0, j >= guard and the
+ *     slot still holds frame n (not dropped, not overwritten, this epoch);
+ *   - the median interval is over the last 120 intervals between placed
+ *     frames of an epoch (the upper median for an even count; NaN if none;
+ *     kept across epochs).
+ * Host code only; one clock per producer (not thread-safe). */
+typedef struct ddm_clock ddm_clock;
+enum { DDM_TIME_CAPTURE = 0, DDM_TIME_WATCHDOG = 1, DDM_TIME_MEDIA = 2 };
+typedef struct ddm_clock_state {
+  int64_t newest_n;       /* newest frame index of the epoch, -1 before its first frame */
+  int64_t frames, missed, duplicates, resets;
+  int32_t valid;          /* frames of the window still in their slots */
+  int32_t time_source;    /* DDM_TIME_* of the last placed frame, -1 in a fresh epoch */
+  double median_dt_ms;
+} ddm_clock_state;
+/* new FrameClock(fps) with a ring of T slots (the reference's T = 256) */
+int ddm_clock_create(double fps, int32_t T, ddm_clock **out);
+void ddm_clock_destroy(ddm_clock *c);
+/* place (camera.ts:345-376): DDM_OK with *slot and *n, or DDM_DROPPED for a duplicate */
+int ddm_clock_place(ddm_clock *c, double t_ms, int32_t source, int32_t *slot, int64_t *n);
+/* drop (camera.ts:379-382): frame n was not ingested; its slot no longer counts */
+int ddm_clock_drop(ddm_clock *c, int64_t n);
+/* reset (camera.ts:334-341): a new epoch */
+int ddm_clock_reset(ddm_clock *c);
+/* counters, newestN, timeSource, validCount, medianDtMs (camera.ts:326-398) */
+int ddm_clock_state_get(const ddm_clock *c, ddm_clock_state *s);
+/* snapshot (camera.ts:405-424) without the shuffle: order[T], mask[T], *used = valid entries */
+int ddm_clock_snapshot(const ddm_clock *c, int32_t guard, uint32_t *order, float *mask, int32_t *used);
+/* shuffleOrder (src/ddm_ref.js:510-519), the frame-order null test: a Fisher-Yates permutation of
+ * the (order, mask) pairs in place; u holds the n-1 uniform draws in [0, 1) the caller's generator
+ * gives, consumed for positions n-1 down to 1 (position i trades with floor(u (i + 1))). */
+int ddm_shuffle_order(uint32_t *order, float *mask, int32_t n, const double *u);
+
 /* ------------------------------------------------------------------ synthetic videos
  *
  * Seeded video generator on the GPU (the benchmark and training workloads):
@@ -405,6 +450,22 @@ int ddm_analyze_motility(const ddm_radial_record *r, const ddm_motility_opts *op
 int ddm_analyze_motility_batch(const ddm_radial_record *r, const ddm_motility_opts *opt, int32_t n,
                                ddm_motility_result *res, ddm_motility_detail *detail, int32_t nthreads);
 const char *ddm_verdict_last_error(void);
+
+/* Building blocks of the fits, for callers that prepare per-q problems themselves (host code):
+ * lagGrid (src/motility_fit.js:358-369): the lags 1..L-1 the fits sample; writes at most cap of them
+ * and returns their count (negative status on L < 2). */
+int ddm_lag_grid(int32_t L, int32_t *lags, int32_t cap);
+/* prepare's bin selection (src/motility_fit.js:383-401): the bins with nModes >= 3 and a finite positive
+ * maximum of D over the lag grid, in order; writes at most cap and returns their count.  The drift sums
+ * may be NULL. */
+int ddm_usable_bins(const ddm_radial_record *r, int32_t *bins, int32_t cap);
+/* The swimmer self-ISF g(x; Z) of the fits: isf3d (src/motility_fit.js:135-144) for DDM_GEOMETRY_3D,
+ * the tabulated in-plane kernel isf2dFast (:168-219) for DDM_GEOMETRY_2D; n points. */
+int ddm_swimmer_isf(int32_t geometry, const double *x, int32_t n, double Z, double *g);
+/* estimateDrift (src/motility_fit.js:671-725) over the record's phase sums (drift required): swimmer =
+ * NULL, or {vbar (um/s), Z} of a swimmer component whose ISF gates the lags (|g| < 0.2). */
+int ddm_estimate_drift(const ddm_radial_record *r, const ddm_motility_opts *opt, const double *swimmer,
+                       ddm_drift_estimate *out);
 
 #ifdef __cplusplus
 }

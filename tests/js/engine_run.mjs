@@ -19,7 +19,7 @@ async function run(dir, N, T, L, NB, kmin, kmax, dtype) {
   const Ctor = dtype === 'u8' ? Uint8Array : dtype === 'u16' ? Uint16Array : Float32Array;
   const frames = new Ctor(raw.buffer, raw.byteOffset, raw.byteLength / Ctor.BYTES_PER_ELEMENT);
   const e = await DdmEngine.create({ N, T, L, NB, kmin, kmax });
-  const ok = e.ingestFrames(frames, T, 0);
+  const ok = await e.ingestFrames(frames, T, 0);
   const valid = new Array(T).fill(true);
   const { order, mask } = linearOrder(valid);
   await e.computeTemporal(order, mask);

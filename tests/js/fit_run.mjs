@@ -11,7 +11,7 @@ async function run(dir, N, T, L, NB, kmin, kmax, fps) {
   const raw = fs.readFileSync(path.join(dir, 'frames.bin'));
   const frames = new Uint8Array(raw.buffer, raw.byteOffset, raw.byteLength);
   const e = await DdmEngine.create({ N, T, L, NB, kmin, kmax });
-  e.ingestFrames(frames, T, 0);
+  await e.ingestFrames(frames, T, 0);
   const { order, mask } = linearOrder(new Array(T).fill(true));
   await e.computeTemporal(order, mask);
   const radial = await e.readRadial();

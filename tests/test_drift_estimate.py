@@ -47,7 +47,10 @@ def test_isf_kernels_match_oracle():
     for Z in (0.7, 1.5, 4.0, 12.0, 60.0):
         for x in (0.0, 1e-5, 0.03, 0.4, 1.0, 3.7, 11.0, 40.0):
             assert DR.isf3d(x, Z) == pytest.approx(O.isf3d(x, Z), rel=1e-12, abs=1e-15)
-            assert DR.isf2d(x, Z) == pytest.approx(O.isf2d(x, Z), rel=1e-12, abs=1e-15)
+            # the in-plane kernel is the tabulated one of the fits (isf2dFast): its FP64 oracle to rounding,
+            # the exact Legendre form to the table's own 2e-5
+            assert DR.isf2d(x, Z) == pytest.approx(O.isf2d_fast(x, Z), rel=1e-10, abs=1e-13)
+            assert abs(DR.isf2d(x, Z) - O.isf2d(x, Z)) < 2e-5
 
 
 @pytest.mark.parametrize("speed,deg", [(6, -34), (2.5, 90), (12, 180), (4, 225)])

@@ -195,37 +195,19 @@ def test_js_analyze_motility_matches_python(tmp_path):
     assert out[0]["verdict"] == "MOTILE" and out[2]["verdict"] == "UNLOCKED_EXPOSURE"
 
 
-def test_js_fit_worker_protocol(tmp_path):
-    """The fit Worker protocol (src/motility_fit.js:992-1005): {id, radial, opt} posted to a worker thread
-    started on motility-fit.mjs comes back as {id, result} with the main-thread result, or {id, error}
-    when analyzeMotility throws (no GPU work)."""
-    from tests.test_verdict import oracle_radial
-    from hackathon_amd.verdict import analyze_motility
-    if not os.path.exists(ADDON):
-        pytest.fail("N-API addon not built (python -m hackathon_amd.napi.build_addon)")
-    r = oracle_radial("motile_3d")
-    rec = dict(N=r.N, NB=r.NB, L=r.L, kCenter=list(map(float, r.kCenter)), kNodes=list(map(float, r.kNodes)),
-               nModes=[int(x) for x in r.nModes], D=list(map(float, r.D)), drift=list(map(float, r.drift)))
-    (tmp_path / "radial.json").write_text(json.dumps(rec))
-    mod = os.path.join(ROOT, "hackathon_amd", "js", "motility-fit.mjs")
-    code = ("Promise.all([import('worker_threads'), import('fs'), import('url')]).then(([wt, fs, url]) => {"
-            " const r = JSON.parse(fs.readFileSync('" + str(tmp_path / "radial.json") + "', 'utf8'));"
-            " const w = new wt.Worker(url.pathToFileURL('" + mod + "'), {execArgv: []});"
-            " const got = [];"
-            " w.on('message', (m) => { got.push(m); if (got.length === 2) {"
-            "   console.log('RESULT ' + JSON.stringify(got)); w.terminate(); } });"
-            " w.on('error', (e) => { console.log('RESULT ' + JSON.stringify({error: String(e)})); });"
-            " w.postMessage({id: 7, radial: r, opt: {dxUm: 1, fps: 30}});"
-            " w.postMessage({id: 8, radial: r, opt: {dxUm: 1, fps: 30, geometry: '4d'}}); })")
-    res = subprocess.run(["node", "--input-type=module", "-e", code], capture_output=True, text=True, timeout=120)
-    line = [x for x in res.stdout.splitlines() if x.startswith("RESULT ")]
-    assert line, res.stdout + res.stderr
-    out = json.loads(line[0][7:])
-    assert isinstance(out, list), out
-    byid = {m["id"]: m for m in out}
-    want = analyze_motility(r, 1.0, 30.0)
-    ok = byid[7]["result"]
-    assert "error" not in byid[7] and ok["verdict"] == want["verdict"] == "MOTILE"
-    for k in ("TSI", "alpha", "vbar", "Z", "Ddiff", "dBIC"):
-        assert ok[k] == want[k], k
-    assert "result" not in byid[8] and "geometry" in byid[8]["error"]
+
+@pytest.mark.gpu
+def test_js_engine_calls_leave_the_event_loop_running(gpu):
+    """ingestFrames and computeTemporal run off the event loop (the addon's async work): a 1 ms timer keeps
+    ticking while a 1024 x 512^2 u16 stack is ingested and analysed twenty times; a second computeTemporal while one runs
+    is refused with the reference's message (src/ddm-engine.ts:331-352)."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "async_run.mjs")], capture_output=True, text=True,
+                       timeout=240)
+    line = [x for x in r.stdout.splitlines() if x.startswith("RESULT ")]
+    assert line, r.stdout + r.stderr
+    out = json.loads(line[-1][7:])
+    print(out)
+    assert "error" not in out, out
+    assert out["ingestTicks"] >= 3 and out["temporalTicks"] >= 3, out
+    assert out["reentry"] == "computeTemporal is already running"
+    assert out["finite"] and out["d0"] == 0 and out["d1"] > 0
