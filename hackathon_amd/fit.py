@@ -47,18 +47,25 @@ def _declare(L):
     return L
 
 
+def _helpers(L):
+    if not getattr(L, "_fit_helpers_declared", False):
+        L.ddm_lag_grid.argtypes = [C.c_int32, C.c_void_p, C.c_int32]
+        L.ddm_lag_grid.restype = C.c_int
+        L.ddm_usable_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.ddm_usable_bins.restype = C.c_int
+        L._fit_helpers_declared = True
+    return L
+
+
 def lag_grid(L: int) -> np.ndarray:
-    """Lags 1..10, then steps of x1.08 (src/motility_fit.js:358-369)."""
-    lags = list(range(1, min(10, L - 1) + 1))
-    x = 10.0
-    while True:
-        x *= 1.08
-        t = int(np.floor(x + 0.5))          # Math.round for positive x
-        if t > L - 1:
-            break
-        if t > lags[-1]:
-            lags.append(t)
-    return np.array(lags, dtype=np.int64)
+    """The lags the fits sample (src/motility_fit.js:358-369): the library's ddm_lag_grid."""
+    lib = _helpers(load_library())
+    n = lib.ddm_lag_grid(int(L), None, 0)
+    if n < 0:
+        raise DdmError("lagGrid: L must be >= 2")
+    out = np.zeros(n, np.int32)
+    lib.ddm_lag_grid(int(L), out.ctypes.data, n)
+    return out.astype(np.int64)
 
 
 @dataclass
@@ -80,26 +87,38 @@ class Prepared:
     tauMax: float
 
 
+def usable_bins(radial: Radial) -> np.ndarray:
+    """prepare's bin selection (src/motility_fit.js:383-401), the library's ddm_usable_bins: nModes >= 3 and
+    a finite positive maximum of D over the lag grid."""
+    from .verdict import RadialRecord
+    NB, L = int(radial.NB), int(radial.L)
+    kc = np.ascontiguousarray(radial.kCenter, dtype=np.float64)
+    D = np.ascontiguousarray(radial.D, dtype=np.float64)
+    nm = np.ascontiguousarray(radial.nModes, dtype=np.uint32)
+    rec = RadialRecord(int(radial.N), NB, L, 0, kc.ctypes.data, None, D.ctypes.data, None, nm.ctypes.data)
+    lib = _helpers(load_library())
+    out = np.zeros(NB, np.int32)
+    n = lib.ddm_usable_bins(C.byref(rec), out.ctypes.data, NB)
+    if n < 0:
+        raise DdmError("prepare: invalid Radial record")
+    return out[:n]
+
+
 def prepare(radial: Radial, dxUm: float, fps: float) -> Prepared:
-    """prepare (src/motility_fit.js:383-401)."""
-    N, NB, L = radial.N, radial.NB, radial.L
+    """prepare (src/motility_fit.js:383-401): the usable bins with their data on the lag grid, q from kCenter,
+    the q-nodes from kNodes (in rad/um) and the weight 1 / max D."""
+    NB, L = radial.NB, radial.L
     dt = 1.0 / fps
     lags = lag_grid(L)
     D = np.asarray(radial.D, dtype=np.float64).reshape(NB, L)
-    qs = 2 * np.pi / (N * dxUm)
+    per_k = 2 * np.pi / (radial.N * dxUm)
     kn = radial.kNodes
     K = len(kn) // NB if kn is not None and len(kn) else 0
-    bins = []
-    for b in range(NB):
-        if radial.nModes[b] < 3:
-            continue
-        d = D[b, lags]
-        scale = float(d.max())
-        if not (scale > 0) or not np.isfinite(scale):
-            continue
-        q = qs * float(radial.kCenter[b])
-        qn = np.asarray(kn[b * K:(b + 1) * K], dtype=np.float64) * qs if K else np.array([q])
-        bins.append(PreparedBin(b, q, qn, d, 1.0 / scale))
+    nodes = np.asarray(kn, dtype=np.float64).reshape(NB, K) * per_k if K else None
+    bins = [PreparedBin(int(b), per_k * float(radial.kCenter[b]),
+                        nodes[b] if K else np.array([per_k * float(radial.kCenter[b])]), D[b, lags],
+                        1.0 / float(D[b, lags].max()))
+            for b in usable_bins(radial)]
     return Prepared(bins, lags, lags * dt, dt, L, (L - 1) * dt)
 
 
