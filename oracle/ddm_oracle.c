@@ -24,10 +24,13 @@
  *                               src/ddm_ref.js (tests/golden/make_golden.mjs)
  *   special functions           reference KAT values (t_kernels.mjs:39-51), scipy
  *   isf3d / isf2d               Simpson quadrature over the Schulz pdf (t_kernels.mjs:60-72)
- *   orc_isf2d_fast              |table - exact| < 2e-5 (t_kernels.mjs:85-102); exact table
- *                               values: parity unpinned (motility_fit.js does not parse
- *                               on this Node, so it cannot be imported)
+ *   orc_isf2d_fast              |table - exact| < 2e-5 (t_kernels.mjs:85-102); the exact
+ *                               table values against the reference's isf2dFast outputs
+ *                               (tests/golden/fits_reference.json, make_golden_fits.mjs)
  *   orc_nnls3                   KKT conditions (t_kernels.mjs:107-134)
+ *   orc_fit_brownian/_swimmer   the reference's fitBrownian / fitSwimmer outputs on six
+ *                               noisy oracle records (same fixture file)
+ *   orc_simulate                the reference's simulate() movies, byte for byte (same file)
  *   orc_lm_fit                  noise-free recovery, closed-form Brownian data; with a
  *                               drift speed, recovery on orc_model_column data
  *
