@@ -20,6 +20,7 @@
 // Brownian grid; 16 vbar x Z{1.5, 8} x alpha{0.3, 0.8} at that D; amplitudes by
 // 2-column NNLS), LM from the best `nstarts`; "seeded" = LM from one given
 // start (the surrogate's), no grid.
+#include "../../include/ddm.h"
 #include "ddm_internal.h"
 #include "ddm_fft.h"
 
@@ -653,6 +654,497 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
     chi_out[f] = bestchi;
     steps_out[f] = steps;
   }
+}
+
+// ----------------------------------------------------------------- global-parameter fits
+//
+// One workgroup per video: (alpha, ln vbar, ln Z, ln D) shared by all its
+// bins, the per-bin amplitudes A_b, B_b >= 0 eliminated exactly by 2-column
+// NNLS at every evaluation -- the reference's variable projection (evalModel,
+// motility-ddm src/motility_fit.js:450-474) and its objective RSS = sum_b
+// w_b^2 rss_b, here with an FP64 model.  The nonlinear parameters take LM
+// steps on the reduced Gauss-Newton matrix of the projected residual
+// (Kaufman's approximation): per bin, the derivative columns v_k = w A_b
+// dc/dtheta_k are projected off the span of the bin's active amplitude
+// columns, H = sum_b (v^T v - m^T G^-1 m) with m = M^T v and G = M^T M, and
+// g = sum_b v^T r (the NNLS residual is orthogonal to M).  A bin whose A_b is
+// 0 does not depend on theta.  The four waves take bins round-robin, lanes
+// take lags; every decision is made on workgroup sums, so all threads agree.
+// The start grids are ranked with the FP32 model (as the per-q kernel's
+// grid); LM accepts and rejects steps on the FP64 objective.
+// Plain seeding follows fitBrownian / fitSwimmer (:500-536, :577-611): a
+// 49-point ln D grid refined by LM in ln D, then the 16 vbar x 2 Z x 2 alpha
+// start grid at that D and LM from the best `nstarts` (Nelder-Mead in the
+// reference).
+namespace {
+
+struct GFitIn {
+  const float* tau;      // [nvid * MAXT]
+  const float* d;        // [nbins * MAXT]
+  const float* w;        // [nbins]
+  const float* qn;       // [nbins * K]
+  const int* bin_off;    // [nvid + 1]
+  const int* ntau;       // [nvid]
+  const float* bounds;   // [nvid * 4] vmin, vmax, Dmin, Dmax
+  const float* seeds;    // [nvid * 4] alpha, ln vbar, ln Z, ln D, or null
+  const float* vmag;     // [nvid] or null
+  int K;
+  Isf2dDev t2d;
+};
+constexpr int GNT = 256;    // 4 waves per video (up to 512 registers per lane: the FP64 model spills below that)
+constexpr int GW = GNT / 64;
+
+// 2-column NNLS as nnls2, amplitudes in FP64
+__device__ double nnls2_d(double g11, double g12, double n, double h1, double h2, double dd, double& A, double& B) {
+  double best = dd, bA = 0, bB = 0;
+  if (g11 > 0) {
+    const double a = h1 / g11, r = dd - a * h1;
+    if (a >= 0 && r < best) best = r, bA = a, bB = 0;
+  }
+  if (n > 0) {
+    const double b = h2 / n, r = dd - b * h2;
+    if (b >= 0 && r < best) best = r, bA = 0, bB = b;
+  }
+  const double det = g11 * n - g12 * g12;
+  if (det > 1e-11 * fmax(g11, n) * fmax(g11, n)) {
+    const double a = (h1 * n - g12 * h2) / det, b = (g11 * h2 - g12 * h1) / det, r = dd - a * h1 - b * h2;
+    if (a >= 0 && b >= 0 && r < best) best = r, bA = a, bB = b;
+  }
+  A = bA;
+  B = bB;
+  return best > 0 ? best : 0;
+}
+
+struct GCtx {
+  int b0, b1, nt, K, lane, wave;
+  float tv[MAXT / 64];
+  float V;
+};
+
+__device__ __forceinline__ void load_q(const GFitIn& in, int b, int K, float* q) {
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) q[i] = i < K ? in.qn[(long)b * K + i] : 0.f;
+}
+
+// RSS = sum_b w_b^2 rss_b at th (FP64 model); the bins' NNLS amplitudes go to ab.
+template <int GEO, bool SWIM>
+__device__ double g_eval(const GFitIn& in, const GCtx& cx, const double* th, double* ab, double* red) {
+  const double alpha = SWIM ? th[0] : 0.0, vbar = exp(th[1]), Z = exp(th[2]), Dd = exp(th[3]);
+  double acc = 0;
+  for (int b = cx.b0 + cx.wave; b < cx.b1; b += GW) {
+    float q[MAXK];
+    load_q(in, b, cx.K, q);
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < MAXT / 64; r++) {
+      const int j = cx.lane + 64 * r;
+      if (j < cx.nt) {
+        const double c = column_d<GEO>(cx.tv[r], q, cx.K, alpha, vbar, Z, Dd, cx.V, in.t2d, SWIM);
+        const double dv = in.d[(long)b * MAXT + j];
+        a[0] += c * c;
+        a[1] += c;
+        a[2] += c * dv;
+        a[3] += dv;
+        a[4] += dv * dv;
+      }
+    }
+    const double t = wave_sum_split<3>(a, cx.lane);
+    double A, B;
+    const double rss = nnls2_d(readlane_d(t, split_lane(0, 3)), readlane_d(t, split_lane(1, 3)), (double)cx.nt,
+                               readlane_d(t, split_lane(2, 3)), readlane_d(t, split_lane(3, 3)),
+                               readlane_d(t, split_lane(4, 3)), A, B);
+    const double w = in.w[b];
+    acc += w * w * rss;
+    if (cx.lane == 0) {
+      ab[2 * (b - cx.b0)] = A;
+      ab[2 * (b - cx.b0) + 1] = B;
+    }
+  }
+  if (cx.lane == 0) red[cx.wave] = acc;
+  __syncthreads();
+  double chi = 0;
+#pragma unroll
+  for (int k = 0; k < GW; k++) chi += red[k];
+  __syncthreads();
+  return chi;
+}
+
+// The same objective with the FP32 model (ranking the start grids only).
+template <int GEO, bool SWIM>
+__device__ double g_eval32(const GFitIn& in, const GCtx& cx, const double* th, double* red) {
+  const float alpha = SWIM ? (float)th[0] : 0.f, vbar = (float)exp(th[1]), Z = (float)exp(th[2]),
+              Dd = (float)exp(th[3]);
+  double acc = 0;
+  for (int b = cx.b0 + cx.wave; b < cx.b1; b += GW) {
+    float q[MAXK];
+    load_q(in, b, cx.K, q);
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < MAXT / 64; r++) {
+      const int j = cx.lane + 64 * r;
+      if (j < cx.nt) {
+        float c, u0, u1, u2, u3;
+        column<GEO>(cx.tv[r], q, cx.K, alpha, vbar, Z, Dd, cx.V, in.t2d, SWIM, false, c, u0, u1, u2, u3);
+        const double dv = in.d[(long)b * MAXT + j];
+        a[0] += (double)c * c;
+        a[1] += c;
+        a[2] += c * dv;
+        a[3] += dv;
+        a[4] += dv * dv;
+      }
+    }
+    const double t = wave_sum_split<3>(a, cx.lane);
+    double A, B;
+    const double rss = nnls2_d(readlane_d(t, split_lane(0, 3)), readlane_d(t, split_lane(1, 3)), (double)cx.nt,
+                               readlane_d(t, split_lane(2, 3)), readlane_d(t, split_lane(3, 3)),
+                               readlane_d(t, split_lane(4, 3)), A, B);
+    const double w = in.w[b];
+    acc += w * w * rss;
+  }
+  if (cx.lane == 0) red[cx.wave] = acc;
+  __syncthreads();
+  double chi = 0;
+#pragma unroll
+  for (int k = 0; k < GW; k++) chi += red[k];
+  __syncthreads();
+  return chi;
+}
+
+// Reduced normal equations at th from the amplitudes ab of the evaluation at
+// th: H (packed lower 4x4) and g.
+template <int GEO, bool SWIM>
+__device__ void g_normal(const GFitIn& in, const GCtx& cx, const double* th, const double* ab, double* H, double* g,
+                         double* red) {
+  const float alpha = SWIM ? (float)th[0] : 0.f, vbar = (float)exp(th[1]), Z = (float)exp(th[2]),
+              Dd = (float)exp(th[3]);
+  double Hw[10], gw[4];
+#pragma unroll
+  for (int k = 0; k < 10; k++) Hw[k] = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) gw[k] = 0;
+  for (int b = cx.b0 + cx.wave; b < cx.b1; b += GW) {
+    const double A = ab[2 * (b - cx.b0)], B = ab[2 * (b - cx.b0) + 1];
+    if (!(A > 0)) continue;   // the bin's fit does not depend on theta
+    float q[MAXK];
+    load_q(in, b, cx.K, q);
+    // S_kl (10), c dc_k (4), dc_k (4), d dc_k (4), c^2, c: 24 sums.  A lane
+    // holds at most MAXT/64 = 2 lags, so its partial sums are FP32 (its terms
+    // are FP32 products already); the wave sums are FP64, eight at a time.
+    float a[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) a[k] = 0.f;
+#pragma unroll
+    for (int r = 0; r < MAXT / 64; r++) {
+      const int j = cx.lane + 64 * r;
+      if (j < cx.nt) {
+        float c, dc[4];
+        column<GEO>(cx.tv[r], q, cx.K, alpha, vbar, Z, Dd, cx.V, in.t2d, SWIM, true, c, dc[0], dc[1], dc[2], dc[3]);
+        const float dv = in.d[(long)b * MAXT + j];
+        int h = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+          for (int l = 0; l <= k; l++) a[h++] += dc[k] * dc[l];
+          a[10 + k] += c * dc[k];
+          a[14 + k] += dc[k];
+          a[18 + k] += dv * dc[k];
+        }
+        a[22] += c * c;
+        a[23] += c;
+      }
+    }
+    double S[24];
+#pragma unroll
+    for (int grp = 0; grp < 3; grp++) {
+      double t8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) t8[k] = a[8 * grp + k];
+      const double t = wave_sum_split<3>(t8, cx.lane);
+#pragma unroll
+      for (int k = 0; k < 8; k++) S[8 * grp + k] = readlane_d(t, split_lane(k, 3));
+    }
+    const double w = in.w[b], w2 = w * w;
+    // projection on the active amplitude columns: (w c, w) when B > 0, else w c alone
+    const double G00 = w2 * S[22], G01 = w2 * S[23], G11 = w2 * cx.nt;
+    const double det = G00 * G11 - G01 * G01;
+    const bool two = B > 0 && det > 1e-11 * fmax(G00, G11) * fmax(G00, G11);
+    double m0[4], m1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      m0[k] = w2 * A * S[10 + k];
+      m1[k] = w2 * A * S[14 + k];
+      gw[k] += w2 * A * (S[18 + k] - A * S[10 + k] - B * S[14 + k]);
+    }
+    int h = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int l = 0; l <= k; l++, h++) {
+        const double proj = two ? (m0[k] * (G11 * m0[l] - G01 * m1[l]) + m1[k] * (G00 * m1[l] - G01 * m0[l])) / det
+                                : (G00 > 0 ? m0[k] * m0[l] / G00 : 0.0);
+        Hw[h] += w2 * A * A * S[h] - proj;
+      }
+  }
+  // the four waves' partials
+  if (cx.lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) red[cx.wave * 16 + k] = Hw[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[cx.wave * 16 + 10 + k] = gw[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    double sum = 0;
+#pragma unroll
+    for (int w = 0; w < GW; w++) sum += red[w * 16 + k];
+    if (k < 10) H[k] = sum;
+    else g[k - 10] = sum;
+  }
+  __syncthreads();
+}
+
+// LM over (alpha, ln vbar, ln Z, ln D) (SWIM) or ln D alone, from th; the
+// amplitudes of the returned point are left in *abp.  Returns RSS.
+template <int GEO, bool SWIM>
+__device__ double g_lm(const GFitIn& in, const GCtx& cx, double* th, const double* lo, const double* hi, int maxit,
+                       double** abp, double** abq, double* red, int* steps) {
+  for (int k = 0; k < 4; k++) th[k] = fmin(hi[k], fmax(lo[k], th[k]));
+  double chi = g_eval<GEO, SWIM>(in, cx, th, *abp, red);
+  double lam = 1e-3, nu = 2;
+  int it = 0;
+  bool done = false;
+  while (it < maxit && !done) {
+    double H[10], g[4];
+    g_normal<GEO, SWIM>(in, cx, th, *abp, H, g, red);
+    bool frozen[4];
+    double diag[4], dmax = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      frozen[a] = (!SWIM && a < 3) || (th[a] <= lo[a] && g[a] < 0) || (th[a] >= hi[a] && g[a] > 0);
+      const double Haa = H[a * (a + 1) / 2 + a];
+      if (!frozen[a]) dmax = fmax(dmax, Haa);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const double Haa = H[a * (a + 1) / 2 + a], fl = fmax(1e-12 * dmax, 1e-300);
+      diag[a] = Haa > fl ? Haa : fl;
+      if (frozen[a]) g[a] = 0;
+    }
+    for (;;) {
+      it++;
+      double Lc[10];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b <= a; b++) {
+          const int h = a * (a + 1) / 2 + b;
+          Lc[h] = (frozen[a] || frozen[b]) ? (a == b ? 1.0 : 0.0) : H[h] + (a == b ? lam * diag[a] : 0.0);
+        }
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const int ra = a * (a + 1) / 2;
+#pragma unroll
+        for (int b = 0; b <= a; b++) {
+          const int rb = b * (b + 1) / 2;
+          double s = Lc[ra + b];
+#pragma unroll
+          for (int k = 0; k < b; k++) s -= Lc[ra + k] * Lc[rb + k];
+          if (a == b) {
+            ok = ok && s > 0;
+            Lc[ra + a] = sqrt(s > 0 ? s : 1.0);
+          } else {
+            Lc[ra + b] = s / Lc[rb + b];
+          }
+        }
+      }
+      if (ok) {
+        double y[4], x[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          double s = g[a];
+#pragma unroll
+          for (int k = 0; k < a; k++) s -= Lc[a * (a + 1) / 2 + k] * y[k];
+          y[a] = s / Lc[a * (a + 1) / 2 + a];
+        }
+#pragma unroll
+        for (int a = 3; a >= 0; a--) {
+          double s = y[a];
+#pragma unroll
+          for (int k = a + 1; k < 4; k++) s -= Lc[k * (k + 1) / 2 + a] * x[k];
+          x[a] = s / Lc[a * (a + 1) / 2 + a];
+        }
+        double nth[4], pred = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          nth[a] = frozen[a] ? th[a] : fmin(hi[a], fmax(lo[a], th[a] + x[a]));
+          const double dk = nth[a] - th[a];
+          pred += dk * (lam * diag[a] * dk + g[a]);
+        }
+        const double nchi = g_eval<GEO, SWIM>(in, cx, nth, *abq, red);
+        const double rho = pred > 0 ? (chi - nchi) / pred : -1;
+        if (nchi < chi && rho > 0) {
+          const double rel = (chi - nchi) / fmax(chi, 1e-300);
+#pragma unroll
+          for (int a = 0; a < 4; a++) th[a] = nth[a];
+          chi = nchi;
+          double* tmp = *abp;
+          *abp = *abq;
+          *abq = tmp;
+          const double f = 1 - (2 * rho - 1) * (2 * rho - 1) * (2 * rho - 1);
+          lam *= f > 1.0 / 3 ? f : 1.0 / 3;
+          if (lam < 1e-15) lam = 1e-15;
+          nu = 2;
+          if ((rel < 1e-9 && pred < 1e-9 * chi) || rel < 1e-12 || chi < 1e-300) done = true;
+          break;
+        }
+      }
+      lam *= nu;
+      nu *= 2;
+      if (lam > 1e15 || it >= maxit) {
+        done = true;
+        break;
+      }
+    }
+  }
+  *steps += it;
+  return chi;
+}
+
+// mode 0: LM from the seeds; 1: plain (the reference's grids).  swim 0: the
+// Brownian model only.  Outputs per video: theta (alpha, ln vbar, ln Z, ln D),
+// RSS, LM steps, the Brownian fit (D, RSS; plain mode), per bin (A, B).
+// split > 0 (plain swimmer fits of small batches): `split` workgroups per
+// video, workgroup s running the grids and the LM from start s only, each
+// writing its own output slot (blockIdx.x); the caller keeps the best start,
+// which is what the sequential loop over starts returns.
+template <int GEO>
+__global__ __launch_bounds__(GNT) void k_lm_global(GFitIn in, int mode, int swim, int nstarts, int maxit, int split,
+                                                   double* __restrict__ theta_out, double* __restrict__ chi_out,
+                                                   int* __restrict__ steps_out, double* __restrict__ brown_out,
+                                                   double* __restrict__ ab_out) {
+  __shared__ double ab0[2 * DDM_GLOBAL_MAX_BINS], ab1[2 * DDM_GLOBAL_MAX_BINS], red[16 * GW];
+  const int v = split > 0 ? (int)blockIdx.x / split : (int)blockIdx.x;
+  const int s_only = split > 0 ? (int)blockIdx.x % split : -1;
+  GCtx cx;
+  cx.lane = threadIdx.x & 63;
+  cx.wave = threadIdx.x >> 6;
+  cx.b0 = in.bin_off[v];
+  cx.b1 = in.bin_off[v + 1];
+  cx.nt = min(in.ntau[v], MAXT);
+  cx.K = min(in.K, MAXK);
+  cx.V = in.vmag ? in.vmag[v] : 0.f;
+#pragma unroll
+  for (int r = 0; r < MAXT / 64; r++) {
+    const int j = cx.lane + 64 * r;
+    cx.tv[r] = j < cx.nt ? in.tau[(long)v * MAXT + j] : 0.f;
+  }
+  const double lo[4] = {0.0, log((double)in.bounds[4 * v]), log(0.5), log((double)in.bounds[4 * v + 2])};
+  const double hi[4] = {1.0, log((double)in.bounds[4 * v + 1]), log(100.0), log((double)in.bounds[4 * v + 3])};
+  double* abp = ab0;
+  double* abq = ab1;
+  int steps = 0;
+  double th[4], chi;
+  if (mode == 0) {
+    for (int k = 0; k < 4; k++) th[k] = in.seeds[4 * v + k];
+    chi = swim ? g_lm<GEO, true>(in, cx, th, lo, hi, maxit, &abp, &abq, red, &steps)
+               : g_lm<GEO, false>(in, cx, th, lo, hi, maxit, &abp, &abq, red, &steps);
+  } else {
+    // fitBrownian: 49-point ln D grid, then LM in ln D
+    double tb[4] = {0.0, lo[1], lo[2], lo[3]}, cb = 1e300;
+    for (int i = 0; i <= 48; i++) {
+      const double t[4] = {0.0, lo[1], lo[2], lo[3] + (hi[3] - lo[3]) * i / 48};
+      const double c = g_eval32<GEO, false>(in, cx, t, red);
+      if (c < cb) {
+        cb = c;
+        tb[3] = t[3];
+      }
+    }
+    cb = g_lm<GEO, false>(in, cx, tb, lo, hi, maxit, &abp, &abq, red, &steps);
+    if (s_only > 0) steps = 0;   // the Brownian steps count once per video
+    if (threadIdx.x == 0 && brown_out && s_only <= 0) {
+      brown_out[2 * v] = exp(tb[3]);
+      brown_out[2 * v + 1] = cb;
+    }
+    if (!swim) {
+      for (int k = 0; k < 4; k++) th[k] = tb[k];
+      chi = cb;
+    } else {
+      // fitSwimmer's start grid at the Brownian D: the best nstarts (<= 3),
+      // kept as grid indices (i, zi, ai) in a sorted insertion where an
+      // earlier start wins ties, as the reference's stable sort keeps it first
+      int si[3] = {-1, -1, -1};
+      double sc[3] = {1e300, 1e300, 1e300};
+      const int ns = nstarts < 1 ? 1 : (nstarts > 3 ? 3 : nstarts);
+      for (int gi = 0; gi < 64; gi++) {
+        const int i = gi >> 2, zi = (gi >> 1) & 1, ai = gi & 1;
+        const double t[4] = {ai ? 0.8 : 0.3, lo[1] + (hi[1] - lo[1]) * (i + 0.5) / 16, log(zi ? 8.0 : 1.5), tb[3]};
+        const double c = g_eval32<GEO, true>(in, cx, t, red);
+        if (c < sc[2]) {
+          if (c < sc[1]) {
+            sc[2] = sc[1];
+            si[2] = si[1];
+            if (c < sc[0]) {
+              sc[1] = sc[0];
+              si[1] = si[0];
+              sc[0] = c;
+              si[0] = gi;
+            } else {
+              sc[1] = c;
+              si[1] = gi;
+            }
+          } else {
+            sc[2] = c;
+            si[2] = gi;
+          }
+        }
+      }
+      chi = 1e300;
+      for (int k = 0; k < 4; k++) th[k] = tb[k];
+      for (int s = 0; s < ns; s++) {
+        const int gi = s == 0 ? si[0] : (s == 1 ? si[1] : si[2]);
+        if (gi < 0 || (s_only >= 0 && s != s_only)) continue;
+        const int i = gi >> 2, zi = (gi >> 1) & 1, ai = gi & 1;
+        double t[4] = {ai ? 0.8 : 0.3, lo[1] + (hi[1] - lo[1]) * (i + 0.5) / 16, log(zi ? 8.0 : 1.5), tb[3]};
+        const double c = g_lm<GEO, true>(in, cx, t, lo, hi, maxit, &abp, &abq, red, &steps);
+        if (c < chi) {
+          chi = c;
+          for (int k = 0; k < 4; k++) th[k] = t[k];
+        }
+      }
+    }
+  }
+  // the amplitudes at the reported point
+  chi = swim ? g_eval<GEO, true>(in, cx, th, abp, red) : g_eval<GEO, false>(in, cx, th, abp, red);
+  const int slot = blockIdx.x;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; k++) theta_out[4 * slot + k] = th[k];
+    chi_out[slot] = chi;
+    steps_out[slot] = steps;
+  }
+  const long ab0ff = split > 0 ? 2L * ((long)cx.b0 * split + (long)s_only * (cx.b1 - cx.b0)) : 2L * cx.b0;
+  for (int i = threadIdx.x; i < 2 * (cx.b1 - cx.b0); i += GNT) ab_out[ab0ff + i] = abp[i];
+}
+
+}  // namespace
+
+hipError_t launch_lm_global(const float* tau, const float* d, const float* w, const float* qn, int K,
+                            const int* bin_off, const int* ntau, const float* bounds, const float* seeds,
+                            const float* vmag, int nvid, int model, const Isf2dDev* t2d, int mode, int nstarts,
+                            int maxit, int split, double* theta, double* chi, int* steps, double* brown, double* ab,
+                            hipStream_t st) {
+  if (K > MAXK || model < 0 || model > 2 || (model == 2 && !t2d) || nvid < 1 || split < 0 || split > 3)
+    return hipErrorInvalidValue;
+  GFitIn in{tau, d, w, qn, bin_off, ntau, bounds, seeds, vmag, K, model == 2 ? *t2d : Isf2dDev{}};
+  const dim3 grid(split > 0 ? nvid * split : nvid);
+  if (model == 2)
+    hipLaunchKernelGGL(k_lm_global<1>, grid, dim3(GNT), 0, st, in, mode, 1, nstarts, maxit, split, theta, chi, steps,
+                       brown, ab);
+  else
+    hipLaunchKernelGGL(k_lm_global<0>, grid, dim3(GNT), 0, st, in, mode, model == 1 ? 1 : 0, nstarts, maxit, split,
+                       theta, chi, steps, brown, ab);
+  return hipGetLastError();
 }
 
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,

@@ -25,6 +25,12 @@ struct ddm_fitter {
   float* theta = nullptr;
   double* chi = nullptr;
   int* steps = nullptr;
+  // global-parameter fits: device arrays (grown on demand)
+  size_t gcap_vid = 0, gcap_bins = 0;
+  float *gtau = nullptr, *gd = nullptr, *gw = nullptr, *gqn = nullptr, *gbounds = nullptr, *gseeds = nullptr,
+        *gvmag = nullptr;
+  int *gboff = nullptr, *gntau = nullptr, *gsteps = nullptr;
+  double *gtheta = nullptr, *gchi = nullptr, *gbrown = nullptr, *gab = nullptr;
   // in-plane ISF table (uploaded on the first in-plane fit)
   double *t2v = nullptr, *t2z = nullptr, *t2u0 = nullptr, *t2du = nullptr;
   int *t2off = nullptr, *t2n = nullptr;
@@ -43,6 +49,45 @@ void release(ddm_fitter* f) {
   f->ntau = f->steps = nullptr;
   f->chi = nullptr;
   f->cap = 0;
+}
+
+void release_global(ddm_fitter* f) {
+  void* ptrs[] = {f->gtau, f->gd, f->gw, f->gqn, f->gbounds, f->gseeds, f->gvmag, f->gboff, f->gntau, f->gsteps,
+                  f->gtheta, f->gchi, f->gbrown, f->gab};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  f->gtau = f->gd = f->gw = f->gqn = f->gbounds = f->gseeds = f->gvmag = nullptr;
+  f->gboff = f->gntau = f->gsteps = nullptr;
+  f->gtheta = f->gchi = f->gbrown = f->gab = nullptr;
+  f->gcap_vid = f->gcap_bins = 0;
+}
+
+hipError_t grow_global(ddm_fitter* f, size_t nv, size_t nb) {
+  if (nv <= f->gcap_vid && nb <= f->gcap_bins) return hipSuccess;
+  release_global(f);
+  nv = std::max(nv, (size_t)1);
+  nb = std::max(nb, (size_t)1);
+  hipError_t e;
+  if ((e = hipMalloc((void**)&f->gtau, nv * kMaxT * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gd, nb * kMaxT * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gw, nb * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gqn, nb * kMaxK * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gbounds, nv * 16)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gseeds, nv * 16)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gvmag, nv * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gboff, (nv + 1) * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gntau, nv * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gsteps, nv * 4)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gtheta, nv * 32)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gchi, nv * 8)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gbrown, nv * 16)) != hipSuccess ||
+      (e = hipMalloc((void**)&f->gab, nb * 16)) != hipSuccess) {
+    release_global(f);
+    return e;
+  }
+  f->gcap_vid = nv;
+  f->gcap_bins = nb;
+  return hipSuccess;
 }
 
 void free_isf2d(ddm_fitter* f) {
@@ -121,6 +166,7 @@ void ddm_fitter_destroy(ddm_fitter* f) {
   if (!f) return;
   hipStreamSynchronize(f->stream);
   release(f);
+  release_global(f);
   free_isf2d(f);
   hipEventDestroy(f->ev0);
   hipEventDestroy(f->ev1);
@@ -196,6 +242,120 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
     return ffail(f, DDM_EHIP, std::string("fit: ") + hipGetErrorString(e));
   for (size_t i = 0; i < n * 6; i++) theta[i] = th[i];
   for (size_t i = 0; i < n; i++) iters[i] = st[i];
+  if (kernel_ms) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, f->ev0, f->ev1);
+    *kernel_ms = ms;
+  }
+  return DDM_OK;
+}
+
+int ddm_fit_global(ddm_fitter* f, const ddm_global_fit_batch* b, double* theta, double* rss, int32_t* iters,
+                   double* brown, double* amps, double* kernel_ms) {
+  if (!f || !b) return DDM_EINVAL;
+  if (b->nvideos < 0) return ffail(f, DDM_EINVAL, "global fit: negative batch size");
+  if (b->nvideos == 0) return DDM_OK;
+  if (b->ntau_max < 1 || b->ntau_max > kMaxT) return ffail(f, DDM_EINVAL, "global fit: ntau_max must be in [1, 128]");
+  if (b->knodes < 1 || b->knodes > kMaxK) return ffail(f, DDM_EINVAL, "global fit: knodes must be in [1, 8]");
+  if (b->model != DDM_MODEL_BROWNIAN && b->model != DDM_MODEL_SWIMMER_3D && b->model != DDM_MODEL_SWIMMER_2D)
+    return ffail(f, DDM_EINVAL, "global fit: model");
+  if (b->seed_mode != DDM_SEED_GRID && b->seed_mode != DDM_SEED_GIVEN)
+    return ffail(f, DDM_EINVAL, "global fit: seed_mode must be DDM_SEED_GRID or DDM_SEED_GIVEN");
+  if (b->seed_mode == DDM_SEED_GIVEN && !b->seeds) return ffail(f, DDM_EINVAL, "global fit: seeds required");
+  if (!b->bin_offset || !b->ntau || !b->tau || !b->data || !b->weight || !b->qnodes || !b->bounds || !theta || !rss ||
+      !iters)
+    return ffail(f, DDM_EINVAL, "global fit: null array");
+  const size_t nv = (size_t)b->nvideos;
+  if (b->bin_offset[0] != 0) return ffail(f, DDM_EINVAL, "global fit: bin_offset[0] must be 0");
+  for (size_t v = 0; v < nv; v++) {
+    const int nbv = b->bin_offset[v + 1] - b->bin_offset[v];
+    if (nbv < 1 || nbv > DDM_GLOBAL_MAX_BINS)
+      return ffail(f, DDM_EINVAL, "global fit: every video needs 1.." + std::to_string(DDM_GLOBAL_MAX_BINS) + " bins");
+    if (b->ntau[v] < 1 || b->ntau[v] > b->ntau_max) return ffail(f, DDM_EINVAL, "global fit: ntau[v] out of range");
+    const double* bd = b->bounds + 4 * v;
+    if (!(bd[0] > 0 && bd[1] > bd[0] && bd[2] > 0 && bd[3] > bd[2]))
+      return ffail(f, DDM_EINVAL, "global fit: bounds must satisfy 0 < vmin < vmax, 0 < Dmin < Dmax");
+    if (b->vmag && !(b->vmag[v] >= 0 && std::isfinite(b->vmag[v])))
+      return ffail(f, DDM_EINVAL, "global fit: vmag must be >= 0");
+  }
+  const size_t nb = (size_t)b->bin_offset[nv];
+  std::vector<float> tau(nv * kMaxT, 0.f), d(nb * kMaxT, 0.f), w(nb), qn(nb * b->knodes), bd(nv * 4), sd(nv * 4, 0.f),
+      vm(nv, 0.f);
+  std::vector<int> nt(nv), boff(nv + 1);
+  for (size_t v = 0; v < nv; v++) {
+    nt[v] = b->ntau[v];
+    for (int j = 0; j < nt[v]; j++) tau[v * kMaxT + j] = (float)b->tau[v * b->ntau_max + j];
+    for (int k = 0; k < 4; k++) bd[v * 4 + k] = (float)b->bounds[v * 4 + k];
+    if (b->seeds)
+      for (int k = 0; k < 4; k++) sd[v * 4 + k] = (float)b->seeds[v * 4 + k];
+    if (b->vmag) vm[v] = (float)b->vmag[v];
+    for (int i = b->bin_offset[v]; i < b->bin_offset[v + 1]; i++)
+      for (int j = 0; j < nt[v]; j++) d[(size_t)i * kMaxT + j] = (float)b->data[(size_t)i * b->ntau_max + j];
+  }
+  for (size_t v = 0; v <= nv; v++) boff[v] = b->bin_offset[v];
+  for (size_t i = 0; i < nb; i++) {
+    w[i] = (float)b->weight[i];
+    for (int k = 0; k < b->knodes; k++) qn[i * b->knodes + k] = (float)b->qnodes[i * b->knodes + k];
+  }
+  // small plain swimmer batches: one workgroup per (video, start), so a batch
+  // of few videos is not held up by each video's starts in sequence
+  const int ns = std::max(1, std::min(3, b->nstarts));
+  const int split = (b->seed_mode == DDM_SEED_GRID && b->model != DDM_MODEL_BROWNIAN && ns > 1 && nv * ns <= 512)
+                        ? ns : 0;
+  const size_t slots = split ? nv * split : nv;
+  hipError_t e = grow_global(f, slots, split ? nb * split : nb);
+  if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("global fit: allocation: ") + hipGetErrorString(e));
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, f->stream);
+  };
+  if ((e = up(f->gtau, tau.data(), tau.size() * 4)) != hipSuccess || (e = up(f->gd, d.data(), d.size() * 4)) != hipSuccess ||
+      (e = up(f->gw, w.data(), nb * 4)) != hipSuccess || (e = up(f->gqn, qn.data(), qn.size() * 4)) != hipSuccess ||
+      (e = up(f->gbounds, bd.data(), nv * 16)) != hipSuccess || (e = up(f->gseeds, sd.data(), nv * 16)) != hipSuccess ||
+      (e = up(f->gvmag, vm.data(), nv * 4)) != hipSuccess || (e = up(f->gboff, boff.data(), (nv + 1) * 4)) != hipSuccess ||
+      (e = up(f->gntau, nt.data(), nv * 4)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("global fit: upload: ") + hipGetErrorString(e));
+  if (b->model == DDM_MODEL_SWIMMER_2D && !f->t2v && (e = upload_isf2d(f)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("global fit: in-plane ISF table: ") + hipGetErrorString(e));
+  hipEventRecord(f->ev0, f->stream);
+  e = ddm::launch_lm_global(f->gtau, f->gd, f->gw, f->gqn, b->knodes, f->gboff, f->gntau, f->gbounds,
+                            b->seed_mode == DDM_SEED_GIVEN ? f->gseeds : nullptr, b->vmag ? f->gvmag : nullptr,
+                            (int)nv, b->model, &f->t2d, b->seed_mode == DDM_SEED_GIVEN ? 0 : 1,
+                            ns, std::max(0, b->max_iter), split, f->gtheta, f->gchi, f->gsteps, f->gbrown, f->gab,
+                            f->stream);
+  if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("global fit: launch: ") + hipGetErrorString(e));
+  hipEventRecord(f->ev1, f->stream);
+  const size_t nab = split ? nb * split : nb;
+  std::vector<int> st(slots);
+  std::vector<double> th(slots * 4), ch(slots), ab(amps ? nab * 2 : 0), br(brown ? nv * 2 : 0);
+  if ((e = hipMemcpyAsync(th.data(), f->gtheta, slots * 32, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(ch.data(), f->gchi, slots * 8, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(st.data(), f->gsteps, slots * 4, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (amps && (e = hipMemcpyAsync(ab.data(), f->gab, nab * 16, hipMemcpyDeviceToHost, f->stream)) != hipSuccess) ||
+      (brown && (e = hipMemcpyAsync(br.data(), f->gbrown, nv * 16, hipMemcpyDeviceToHost, f->stream)) != hipSuccess) ||
+      (e = hipStreamSynchronize(f->stream)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("global fit: ") + hipGetErrorString(e));
+  for (size_t v = 0; v < nv; v++) {
+    // the best start (the first of equal ones, as the sequential loop keeps it)
+    const int per = split ? split : 1;
+    int best = 0, total = 0;
+    for (int s2 = 0; s2 < per; s2++) {
+      total += st[v * per + s2];
+      if (ch[v * per + s2] < ch[v * per + best]) best = s2;
+    }
+    const size_t sl = v * per + best;
+    for (int k = 0; k < 4; k++) theta[v * 4 + k] = th[sl * 4 + k];
+    rss[v] = ch[sl];
+    iters[v] = total;
+    if (amps) {
+      const size_t b0 = b->bin_offset[v], nbv = b->bin_offset[v + 1] - b0;
+      const size_t src = split ? 2 * (b0 * split + best * nbv) : 2 * b0;
+      std::copy(ab.begin() + src, ab.begin() + src + 2 * nbv, amps + 2 * b0);
+    }
+  }
+  if (brown) {
+    if (b->seed_mode == DDM_SEED_GRID) std::copy(br.begin(), br.end(), brown);
+    else std::fill(brown, brown + nv * 2, NAN);
+  }
   if (kernel_ms) {
     float ms = 0;
     hipEventElapsedTime(&ms, f->ev0, f->ev1);

@@ -62,6 +62,17 @@ hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, cons
                          const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
                          int* steps, hipStream_t st);
 
+// global-parameter fits (one workgroup per video); model 0 Brownian, 1 3D
+// swimmer, 2 in-plane swimmer (t2d required); mode 0 from seeds, 1 plain;
+// split > 0: that many workgroups per video, one LM start each (outputs per
+// workgroup: theta[4], chi, steps, and (A, B) of the video's bins at
+// 2 (b0 split + s nbins))
+hipError_t launch_lm_global(const float* tau, const float* d, const float* w, const float* qn, int K,
+                            const int* bin_off, const int* ntau, const float* bounds, const float* seeds,
+                            const float* vmag, int nvid, int model, const Isf2dDev* t2d, int mode, int nstarts,
+                            int maxit, int split, double* theta, double* chi, int* steps, double* brown, double* ab,
+                            hipStream_t st);
+
 // allocation-free fits (ddm_fit.hip): per-device set-up of the fixed scratch
 // array (current device), and pack -> LM -> unpack on device-resident arrays
 hipError_t noalloc_setup();

@@ -31,10 +31,20 @@ class FitBatch(C.Structure):
                 ("qnodes", C.c_void_p), ("bounds", C.c_void_p), ("seeds", C.c_void_p), ("vmag", C.c_void_p)]
 
 
+class GlobalFitBatch(C.Structure):
+    _fields_ = [("nvideos", C.c_int32), ("ntau_max", C.c_int32), ("knodes", C.c_int32), ("model", C.c_int32),
+                ("seed_mode", C.c_int32), ("nstarts", C.c_int32), ("max_iter", C.c_int32),
+                ("bin_offset", C.c_void_p), ("ntau", C.c_void_p), ("tau", C.c_void_p), ("data", C.c_void_p),
+                ("weight", C.c_void_p), ("qnodes", C.c_void_p), ("bounds", C.c_void_p), ("seeds", C.c_void_p),
+                ("vmag", C.c_void_p)]
+
+
 def _declare(L):
     if getattr(L, "_fit_declared", False):
         return L
     vp = C.c_void_p
+    L.ddm_fit_global.argtypes = [vp, C.POINTER(GlobalFitBatch), vp, vp, vp, vp, vp, vp]
+    L.ddm_fit_global.restype = C.c_int
     L.ddm_fitter_create.argtypes = [C.POINTER(vp)]
     L.ddm_fitter_create.restype = C.c_int
     L.ddm_fitter_destroy.argtypes = [vp]
@@ -210,6 +220,81 @@ class Fitter:
         if rc != 0:
             raise DdmError(self._lib.ddm_fitter_last_error(self._h).decode())
         return FitResult(th, chi, it, ms.value)
+
+
+@dataclass
+class GlobalFitResult:
+    theta: np.ndarray     # (nvideos, 4): alpha, ln vbar, ln Z, ln D shared by the video's bins
+    rss: np.ndarray       # the reference's objective sum_b w_b^2 rss_b
+    iters: np.ndarray     # LM steps (all starts)
+    brown: np.ndarray     # (nvideos, 2): D and RSS of the Brownian fit (plain seeding; NaN when seeded)
+    amps: list            # per video: (nbins, 2) amplitudes A_b, B_b at theta
+    kernel_ms: float
+
+    @property
+    def alpha(self):
+        return self.theta[:, 0]
+
+    @property
+    def vbar(self):
+        return np.exp(self.theta[:, 1])
+
+    @property
+    def Z(self):
+        return np.exp(self.theta[:, 2])
+
+    @property
+    def D(self):
+        return np.exp(self.theta[:, 3])
+
+
+def fit_global(fitter: "Fitter", prs, model=SWIMMER_3D, seeds=None, nstarts=3, max_iter=200, vmag=None
+               ) -> GlobalFitResult:
+    """Global-parameter fits of prepared records (one per video): alpha, vbar, Z, D shared by a video's bins,
+    A_b, B_b >= 0 per bin, the reference's objective (fitSwimmer / fitBrownian over evalModel,
+    src/motility_fit.js:450-474, :500-611), all videos in one launch.  seeds: (nvideos, 4) = (alpha, ln vbar,
+    ln Z, ln D) for LM from one start; None = the reference's grids.  vmag: drift speed per video."""
+    nv = len(prs)
+    if nv == 0:
+        return GlobalFitResult(np.zeros((0, 4)), np.zeros(0), np.zeros(0, np.int32), np.zeros((0, 2)), [], 0.0)
+    nbs = [len(p.bins) for p in prs]
+    if min(nbs) < 1:
+        raise DdmError("fit_global: every record needs at least one usable bin")
+    off = np.concatenate([[0], np.cumsum(nbs)]).astype(np.int32)
+    tmax = max(len(p.tau) for p in prs)
+    if tmax > MAXT:
+        raise DdmError(f"at most {MAXT} lags per fit")
+    K = max(len(b.qn) for p in prs for b in p.bins)
+    nb = int(off[-1])
+    T = np.zeros((nv, tmax))
+    Dd = np.zeros((nb, tmax))
+    w = np.zeros(nb)
+    qn = np.zeros((nb, K))
+    bd = np.zeros((nv, 4))
+    nt = np.array([len(p.tau) for p in prs], np.int32)
+    for v, p in enumerate(prs):
+        T[v, :nt[v]] = p.tau
+        bd[v] = bounds(p)
+        for i, b in enumerate(p.bins):
+            Dd[off[v] + i, :nt[v]] = b.d
+            w[off[v] + i] = b.w
+            qn[off[v] + i] = np.resize(b.qn, K)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float64).reshape(nv, 4)
+    vm = None if vmag is None else np.ascontiguousarray(np.broadcast_to(np.asarray(vmag, np.float64), (nv,)))
+    b = GlobalFitBatch(nv, tmax, K, model, SEED_GRID if sd is None else SEED_GIVEN, nstarts, max_iter,
+                       off.ctypes.data, nt.ctypes.data, T.ctypes.data, Dd.ctypes.data, w.ctypes.data, qn.ctypes.data,
+                       bd.ctypes.data, None if sd is None else sd.ctypes.data, None if vm is None else vm.ctypes.data)
+    th = np.zeros((nv, 4))
+    rss = np.zeros(nv)
+    it = np.zeros(nv, np.int32)
+    br = np.zeros((nv, 2))
+    ab = np.zeros((nb, 2))
+    ms = C.c_double()
+    rc = fitter._lib.ddm_fit_global(fitter._h, C.byref(b), th.ctypes.data, rss.ctypes.data, it.ctypes.data,
+                                    br.ctypes.data, ab.ctypes.data, C.byref(ms))
+    if rc != 0:
+        raise DdmError(fitter._lib.ddm_fitter_last_error(fitter._h).decode())
+    return GlobalFitResult(th, rss, it, br, [ab[off[v]:off[v + 1]] for v in range(nv)], ms.value)
 
 
 def problems_from(pr: Prepared):

@@ -112,11 +112,13 @@ class Problems:
     def __init__(self):
         self.tau = None
         self.data, self.w, self.qn, self.bd, self.video = [], [], [], [], []
+        self.prepared = []   # per video, for the global-parameter fits
 
     def add(self, radial, vid: int):
         pr = prepare(radial, DX_UM, FPS)
         if len(pr.bins) != NQ:
             raise DdmError(f"video {vid}: {len(pr.bins)} usable bins, expected {NQ}")
+        self.prepared.append(pr)
         self.tau = pr.tau
         bd = bounds(pr)
         K = max(len(b.qn) for b in pr.bins)
@@ -287,6 +289,55 @@ def compare(fitter: Fitter, probs: Problems, seeds: np.ndarray, truth4=None, mlp
     if truth4 is not None:
         vid = np.array(probs.video)
         t = np.asarray(truth4)[vid]
+        tp = np.stack([t[:, 0], t[:, 1] * DX_UM, t[:, 2], t[:, 3] * DX_UM ** 2 * FPS], axis=1)
+        for name, r in (("plain", pp), ("seeded", ps)):
+            err = np.abs(r - tp) / np.maximum(np.abs(tp), 1e-2)
+            out[f"{name}_param_err_median"] = dict(zip(("alpha", "vbar", "Z", "D"), map(float, np.median(err, 0))))
+    return out, plain, seeded
+
+
+def global_seeds(pred: np.ndarray) -> np.ndarray:
+    """Per-video (alpha, ln vbar, ln Z, ln D) starts of the global fit from the network's prediction."""
+    th = np.array(pred[:, :4], dtype=np.float64)
+    th[:, 0] = np.clip(th[:, 0], 0.0, 1.0)
+    return th
+
+
+def compare_global(fitter: Fitter, prs, seeds4: np.ndarray, truth4=None, mlp_ms=0.0, max_iter=200, nstarts=3):
+    """Plain grid-seeded vs surrogate-seeded global-parameter LM (alpha, vbar, Z, D shared by a video's bins,
+    A_b, B_b per bin: the reference's fitSwimmer objective) on the same videos.  A seeded fit is accepted
+    only if its RSS is at most the plain RSS (1 + 1e-6); rejected videos fall back to (and are timed as) the
+    plain fit.  One fit = one video."""
+    from .fit import fit_global
+    plain = fit_global(fitter, prs, model=SWIMMER_3D, nstarts=nstarts, max_iter=max_iter)
+    seeded = fit_global(fitter, prs, model=SWIMMER_3D, seeds=seeds4, max_iter=max_iter)
+    acc = seeded.rss <= plain.rss * (1 + ACCEPT_RTOL)
+    rej = np.nonzero(~acc)[0]
+    fb_ms = fit_global(fitter, [prs[i] for i in rej], model=SWIMMER_3D, nstarts=nstarts,
+                       max_iter=max_iter).kernel_ms if len(rej) else 0.0
+    n = len(prs)
+
+    def params(r):
+        return np.stack([r.alpha, r.vbar, r.Z, r.D], axis=1)
+
+    pp, ps = params(plain), params(seeded)
+    rel = np.abs(ps - pp) / np.maximum(np.abs(pp), 1e-3)
+    out = dict(
+        fits=n, bins_per_fit=NQ,
+        plain_kernel_ms=plain.kernel_ms, seeded_kernel_ms=seeded.kernel_ms, mlp_ms=mlp_ms, fallback_kernel_ms=fb_ms,
+        plain_fits_per_s=n / (plain.kernel_ms * 1e-3),
+        seeded_fits_per_s=n / ((seeded.kernel_ms + mlp_ms) * 1e-3),
+        seeded_with_fallback_fits_per_s=n / ((seeded.kernel_ms + mlp_ms + fb_ms) * 1e-3),
+        accept_rate=float(acc.mean()),
+        accepted_param_within_1e3=float(np.mean(rel[acc].max(axis=1) < 1e-3)) if acc.any() else None,
+        accepted_param_agreement_p50=float(np.median(rel[acc].max(axis=1))) if acc.any() else None,
+        seeded_better_rss=float(np.mean(seeded.rss < plain.rss * (1 - ACCEPT_RTOL))),
+        plain_iters_median=float(np.median(plain.iters)), seeded_iters_median=float(np.median(seeded.iters)),
+    )
+    out["speedup"] = out["seeded_fits_per_s"] / out["plain_fits_per_s"]
+    out["speedup_with_fallback"] = out["seeded_with_fallback_fits_per_s"] / out["plain_fits_per_s"]
+    if truth4 is not None:
+        t = np.asarray(truth4)
         tp = np.stack([t[:, 0], t[:, 1] * DX_UM, t[:, 2], t[:, 3] * DX_UM ** 2 * FPS], axis=1)
         for name, r in (("plain", pp), ("seeded", ps)):
             err = np.abs(r - tp) / np.maximum(np.abs(tp), 1e-2)

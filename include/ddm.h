@@ -220,6 +220,41 @@ const char *ddm_fitter_last_error(const ddm_fitter *f);
  * NULL) is the fit kernel's duration from HIP events. */
 int ddm_fit(ddm_fitter *f, const ddm_fit_batch *b, double *theta, double *chi2, int32_t *iters, double *kernel_ms);
 
+/* Global-parameter fits: the reference's model with alpha, vbar, Z and D SHARED by all bins of a video and
+ * per-bin amplitudes A_b, B_b >= 0 (fitSwimmer / fitBrownian, src/motility_fit.js:500-536, :577-611, over
+ * evalModel's variable projection :450-474), one workgroup per video, all videos in one launch.  The
+ * objective is the reference's RSS = sum_b w_b^2 min_{A_b,B_b >= 0} |d_b - A_b c_b - B_b|^2.  The reference
+ * minimises it by Nelder-Mead; here LM on (alpha, ln vbar, ln Z, ln D) with the amplitudes eliminated
+ * exactly at every step.  seed_mode DDM_SEED_GRID: the reference's seeding (49-point ln D grid refined in
+ * ln D = the Brownian fit, then the 16 vbar x 2 Z x 2 alpha start grid at that D, LM from the best
+ * nstarts); DDM_SEED_GIVEN: LM from seeds[4 v ..] = (alpha, ln vbar, ln Z, ln D).  model
+ * DDM_MODEL_BROWNIAN fits D alone. */
+enum { DDM_GLOBAL_MAX_BINS = 1024 };
+typedef struct ddm_global_fit_batch {
+  int32_t nvideos;
+  int32_t ntau_max;         /* row stride of tau/data, <= 128 */
+  int32_t knodes;           /* q nodes per bin, <= 8 */
+  int32_t model;            /* DDM_MODEL_* */
+  int32_t seed_mode;        /* DDM_SEED_GRID or DDM_SEED_GIVEN */
+  int32_t nstarts;          /* 1..3 */
+  int32_t max_iter;         /* LM steps per start */
+  const int32_t *bin_offset; /* [nvideos + 1]: video v owns bins bin_offset[v] .. bin_offset[v+1]-1
+                               (1..DDM_GLOBAL_MAX_BINS of them) */
+  const int32_t *ntau;      /* [nvideos] lags of the video's bins (the same lag grid for all of them) */
+  const double *tau;        /* [nvideos * ntau_max] lag times (s) */
+  const double *data;       /* [nbins * ntau_max] D_b(tau) */
+  const double *weight;     /* [nbins] w_b = 1 / max_tau D_b */
+  const double *qnodes;     /* [nbins * knodes] */
+  const double *bounds;     /* [nvideos * 4] vmin, vmax, Dmin, Dmax (bounds, :480-491) */
+  const double *seeds;      /* [nvideos * 4] (DDM_SEED_GIVEN), else NULL */
+  const double *vmag;       /* [nvideos] uniform drift speed in the model, or NULL */
+} ddm_global_fit_batch;
+/* theta[nvideos * 4] = (alpha, ln vbar, ln Z, ln D); rss[nvideos]; iters[nvideos] (LM steps); brown
+ * [nvideos * 2] = (D, RSS) of the Brownian fit (DDM_SEED_GRID only; may be NULL); amps[nbins * 2] = (A_b,
+ * B_b) at theta (may be NULL); *kernel_ms (may be NULL) from HIP events.  Blocks until done. */
+int ddm_fit_global(ddm_fitter *f, const ddm_global_fit_batch *b, double *theta, double *rss, int32_t *iters,
+                   double *brown, double *amps, double *kernel_ms);
+
 /* ddm_fit on device-resident arrays, allocating nothing.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -87,6 +87,29 @@ def test_trained_network_gains_over_plain_lm(gpu):
 
 
 @pytest.mark.gpu
+def test_trained_network_seeds_the_global_fit(gpu):
+    """The network's per-video prediction as the start of the global-parameter fit (alpha, vbar, Z, D shared
+    by the video's 32 bins, the reference's fitSwimmer objective) on 1024 held-out videos: a seeded fit is
+    accepted only at RSS <= plain RSS (1 + 1e-6), rejected videos are refitted plain and that time is
+    charged to the seeded side.  The seeded LM takes a third of the plain LM's steps, nearly every seeded
+    fit is accepted, accepted fits agree with plain to 1e-5 (median), and fits/s with the fallback charged
+    beat the plain grid-seeded LM."""
+    from hackathon_amd.fit import Fitter
+    sim, fitter = S.Simulator(), Fitter()
+    X, Y, truth, probs = S.build_dataset(1024, S.HELDOUT_SEED0 + 9999, sim, fitter, keep_problems=True,
+                                         log=lambda m: None)
+    sur = S.Surrogate()
+    pred, mlp_ms = S.timed_predict(sur, X)
+    res, plain, seeded = S.compare_global(fitter, probs.prepared, S.global_seeds(pred), truth, mlp_ms)
+    print({k: v for k, v in res.items() if not isinstance(v, dict)})
+    assert res["seeded_iters_median"] <= 0.4 * res["plain_iters_median"]
+    assert res["accept_rate"] > 0.95
+    assert res["accepted_param_agreement_p50"] < 1e-4
+    assert res["speedup"] > 3.0
+    assert res["speedup_with_fallback"] > 2.0
+
+
+@pytest.mark.gpu
 def test_short_training_lowers_validation_loss(gpu):
     from hackathon_amd.fit import Fitter
     sim, fitter = S.Simulator(), Fitter()
