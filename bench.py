@@ -14,13 +14,17 @@ Prints one JSON line.  value = frame-pixels/s through to D(q, tau).
 `roofline` is for the dominant kernel -- the longest launch, k_temporal_w
 (the row transform takes about as long per step, in ten short launches; every
 kernel's algorithmic GB/s is listed under pipeline.kernels_serial) -- timed
-live with HIP events on the engine's stream; `cpu_baseline` is the FP64 oracle (one thread) on a bounded
-slice of the same workload.  `fits` is the second half of the metric: the
-global-parameter fit of each of 4096 held-out simulated videos (256x256x512,
-32 bins each; alpha, vbar, Z, D shared by a video's bins), plain grid-seeded
-LM against surrogate-seeded LM, with the acceptance rule RSS_seeded <=
-RSS_plain (1 + 1e-6) and a plain refit of rejected videos charged to the
-seeded side; the per-q comparison on the same videos beside it.
+live with HIP events on the engine's stream, on the binned modes' spectra it
+must read (its `limit` says what sets its time).  `pipeline.ring` times the
+live-ring case (rotated order, guard mask) beside the identity order.
+`cpu_baseline` is the reference's CPU path restated in JavaScript under Node
+(one thread and over worker_threads) on config 1 and a 64-frame slice of this
+workload.  `fits` is the second half of the metric: the global-parameter fit
+of each of 4096 held-out simulated videos (256x256x512, 32 bins each; alpha,
+vbar, Z, D shared by a video's bins), plain grid-seeded LM against
+surrogate-seeded LM, with the acceptance rule RSS_seeded <= RSS_plain
+(1 + 1e-6) and a plain refit of rejected videos charged to the seeded side;
+the per-q comparison on the same videos beside it.
 """
 from __future__ import annotations
 
@@ -33,6 +37,11 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# what sets each kernel's time, from its counters (rocprofv3 --pmc summaries under profiles/): the
+# temporal kernel's HBM traffic is ~1.06x its algorithmic bytes, and its time is FP32 VALU issue
+LIMIT = {"temporal_wk": "VALU issue (FP32): counter traffic ~1.06x the algorithmic bytes, duration set by "
+                        "instruction issue, not by HBM",
+         "spatial_rows": "HBM", "spatial_cols_cornerturn": "HBM", "radial_finalize": "latency"}
 METRIC = "frame-pixels/s to D(q,τ) (%HBM roofline); fits/s seeded vs plain LM @equal χ²"
 
 
@@ -45,7 +54,7 @@ def parse():
     p.add_argument("--size", type=int, default=512)
     p.add_argument("--bins", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-frames", type=int, default=256, help="frames in the CPU-baseline slice")
+    p.add_argument("--cpu-frames", type=int, default=64, help="frames in the CPU-baseline slice")
     p.add_argument("--fit-videos", type=int, default=4096, help="held-out videos for the fits/s comparison")
     p.add_argument("--no-fits", action="store_true")
     p.add_argument("--no-batch", action="store_true")
@@ -135,23 +144,55 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 
 def cpu_baseline(N, frames, nbins):
-    """FP64 oracle (C, one thread): History.ingest + analyzeRadial on a slice of the workload."""
+    """The reference's CPU path (History.ingest -> analyzeRadial, restated in JavaScript:
+    oracle/cpu_path.mjs) under Node on the host cores: C1 in full (64 x 128^2 u8, 64 bins) and a slice of
+    this workload (the first `frames` frames of N^2 u16 with T = L = frames, `nbins` bins), each on one
+    thread and sharded over worker_threads; the FP64 C oracle's fitBrownian + fitSwimmer of the C1 record
+    (one thread).  value = the slice's frame-pixels/s on all threads used."""
+    import platform
     import oracle as O
     from hackathon_amd import synth
+    from hackathon_amd.engine import Radial
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+    cpu = platform.processor() or ""
+    try:
+        cpu = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    strip = lambda r: {k: v for k, v in r.items() if k not in ("D", "P", "drift", "node", "cpu", "cpus")}
+    c1 = synth.config1()
+    b1 = O.make_bins(128, 64, 0, 64)
+    runs = {"c1_1": strip(O.cpu_path(c1, b1, 64, threads=1)),
+            f"c1_{threads}": strip(O.cpu_path(c1, b1, 64, threads=threads))}
     c2 = dict(synth.C2)
     c2.update(T=frames, N=N)
     fr = synth.video_numpy(dtype=np.uint16, seed=2, **c2)
-    T = L = frames
-    bins = O.make_bins(N, nbins, 0, N // 2, with_nodes=False)
+    b2 = O.make_bins(N, nbins, 0, N // 2, with_nodes=False)
+    one = O.cpu_path(fr, b2, frames, threads=1)
+    many = O.cpu_path(fr, b2, frames, threads=threads)
+    runs["c2_slice_1"], runs[f"c2_slice_{threads}"] = strip(one), strip(many)
+    # full C2 (1024 frames, lags to 1023): ingest scales with the frame count, the temporal stage with the
+    # per-mode transform length n log n (n = 2T)
+    T2 = 1024
+    lg = lambda t: (2 * t) * np.log2(2 * t)
+    ext = {k: r["ingest_s"] * T2 / frames + r["temporal_s"] * lg(T2) / lg(frames)
+           for k, r in (("1", one), (str(threads), many))}
+    # the reference's fits (fitBrownian + fitSwimmer, FP64 C oracle) on the C1 record
+    hist = O.history(c1, 64)
+    D, P = O.analyze_radial(hist, np.arange(64), np.ones(64, np.float32), b1, 64)
+    rec = dict(N=128, NB=64, L=64, D=D.ravel(), nModes=b1["nModes"], kCenter=b1["kCenter"], kNodes=b1["kNodes"])
     t0 = time.perf_counter()
-    hist = O.history(fr, T)
-    t1 = time.perf_counter()
-    O.analyze_radial(hist, np.arange(T), np.ones(T, np.float32), bins, L)
-    t2 = time.perf_counter()
-    fpx = T * N * N
-    return {"value": fpx / (t2 - t0), "unit": "frame-pixels/s", "cores": 1, "kind": "port",
-            "sample": f"{T} frames of {N}x{N} u16 (same generator, T=L={T}, {nbins} bins), FP64 C oracle, "
-                      f"ingest {t1 - t0:.2f} s + temporal/radial {t2 - t1:.2f} s"}
+    pr = O.prepare(rec, 1.0, 100.0)
+    Dd, _ = O.fit_brownian(pr)
+    O.fit_swimmer(pr, "3d", 0.0, Dd)
+    fit_s = time.perf_counter() - t0
+    return {"value": many["frame_pixels_per_s"], "unit": "frame-pixels/s", "cores": threads, "kind": "port",
+            "sample": f"the reference's CPU path restated in JavaScript (oracle/cpu_path.mjs, Node "
+                      f"{one['node']}, FP64 radix-2 Stockham, packed z = m + i p WK, drift sums) on {frames} "
+                      f"frames of {N}x{N} u16 (same generator, T=L={frames}, {nbins} bins) over {threads} "
+                      f"worker_threads; one thread: {one['frame_pixels_per_s']:.3g} frame-pixels/s",
+            "cpu_model": cpu, "host_cpus": os.cpu_count(), "runs": runs,
+            "c2_full_extrapolated_s": ext, "c1_fit_s": fit_s, "c1_fits_per_s": 1.0 / fit_s}
 
 
 def main():
@@ -215,17 +256,41 @@ def main():
         r2 = e.readRadial()
         assert np.array_equal(r2.D, r.D) and np.array_equal(r2.P, r.P), "in-flight engines disagree"
     ms_step = (t1 - t0) * 1e3 / a.steps
+    # the live ring: the write head mid-ring (a rotated order, so the temporal kernel gathers) and a
+    # guard mask over the 4 slots nearest it (masked mean, masked pair counts and tail sums)
+    ring_order = ((np.arange(T) + T // 3) % T).astype(np.uint32)
+    ring_mask = np.ones(T, np.float32)
+    ring_mask[-4:] = 0
+
+    def ring_step(e):
+        e.ingestDevice(frames.data_ptr(), np.uint16, T, 0)
+        e.computeTemporal(ring_order, ring_mask)
+
+    for i in range(a.inflight):
+        ring_step(engs[i])
+    for e in engs:
+        e.idle()
+    tr0 = time.perf_counter()
+    for i in range(a.steps):
+        ring_step(engs[i % a.inflight])
+    for e in engs:
+        e.idle()
+    ring_ms = (time.perf_counter() - tr0) * 1e3 / a.steps
+    assert np.all(np.isfinite(engs[0].readRadial().D)), "non-finite D (ring)"
     fpx = T * N * N
     value = fpx / (ms_step * 1e-3)
 
-    # Algorithmic bytes per frame-pixel: the raw frame read once plus the f32
-    # complex half-plane spectrum written once and read once.  Per launch:
+    # Algorithmic bytes: the raw frames read once, the f32 complex half-plane
+    # spectrum of every mode written once (the history holds all of them, as
+    # readHistory returns), and the binned modes' spectra read once by the
+    # temporal stage (modes outside every bin are never needed).  Per launch:
     # spatial rows = frames read, spatial cols = spectrum write, temporal =
-    # spectrum read (all modes, by definition; it reads only the binned ones).
+    # binned spectrum read.
     NH = N // 2 + 1
     spec_bytes = T * N * NH * 8
+    binned_bytes = T * int(np.sum(r.nModes)) * 8
     raw_bytes = T * N * N * 2
-    per_step_algo = {"spatial_rows": raw_bytes, "spatial_cols_cornerturn": spec_bytes, "temporal_wk": spec_bytes,
+    per_step_algo = {"spatial_rows": raw_bytes, "spatial_cols_cornerturn": spec_bytes, "temporal_wk": binned_bytes,
                      "radial_finalize": 0}
     cols_f = engs[0].tuning()["cols_f"]
     kernel_symbol = {"spatial_rows": "ddm::k_rows_w<9, unsigned short>",
@@ -257,7 +322,8 @@ def main():
             "spatial_cols_cornerturn": "k_cols_w (column FFT + corner turn into the history)",
             "spatial_rows": "k_rows_w (decode, row-mean removal, window, R2C row FFT)",
             "radial_finalize": "k_bin_sum + k_bin_ifft + k_bin_lags(_full)"}[dom]
-    pipeline_gbs = (raw_bytes + 2 * spec_bytes) / (ms_step * 1e-3) / 1e9
+    pipe_bytes = raw_bytes + spec_bytes + binned_bytes
+    pipeline_gbs = pipe_bytes / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -277,10 +343,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": algo_launch,
-                     "avg_launch_ms": per_kernel[dom]["avg_launch_ms"]},
-        "pipeline": {"algorithmic_bytes": raw_bytes + 2 * spec_bytes, "GBs": pipeline_gbs,
+                     "avg_launch_ms": per_kernel[dom]["avg_launch_ms"],
+                     "limit": LIMIT.get(dom)},
+        "pipeline": {"algorithmic_bytes": pipe_bytes, "GBs": pipeline_gbs,
                      "frac_of_hbm_peak": pipeline_gbs / HBM_PEAK_GBS, "inflight": a.inflight,
                      "serial_ms_per_step": serial_ms,
+                     "ring": {"ms_per_step": ring_ms, "frame_pixels_per_s": fpx / (ring_ms * 1e-3),
+                              "order": f"rotated by {T // 3}", "guard": 4},
                      "serial_frame_pixels_per_s": fpx / (serial_ms * 1e-3),
                      "kernels_serial": per_kernel},
     }

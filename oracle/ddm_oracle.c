@@ -1413,9 +1413,35 @@ double orc_lm_fit(int ntau, const double *tau, const double *d, double w, int K,
 /* Plain grid-seeded per-q LM: the 64 grid starts of orc_lm_seed are ranked by
  * chi^2 at NNLS amplitudes, LM runs from the best `nstarts`, the lowest final
  * chi^2 wins.  out6 = theta; returns chi^2; *iters = total LM steps. */
+/* The start grid of orc_lm_plain, ranked: starts[s*6..] (A, B, alpha, ln v, ln Z, ln D) in increasing
+ * grid chi^2 (sf[s]); returns the number of starts (64 for the swimmer, 1 for Brownian). */
+int orc_lm_starts(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                  const double *bounds4, int swim, int geometry, double vmag, double *starts_out, double *sf_out);
+
 double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
                     const double *bounds4, int swim, int geometry, double vmag, int nstarts, int maxit,
                     double *out6, int *iters) {
+  double starts[64 * 6], sf[64];
+  int ns = orc_lm_starts(ntau, tau, d, w, K, qn, bounds4, swim, geometry, vmag, starts, sf);
+  double best = INFINITY;
+  int tot = 0;
+  for (int s = 0; s < (nstarts < ns ? nstarts : ns); s++) {
+    double th[6];
+    int it;
+    memcpy(th, starts + 6 * s, sizeof(th));
+    double chi = orc_lm_fit(ntau, tau, d, w, K, qn, bounds4, swim, geometry, vmag, maxit, th, &it);
+    tot += it;
+    if (chi < best) {
+      best = chi;
+      memcpy(out6, th, sizeof(th));
+    }
+  }
+  if (iters) *iters = tot;
+  return best;
+}
+
+int orc_lm_starts(int ntau, const double *tau, const double *d, double w, int K, const double *qn,
+                  const double *bounds4, int swim, int geometry, double vmag, double *starts_out, double *sf_out) {
   double c[128];
   double starts[64][6], sf[64];
   int ns = 0;
@@ -1451,21 +1477,11 @@ double orc_lm_plain(int ntau, const double *tau, const double *d, double w, int 
     }
     idx[k + 1] = v;
   }
-  double best = INFINITY;
-  int tot = 0;
-  for (int s = 0; s < (nstarts < ns ? nstarts : ns); s++) {
-    double th[6];
-    int it;
-    memcpy(th, starts[idx[s]], sizeof(th));
-    double chi = orc_lm_fit(ntau, tau, d, w, K, qn, bounds4, swim, geometry, vmag, maxit, th, &it);
-    tot += it;
-    if (chi < best) {
-      best = chi;
-      memcpy(out6, th, sizeof(th));
-    }
+  for (int s = 0; s < ns; s++) {
+    memcpy(starts_out + 6 * s, starts[idx[s]], sizeof(double) * 6);
+    sf_out[s] = sf[idx[s]];
   }
-  if (iters) *iters = tot;
-  return best;
+  return ns;
 }
 
 /* ---------------------------------------------------------------- scenario simulator

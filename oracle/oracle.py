@@ -410,6 +410,23 @@ def lm_plain(tau, d, w, qn, bounds4, swim=True, geometry="3d", nstarts=3, maxit=
     return th, chi, it.value
 
 
+def lm_starts(tau, d, w, qn, bounds4, swim=True, geometry="3d", vmag=0.0):
+    """orc_lm_plain's start grid ranked by grid chi2 -> (starts (n, 6), chi2 (n,))."""
+    L = _fit_decls()
+    if not hasattr(L, "_starts"):
+        L.orc_lm_starts.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                    C.c_double, _dp, _dp]
+        L.orc_lm_starts.restype = C.c_int
+        L._starts = True
+    st = np.zeros(64 * 6)
+    sf = np.zeros(64)
+    n = L.orc_lm_starts(len(tau), np.ascontiguousarray(tau, dtype=np.float64), np.ascontiguousarray(d, dtype=np.float64),
+                        w, len(qn), np.ascontiguousarray(qn, dtype=np.float64),
+                        np.ascontiguousarray(bounds4, dtype=np.float64), int(swim), GEOMETRY[geometry], float(vmag),
+                        st, sf)
+    return st[:6 * n].reshape(n, 6), sf[:n]
+
+
 # ------------------------------------------------------------------ scenario simulator
 
 class SimParams(C.Structure):
@@ -438,3 +455,29 @@ def simulate(sim: dict):
     valid = np.zeros(p.T, dtype=np.uint8)
     ms = L.orc_simulate(C.byref(p), frames.ctypes.data, valid.ctypes.data)
     return frames, valid.astype(bool), ms
+
+
+# ------------------------------------------------------------------ Node CPU path (baseline)
+
+def cpu_path(frames: np.ndarray, bins: dict, L: int, threads: int = 1, timeout: float = 600.0) -> dict:
+    """Run oracle/cpu_path.mjs (the reference's CPU path restated in JavaScript) under Node on `frames`
+    (T x N x N, uint8 or uint16; frame t into slot t, every frame valid) with `threads` threads.  Returns
+    its timings (JSON) plus D[NB, L], P[NB] and drift[NB, L, 8]."""
+    import json
+    import tempfile
+    T, N = frames.shape[0], frames.shape[1]
+    NB = bins["NB"]
+    dt = {np.dtype(np.uint8): "u8", np.dtype(np.uint16): "u16"}[frames.dtype]
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "spec.json"), "w") as f:
+            json.dump(dict(N=N, T=T, L=L, NB=NB, dtype=dt, threads=int(threads)), f)
+        np.ascontiguousarray(frames).tofile(os.path.join(d, "frames.bin"))
+        np.ascontiguousarray(bins["binOffsets"], np.uint32).tofile(os.path.join(d, "binOffsets.bin"))
+        np.ascontiguousarray(bins["binModes"], np.int32).ravel().tofile(os.path.join(d, "binModes.bin"))
+        p = subprocess.run(["node", os.path.join(_HERE, "cpu_path.mjs"), d], check=True, capture_output=True,
+                           text=True, timeout=timeout)
+        out = json.loads(p.stdout.strip().splitlines()[-1])
+        out["D"] = np.fromfile(os.path.join(d, "D.bin")).reshape(NB, L)
+        out["P"] = np.fromfile(os.path.join(d, "P.bin"))
+        out["drift"] = np.fromfile(os.path.join(d, "drift.bin")).reshape(NB, L, 8)
+    return out
