@@ -285,17 +285,16 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
 // compared, so LM accept/reject and termination decisions are not driven by
 // FP32 rounding of the model.
 template <int GEO>
-__device__ __noinline__ double chi2_at(const float* th, const float* tv, const float* dv, int nt, float w, const float* q,
+__device__ __noinline__ double chi2_at(const double* th, const float* tv, const float* dv, int nt, float w, const float* q,
                                        int K, float V, const Isf2dDev& T, bool swim, int lane) {
   double s = 0;
-  const double alpha = swim ? th[2] : 0.0, vbar = exp((double)th[3]), Z = exp((double)th[4]),
-               Dd = exp((double)th[5]);
+  const double alpha = swim ? th[2] : 0.0, vbar = exp(th[3]), Z = exp(th[4]), Dd = exp(th[5]);
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
       const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim);
-      const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
+      const double res = (double)w * ((double)dv[r] - (th[0] * c + th[1]));
       s += res * res;
     }
   }
@@ -339,17 +338,19 @@ __device__ double nnls2(double g11, double g12, double n, double h1, double h2, 
   return best > 0 ? best : 0;
 }
 
+// One start-grid point: the model column in FP64 (as the objective), so the
+// grid's ranking -- which starts the LM runs from -- is not decided by FP32
+// rounding of near-equal grid chi^2 values.
 template <int GEO>
-__device__ double grid_point(const float* tv, const float* dv, int nt, const float* q, int K, float alpha, float vbar,
-                             float Z, float Dd, float V, const Isf2dDev& T, bool swim, int lane, float& A, float& B) {
+__device__ double grid_point(const float* tv, const float* dv, int nt, const float* q, int K, double alpha, double vbar,
+                             double Z, double Dd, float V, const Isf2dDev& T, bool swim, int lane, float& A, float& B) {
   double g11 = 0, g12 = 0, h1 = 0, h2 = 0, dd = 0;
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
     if (j < nt) {
-      float c, a, b, e, f;
-      column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, false, c, a, b, e, f);
-      g11 += (double)c * c;
+      const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim);
+      g11 += c * c;
       g12 += c;
       h1 += (double)c * dv[r];
       h2 += dv[r];
@@ -367,15 +368,18 @@ __device__ double grid_point(const float* tv, const float* dv, int nt, const flo
 }
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+__device__ __forceinline__ double clampd(double x, float lo, float hi) {
+  return fmin((double)hi, fmax((double)lo, x));
+}
 
 // LM from theta (in/out); returns chi^2, adds the steps taken to *steps.
 template <bool SWIM, int GEO>
-__device__ double lm_run(float* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
+__device__ double lm_run(double* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
                          const float* q, int K, float V, const Isf2dDev& T, int maxit, int lane, int* steps) {
   constexpr bool swim = SWIM;
   constexpr int p = SWIM ? 6 : 3;
   constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
-  for (int k = 0; k < 6; k++) th[k] = clampf(th[k], lo[k], hi[k]);
+  for (int k = 0; k < 6; k++) th[k] = clampd(th[k], lo[k], hi[k]);
   double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane);
   double lam = 1e-3, nu = 2;
   int it = 0;
@@ -386,7 +390,10 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
     for (int k = 0; k < 21; k++) H[k] = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) g[k] = 0;
-    const float alpha = swim ? th[2] : 0.f, vbar = expf(th[3]), Z = expf(th[4]), Dd = expf(th[5]);
+    // the Jacobian in FP32 at the FP32-rounded parameters (theta itself stays FP64, so steps below
+    // an FP32 ulp of a parameter are kept and the FP64 objective sees them)
+    const float alpha = swim ? (float)th[2] : 0.f, vbar = expf((float)th[3]), Z = expf((float)th[4]),
+                Dd = expf((float)th[5]);
 #pragma unroll
     for (int r = 0; r < MAXT / 64; r++) {
       const int j = lane + 64 * r;
@@ -490,15 +497,15 @@ __device__ double lm_run(float* th, const float* lo, const float* hi, const floa
           for (int k = a + 1; k < p; k++) s -= Lc[k * (k + 1) / 2 + a] * x[k];
           x[a] = s / Lc[a * (a + 1) / 2 + a];
         }
-        float nth[6];
+        double nth[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) nth[k] = th[k];
         double pred = 0;
 #pragma unroll
         for (int a = 0; a < p; a++) {
           const int k = idx[a];
-          nth[k] = clampf((float)(th[k] + x[a]), lo[k], hi[k]);
-          const double dk = (double)nth[k] - th[k];
+          nth[k] = clampd(th[k] + x[a], lo[k], hi[k]);
+          const double dk = nth[k] - th[k];
           pred += dk * (lam * diag[a] * dk + g[a]);
         }
         const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane);
@@ -572,19 +579,22 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
 #pragma unroll
       for (int k = 3; k < 6; k++) st[0][k] = fminf(hi[k], fmaxf(lo[k], st[0][k]));
       float A, B;
-      grid_point<GEO>(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.f, expf(st[0][3]), expf(st[0][4]), expf(st[0][5]), V,
-                      in.t2d, SWIM, lane, A, B);
+      grid_point<GEO>(tv, dv, nt, q, K, SWIM ? st[0][2] : 0.0, exp((double)st[0][3]), exp((double)st[0][4]),
+                      exp((double)st[0][5]), V, in.t2d, SWIM, lane, A, B);
       st[0][0] = A;
       st[0][1] = B;
     }
   } else {
-    // Brownian 49-point log-D grid
-    float bA = 0, bB = 0, blD = lo[5];
+    // Brownian 49-point log-D grid (grid coordinates in FP64, as the ranking)
+    const double lDmin = log((double)Dmin), lDmax = log((double)Dmax), lvmin = log((double)vmin),
+                 lvmax = log((double)vmax);
+    float bA = 0, bB = 0;
+    double blD = lDmin;
     double bchi = 1e300;
     for (int i = 0; i <= 48; i++) {
-      const float lD = lo[5] + ((hi[5] - lo[5]) * i) / 48.f;
+      const double lD = lDmin + ((lDmax - lDmin) * i) / 48;
       float A, B;
-      const double r = grid_point<GEO>(tv, dv, nt, q, K, 0.f, 1.f, 1.f, expf(lD), V, in.t2d, false, lane, A, B);
+      const double r = grid_point<GEO>(tv, dv, nt, q, K, 0.0, 1.0, 1.0, exp(lD), V, in.t2d, false, lane, A, B);
       if (r < bchi) {
         bchi = r;
         bA = A;
@@ -598,24 +608,24 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
       st[0][2] = 0.f;
       st[0][3] = 0.5f * (lo[3] + hi[3]);
       st[0][4] = logf(4.f);
-      st[0][5] = blD;
+      st[0][5] = (float)blD;
     } else {
       double sf[3] = {1e300, 1e300, 1e300};
-      const float lZs[2] = {logf(1.5f), logf(8.f)}, as[2] = {0.3f, 0.8f};
+      const double lZs[2] = {log(1.5), log(8.0)}, as[2] = {0.3, 0.8};
       for (int i = 0; i < 16; i++) {
-        const float lv = lo[3] + ((hi[3] - lo[3]) * (i + 0.5f)) / 16.f;
+        const double lv = lvmin + ((lvmax - lvmin) * (i + 0.5)) / 16;
 #pragma unroll
         for (int z = 0; z < 2; z++)
 #pragma unroll
           for (int a = 0; a < 2; a++) {
             float A, B;
             const double r =
-                grid_point<GEO>(tv, dv, nt, q, K, as[a], expf(lv), expf(lZs[z]), expf(blD), V, in.t2d, true, lane, A,
+                grid_point<GEO>(tv, dv, nt, q, K, as[a], exp(lv), exp(lZs[z]), exp(blD), V, in.t2d, true, lane, A,
                                 B) *
                 w * w;
             // keep the best 3 (stable: earlier grid points win ties); an
             // insertion step with static register indices
-            float cand[6] = {A, B, as[a], lv, lZs[z], blD};
+            float cand[6] = {A, B, (float)as[a], (float)lv, (float)lZs[z], (float)blD};
             double cr = r;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -639,14 +649,14 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
   float best[6];
   double bestchi = 1e300;
   for (int s = 0; s < nrun; s++) {
-    float th[6];
+    double th[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
     const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps);
     if (chi < bestchi) {
       bestchi = chi;
 #pragma unroll
-      for (int k = 0; k < 6; k++) best[k] = th[k];
+      for (int k = 0; k < 6; k++) best[k] = (float)th[k];
     }
   }
   if (lane == 0) {

@@ -76,6 +76,13 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise DdmError(f"libddm.so is not built ({path}); run python -m hackathon_amd.build")
+    # torch's HIP runtime first: torch loads its own libamdhip64 by file name, so a copy of the runtime
+    # loaded earlier through libddm's dependency would leave two runtimes in the process and torch without
+    # a device; loaded after torch's, libddm's dependency resolves to torch's copy (same soname)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     vp = C.c_void_p
     L.ddm_create.argtypes = [C.POINTER(Geometry), C.POINTER(vp)]

@@ -72,9 +72,10 @@ def test_gpu_swimmer_noisy_matches_oracle(gpu, geometry):
     print("bins agreeing with the oracle (chi2 1e-4, params 1e-3):", agree, "of", len(rows))
     for row in rows:
         print("bin %d chi2 %.3e rel_chi %.2e rel_p %.2e" % row)
-    # never a clearly worse minimum than the FP64 oracle from the same starts
+    # never a clearly worse minimum than the FP64 oracle from the same starts, and at most one bin of 32
+    # (a near-tie of grid starts, or a flat direction) off the oracle's chi2 (1e-4) or parameters (1e-3)
     assert all(r[2] < 1e-3 for r in rows)
-    assert agree >= 0.75 * len(rows)
+    assert agree >= len(rows) - 1
 
 
 @pytest.mark.parametrize("geometry", ["3d", "2d"])
@@ -130,12 +131,18 @@ def test_gpu_drift_noisy_matches_oracle(gpu, swim, geometry, model):
         probs.append((tau, d, 1 / d.max(), bd))
     res = F.Fitter().fit([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], qs[:, None],
                          np.array([p[3] for p in probs]), model=model, vmag=4.0)
-    worse = []
+    worse, agree = [], 0
     for i, q in enumerate(qs):
         tau, d, w, bd = probs[i]
         th, chi, _ = O.lm_plain(tau, d, w, [q], bd, swim, geometry, vmag=4.0)
         rel = (res.chi2[i] - chi) / chi
-        print(f"q {q:.2f}: GPU chi2 {res.chi2[i]:.4e} oracle {chi:.4e} rel {rel:+.1e}")
+        p_gpu = np.array([res.alpha[i], res.vbar[i], res.Z[i], res.D[i]] if swim else [res.D[i]])
+        p_orc = np.exp(th[5:6]) if not swim else np.array([th[2], np.exp(th[3]), np.exp(th[4]), np.exp(th[5])])
+        rel_p = np.max(np.abs(p_gpu - p_orc) / np.maximum(np.abs(p_orc), 1e-3))
+        print(f"q {q:.2f}: GPU chi2 {res.chi2[i]:.4e} oracle {chi:.4e} rel {rel:+.1e} params {rel_p:.1e}")
         if rel > 1e-3:
             worse.append((q, rel))
+        agree += abs(rel) < 1e-4 and rel_p < 1e-3
+    print(f"fits agreeing with the oracle (chi2 1e-4, params 1e-3): {agree} of {len(qs)}")
     assert worse == []
+    assert agree >= len(qs) - 2
