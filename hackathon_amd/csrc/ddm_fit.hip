@@ -23,6 +23,7 @@
 #include "../../include/ddm.h"
 #include "ddm_internal.h"
 #include "ddm_fft.h"
+#include "ddm_isf2d.h"
 
 namespace ddm {
 
@@ -666,6 +667,96 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
   }
 }
 
+// ----------------------------------------------------------------- in-plane ISF, FP64 quadrature
+//
+// The Schulz-averaged in-plane swimmer ISF g(x; Z) = P_Z(1/sqrt(1+L^2)) /
+// (1+L^2)^((Z+1)/2), L = x/(Z+1) (isf2d, motility-ddm src/motility_fit.js:
+// 152-159), with the Legendre function from Laplace's first integral
+// P_nu(cos t) = (1/pi) int_0^pi (cos t + i sin t cos phi)^nu dphi by the
+// trapezoid rule on 16 + ceil(nu t) panels (legendreP, :120-132): the
+// integrand is smooth, even and periodic in phi, so the rule converges
+// geometrically, and the phase nu arg(.) spans at most nu t.  In FP64
+// throughout: near L -> 0 the terms cancel to 1 - O(L^2).  The device builds
+// the fitter's isf2dFast table from it (every node of every row) and
+// evaluates it for ddm_isf2d_eval.
+__device__ double legendre_p_d(double nu, double c) {
+  const double s = sqrt(fmax(0.0, 1 - c * c));
+  const int n = 16 + (int)ceil(nu * atan2(s, c));
+  double acc = 0;
+  for (int i = 0; i <= n; i++) {
+    const double im = s * cos(M_PI * i / n);
+    const double val = exp(0.5 * nu * log(c * c + im * im)) * cos(nu * atan2(im, c));
+    acc += (i == 0 || i == n) ? 0.5 * val : val;
+  }
+  return acc / n;
+}
+
+// g at L = x/(Z+1) >= 1e-4 (a table node); 0 once the envelope is below e^-25
+__device__ double isf2d_lambda_d(double Lm, double Z) {
+  const double u = 1 + Lm * Lm;
+  const double ln_pre = -0.5 * (Z + 1) * log(u);
+  if (ln_pre < -25) return 0;
+  return legendre_p_d(Z, 1 / sqrt(u)) * exp(ln_pre);
+}
+
+__device__ double isf2d_exact_d(double x, double Z) {
+  const double Lm = x / (Z + 1);
+  if (Lm < 1e-4) return 1 - x * x * (Z + 2) / (4 * (Z + 1));
+  return isf2d_lambda_d(Lm, Z);
+}
+
+// One thread per table node: v[off[i] + k] = g(exp(u0[i] + k du[i]); Z[i]) for
+// k < n[i] + 2, the rows stored back to back (row i found by bisection on off).
+__global__ __launch_bounds__(256) void k_isf2d_nodes(const double* __restrict__ Z, const double* __restrict__ u0,
+                                                     const double* __restrict__ du, const int* __restrict__ off,
+                                                     int nz, int total, double* __restrict__ v) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = nz - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  v[g] = isf2d_lambda_d(exp(u0[lo] + (g - off[lo]) * du[lo]), Z[lo]);
+}
+
+// exact = 1: isf2d by the quadrature; 0: isf2dFast, the table lookup the
+// in-plane fits use (Catmull-Rom along ln L, quadratic Lagrange across Z)
+__global__ __launch_bounds__(256) void k_isf2d_eval(const double* __restrict__ x, const double* __restrict__ Z, int n,
+                                                    int exact, Isf2dDev T, double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double a, b;
+  out[i] = exact ? isf2d_exact_d(x[i], Z[i]) : 1 - one_minus_isf2d(x[i], Z[i], T, a, b);
+}
+
+// The table of ddm_isf2d.h built on the device: its layout (Z, u0, du, off, n
+// of each row) copied from the host, its values computed by k_isf2d_nodes.
+// Synchronous (the caller's next launch may read it from another stream).
+hipError_t build_isf2d_device(const Isf2dTable& T, double* v, double* z, double* u0, double* du, int* off, int* n,
+                              hipStream_t st) {
+  const size_t nz = T.Z.size();
+  const int total = (int)T.v.size();
+  hipError_t e;
+  if ((e = hipMemcpyAsync(z, T.Z.data(), nz * 8, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(u0, T.u0.data(), nz * 8, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(du, T.du.data(), nz * 8, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(off, T.off.data(), nz * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(n, T.n.data(), nz * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_isf2d_nodes, dim3((total + 255) / 256), dim3(256), 0, st, z, u0, du, off, (int)nz, total, v);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t launch_isf2d_eval(const double* x, const double* Z, int n, int exact, const Isf2dDev* T, double* out,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_isf2d_eval, dim3((n + 255) / 256), dim3(256), 0, st, x, Z, n, exact, *T, out);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- global-parameter fits
 //
 // One workgroup per video: (alpha, ln vbar, ln Z, ln D) shared by all its
@@ -680,8 +771,9 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
 // g = sum_b v^T r (the NNLS residual is orthogonal to M).  A bin whose A_b is
 // 0 does not depend on theta.  The four waves take bins round-robin, lanes
 // take lags; every decision is made on workgroup sums, so all threads agree.
-// The start grids are ranked with the FP32 model (as the per-q kernel's
-// grid); LM accepts and rejects steps on the FP64 objective.
+// The start grids are ranked with the FP32 model (113 points over every bin
+// of the video; the per-q kernel ranks its few lags in FP64); LM accepts and
+// rejects steps on the FP64 objective.
 // Plain seeding follows fitBrownian / fitSwimmer (:500-536, :577-611): a
 // 49-point ln D grid refined by LM in ln D, then the 16 vbar x 2 Z x 2 alpha
 // start grid at that D and LM from the best `nstarts` (Nelder-Mead in the
@@ -1277,7 +1369,7 @@ __global__ __launch_bounds__(256) void k_fit_unpack(const unsigned char* __restr
 }  // namespace
 
 // Set-up on the current device, once: the address of `scratch` and the
-// in-plane ISF table copied into it.
+// in-plane ISF table built into it (by the FP64 quadrature on the device).
 hipError_t noalloc_setup() {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -1297,13 +1389,7 @@ hipError_t noalloc_setup() {
   double* du = u0 + nz;
   int* off = reinterpret_cast<int*>(du + nz);
   int* n = off + nz;
-  if ((e = hipMemcpy(v, T.v.data(), T.v.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(z, T.Z.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(u0, T.u0.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(du, T.du.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(off, T.off.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(n, T.n.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
-    return e;
+  if ((e = build_isf2d_device(T, v, z, u0, du, off, n, nullptr)) != hipSuccess) return e;
   g_na_t2d[dev] = Isf2dDev{v, z, u0, du, off, n, T.lz0, T.dlz};
   g_na_ptr[dev] = b;
   return hipSuccess;

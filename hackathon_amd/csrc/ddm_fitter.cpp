@@ -98,8 +98,9 @@ void free_isf2d(ddm_fitter* f) {
   }
 }
 
-// The in-plane ISF table of ddm_isf2d.h, copied to device memory once (a
-// failed upload leaves nothing behind, so the next in-plane fit retries).
+// The in-plane ISF table of ddm_isf2d.h (its row layout), built in device
+// memory once by the FP64 quadrature on the GPU (a failed build leaves nothing
+// behind, so the next in-plane fit retries).
 hipError_t upload_isf2d(ddm_fitter* f) {
   const ddm::Isf2dTable& T = ddm::isf2d_table();
   const size_t nz = T.Z.size();
@@ -109,12 +110,7 @@ hipError_t upload_isf2d(ddm_fitter* f) {
       (e = hipMalloc((void**)&f->t2du, nz * 8)) != hipSuccess || (e = hipMalloc((void**)&f->t2off, nz * 4)) != hipSuccess ||
       (e = hipMalloc((void**)&f->t2n, nz * 4)) != hipSuccess)
     return free_isf2d(f), e;
-  if ((e = hipMemcpy(f->t2v, T.v.data(), T.v.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(f->t2z, T.Z.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(f->t2u0, T.u0.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(f->t2du, T.du.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(f->t2off, T.off.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(f->t2n, T.n.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = ddm::build_isf2d_device(T, f->t2v, f->t2z, f->t2u0, f->t2du, f->t2off, f->t2n, f->stream)) != hipSuccess)
     return free_isf2d(f), e;
   f->t2d = ddm::Isf2dDev{f->t2v, f->t2z, f->t2u0, f->t2du, f->t2off, f->t2n, T.lz0, T.dlz};
   return hipSuccess;
@@ -361,6 +357,28 @@ int ddm_fit_global(ddm_fitter* f, const ddm_global_fit_batch* b, double* theta, 
     hipEventElapsedTime(&ms, f->ev0, f->ev1);
     *kernel_ms = ms;
   }
+  return DDM_OK;
+}
+
+int ddm_isf2d_eval(ddm_fitter* f, const double* x, const double* Z, int32_t n, int32_t exact, double* out) {
+  if (!f) return DDM_EINVAL;
+  if (n < 0 || (n > 0 && (!x || !Z || !out))) return ffail(f, DDM_EINVAL, "isf2d: bad arguments");
+  if (n == 0) return DDM_OK;
+  hipError_t e;
+  if (!f->t2v && (e = upload_isf2d(f)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("isf2d: table: ") + hipGetErrorString(e));
+  double* d = nullptr;
+  if ((e = hipMalloc((void**)&d, (size_t)n * 24)) != hipSuccess)
+    return ffail(f, DDM_EHIP, std::string("isf2d: allocation: ") + hipGetErrorString(e));
+  if ((e = hipMemcpyAsync(d, x, (size_t)n * 8, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(d + n, Z, (size_t)n * 8, hipMemcpyHostToDevice, f->stream)) != hipSuccess ||
+      (e = ddm::launch_isf2d_eval(d, d + n, n, exact ? 1 : 0, &f->t2d, d + 2 * (size_t)n, f->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(out, d + 2 * (size_t)n, (size_t)n * 8, hipMemcpyDeviceToHost, f->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(f->stream)) != hipSuccess) {
+    hipFree(d);
+    return ffail(f, DDM_EHIP, std::string("isf2d: ") + hipGetErrorString(e));
+  }
+  hipFree(d);
   return DDM_OK;
 }
 

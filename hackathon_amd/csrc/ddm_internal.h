@@ -55,6 +55,15 @@ struct Isf2dDev {
   double lz0, dlz;
 };
 
+struct Isf2dTable;
+// the in-plane ISF table built on the device (layout from T, every value by
+// the FP64 Legendre quadrature); synchronous
+hipError_t build_isf2d_device(const Isf2dTable& T, double* v, double* z, double* u0, double* du, int* off, int* n,
+                              hipStream_t st);
+// out[i] = isf2d(x[i], Z[i]) by the quadrature (exact = 1) or the table (0); device arrays
+hipError_t launch_isf2d_eval(const double* x, const double* Z, int n, int exact, const Isf2dDev* T, double* out,
+                             hipStream_t st);
+
 // model: 0 Brownian, 1 3D swimmer, 2 in-plane swimmer (t2d required); vmag:
 // per-fit uniform drift speed or null
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,

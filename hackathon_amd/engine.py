@@ -39,7 +39,7 @@ EXPORTS = ("ddm_create", "ddm_destroy", "ddm_last_error", "ddm_get_sizes", "ddm_
            "ddm_verdict_last_error", "ddm_analyze_motility_batch", "ddm_read_mean", "ddm_fit_device_noalloc",
            "ddm_clock_create", "ddm_clock_destroy", "ddm_clock_place", "ddm_clock_drop", "ddm_clock_reset",
            "ddm_clock_state_get", "ddm_clock_snapshot", "ddm_shuffle_order", "ddm_lag_grid", "ddm_usable_bins",
-           "ddm_swimmer_isf", "ddm_estimate_drift", "ddm_fit_global")
+           "ddm_swimmer_isf", "ddm_estimate_drift", "ddm_fit_global", "ddm_isf2d_eval")
 
 
 class DdmError(RuntimeError):

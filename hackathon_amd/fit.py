@@ -53,6 +53,8 @@ def _declare(L):
     L.ddm_fitter_last_error.restype = C.c_char_p
     L.ddm_fit.argtypes = [vp, C.POINTER(FitBatch), vp, vp, vp, vp]
     L.ddm_fit.restype = C.c_int
+    L.ddm_isf2d_eval.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
+    L.ddm_isf2d_eval.restype = C.c_int
     L._fit_declared = True
     return L
 
@@ -184,6 +186,17 @@ class Fitter:
             self.close()
         except Exception:
             pass
+
+    def isf2d(self, x, Z, exact=True) -> np.ndarray:
+        """The in-plane swimmer ISF on the GPU in FP64: isf2d by the Legendre quadrature (exact) or
+        isf2dFast, the device-built table the in-plane fits interpolate (src/motility_fit.js:120-220)."""
+        x, Z = np.broadcast_arrays(np.asarray(x, np.float64), np.asarray(Z, np.float64))
+        x, Z = np.ascontiguousarray(x.ravel()), np.ascontiguousarray(Z.ravel())
+        out = np.zeros(len(x))
+        if self._lib.ddm_isf2d_eval(self._h, x.ctypes.data, Z.ctypes.data, len(x), int(bool(exact)),
+                                    out.ctypes.data) != 0:
+            raise DdmError(self._lib.ddm_fitter_last_error(self._h).decode())
+        return out
 
     def fit(self, tau, data, weight, qnodes, bnds, model=SWIMMER_3D, seeds=None, nstarts=3,
             max_iter=100, seed_ab=True, vmag=None) -> FitResult:

@@ -255,6 +255,12 @@ typedef struct ddm_global_fit_batch {
 int ddm_fit_global(ddm_fitter *f, const ddm_global_fit_batch *b, double *theta, double *rss, int32_t *iters,
                    double *brown, double *amps, double *kernel_ms);
 
+/* The in-plane swimmer ISF on the GPU, FP64: exact != 0 evaluates isf2d (src/motility_fit.js:152-159) with
+ * legendreP's trapezoid quadrature of Laplace's integral (:120-132); exact == 0 evaluates isf2dFast
+ * (:161-220), the table the in-plane fits interpolate, which the fitter builds on the device from the same
+ * quadrature.  x, Z, out: host arrays of n values.  Blocks until done. */
+int ddm_isf2d_eval(ddm_fitter *f, const double *x, const double *Z, int32_t n, int32_t exact, double *out);
+
 /* ddm_fit on device-resident arrays, allocating nothing.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
