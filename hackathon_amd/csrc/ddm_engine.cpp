@@ -385,8 +385,11 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
     return fail(nullptr, DDM_EINVAL, "ddm_create: tuning out of range");
   *out = nullptr;
   const ddm_geometry g = *gp;
-  if (!is_pow2(g.N) || g.N < 16 || g.N > 2048)
-    return fail(nullptr, DDM_EINVAL, "ddm_create: N must be a power of two in [16, 2048], got " + std::to_string(g.N));
+  const bool n3 = g.N % 3 == 0 && is_pow2(g.N / 3) && g.N >= 48 && g.N <= 1536;
+  if (!((is_pow2(g.N) && g.N >= 16 && g.N <= 2048) || n3))
+    return fail(nullptr, DDM_EINVAL,
+                "ddm_create: N must be a power of two in [16, 2048] or 3 * 2^m in [48, 1536], got " +
+                    std::to_string(g.N));
   if (g.T < 2 || g.T > 4096) return fail(nullptr, DDM_EINVAL, "ddm_create: T must be in [2, 4096]");
   if (g.L < 1 || g.L > g.T) return fail(nullptr, DDM_EINVAL, "ddm_create: L must be in [1, T]");
   if (g.NB < 1) return fail(nullptr, DDM_EINVAL, "ddm_create: NB must be >= 1");
@@ -441,7 +444,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   // budget (default 128 MB: the scratch stays in the 256 MB last-level cache).
   // wave_spatial: 0 default (environment, else on where built), 1 off, 2 on
   const long ws = tn.wave_spatial ? tn.wave_spatial - 1 : env_long("DDM_WAVE_SPATIAL", 1, 0, 1);
-  e->wave_spatial = (e->logn == 9 || e->logn == 10) && ws;
+  e->wave_spatial = is_pow2(g.N) && (e->logn == 9 || e->logn == 10) && ws;
   e->cols_f = (tn.cols_f ? tn.cols_f : env_long("DDM_COLS_F", 16, 8, 16)) >= 16 ? 16 : 8;
   const int F = 16;   // chunks are a multiple of every column-kernel frame tile
   long per_frame = (long)e->NH * g.N * 8;
@@ -595,7 +598,7 @@ static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframe
     if (e->wave_spatial)
       return ddm::launch_rows_w(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
                                 e->stream);
-    return ddm::launch_rows(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
+    return ddm::launch_rows(dtype, e->g.N, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
                             e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial rows launch");
@@ -603,7 +606,7 @@ static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframe
     if (e->wave_spatial)
       return ddm::launch_cols_w(e->logn, e->cols_f, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win,
                                 e->tw_n, e->wx, e->hist, e->g.T, slot % e->g.T, e->stream);
-    return ddm::launch_cols(e->logn, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
+    return ddm::launch_cols(e->g.N, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
                             e->hist, e->g.T, slot % e->g.T, e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial cols launch");

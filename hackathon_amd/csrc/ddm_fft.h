@@ -1,7 +1,7 @@
 // Device FFT building blocks for gfx950 (CDNA4, wave64).
 //
 // Stockham autosort FFTs staged through LDS, radix 8 where possible, then 4/2
-// (n = 2^LOGN).  Natural-order input and output, unnormalised both ways, the
+// (n = 2^LOGN), and a radix-3 pass for n = 3 * 2^m.  Natural-order input and output, unnormalised both ways, the
 // same sign and twiddle convention as the reference's radix-2 Stockham
 // (motility-ddm src/fft.js:59-93, src/ddm.wgsl:78-101): forward uses
 // exp(-2 pi i m / n), inverse its conjugate.  Twiddles come from a host FP64
@@ -70,6 +70,20 @@ template <bool INV> __device__ __forceinline__ void dft8(float2* a) {
 template <> __device__ __forceinline__ void dft_small<8, false>(float2* a) { dft8<false>(a); }
 template <> __device__ __forceinline__ void dft_small<8, true>(float2* a) { dft8<true>(a); }
 
+// 3-point DFT: X0 = a0 + (a1 + a2), X1,2 = a0 - (a1 + a2)/2 -+ i (sqrt3/2)(a1 - a2) (forward; the
+// inverse flips the sign of the i term)
+template <bool INV> __device__ __forceinline__ void dft3(float2* a) {
+  const float h = 0.86602540378443864676f;
+  const float2 t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
+  const float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+  const float2 r = INV ? make_float2(-h * t2.y, h * t2.x) : make_float2(h * t2.y, -h * t2.x);
+  a[0] = cadd(a[0], t1);
+  a[1] = cadd(m, r);
+  a[2] = csub(m, r);
+}
+template <> __device__ __forceinline__ void dft_small<3, false>(float2* a) { dft3<false>(a); }
+template <> __device__ __forceinline__ void dft_small<3, true>(float2* a) { dft3<true>(a); }
+
 // log2 of the radix of the next pass given REM = log2(n) - log2(Ns): radix 8
 // while possible, finishing with 4x4 or 4 rather than a lone radix 2.
 __host__ __device__ constexpr int next_lr(int rem) { return (rem == 4 || rem == 2) ? 2 : (rem == 1 ? 1 : 3); }
@@ -129,6 +143,70 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) 
     stockham_pass<LOGN, B, NT, LR, NSLOG, INV>(buf, tw, tid);
     lds_fft<LOGN, B, NT, INV, NSLOG + LR>(buf, tw, tid);
   }
+}
+
+// ----------------------------------------------------------------- n = 3 * 2^m
+//
+// The same autosort passes for transform lengths with one factor 3 (frame
+// sides such as 384 = 3 * 128): a radix-3 pass first (sub-transform length
+// Ns = 1, no twiddles), then radix 8/4/2 passes on sub-transform lengths 3,
+// 3R, ...; the pass formulas above hold for any radix whose product is n, with
+// twiddle w^(r k n/(Ns R)) from the n-point table.
+template <int n, int B, int NT, int R, int Ns, bool INV>
+__device__ __forceinline__ void stockham_pass_n(float2* buf, const float2* tw, int tid) {
+  constexpr int NBF = B * (n / R);
+  constexpr int ITEMS = (NBF + NT - 1) / NT;
+  float2 a[ITEMS][R];
+#pragma unroll
+  for (int it = 0; it < ITEMS; it++) {
+    const int g = tid + it * NT;
+    if (NBF % NT == 0 || g < NBF) {
+      const int f = g / (n / R), j = g % (n / R), k = j % Ns;
+      const float2* src = buf + f * n + j;
+#pragma unroll
+      for (int r = 0; r < R; r++) a[it][r] = src[r * (n / R)];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          float2 w = tw[r * k * (n / (Ns * R))];
+          if (INV) w.y = -w.y;
+          a[it][r] = cmul(a[it][r], w);
+        }
+      }
+      dft_small<R, INV>(a[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITEMS; it++) {
+    const int g = tid + it * NT;
+    if (NBF % NT == 0 || g < NBF) {
+      const int f = g / (n / R), j = g % (n / R), k = j % Ns;
+      float2* dst = buf + f * n + (j - k) * R + k;
+#pragma unroll
+      for (int s = 0; s < R; s++) dst[s * Ns] = a[it][s];
+    }
+  }
+  __syncthreads();
+}
+
+__host__ __device__ constexpr int ilog2n(int n) { return n <= 1 ? 0 : 1 + ilog2n(n >> 1); }
+
+template <int n, int B, int NT, bool INV, int Ns = 1>
+__device__ __forceinline__ void lds_fft_n(float2* buf, const float2* tw, int tid) {
+  if constexpr (Ns < n) {
+    constexpr int rem = n / Ns;
+    constexpr int R = rem % 3 == 0 ? 3 : (1 << next_lr(ilog2n(rem)));
+    stockham_pass_n<n, B, NT, R, Ns, INV>(buf, tw, tid);
+    lds_fft_n<n, B, NT, INV, Ns * R>(buf, tw, tid);
+  }
+}
+
+// Any supported length: the power-of-two passes above, or the radix-3 form.
+template <int n, int B, int NT, bool INV>
+__device__ __forceinline__ void lds_fft_any(float2* buf, const float2* tw, int tid) {
+  if constexpr ((n & (n - 1)) == 0) lds_fft<ilog2n(n), B, NT, INV>(buf, tw, tid);
+  else lds_fft_n<n, B, NT, INV>(buf, tw, tid);
 }
 
 // ----------------------------------------------------------------- FP64 Stockham

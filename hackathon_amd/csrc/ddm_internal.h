@@ -8,10 +8,11 @@ struct ddm_fit_batch;
 
 namespace ddm {
 
-hipError_t launch_rows(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
+// block spatial kernels: n = frame side, a power of two in [16, 2048] or 3 * 2^m in [48, 1536]
+hipError_t launch_rows(int dtype, int n, const void* frames, int nframes, float scale, const float* win,
                        const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);
-int cols_frames_per_wg(int logn);
-hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+int cols_frames_per_wg(int n);
+hipError_t launch_cols(int n, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                        const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
                        hipStream_t st);
 hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,

@@ -27,11 +27,22 @@ namespace ddm {
 
 constexpr int NT = 256;
 
-// Real rows per spatial-row workgroup = 2 * rows_pairs_per_wg: about 32 KB of
-// complex LDS, at most N rows.
-__host__ __device__ constexpr int rows_pairs_per_wg(int logn) {
-  return logn >= 11 ? 4 : ((4096 >> logn) < (1 << (logn - 1)) ? ((4096 >> logn) < 64 ? (4096 >> logn) : 64)
-                                                                : (1 << (logn - 1)));
+// Real rows per spatial-row workgroup = 2 * rows_pairs_n(N): about 32 KB of
+// complex LDS, at most N rows, and 2B dividing N (a power of two B: for
+// N = 3 * 2^m every power of two up to 2^(m-1) qualifies).
+__host__ __device__ constexpr int rows_pairs_n(int n) {
+  int b = 1;
+  while (2 * b * n <= 4096 && 2 * b <= 64 && 4 * b <= n && n % (4 * b) == 0) b *= 2;
+  return n >= 2048 ? 4 : b;
+}
+// lanes per row segment of the row-sum reduction: the largest power of two
+// <= 64 dividing the row's vector loads
+__host__ __device__ constexpr int seg_lanes(int nv) { return (nv & -nv) < 64 ? (nv & -nv) : 64; }
+// frame sides the spatial kernels take: powers of two in [16, 2048] and
+// 3 * 2^m in [48, 1536] (radix-3 first pass)
+__host__ __device__ constexpr bool spatial_size_ok(int n) {
+  return (n >= 16 && n <= 2048 && (n & (n - 1)) == 0) ||
+         (n >= 48 && n <= 1536 && n % 3 == 0 && ((n / 3) & (n / 3 - 1)) == 0);
 }
 
 // float2 slots per wave of k_temporal_w: the sub-transform exchange buffer,
@@ -80,15 +91,16 @@ template <int SEG, typename T> __device__ __forceinline__ T seg_sum(T v) {
 // the row mean is removed exactly.  f32 pixels: the row mean is removed as an
 // unevaluated (hi, lo) pair, like reduce_mean does per frame (src/ddm.wgsl:129-150).
 // rowinfo[y] keeps (S_y, 0) or (hi_y, lo_y) for the column pass.
-template <int LOGN, int B, typename Tin>
+template <int N, int B, typename Tin>
 __global__ __launch_bounds__(NT) void k_rows(const Tin* __restrict__ frames, int nframes, float scale,
                                              const float* __restrict__ win, const float2* __restrict__ tw_g,
                                              float2* __restrict__ scratch, int2* __restrict__ rowinfo) {
-  constexpr int N = 1 << LOGN, NH = N / 2 + 1, RW = 2 * B;
+  constexpr int NH = N / 2 + 1, RW = 2 * B;
+  static_assert(N % RW == 0, "rows per workgroup must divide N");
   constexpr int VEC = vec8b<Tin>::n;              // pixels per 8-byte load
   constexpr int NV = RW * N / VEC;                // vector loads per workgroup
   constexpr int LPT = (NV + NT - 1) / NT;         // loads per thread
-  constexpr int SEG = (N / VEC) < 64 ? (N / VEC) : 64;  // lanes per row within a wave
+  constexpr int SEG = seg_lanes(N / VEC);         // lanes per row segment within a wave
   constexpr bool IS_INT = !std::is_same<Tin, float>::value;
   extern __shared__ float2 lds[];
   float2* buf = lds;                               // B * N
@@ -205,7 +217,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Tin* __restrict__ frames, int
           make_int2(__float_as_int(rowsum_f[tid] * (1.0f / N)), __float_as_int(rowsum_lo[tid] * (1.0f / N)));
   }
   __syncthreads();
-  lds_fft<LOGN, B, NT, false>(buf, tw, tid);
+  lds_fft_any<N, B, NT, false>(buf, tw, tid);
   // untangle the packed real transforms, store kx-major: scratch[(f*NH + kx)*N + y]
   constexpr int NOUT = NH * RW;
   for (int e = tid; e < NOUT; e += NT) {
@@ -213,7 +225,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Tin* __restrict__ frames, int
     int kx = e / RW;
     int p = r >> 1;
     float2 z = buf[p * N + kx];
-    float2 zc = cconj(buf[p * N + ((N - kx) & (N - 1))]);
+    float2 zc = cconj(buf[p * N + (N - kx) % N]);
     float2 o;
     if (r & 1) {
       float2 u = csub(z, zc);
@@ -233,12 +245,12 @@ __global__ __launch_bounds__(NT) void k_rows(const Tin* __restrict__ frames, int
 // only touches the columns where the window's DFT Wx(kx) is nonzero, kx <= 3
 // for the 4-term window: row spectrum += (c_y - mu) * s * Wx(kx) * w(y).
 // Integer frames: c_y - mu = (N*S_y - sum_y S_y) / N^2, exact in int64.
-template <int LOGN, int F>
+template <int N, int F>
 __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch, const int2* __restrict__ rowinfo,
                                              int nframes, int is_int, float scale, const float* __restrict__ win,
                                              const float2* __restrict__ tw_g, const float2* __restrict__ wx,
                                              float2* __restrict__ hist, int T, int first_slot) {
-  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
+  constexpr int NH = N / 2 + 1;
   extern __shared__ float2 lds[];
   float2* buf = lds;                  // F * N
   float2* tw = lds + F * N;           // N
@@ -265,10 +277,13 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
           comp_add(sh, ch, __int_as_float(ri[y].x));
           comp_add(sl, cl, __int_as_float(ri[y].y));
         }
-        // N is a power of two: dividing by it is exact, so the frame mean is
-        // kept unrounded as sh/N + ch/N + (sl + cl)/N
-        mu[tid] = sh * (1.0f / N);
-        mu[F + tid] = ch * (1.0f / N);
+        // the frame mean as m0 + (ch + r0)/N + (sl + cl)/N with m0 = sh/N
+        // rounded and r0 = sh - N m0 its exact residual (0 when N is a power
+        // of two), so the leading term's rounding does not reach the mean
+        const float m0 = sh * (1.0f / N);
+        const float r0 = fmaf(-m0, (float)N, sh);
+        mu[tid] = m0;
+        mu[F + tid] = (ch + r0) * (1.0f / N);
         mu[2 * F + tid] = (sl + cl) * (1.0f / N);
       }
     }
@@ -290,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
     }
   }
   __syncthreads();
-  lds_fft<LOGN, F, NT, false>(buf, tw, tid);
+  lds_fft_any<N, F, NT, false>(buf, tw, tid);
   const float s = 1.0f / N;
   for (int e = tid; e < F * N; e += NT) {
     int f = e % F, ky = e / F;
@@ -1443,19 +1458,18 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
 
 // ----------------------------------------------------------------- launchers
 
-template <typename Tin, int LOGN>
+template <typename Tin, int N>
 static hipError_t launch_rows_t(const void* frames, int nframes, float scale, const float* win, const float2* tw,
                                 float2* scratch, int2* rowinfo, hipStream_t st) {
-  constexpr int N = 1 << LOGN;
-  constexpr int B = rows_pairs_per_wg(LOGN);
+  constexpr int B = rows_pairs_n(N);
   constexpr int RW = 2 * B;
-  static_assert(RW <= N, "rows per workgroup exceed N");
+  static_assert(RW <= N && N % RW == 0, "rows per workgroup must divide N");
   constexpr int VEC = vec8b<Tin>::n;
-  constexpr int SEG = (N / VEC) < 64 ? (N / VEC) : 64;
+  constexpr int SEG = seg_lanes(N / VEC);
   constexpr int NSEG = (N / VEC) / SEG;
   size_t lds = sizeof(float2) * (B * N + N) + sizeof(float) * 2 * RW + sizeof(int) * RW +
                (sizeof(float) + sizeof(int)) * RW * NSEG;
-  auto kern = k_rows<LOGN, B, Tin>;
+  auto kern = k_rows<N, B, Tin>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid(nframes * (N / RW));
@@ -1464,37 +1478,38 @@ static hipError_t launch_rows_t(const void* frames, int nframes, float scale, co
 }
 
 template <typename Tin>
-static hipError_t launch_rows_dt(int logn, const void* frames, int nframes, float scale, const float* win,
+static hipError_t launch_rows_dt(int n, const void* frames, int nframes, float scale, const float* win,
                                  const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
-  switch (logn) {
-#define CASE(LG_) \
-  case LG_: return launch_rows_t<Tin, LG_>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
-    CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
+  switch (n) {
+#define CASE(N_) \
+  case N_: return launch_rows_t<Tin, N_>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
+    CASE(48) CASE(96) CASE(192) CASE(384) CASE(768) CASE(1536)
 #undef CASE
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_rows(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
+hipError_t launch_rows(int dtype, int n, const void* frames, int nframes, float scale, const float* win,
                        const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
-  if (dtype == 0) return launch_rows_dt<uint8_t>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
-  if (dtype == 1) return launch_rows_dt<uint16_t>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
-  return launch_rows_dt<float>(logn, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  if (dtype == 0) return launch_rows_dt<uint8_t>(n, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  if (dtype == 1) return launch_rows_dt<uint16_t>(n, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  return launch_rows_dt<float>(n, frames, nframes, scale, win, tw, scratch, rowinfo, st);
 }
 
 // Frames per column workgroup: the corner turn writes runs of this many slots
 // per mode.  At N = 2048 eight (64 B runs, one 147 KB workgroup per CU)
 // measured 20 % faster than four (32 B runs, two per CU).
-int cols_frames_per_wg(int logn) { return logn >= 10 ? 8 : 16; }
+int cols_frames_per_wg(int n) { return n >= 1024 ? 8 : 16; }
 
-template <int LOGN>
+template <int N>
 static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                                 const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                                 int first_slot, hipStream_t st) {
-  constexpr int N = 1 << LOGN, NH = N / 2 + 1;
-  constexpr int F = LOGN >= 10 ? 8 : 16;
+  constexpr int NH = N / 2 + 1;
+  constexpr int F = N >= 1024 ? 8 : 16;
   size_t lds = sizeof(float2) * (F * N + N) + sizeof(float) * 4 * F + sizeof(long long) * F;
-  auto kern = k_cols<LOGN, F>;
+  auto kern = k_cols<N, F>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid(NH, (nframes + F - 1) / F);
@@ -1503,13 +1518,14 @@ static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int 
   return hipGetLastError();
 }
 
-hipError_t launch_cols(int logn, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+hipError_t launch_cols(int n, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                        const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
                        hipStream_t st) {
-  switch (logn) {
-#define CASE(LG_) \
-  case LG_: return launch_cols_t<LG_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
-    CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11)
+  switch (n) {
+#define CASE(N_) \
+  case N_: return launch_cols_t<N_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
+    CASE(48) CASE(96) CASE(192) CASE(384) CASE(768) CASE(1536)
 #undef CASE
   }
   return hipErrorInvalidValue;

@@ -51,7 +51,7 @@ enum { DDM_U8 = 0, DDM_U16 = 1, DDM_F32 = 2 };
 
 /* Geometry; the reference fixes it (DDM, src/ddm_ref.js:18-31: N=256, T=256,
  * L=128, NB=32, kmin=4, kmax=100, KNODES=4).  Here every field is a runtime
- * parameter.  N: power of two in [16, 2048].  T: ring slots, 2..4096 (the
+ * parameter.  N: power of two in [16, 2048] or 3 * 2^m in [48, 1536].  T: ring slots, 2..4096 (the
  * temporal FFT length is the next power of two >= 2T).  L: lags 1..T.
  * NB >= 1 bins linear in |k| on [kmin, kmax).  hypot_mode 0 bins by V8's
  * Math.hypot (bit-exact with the reference), 1 by the correctly rounded

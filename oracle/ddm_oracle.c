@@ -106,17 +106,27 @@ int orc_fft(int n, double *data, int inverse) {
  * radix-3 geometry N=384, which the reference's fft() rejects). */
 void orc_dft(int n, const double *in, double *out, int inverse) {
   double sgn = inverse ? 1.0 : -1.0;
+  /* the n distinct angles sgn 2 pi m / n, m = (k t) mod n, evaluated once */
+  double *cs = (double *)malloc(sizeof(double) * 2 * n);
+  for (int m = 0; m < n; m++) {
+    double ang = (sgn * 2 * ORC_PI * (double)m) / n;
+    cs[2 * m] = cos(ang);
+    cs[2 * m + 1] = sin(ang);
+  }
   for (int k = 0; k < n; k++) {
     double sr = 0, si = 0;
+    long m = 0;
     for (int t = 0; t < n; t++) {
-      double ang = (sgn * 2 * ORC_PI * (double)(((long)k * t) % n)) / n;
-      double c = cos(ang), s = sin(ang);
+      double c = cs[2 * m], s = cs[2 * m + 1];
       sr += in[2 * t] * c - in[2 * t + 1] * s;
       si += in[2 * t] * s + in[2 * t + 1] * c;
+      m += k;
+      if (m >= n) m -= n;
     }
     out[2 * k] = sr;
     out[2 * k + 1] = si;
   }
+  free(cs);
 }
 
 /* FFT for any n: Stockham when n is a power of two, direct DFT otherwise. */
