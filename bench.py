@@ -117,9 +117,15 @@ def batch_section(nvideos=64, seed=3, reps=3):
     rads = b.radials()
     prs, res = BA.fit_all(rads, 1.0, 100.0, nstarts=3)
     b.close()
+    # the same videos with the temporal stages as persistent work-queue kernels (one per transform length)
+    bq = BA.Batch(vids, mode=BA.GRAPH | BA.QUEUE, chains=8)
+    bq.run()
+    msq = min(bq.run() for _ in range(reps))
+    bq.close()
     return {"workload": f"{nvideos} videos, N in {{128, 256, 384, 512}}, T uniform in [200, 2000], u8, lags "
                         f"1..T-1, N/2 unit-width bins; one HIP-graph launch (8 concurrent chains)",
             "frame_pixels": fpx, "ms": ms, "frame_pixels_per_s": fpx / (ms * 1e-3),
+            "work_queue_ms": msq, "work_queue_frame_pixels_per_s": fpx / (msq * 1e-3),
             "fits": int(len(res.chi2)), "fit_kernel_ms": res.kernel_ms,
             "fits_per_s": len(res.chi2) / (res.kernel_ms * 1e-3)}
 

@@ -17,7 +17,7 @@ import numpy as np
 from .engine import DdmEngine, DdmError, Geometry, Tuning, load_library
 from .fit import SWIMMER_3D, Fitter, bounds, prepare
 
-EAGER, GRAPH = 0, 1
+EAGER, GRAPH, QUEUE = 0, 1, 2   # QUEUE is or-ed with EAGER or GRAPH: persistent work-queue temporal stage
 _DT = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}
 
 

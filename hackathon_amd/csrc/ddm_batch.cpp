@@ -1,12 +1,15 @@
 // Batches of uneven videos (ddm_batch_* in include/ddm.h): one engine per
 // video; the videos' ingest + temporal + radial work is recorded into one HIP
 // graph with `chains` concurrent branches (fork/join through events on the
-// batch stream), or enqueued eagerly with the same dependencies.
+// batch stream), or enqueued eagerly with the same dependencies.  In
+// work-queue mode the temporal stages of all videos of one transform length
+// run as one persistent kernel fed from a queue of their work items.
 #include "../../include/ddm.h"
 #include "ddm_internal.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -22,6 +25,16 @@ struct ddm_batch {
   std::vector<char> meta_identity;
   std::vector<float> meta_n0;
   int mode = DDM_BATCH_EAGER, chains = 1;
+  // work-queue mode: one persistent temporal launch per transform length
+  struct Queue {
+    int logt = 0, nwork = 0, grid = 0;
+    std::vector<int> vids;                // videos in the queue, in item order
+    ddm::TemporalArgs* dargs = nullptr;   // device table, one entry per video
+    const float2 *tw = nullptr, *tw_f = nullptr;
+  };
+  std::vector<Queue> queues;
+  std::vector<char> queued;               // video i's temporal stage runs from a queue
+  int* counters = nullptr;                // one device int per queue
   hipStream_t s0 = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   hipGraph_t graph = nullptr;
@@ -43,30 +56,96 @@ hipEvent_t new_event(ddm_batch* b) {
 }
 
 // Enqueue every video on its engine's stream, chained per `chains`, forked from
-// and joined back into b->s0 (identical when recorded into a graph).
+// and joined back into b->s0 (identical when recorded into a graph).  In
+// work-queue mode the queued videos stop after their metadata uploads; the
+// persistent temporal kernels follow on b->s0 once every chain has joined,
+// then the queued videos' radial finalisations, chained again.
 int enqueue_all(ddm_batch* b, bool capturing) {
   const int n = (int)b->eng.size();
   hipEvent_t fork = new_event(b);
   if (!fork || hipEventRecord(fork, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: fork event");
-  std::vector<hipEvent_t> tail(b->chains, fork);
-  for (int i = 0; i < n; i++) {
-    ddm_engine* e = b->eng[i];
-    hipStream_t es = ddm::engine_stream(e);
-    const int c = i % b->chains;
-    if (hipStreamWaitEvent(es, tail[c], 0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain wait");
-    ddm::engine_set_capturing(e, capturing);
-    const ddm_batch_video& v = b->vids[i];
-    int rc = ddm_ingest_device(e, v.dframes, v.dtype, v.nframes, 0);
-    if (!rc) rc = ddm_compute_temporal(e, b->order[i].data(), b->mask[i].data());
-    ddm::engine_set_capturing(e, false);
-    if (rc) return bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(e));
-    hipEvent_t done = new_event(b);
-    if (!done || hipEventRecord(done, es) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain event");
-    tail[c] = done;
+  auto run_chains = [&](hipEvent_t start, bool second) -> int {
+    std::vector<hipEvent_t> tail(b->chains, start);
+    for (int i = 0; i < n; i++) {
+      if (second && !b->queued[i]) continue;
+      ddm_engine* e = b->eng[i];
+      hipStream_t es = ddm::engine_stream(e);
+      const int c = i % b->chains;
+      if (hipStreamWaitEvent(es, tail[c], 0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain wait");
+      ddm::engine_set_capturing(e, capturing);
+      const ddm_batch_video& v = b->vids[i];
+      int rc = DDM_OK;
+      if (second) {
+        rc = ddm::engine_temporal_end(e);
+      } else {
+        rc = ddm_ingest_device(e, v.dframes, v.dtype, v.nframes, 0);
+        if (!rc) rc = b->queued[i] ? ddm::engine_temporal_begin(e, b->order[i].data(), b->mask[i].data())
+                                   : ddm_compute_temporal(e, b->order[i].data(), b->mask[i].data());
+      }
+      ddm::engine_set_capturing(e, false);
+      if (rc) return bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(e));
+      hipEvent_t done = new_event(b);
+      if (!done || hipEventRecord(done, es) != hipSuccess) return bfail(b, DDM_EHIP, "batch: chain event");
+      tail[c] = done;
+    }
+    for (int c = 0; c < b->chains; c++)
+      if (tail[c] != start && hipStreamWaitEvent(b->s0, tail[c], 0) != hipSuccess)
+        return bfail(b, DDM_EHIP, "batch: join");
+    return DDM_OK;
+  };
+  int rc = run_chains(fork, false);
+  if (rc || b->queues.empty()) return rc;
+  for (size_t q = 0; q < b->queues.size(); q++) {
+    const ddm_batch::Queue& Q = b->queues[q];
+    hipError_t he = ddm::launch_temporal_wq(Q.logt, Q.dargs, (int)Q.vids.size(), Q.nwork, b->counters + q, Q.tw,
+                                            Q.tw_f, Q.grid, b->s0);
+    if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: work-queue launch: ") + hipGetErrorString(he));
   }
-  for (int c = 0; c < b->chains; c++)
-    if (tail[c] != fork && hipStreamWaitEvent(b->s0, tail[c], 0) != hipSuccess)
-      return bfail(b, DDM_EHIP, "batch: join");
+  hipEvent_t mid = new_event(b);
+  if (!mid || hipEventRecord(mid, b->s0) != hipSuccess) return bfail(b, DDM_EHIP, "batch: queue event");
+  return run_chains(mid, true);
+}
+
+// Work-queue set-up: each video's temporal arguments planned on the host (no
+// GPU work), grouped by transform length into device tables.
+int build_queues(ddm_batch* b) {
+  const int n = (int)b->eng.size();
+  b->queued.assign(n, 0);
+  std::vector<std::vector<ddm::TemporalArgs>> args(13);
+  std::vector<ddm_batch::Queue> qs(13);
+  for (int i = 0; i < n; i++) {
+    ddm::TemporalArgs a{};
+    int logt = 0;
+    int rc = ddm::engine_temporal_plan(b->eng[i], b->order[i].data(), b->mask[i].data(), &a, &logt);
+    if (rc) return bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(b->eng[i]));
+    const int ni = ddm::engine_nitems(b->eng[i]);
+    if (logt < 8 || logt > 12 || ni < 1) continue;   // the video's own computeTemporal
+    ddm_batch::Queue& Q = qs[logt];
+    a.item0 = Q.nwork;
+    Q.nwork += ni;
+    Q.logt = logt;
+    Q.vids.push_back(i);
+    if (!Q.tw) {
+      Q.tw = ddm::engine_tw_t(b->eng[i]);
+      Q.tw_f = ddm::engine_tw_f(b->eng[i]);
+    }
+    args[logt].push_back(a);
+    b->queued[i] = 1;
+  }
+  for (int l = 8; l <= 12; l++) {
+    if (qs[l].vids.empty()) continue;
+    ddm_batch::Queue& Q = qs[l];
+    const int g = ddm::temporal_wq_grid(l);
+    if (g < 1) return bfail(b, DDM_EHIP, "batch: work-queue occupancy query");
+    Q.grid = std::min(g, Q.nwork);
+    hipError_t he = hipMalloc((void**)&Q.dargs, args[l].size() * sizeof(ddm::TemporalArgs));
+    if (he == hipSuccess)
+      he = hipMemcpy(Q.dargs, args[l].data(), args[l].size() * sizeof(ddm::TemporalArgs), hipMemcpyHostToDevice);
+    b->queues.push_back(Q);
+    if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: work-queue table: ") + hipGetErrorString(he));
+  }
+  if (!b->queues.empty() && hipMalloc((void**)&b->counters, b->queues.size() * sizeof(int)) != hipSuccess)
+    return bfail(b, DDM_EHIP, "batch: work-queue counters");
   return DDM_OK;
 }
 
@@ -81,6 +160,9 @@ void release(ddm_batch* b) {
   if (b->s0) hipStreamDestroy(b->s0);
   for (void* p : b->meta_owned)
     if (p) hipHostFree(p);
+  for (auto& q : b->queues)
+    if (q.dargs) hipFree(q.dargs);
+  if (b->counters) hipFree(b->counters);
 }
 }  // namespace
 
@@ -91,12 +173,12 @@ static std::string g_batch_create_error;
 int ddm_batch_create(const ddm_batch_video* v, int32_t n, int32_t mode, int32_t chains, ddm_batch** out) {
   if (!out) return DDM_EINVAL;
   *out = nullptr;
-  if (!v || n < 1 || (mode != DDM_BATCH_EAGER && mode != DDM_BATCH_GRAPH) || chains < 1) {
+  if (!v || n < 1 || mode < 0 || mode > (DDM_BATCH_GRAPH | DDM_BATCH_QUEUE) || chains < 1) {
     g_batch_create_error = "batch: bad arguments";
     return DDM_EINVAL;
   }
   ddm_batch* b = new ddm_batch();
-  b->mode = mode;
+  b->mode = mode & DDM_BATCH_GRAPH;
   b->chains = std::min<int>(chains, n);
   auto bail = [&](int rc) {
     g_batch_create_error = b->err;
@@ -127,7 +209,12 @@ int ddm_batch_create(const ddm_batch_video* v, int32_t n, int32_t mode, int32_t 
     b->order.push_back(std::move(o));
     b->mask.push_back(std::move(m));
   }
-  if (mode == DDM_BATCH_GRAPH) {
+  b->queued.assign(n, 0);
+  if (mode & DDM_BATCH_QUEUE) {
+    const int qrc = build_queues(b);
+    if (qrc) return bail(qrc);
+  }
+  if (b->mode == DDM_BATCH_GRAPH) {
     if (hipStreamBeginCapture(b->s0, hipStreamCaptureModeRelaxed) != hipSuccess)
       return (bfail(b, DDM_EHIP, "batch: begin capture"), bail(DDM_EHIP));
     int rc = enqueue_all(b, true);

@@ -279,6 +279,10 @@ struct ddm_engine {
   // the last computeTemporal's metadata, for readTemporalFull
   bool meta_valid = false, last_identity = true;
   float last_n0 = 0.f;
+  // the analysis begun by computeTemporal's first half (temporal_begin), for
+  // its temporal launch and radial finalisation
+  int cur_KT = 1;
+  bool cur_full = true;
   float *fullD = nullptr, *fullP = nullptr;   // per-mode outputs of one chunk of modes
   float2* fullC = nullptr;
   double* drift_acc = nullptr;   // NB*L*8 phase-drift sums (FP64)
@@ -644,100 +648,36 @@ int ddm_ingest_device(ddm_engine* e, const void* dframes, int32_t dtype, int32_t
   return DDM_OK;
 }
 
+// computeTemporal: metadata, the temporal kernel, the radial finalisation
+static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask) {
+  int rc = ddm::engine_temporal_begin(e, order, mask);
+  if (rc) return rc;
+  hipError_t he;
+  if (e->nitems > 0 &&
+      (he = timed(e, DDM_K_TEMPORAL, [&] {
+         const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
+         const int identity = e->last_identity ? 1 : 0;
+         if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
+           return ddm::launch_temporal_w(e->logt, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
+                                         e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part, e->p_part,
+                                         e->P_part, e->cur_KT, e->G_part, e->stream);
+         return ddm::launch_temporal(e->logt, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
+                                     identity, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part, e->cur_KT,
+                                     e->G_part, e->stream);
+       })) != hipSuccess)
+    return hip_fail(e, he, "temporal launch");
+  return ddm::engine_temporal_end(e);
+}
+
 int ddm_compute_temporal(ddm_engine* e, const uint32_t* order, const float* mask) {
   if (!e) return DDM_EINVAL;
-  if (!order || !mask) return fail(e, DDM_EINVAL, "computeTemporal: order/mask must have length T");
   if (e->temporal_busy) return fail(e, DDM_ESTATE, "computeTemporal is already running");
   e->temporal_busy = true;
-  const int T = e->g.T, L = e->g.L;
-  for (int j = 0; j < T; j++) {
-    if (order[j] >= (uint32_t)T) {
-      e->temporal_busy = false;
-      return fail(e, DDM_EINVAL, "computeTemporal: order[" + std::to_string(j) + "] is not a slot < T");
-    }
-  }
-  // the pinned staging buffers may still feed the previous upload
-  hipError_t he = e->capturing ? hipSuccess : hipEventSynchronize(e->meta_ev);
-  if (he != hipSuccess) {
-    e->temporal_busy = false;
-    return hip_fail(e, he, "computeTemporal: metadata staging");
-  }
-  bool identity = true, full = true;
-  float* m01 = e->meta_mask;
-  for (int j = 0; j < T; j++) {
-    identity = identity && order[j] == (uint32_t)j;
-    full = full && mask[j] != 0.f;
-    e->meta_order[j] = order[j];
-    m01[j] = mask[j] != 0.f ? 1.f : 0.f;
-  }
-  // exact valid-pair counts N(tau) = sum_t m(t) m(t + tau) (temporalMeta,
-  // src/ddm_ref.js:552-556): T - tau when every frame is valid, else popcounts
-  // of the mask packed into 64-bit words (T/64 word operations per lag)
-  const int nw = (T + 63) / 64;
-  std::vector<uint64_t> bits(full ? 0 : nw, 0);
-  if (!full)
-    for (int j = 0; j < T; j++)
-      if (m01[j] != 0.f) bits[j >> 6] |= 1ull << (j & 63);
-  auto pairs = [&](int tau) -> int {
-    if (full) return T - tau;
-    const int ws = tau >> 6, bs = tau & 63;
-    int c = 0;
-    for (int w = 0; w + ws < nw; w++) {
-      uint64_t sh = bits[w + ws] >> bs;   // bit t of sh = m(64 w + t + tau)
-      if (bs && w + ws + 1 < nw) sh |= bits[w + ws + 1] << (64 - bs);
-      c += __builtin_popcountll(bits[w] & sh);
-    }
-    return c;
-  };
-  float* np = e->meta_np;
-  for (int tau = 0; tau < L; tau++) np[tau] = (float)pairs(tau);
-  const float n0 = np[0];
-  // direct tail: the lags from the first one (within reach of KT_max) with
-  // N(tau) < need on, so that every transform lag keeps N(tau) >= need =
-  // max(T/32, min(32, T/4)) (with dropped frames N(tau) is not monotone: a look
-  // at the boundary lag alone would leave single-pair lags to the transform)
-  int KT = e->KT;
-  const int need = std::max((T + 31) / 32, std::min(32, T / 4));
-  for (int tau = std::max(1, T - e->KT_max); tau < T - KT; tau++)
-    if (pairs(tau) < need) {
-      KT = T - tau;
-      break;
-    }
-  e->meta_valid = true;
-  e->last_identity = identity;
-  e->last_n0 = n0;
-  int rc = DDM_OK;
-  if ((he = hipMemcpyAsync(e->order, e->meta_order, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-      (he = hipMemcpyAsync(e->mask, m01, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-      (he = hipMemcpyAsync(e->npairs, np, L * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-      (!e->capturing && (he = hipEventRecord(e->meta_ev, e->stream)) != hipSuccess)) {
-    rc = hip_fail(e, he, "computeTemporal: metadata upload");
-  } else if (e->nitems > 0 &&
-             (he = timed(e, DDM_K_TEMPORAL, [&] {
-                const float inv_n0 = n0 > 0.5f ? 1.0f / n0 : 0.0f;
-                if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
-                  return ddm::launch_temporal_w(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order,
-                                                e->mask, identity ? 1 : 0, inv_n0, e->tw_t, e->tw_f, e->S_part,
-                                                e->p_part, e->P_part, KT, e->G_part, e->stream);
-                return ddm::launch_temporal(e->logt, e->hist, T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
-                                            identity ? 1 : 0, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part,
-                                            KT, e->G_part, e->stream);
-              })) != hipSuccess) {
-    rc = hip_fail(e, he, "temporal launch");
-  } else if ((he = timed(e, DDM_K_FINALIZE, [&] {
-                // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
-                // sub-transforms and stores their spectra r-major
-                const int lm = e->logt <= 12 && e->logt > 9 ? e->logt - 9 : 0;
-                return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, KT,
-                                                e->bin_items, e->g.NB,
-                                                e->nmodes, e->mask, full ? 1 : 0, e->npairs, T, L, n0, e->tw_t64,
-                                                e->Sbin, e->Gbin, e->pbin, e->Pbin, e->radD, e->radP, e->stream);
-              })) != hipSuccess) {
-    rc = hip_fail(e, he, "radial finalize launch");
-  }
+  int rc = compute_temporal(e, order, mask);
   e->temporal_busy = false;
   return rc;
 }
+
 
 // Phase-drift sums (opt-in: only when the caller asks for them): C(tau) of
 // every binned mode from the per-mode kernel, in chunks of the binned-mode
@@ -935,6 +875,122 @@ void engine_get_meta(const ddm_engine* e, bool* identity, float* n0) {
   *n0 = e->last_n0;
 }
 // after a graph run: the engine's device metadata is the graph's again
+// computeTemporal's host planning (temporalMeta, src/ddm_ref.js:522-560):
+// validation, identity/full flags, the exact valid-pair counts N(tau) into np
+// (when not null), N(0) and the direct-tail length KT.  No GPU work.
+static int plan_temporal(ddm_engine* e, const uint32_t* order, const float* mask, float* m01, uint32_t* ord_out,
+                         float* np, bool* identity_out, bool* full_out, float* n0_out, int* KT_out) {
+  const int T = e->g.T, L = e->g.L;
+  for (int j = 0; j < T; j++)
+    if (order[j] >= (uint32_t)T)
+      return fail(e, DDM_EINVAL, "computeTemporal: order[" + std::to_string(j) + "] is not a slot < T");
+  bool identity = true, full = true;
+  for (int j = 0; j < T; j++) {
+    identity = identity && order[j] == (uint32_t)j;
+    full = full && mask[j] != 0.f;
+    if (ord_out) ord_out[j] = order[j];
+    if (m01) m01[j] = mask[j] != 0.f ? 1.f : 0.f;
+  }
+  // exact valid-pair counts N(tau) = sum_t m(t) m(t + tau) (temporalMeta,
+  // src/ddm_ref.js:552-556): T - tau when every frame is valid, else popcounts
+  // of the mask packed into 64-bit words (T/64 word operations per lag)
+  const int nw = (T + 63) / 64;
+  std::vector<uint64_t> bits(full ? 0 : nw, 0);
+  if (!full)
+    for (int j = 0; j < T; j++)
+      if (mask[j] != 0.f) bits[j >> 6] |= 1ull << (j & 63);
+  auto pairs = [&](int tau) -> int {
+    if (full) return T - tau;
+    const int ws = tau >> 6, bs = tau & 63;
+    int c = 0;
+    for (int w = 0; w + ws < nw; w++) {
+      uint64_t sh = bits[w + ws] >> bs;   // bit t of sh = m(64 w + t + tau)
+      if (bs && w + ws + 1 < nw) sh |= bits[w + ws + 1] << (64 - bs);
+      c += __builtin_popcountll(bits[w] & sh);
+    }
+    return c;
+  };
+  if (np)
+    for (int tau = 0; tau < L; tau++) np[tau] = (float)pairs(tau);
+  // direct tail: the lags from the first one (within reach of KT_max) with
+  // N(tau) < need on, so that every transform lag keeps N(tau) >= need =
+  // max(T/32, min(32, T/4)) (with dropped frames N(tau) is not monotone: a look
+  // at the boundary lag alone would leave single-pair lags to the transform)
+  int KT = e->KT;
+  const int need = std::max((T + 31) / 32, std::min(32, T / 4));
+  for (int tau = std::max(1, T - e->KT_max); tau < T - KT; tau++)
+    if (pairs(tau) < need) {
+      KT = T - tau;
+      break;
+    }
+  *identity_out = identity;
+  *full_out = full;
+  *n0_out = (float)pairs(0);
+  *KT_out = KT;
+  return DDM_OK;
+}
+
+// The first half of computeTemporal: planning and the metadata uploads on the
+// engine's stream (order, 0/1 mask, N(tau)).
+int engine_temporal_begin(ddm_engine* e, const uint32_t* order, const float* mask) {
+  if (!order || !mask) return fail(e, DDM_EINVAL, "computeTemporal: order/mask must have length T");
+  const int T = e->g.T, L = e->g.L;
+  // the pinned staging buffers may still feed the previous upload
+  hipError_t he = e->capturing ? hipSuccess : hipEventSynchronize(e->meta_ev);
+  if (he != hipSuccess) return hip_fail(e, he, "computeTemporal: metadata staging");
+  bool identity = true, full = true;
+  float n0 = 0.f;
+  int KT = e->KT;
+  int rc = plan_temporal(e, order, mask, e->meta_mask, e->meta_order, e->meta_np, &identity, &full, &n0, &KT);
+  if (rc) return rc;
+  e->meta_valid = true;
+  e->last_identity = identity;
+  e->last_n0 = n0;
+  e->cur_KT = KT;
+  e->cur_full = full;
+  if ((he = hipMemcpyAsync(e->order, e->meta_order, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (he = hipMemcpyAsync(e->mask, e->meta_mask, T * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (he = hipMemcpyAsync(e->npairs, e->meta_np, L * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+      (!e->capturing && (he = hipEventRecord(e->meta_ev, e->stream)) != hipSuccess))
+    return hip_fail(e, he, "computeTemporal: metadata upload");
+  return DDM_OK;
+}
+
+// The radial finalisation of the analysis temporal_begin started, once its
+// temporal partials are written (on the engine's stream).
+int engine_temporal_end(ddm_engine* e) {
+  hipError_t he = timed(e, DDM_K_FINALIZE, [&] {
+    // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
+    // sub-transforms and stores their spectra r-major
+    const int lm = e->logt <= 12 && e->logt > 9 ? e->logt - 9 : 0;
+    return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, e->cur_KT,
+                                    e->bin_items, e->g.NB, e->nmodes, e->mask, e->cur_full ? 1 : 0, e->npairs,
+                                    e->g.T, e->g.L, e->last_n0, e->tw_t64, e->Sbin, e->Gbin, e->pbin, e->Pbin,
+                                    e->radD, e->radP, e->stream);
+  });
+  if (he != hipSuccess) return hip_fail(e, he, "radial finalize launch");
+  return DDM_OK;
+}
+
+// The wavefront temporal kernel's arguments for (order, mask) -- what
+// computeTemporal would launch -- without any GPU work; logt 0 when the
+// geometry takes the workgroup kernel instead (TPAD > 4096).
+int engine_temporal_plan(ddm_engine* e, const uint32_t* order, const float* mask, TemporalArgs* a, int* logt) {
+  bool identity = true, full = true;
+  float n0 = 0.f;
+  int KT = e->KT;
+  int rc = plan_temporal(e, order, mask, nullptr, nullptr, nullptr, &identity, &full, &n0, &KT);
+  if (rc) return rc;
+  *a = TemporalArgs{e->hist, e->items, e->bin_modes, e->order, e->mask, e->S_part, e->p_part, e->P_part,
+                    e->G_part, e->g.T, identity ? 1 : 0, KT, 0, n0 > 0.5f ? 1.0f / n0 : 0.0f};
+  *logt = e->logt <= 12 ? e->logt : 0;
+  return DDM_OK;
+}
+
+int engine_nitems(const ddm_engine* e) { return e->nitems; }
+const float2* engine_tw_t(const ddm_engine* e) { return e->tw_t; }
+const float2* engine_tw_f(const ddm_engine* e) { return e->tw_f; }
+
 void engine_set_meta(ddm_engine* e, bool identity, float n0) {
   e->meta_valid = true;
   e->last_identity = identity;

@@ -50,6 +50,22 @@ hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* row
 
 // Device copy of the in-plane ISF table (ddm_isf2d.h): row i's values at
 // v[off[i] .. off[i] + n[i] + 1], sampled at ln Lambda = u0[i] + k du[i].
+// One video's arguments of the wavefront temporal stage (k_temporal_w); in a
+// batch's work queue also item0, the index of its first work item there.
+struct TemporalArgs {
+  const float2* hist;
+  const int4* items;
+  const int* bin_modes;
+  const uint32_t* order;
+  const float* mask;
+  float* S_part;
+  float* p_part;
+  float* P_part;
+  float* G_part;
+  int T, identity, KT, item0;
+  float inv_n0;
+};
+
 struct Isf2dDev {
   const double *v, *Z, *u0, *du;
   const int *off, *n;
@@ -105,6 +121,22 @@ hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp,
                         double4* part, float2* pos, void* out, hipStream_t st);
 
 // engine internals used by the batch runner (ddm_batch.cpp)
+// computeTemporal in parts, for the batch's work queue: begin (planning and
+// metadata uploads on the engine's stream), the temporal kernel (the caller's),
+// end (radial finalisation); plan gives the kernel arguments with no GPU work
+// (logt 0: the geometry takes the workgroup kernel)
+int engine_temporal_begin(::ddm_engine* e, const uint32_t* order, const float* mask);
+int engine_temporal_end(::ddm_engine* e);
+int engine_temporal_plan(::ddm_engine* e, const uint32_t* order, const float* mask, TemporalArgs* a, int* logt);
+int engine_nitems(const ::ddm_engine* e);
+const float2* engine_tw_t(const ::ddm_engine* e);
+const float2* engine_tw_f(const ::ddm_engine* e);
+// the persistent temporal kernel over the work items of nvid videos of one
+// transform length 2^logt (vids: device table, item0 increasing; counter: one
+// device int, zeroed here by a fill kernel); grid from temporal_wq_grid
+hipError_t launch_temporal_wq(int logt, const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
+                              const float2* tw_f, int grid, hipStream_t st);
+int temporal_wq_grid(int logt);
 hipStream_t engine_stream(::ddm_engine* e);
 void engine_set_capturing(::ddm_engine* e, bool on);
 hipError_t engine_detach_meta(::ddm_engine* e, void** owned3);

@@ -478,17 +478,39 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 // the mode loop; the 4 waves' sums are added pairwise once, at the end.  (The
 // same adds as LDS float atomics made the kernel 3.7x slower on gfx950.)  The next round's
 // series is loaded while the current one is transformed.
+// Per-workgroup set-up shared by every item a workgroup takes: the
+// sub-transform twiddle table staged in LDS, the per-lane DIF pre-twiddle
+// bases and the last pass's twiddles (the same for every mode and video of
+// one transform length).
 template <int LOGT, int LM>
-// TPAD = 4096: one workgroup per CU by LDS (the per-wave accumulators), so
-// up to 512 registers per lane instead of spilling at 256
-__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const float2* __restrict__ hist, int T,
-                                                      const int4* __restrict__ items,
-                                                      const int* __restrict__ bin_modes,
-                                                      const uint32_t* __restrict__ order,
-                                                      const float* __restrict__ mask, int identity, float inv_n0,
-                                                      const float2* __restrict__ tw_g, const float2* __restrict__ tw_f,
-                                                      float* __restrict__ S_part, float* __restrict__ p_part,
-                                                      float* __restrict__ P_part, int KT, float* __restrict__ G_part) {
+__device__ __forceinline__ void temporal_w_setup(float2* lds, const float2* __restrict__ tw_g,
+                                                 const float2* __restrict__ tw_f, float2 (&tbase)[(1 << LM) > 1 ? (1 << LM) : 1],
+                                                 float2 (&wlast)[8]) {
+  constexpr int m = 1 << LM;
+  constexpr int LOGF = LOGT - LM;
+  constexpr int NF = 1 << LOGF;
+  constexpr int WS = temporal_wave_lds(LOGT, LM);
+  const int tid = threadIdx.x, lane = tid & 63;
+  // the sub-transform's twiddle table, staged once per workgroup at twpad()
+  // slots: the FFT passes read it from LDS instead of waiting on global loads
+  float2* twl = lds + 4 * WS + 2;
+  for (int i = tid; i < NF; i += NT) twl[twpad(i)] = tw_f[i];
+  // DIF pre-twiddles w_n^{r (lane + 64 q)} = w_n^{r lane} w_n^{64 r q}: the
+  // per-lane factor is loaded once here, the second is wave-uniform
+#pragma unroll
+  for (int r = 0; r < (m > 1 ? m : 1); r++) tbase[r] = tw_g[r * lane];
+  __syncthreads();
+  // last-pass twiddles w_NF^{r lane} of the 512-point sub-transforms, the same
+  // for every mode: kept in registers
+  if constexpr (LOGF == 9) butterfly_twiddles<8, false, true>(twl, lane, wlast);
+}
+
+// One work item (up to 64 modes of one bin) of one video's temporal stage;
+// A.items[item] names it and its partials go to row `item` of A's outputs.
+template <int LOGT, int LM>
+__device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int item, float2* lds,
+                                                const float2 (&tbase)[(1 << LM) > 1 ? (1 << LM) : 1],
+                                                const float2 (&wlast)[8]) {
   constexpr int n = 1 << LOGT;
   constexpr int m = 1 << LM;
   constexpr int LOGF = LOGT - LM;  // sub-transform size
@@ -499,25 +521,22 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
   constexpr int KS = n / NT;       // frequencies per thread in the accumulation
   constexpr int TS = (n / 2) / NT > 0 ? (n / 2) / NT : 1;
   static_assert(PT <= 32, "mask bits");
-  extern __shared__ float2 lds[];
+  const float2* __restrict__ hist = A.hist;
+  const int4* __restrict__ items = A.items;
+  const int* __restrict__ bin_modes = A.bin_modes;
+  const uint32_t* __restrict__ order = A.order;
+  const float* __restrict__ mask = A.mask;
+  float* __restrict__ S_part = A.S_part;
+  float* __restrict__ p_part = A.p_part;
+  float* __restrict__ P_part = A.P_part;
+  float* __restrict__ G_part = A.G_part;
+  const int T = A.T, identity = A.identity, KT = A.KT;
+  const float inv_n0 = A.inv_n0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * WS;
   float* red = reinterpret_cast<float*>(lds + 4 * WS);  // 4
-  // the sub-transform's twiddle table, staged once per workgroup at twpad()
-  // slots: the FFT passes read it from LDS instead of waiting on global loads
-  float2* twl = lds + 4 * WS + 2;
-  for (int i = tid; i < NF; i += NT) twl[twpad(i)] = tw_f[i];
-  // DIF pre-twiddles w_n^{r (lane + 64 q)} = w_n^{r lane} w_n^{64 r q}: the
-  // per-lane factor is loaded once here, the second is wave-uniform
-  float2 tbase[m > 1 ? m : 1];
-#pragma unroll
-  for (int r = 0; r < (m > 1 ? m : 1); r++) tbase[r] = tw_g[r * lane];
-  __syncthreads();
-  // last-pass twiddles w_NF^{r lane} of the 512-point sub-transforms, the same
-  // for every mode: kept in registers
-  float2 wlast[8];
-  if constexpr (LOGF == 9) butterfly_twiddles<8, false, true>(twl, lane, wlast);
-  const int4 it = items[blockIdx.x];
+  const float2* twl = lds + 4 * WS + 2;
+  const int4 it = items[item];
   const int start = it.y, count = it.z;
   uint32_t mbits = 0;
 #pragma unroll
@@ -725,13 +744,13 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
     // the 4 waves' sums, added pairwise (positive terms: relative error <= 2 ulp)
     const float* p0 = reinterpret_cast<const float*>(lds) + 2 * xch_size(NF);
     constexpr int WF = 2 * WS;                     // wave stride, in floats
-    float* Sp = S_part + (long)blockIdx.x * n;
+    float* Sp = S_part + (long)item * n;
 #pragma unroll
     for (int i = 0; i < KS; i++) {
       const int k = tid + NT * i;
       Sp[k] = (p0[k] + p0[WF + k]) + (p0[2 * WF + k] + p0[3 * WF + k]);
     }
-    float* pp = p_part + (long)blockIdx.x * T;
+    float* pp = p_part + (long)item * T;
 #pragma unroll
     for (int i = 0; i < TS; i++) {
       const int t = tid + NT * i;
@@ -743,11 +762,72 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
   // the 4 waves' tail lags meet in their (now idle) exchange buffers
   if (lane % lpl == 0 && lane / lpl < KT) reinterpret_cast<float*>(xch)[lane / lpl] = Gt + Gtc;
   __syncthreads();
-  if (tid == 0) P_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (tid == 0) P_part[item] = (red[0] + red[1]) + (red[2] + red[3]);
   if (tid < KT) {
     const float* g0 = reinterpret_cast<const float*>(lds);
     constexpr int WF = 2 * WS;
-    G_part[(long)blockIdx.x * KT + tid] = (g0[tid] + g0[WF + tid]) + (g0[2 * WF + tid] + g0[3 * WF + tid]);
+    G_part[(long)item * KT + tid] = (g0[tid] + g0[WF + tid]) + (g0[2 * WF + tid] + g0[3 * WF + tid]);
+  }
+}
+
+template <int LOGT, int LM>
+// TPAD = 4096: one workgroup per CU by LDS (the per-wave accumulators), so
+// up to 512 registers per lane instead of spilling at 256
+__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const float2* __restrict__ hist, int T,
+                                                      const int4* __restrict__ items,
+                                                      const int* __restrict__ bin_modes,
+                                                      const uint32_t* __restrict__ order,
+                                                      const float* __restrict__ mask, int identity, float inv_n0,
+                                                      const float2* __restrict__ tw_g, const float2* __restrict__ tw_f,
+                                                      float* __restrict__ S_part, float* __restrict__ p_part,
+                                                      float* __restrict__ P_part, int KT, float* __restrict__ G_part) {
+  constexpr int n = 1 << LOGT;
+  constexpr int m = 1 << LM;
+  constexpr int LOGF = LOGT - LM;  // sub-transform size
+  constexpr int NF = 1 << LOGF;
+  constexpr int PF = NF / 64;      // points per lane of a sub-transform
+  constexpr int PT = n / 128;      // time samples per lane (T <= n/2)
+  constexpr int WS = temporal_wave_lds(LOGT, LM);  // float2 per wave
+  constexpr int KS = n / NT;       // frequencies per thread in the accumulation
+  constexpr int TS = (n / 2) / NT > 0 ? (n / 2) / NT : 1;
+  static_assert(PT <= 32, "mask bits");
+  extern __shared__ float2 lds[];
+  float2 tbase[(1 << LM) > 1 ? (1 << LM) : 1], wlast[8];
+  temporal_w_setup<LOGT, LM>(lds, tw_g, tw_f, tbase, wlast);
+  const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, identity, KT, 0, inv_n0};
+  temporal_w_item<LOGT, LM>(A, blockIdx.x, lds, tbase, wlast);
+}
+
+// The batch form: a persistent grid over the work items of many videos (one
+// transform length), taken from a queue.  vids[v] holds video v's arguments
+// and item0, the index of its first item in the queue (increasing in v);
+// nwork items in all.  A workgroup takes the next index from *counter (zero
+// before the launch) until the queue is empty -- every workgroup sees the
+// same index, so all of them leave the loop -- and computes the item exactly
+// as k_temporal_w does, so the partials are bit-identical.
+template <int LOGT, int LM>
+__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_wq(const TemporalArgs* __restrict__ vids,
+                                                                       int nvid, int nwork, int* __restrict__ counter,
+                                                                       const float2* __restrict__ tw_g,
+                                                                       const float2* __restrict__ tw_f) {
+  extern __shared__ float2 lds[];
+  __shared__ int next;
+  float2 tbase[(1 << LM) > 1 ? (1 << LM) : 1], wlast[8];
+  temporal_w_setup<LOGT, LM>(lds, tw_g, tw_f, tbase, wlast);
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = next;
+    if (w >= nwork) break;
+    int lo = 0, hi = nvid - 1;   // the last video whose first item is <= w
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (vids[mid].item0 <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    const TemporalArgs A = vids[lo];
+    temporal_w_item<LOGT, LM>(A, w - A.item0, lds, tbase, wlast);
+    __syncthreads();   // the item's last LDS reads before the next item's writes
   }
 }
 
@@ -1563,17 +1643,76 @@ hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* item
 
 
 template <int LOGT, int LM>
+static size_t temporal_w_lds() {
+  return sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + twpad((1 << LOGT) >> LM));
+}
+
+template <int LOGT, int LM>
 static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
                                       const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                                       float* P_part, int KT, float* G_part, hipStream_t st) {
-  size_t lds = sizeof(float2) * (4 * temporal_wave_lds(LOGT, LM) + 2 + twpad((1 << LOGT) >> LM));
+  const size_t lds = temporal_w_lds<LOGT, LM>();
   auto kern = k_temporal_w<LOGT, LM>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
                      tw, tw_f, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int v) { p[threadIdx.x] = v; }
+
+template <int LOGT, int LM>
+static hipError_t launch_temporal_wq_t(const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
+                                       const float2* tw_f, int grid, hipStream_t st) {
+  const size_t lds = temporal_w_lds<LOGT, LM>();
+  auto kern = k_temporal_wq<LOGT, LM>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  // the queue's counter is cleared by a kernel (a memset node would not do
+  // in a captured graph)
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, st, counter, 0);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, vids, nvid, nwork, counter, tw, tw_f);
+  return hipGetLastError();
+}
+
+template <int LOGT, int LM>
+static int temporal_wq_grid_t() {
+  const size_t lds = temporal_w_lds<LOGT, LM>();
+  auto kern = k_temporal_wq<LOGT, LM>;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, NT, lds) != hipSuccess)
+    return 0;
+  return cus * (per_cu > 0 ? per_cu : 1);
+}
+
+// Workgroups of the persistent temporal kernel: as many as fit the device at
+// once (0 on a query failure).
+int temporal_wq_grid(int logt) {
+  switch (logt) {
+    case 8: return temporal_wq_grid_t<8, 0>();
+    case 9: return temporal_wq_grid_t<9, 0>();
+    case 10: return temporal_wq_grid_t<10, 1>();
+    case 11: return temporal_wq_grid_t<11, 2>();
+    case 12: return temporal_wq_grid_t<12, 3>();
+  }
+  return 0;
+}
+
+hipError_t launch_temporal_wq(int logt, const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
+                              const float2* tw_f, int grid, hipStream_t st) {
+  if (nvid < 1 || nwork < 1 || grid < 1) return hipErrorInvalidValue;
+  switch (logt) {
+#define CASE(LG_, LM_) \
+  case LG_: return launch_temporal_wq_t<LG_, LM_>(vids, nvid, nwork, counter, tw, tw_f, grid, st);
+    CASE(8, 0) CASE(9, 0) CASE(10, 1) CASE(11, 2) CASE(12, 3)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
 }
 
 // tw: exp(-2 pi i j/n) for the TPAD length n; tw_f: the same for the

@@ -323,7 +323,12 @@ typedef struct ddm_batch_video {
   const uint32_t *order;    /* T entries (host), NULL: linear order */
   const float *mask;        /* T entries (host), NULL: every frame valid */
 } ddm_batch_video;
-enum { DDM_BATCH_EAGER = 0, DDM_BATCH_GRAPH = 1 };
+/* mode: DDM_BATCH_EAGER or DDM_BATCH_GRAPH, optionally | DDM_BATCH_QUEUE: the
+ * videos' temporal stages then run as one persistent work-queue kernel per
+ * temporal transform length (every video's work items in one queue, taken by
+ * a grid that fills the device), between the videos' spatial stages and
+ * their radial finalisations; the results stay bit-identical. */
+enum { DDM_BATCH_EAGER = 0, DDM_BATCH_GRAPH = 1, DDM_BATCH_QUEUE = 2 };
 
 /* Creates the engines (and records the graph).  `chains` (>= 1) splits the
  * videos round-robin into that many independent chains that may run

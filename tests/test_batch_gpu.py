@@ -40,7 +40,8 @@ def _alone(v):
     return r
 
 
-@pytest.mark.parametrize("mode,chains", [(BA.GRAPH, 3), (BA.EAGER, 2), (BA.GRAPH, 1)])
+@pytest.mark.parametrize("mode,chains", [(BA.GRAPH, 3), (BA.EAGER, 2), (BA.GRAPH, 1), (BA.GRAPH | BA.QUEUE, 3),
+                                         (BA.EAGER | BA.QUEUE, 2)])
 def test_batch_bit_identical_to_single_engines(gpu, mode, chains):
     vids = _videos()
     ref = [_alone(v) for v in vids]

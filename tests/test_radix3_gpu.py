@@ -1,7 +1,8 @@
 """Frame sides with a factor 3 (N = 3 * 2^m, e.g. the 384 of the batched uneven workload) against the FP64
 oracle, whose transform falls back to the direct DFT for them (the reference's own FFT takes powers of two
 only, src/fft.js:35-39): the spectra (radix-3 first pass, then radix 8/4/2), D(q, tau) per element, readMean;
-and a 64-video batch over N in {128, 256, 384, 512}, T in [200, 2000] against per-video engines."""
+and a 64-video batch over N in {128, 256, 384, 512}, T in [200, 2000] against per-video engines, with the
+temporal stages per video and as persistent work-queue kernels."""
 import numpy as np
 import pytest
 
@@ -85,7 +86,10 @@ def test_c3_batch_64_videos(gpu):
     b = BA.Batch(vids, mode=BA.GRAPH, chains=8)
     ms = b.run()
     rads = b.radials()
-    print(f"batch {ms:.1f} ms", flush=True)
+    bq = BA.Batch(vids, mode=BA.GRAPH | BA.QUEUE, chains=8)   # persistent work-queue temporal stage
+    msq = bq.run()
+    radq = bq.radials()
+    print(f"batch {ms:.1f} ms, work-queue batch {msq:.1f} ms", flush=True)
     for i, v in enumerate(vids):
         T = v["T"]
         with DdmEngine.create(N=v["frames"].shape[-1], T=T, L=T, NB=v["NB"], kmin=0, kmax=v["kmax"]) as e:
@@ -93,4 +97,6 @@ def test_c3_batch_64_videos(gpu):
             e.computeTemporal(np.arange(T, dtype=np.uint32), np.ones(T, np.float32))
             r = e.readRadial()
         assert np.array_equal(rads[i].D, r.D) and np.array_equal(rads[i].P, r.P), i
+        assert np.array_equal(radq[i].D, r.D) and np.array_equal(radq[i].P, r.P), i
     b.close()
+    bq.close()
