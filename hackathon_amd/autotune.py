@@ -3,8 +3,9 @@
 Candidates cover the engine's tunables (include/ddm.h `ddm_tuning`): the
 temporal work split (target item count: modes per workgroup), the spatial
 chunk (row-spectrum scratch budget, i.e. how long a chunk stays in the
-last-level cache), workgroup- vs wavefront-level spatial FFTs (N = 512, 1024)
-and frames per column workgroup.  Every candidate first has to reproduce the
+last-level cache), workgroup- vs wavefront-level spatial FFTs (N = 512, 1024),
+frames per column workgroup and the FFT radix (radix-8 passes, or radix 4 only,
+in the spatial and temporal transforms).  Every candidate first has to reproduce the
 default configuration's D(q, tau) and P(q) on the caller's frames (relative
 1e-5 of the largest value, the FP32 bar of the path: candidates differ only in
 FP32 summation order); only then is it timed.  Timing runs `inflight` analyses
@@ -34,10 +35,11 @@ def candidates(N: int, quick: bool = False):
     items = (2048, 8192) if quick else (1024, 2048, 4096)
     chunks = (128,) if quick else (64, 96, 128, 192, 256, 512, 1024)
     waves = (1, 2) if N in (512, 1024) else (1,)
+    radices = (8, 4)
     out = []
-    for it, ch, wv in itertools.product(items, chunks, waves):
+    for it, ch, wv, rx in itertools.product(items, chunks, waves, radices):
         for cf in ((8, 16) if wv == 2 else (16,)):
-            out.append(dict(temporal_items=it, chunk_mb=ch, wave_spatial=wv, cols_f=cf))
+            out.append(dict(temporal_items=it, chunk_mb=ch, wave_spatial=wv, cols_f=cf, fft_radix=rx))
     return out
 
 

@@ -26,7 +26,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
               "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
-KERNEL_SOURCES = ["ddm_kernels.hip", "ddm_fit.hip", "ddm_mlp.hip", "ddm_synth.hip"]
+KERNEL_SOURCES = ["ddm_kernels.hip", "ddm_spatial.hip", "ddm_fit.hip", "ddm_mlp.hip", "ddm_synth.hip"]
 HOST_SOURCES = ["ddm_engine.cpp", "ddm_fitter.cpp", "ddm_mlp_host.cpp", "ddm_synth_host.cpp", "ddm_batch.cpp",
                 "ddm_verdict.cpp", "ddm_clock.cpp"]
 
@@ -40,17 +40,18 @@ def _run(cmd):
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every source and link hackathon_amd/libddm.so; returns its path."""
+    """Compile every source (in parallel, one process per source) and link hackathon_amd/libddm.so; returns
+    its path."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
-    for src in KERNEL_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, *DEVICE_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
-        objs.append(obj)
-    for src in HOST_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
-        objs.append(obj)
+    jobs = [([HIPCC, *DEVICE_FLAGS, "-c", "-o", os.path.join(BUILD, src + ".o"), os.path.join(CSRC, src)])
+            for src in KERNEL_SOURCES]
+    jobs += [(["g++", *HOST_FLAGS, "-c", "-o", os.path.join(BUILD, src + ".o"), os.path.join(CSRC, src)])
+             for src in HOST_SOURCES]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(workers) as pool:
+        list(pool.map(_run, jobs))
+    objs = [cmd[cmd.index("-o") + 1] for cmd in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs,
           "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])

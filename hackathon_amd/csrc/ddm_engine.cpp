@@ -232,6 +232,7 @@ struct ddm_engine {
   long modes = 0;
   int chunk = 0;
   int cols_f = 16;        // frames per column workgroup (wavefront kernels)
+  int maxlr = 3;          // FFT radix cap (log2): 3 radix 8, 2 radix 4
   bool wave_spatial = false;
   int mean_frame = -1;    // rowinfo frame of the last frame ingested (-1: none yet)
   int mean_dtype = 0;
@@ -385,7 +386,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   ddm_tuning tn{};
   if (tp) tn = *tp;
   if (tn.temporal_items < 0 || tn.chunk_mb < 0 || tn.wave_spatial < 0 || tn.wave_spatial > 2 ||
-      (tn.cols_f != 0 && tn.cols_f != 8 && tn.cols_f != 16))
+      (tn.cols_f != 0 && tn.cols_f != 8 && tn.cols_f != 16) ||
+      (tn.fft_radix != 0 && tn.fft_radix != 4 && tn.fft_radix != 8))
     return fail(nullptr, DDM_EINVAL, "ddm_create: tuning out of range");
   *out = nullptr;
   const ddm_geometry g = *gp;
@@ -458,6 +460,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->tune.chunk_mb = (int32_t)chunk_mb;
   e->tune.wave_spatial = e->wave_spatial ? 2 : 1;
   e->tune.cols_f = e->cols_f;
+  e->maxlr = (tn.fft_radix ? tn.fft_radix : env_long("DDM_FFT_RADIX", 8, 4, 8)) == 4 ? 2 : 3;
+  e->tune.fft_radix = e->maxlr == 2 ? 4 : 8;
   int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
   chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
   e->chunk = chunk;
@@ -600,17 +604,17 @@ static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframe
   const float scale = dtype == DDM_U8 ? 1.0f / 255.0f : dtype == DDM_U16 ? 1.0f / 65535.0f : 1.0f;
   hipError_t he = timed(e, DDM_K_ROWS, [&] {
     if (e->wave_spatial)
-      return ddm::launch_rows_w(dtype, e->logn, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
+      return ddm::launch_rows_w(dtype, e->logn, e->maxlr, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
                                 e->stream);
-    return ddm::launch_rows(dtype, e->g.N, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
+    return ddm::launch_rows(dtype, e->g.N, e->maxlr, dframes, nframes, scale, e->win, e->tw_n, e->scratch, e->rowinfo,
                             e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial rows launch");
   he = timed(e, DDM_K_COLS, [&] {
     if (e->wave_spatial)
-      return ddm::launch_cols_w(e->logn, e->cols_f, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win,
+      return ddm::launch_cols_w(e->logn, e->cols_f, e->maxlr, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win,
                                 e->tw_n, e->wx, e->hist, e->g.T, slot % e->g.T, e->stream);
-    return ddm::launch_cols(e->g.N, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
+    return ddm::launch_cols(e->g.N, e->maxlr, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
                             e->hist, e->g.T, slot % e->g.T, e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial cols launch");
@@ -658,7 +662,7 @@ static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* m
          const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
          const int identity = e->last_identity ? 1 : 0;
          if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
-           return ddm::launch_temporal_w(e->logt, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
+           return ddm::launch_temporal_w(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
                                          e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part, e->p_part,
                                          e->P_part, e->cur_KT, e->G_part, e->stream);
          return ddm::launch_temporal(e->logt, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
@@ -988,6 +992,7 @@ int engine_temporal_plan(ddm_engine* e, const uint32_t* order, const float* mask
 }
 
 int engine_nitems(const ddm_engine* e) { return e->nitems; }
+int engine_maxlr(const ddm_engine* e) { return e->maxlr; }
 const float2* engine_tw_t(const ddm_engine* e) { return e->tw_t; }
 const float2* engine_tw_f(const ddm_engine* e) { return e->tw_f; }
 

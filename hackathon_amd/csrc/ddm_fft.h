@@ -16,6 +16,16 @@
 
 namespace ddm {
 
+// Threads per workgroup of the DDM kernels (four wavefronts).
+constexpr int NT = 256;
+
+// Sum over aligned segments of SEG lanes (SEG a power of two <= 64).
+template <int SEG, typename T> __device__ __forceinline__ T seg_sum(T v) {
+#pragma unroll
+  for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -87,6 +97,12 @@ template <> __device__ __forceinline__ void dft_small<3, true>(float2* a) { dft3
 // log2 of the radix of the next pass given REM = log2(n) - log2(Ns): radix 8
 // while possible, finishing with 4x4 or 4 rather than a lone radix 2.
 __host__ __device__ constexpr int next_lr(int rem) { return (rem == 4 || rem == 2) ? 2 : (rem == 1 ? 1 : 3); }
+// The same with the radix capped at 2^MAXLR: MAXLR 3 is the policy above;
+// MAXLR 2 takes radix-4 passes, a radix-2 one last when REM is odd (the
+// autotuner's FFT radix choice).
+__host__ __device__ constexpr int next_lr_cap(int rem, int maxlr) {
+  return maxlr >= 3 ? next_lr(rem) : (rem >= 2 ? 2 : rem);
+}
 
 // One in-place Stockham pass over B contiguous transforms of length 2^LOGN in
 // LDS `buf`, executed by NT threads.  `tw[m] = exp(-2 pi i m / n)`, m < n.
@@ -136,12 +152,12 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   __syncthreads();
 }
 
-template <int LOGN, int B, int NT, bool INV, int NSLOG = 0>
+template <int LOGN, int B, int NT, bool INV, int NSLOG = 0, int MAXLR = 3>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) {
   if constexpr (NSLOG < LOGN) {
-    constexpr int LR = next_lr(LOGN - NSLOG);
+    constexpr int LR = next_lr_cap(LOGN - NSLOG, MAXLR);
     stockham_pass<LOGN, B, NT, LR, NSLOG, INV>(buf, tw, tid);
-    lds_fft<LOGN, B, NT, INV, NSLOG + LR>(buf, tw, tid);
+    lds_fft<LOGN, B, NT, INV, NSLOG + LR, MAXLR>(buf, tw, tid);
   }
 }
 
@@ -192,21 +208,22 @@ __device__ __forceinline__ void stockham_pass_n(float2* buf, const float2* tw, i
 
 __host__ __device__ constexpr int ilog2n(int n) { return n <= 1 ? 0 : 1 + ilog2n(n >> 1); }
 
-template <int n, int B, int NT, bool INV, int Ns = 1>
+template <int n, int B, int NT, bool INV, int Ns = 1, int MAXLR = 3>
 __device__ __forceinline__ void lds_fft_n(float2* buf, const float2* tw, int tid) {
   if constexpr (Ns < n) {
     constexpr int rem = n / Ns;
-    constexpr int R = rem % 3 == 0 ? 3 : (1 << next_lr(ilog2n(rem)));
+    constexpr int R = rem % 3 == 0 ? 3 : (1 << next_lr_cap(ilog2n(rem), MAXLR));
     stockham_pass_n<n, B, NT, R, Ns, INV>(buf, tw, tid);
-    lds_fft_n<n, B, NT, INV, Ns * R>(buf, tw, tid);
+    lds_fft_n<n, B, NT, INV, Ns * R, MAXLR>(buf, tw, tid);
   }
 }
 
-// Any supported length: the power-of-two passes above, or the radix-3 form.
-template <int n, int B, int NT, bool INV>
+// Any supported length: the power-of-two passes above, or the radix-3 form;
+// radix capped at 2^MAXLR.
+template <int n, int B, int NT, bool INV, int MAXLR = 3>
 __device__ __forceinline__ void lds_fft_any(float2* buf, const float2* tw, int tid) {
-  if constexpr ((n & (n - 1)) == 0) lds_fft<ilog2n(n), B, NT, INV>(buf, tw, tid);
-  else lds_fft_n<n, B, NT, INV>(buf, tw, tid);
+  if constexpr ((n & (n - 1)) == 0) lds_fft<ilog2n(n), B, NT, INV, 0, MAXLR>(buf, tw, tid);
+  else lds_fft_n<n, B, NT, INV, 1, MAXLR>(buf, tw, tid);
 }
 
 // ----------------------------------------------------------------- FP64 Stockham
@@ -354,8 +371,8 @@ template <> __device__ __forceinline__ void dft_half<8, true>(float2* a) { dft8_
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
 // radix (log2) of a wavefront pass: the usual choice, capped so every lane owns
 // at least one butterfly (R <= n/64)
-__host__ __device__ constexpr int wave_lr(int logn, int nslog) {
-  return next_lr(logn - nslog) < (logn - 6) ? next_lr(logn - nslog) : (logn - 6);
+__host__ __device__ constexpr int wave_lr(int logn, int nslog, int maxlr = 3) {
+  return next_lr_cap(logn - nslog, maxlr) < (logn - 6) ? next_lr_cap(logn - nslog, maxlr) : (logn - 6);
 }
 
 // Twiddle-table slot of entry m in an LDS-staged table: one spare slot per 16
@@ -387,11 +404,11 @@ __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, flo
   }
 }
 
-template <int LOGN, int NSLOG, bool INV, bool HALF, bool TWP = false, bool WL = false>
+template <int LOGN, int NSLOG, bool INV, bool HALF, bool TWP = false, bool WL = false, int MAXLR = 3>
 __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
                                           const float2* wl = nullptr) {
   constexpr int n = 1 << LOGN;
-  constexpr int LR = wave_lr(LOGN, NSLOG);
+  constexpr int LR = wave_lr(LOGN, NSLOG, MAXLR);
   constexpr int R = 1 << LR;
   constexpr int Ns = 1 << NSLOG;
   constexpr int NBL = n / (64 * R);          // butterflies per lane
@@ -437,16 +454,16 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
     wave_sync();
 #pragma unroll
     for (int q = 0; q < n / 64; q++) v[q] = xch[xpad(lane + 64 * q)];
-    wave_pass<LOGN, NSLOG + LR, INV, false, TWP, WL>(v, xch, tw, lane, wl);
+    wave_pass<LOGN, NSLOG + LR, INV, false, TWP, WL, MAXLR>(v, xch, tw, lane, wl);
   }
 }
 
 // In-place canonical-distribution FFT of one wavefront.  HALF: inputs with
 // index >= n/2 are known to be zero (v[q] for q >= n/128 are ignored).  TWP:
 // tw is an LDS table stored at twpad() slots.
-template <int LOGN, bool INV, bool HALF, bool TWP = false>
+template <int LOGN, bool INV, bool HALF, bool TWP = false, int MAXLR = 3>
 __device__ __forceinline__ void wave_fft(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane) {
-  wave_pass<LOGN, 0, INV, HALF, TWP>(v, xch, tw, lane);
+  wave_pass<LOGN, 0, INV, HALF, TWP, false, MAXLR>(v, xch, tw, lane);
 }
 // The same with the last pass's twiddles supplied by the caller (wl[1..7]; a
 // radix-8 last pass with one butterfly per lane, k = lane): a caller that runs

@@ -9,16 +9,17 @@ struct ddm_fit_batch;
 namespace ddm {
 
 // block spatial kernels: n = frame side, a power of two in [16, 2048] or 3 * 2^m in [48, 1536]
-hipError_t launch_rows(int dtype, int n, const void* frames, int nframes, float scale, const float* win,
+// maxlr (every FFT launcher): the radix cap, 3 (radix 8) or 2 (radix 4)
+hipError_t launch_rows(int dtype, int n, int maxlr, const void* frames, int nframes, float scale, const float* win,
                        const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);
 int cols_frames_per_wg(int n);
-hipError_t launch_cols(int n, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
+hipError_t launch_cols(int n, int maxlr, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                        const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
                        hipStream_t st);
 hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                            float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st);
-hipError_t launch_temporal_w(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st);
@@ -42,9 +43,9 @@ hipError_t launch_bin_finalize(int logt, int lm, const float* S_part, const floa
                                const double2* tw64, double* Sbin, double* Gbin, double* pbin, float* Pbin, float* radD,
                                float* radP, hipStream_t st);
 
-hipError_t launch_rows_w(int dtype, int logn, const void* frames, int nframes, float scale, const float* win,
+hipError_t launch_rows_w(int dtype, int logn, int maxlr, const void* frames, int nframes, float scale, const float* win,
                          const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);
-hipError_t launch_cols_w(int logn, int F, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
+hipError_t launch_cols_w(int logn, int F, int maxlr, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
                          float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                          int first_slot, hipStream_t st);
 
@@ -134,9 +135,10 @@ const float2* engine_tw_f(const ::ddm_engine* e);
 // the persistent temporal kernel over the work items of nvid videos of one
 // transform length 2^logt (vids: device table, item0 increasing; counter: one
 // device int, zeroed here by a fill kernel); grid from temporal_wq_grid
-hipError_t launch_temporal_wq(int logt, const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
+hipError_t launch_temporal_wq(int logt, int maxlr, const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
                               const float2* tw_f, int grid, hipStream_t st);
-int temporal_wq_grid(int logt);
+int temporal_wq_grid(int logt, int maxlr);
+int engine_maxlr(const ::ddm_engine* e);
 hipStream_t engine_stream(::ddm_engine* e);
 void engine_set_capturing(::ddm_engine* e, bool on);
 hipError_t engine_detach_meta(::ddm_engine* e, void** owned3);

@@ -83,6 +83,7 @@ typedef struct ddm_tuning {
   int32_t chunk_mb;        /* spatial chunk budget in MiB of row spectra (default 128) */
   int32_t wave_spatial;    /* 1: workgroup-level spatial FFTs, 2: wavefront-level (N = 512, 1024 only) */
   int32_t cols_f;          /* frames per column workgroup of the wavefront kernel: 8 or 16 */
+  int32_t fft_radix;       /* largest FFT radix of the spatial and temporal passes: 8 (default) or 4 */
 } ddm_tuning;
 
 /* DdmEngine.create (src/ddm-engine.ts:125-149): allocate device buffers for the

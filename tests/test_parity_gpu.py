@@ -44,7 +44,10 @@ def test_spectrum_parity(gpu, N, dtype):
     assert err < 1e-5, err
 
 
-@pytest.mark.parametrize("N,tuning", [(1024, dict(wave_spatial=2, cols_f=8)), (1024, dict(wave_spatial=2, cols_f=16)),
+@pytest.mark.parametrize("N,tuning", [(1024, dict(wave_spatial=2, cols_f=8, fft_radix=4)),
+                                      (512, dict(wave_spatial=2, fft_radix=4)), (256, dict(fft_radix=4)),
+                                      (384, dict(fft_radix=4)),
+                                      (1024, dict(wave_spatial=2, cols_f=8)), (1024, dict(wave_spatial=2, cols_f=16)),
                                       (1024, dict(wave_spatial=1)), (512, dict(wave_spatial=1)), (2048, None)])
 def test_spectrum_parity_large(gpu, N, tuning):
     """The wavefront spatial kernels at N = 1024 (radix 8, 8, 4, 4; 8- and 16-frame column tiles), the
@@ -159,9 +162,9 @@ def _radial_errors(gD, gP, refD, refP, nmodes, mask=None):
     return rel, harness, prel
 
 
-def _run_radial(fr, T, L, NB, kmin, kmax, order, mask):
+def _run_radial(fr, T, L, NB, kmin, kmax, order, mask, tuning=None):
     N = fr.shape[1]
-    with DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=kmin, kmax=kmax) as e:
+    with DdmEngine.create(N=N, T=T, L=L, NB=NB, kmin=kmin, kmax=kmax, tuning=tuning) as e:
         e.ingestFrames(fr, 0)
         e.computeTemporal(order, mask)
         r = e.readRadial()
@@ -264,6 +267,22 @@ def test_radial_parity_ring_long(gpu, T, case):
     rel, harness, prel = _radial_errors(r.D.reshape(NB, T), r.P, refD, refP, bins["nModes"], mask)
     print(f"rel p99 {np.quantile(rel, .99):.2e} p99.9 {np.quantile(rel, .999):.2e} max {rel.max():.2e} "
           f"harness {harness:.2e} P {prel:.2e}")
+    assert harness < 1e-5 and prel < 1e-6
+    assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-4, rel.max()
+
+
+@pytest.mark.parametrize("T", [64, 200, 300, 1000, 2000])
+def test_radial_parity_radix4(gpu, T):
+    """The autotuner's other FFT radix (radix-4 passes only, in the wavefront temporal transforms of every
+    length, TPAD 256 .. 4096): the ring case against the FP64 oracle, per element as test_radial_parity_ring_long."""
+    N, NB = 64, 32
+    fr = _gpu_video(T, N, seed=T + 3)
+    order, mask = _ring_meta(T, "ring", seed=T + 1)
+    print(f"radix-4 parity T={T}", flush=True)
+    r, h, bins = _run_radial(fr, T, T, NB, 0, N // 2, order, mask, tuning=dict(fft_radix=4))
+    refD, refP = O.analyze_radial(h.astype(np.complex128), order, mask, bins, T, split=True)
+    rel, harness, prel = _radial_errors(r.D.reshape(NB, T), r.P, refD, refP, bins["nModes"], mask)
+    print(f"rel p99.9 {np.quantile(rel, .999):.2e} max {rel.max():.2e} harness {harness:.2e} P {prel:.2e}")
     assert harness < 1e-5 and prel < 1e-6
     assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-4, rel.max()
 
