@@ -213,6 +213,15 @@ __device__ __forceinline__ void temporal_w_setup(float2* lds, const float2* __re
   if constexpr (LOGF == 9 && MAXLR == 3) butterfly_twiddles<8, false, true>(twl, lane, wlast);
 }
 
+// Register budget switches of the temporal kernel (transform lengths 2^LOGT up
+// to these keep the |Y|^2 sums in registers / prefetch the next series).
+#ifndef DDM_YREG_MAX_LOGT
+#define DDM_YREG_MAX_LOGT 10
+#endif
+#ifndef DDM_PREFETCH_MAX_LOGT
+#define DDM_PREFETCH_MAX_LOGT 11
+#endif
+
 // One work item (up to 64 modes of one bin) of one video's temporal stage;
 // A.items[item] names it and its partials go to row `item` of A's outputs.
 template <int LOGT, int LM, int MAXLR>
@@ -257,11 +266,25 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
   const bool dense = __all(mbits == all_bits);
   // the wave's accumulators behind its exchange buffer: sum |Y(k)|^2 at
   // [0, n) (r-major for LM > 0: index r n/m + j holds k = m j + r; k_bin_sum
-  // undoes the order) and sum p(t) at [n, 3n/2).  A wave sums at most 16
+  // undoes the order) and, at the end, sum p(t) at [n, 3n/2).  A wave sums at most 16
   // modes of an item plainly (positive terms: relative error <= 15 ulp); the
   // items' partials are added with compensation in k_bin_sum.
   float* acc = reinterpret_cast<float*>(xch + xch_size(NF));
-  for (int i = lane; i < n + n / 2; i += 64) acc[i] = 0.f;
+  // sum |Y|^2 in registers too where they fit (lane-owned frequencies: the
+  // sub-transform outputs are canonical), else read-modify-write in LDS
+  constexpr bool YREG = LOGT <= DDM_YREG_MAX_LOGT;
+  float yacc[YREG ? n / 64 : 1];
+  if constexpr (YREG) {
+#pragma unroll
+    for (int i = 0; i < n / 64; i++) yacc[i] = 0.f;
+  } else {
+    for (int i = lane; i < n; i += 64) acc[i] = 0.f;
+  }
+  // sum p(t) of the wave's modes: lane-owned samples t = lane + 64 q, kept in
+  // registers (stored behind the |Y|^2 sums once the item is done)
+  float pacc[PT];
+#pragma unroll
+  for (int q = 0; q < PT; q++) pacc[q] = 0.f;
   float aP = 0.f, cP = 0.f;
   const int lpl = KT <= 16 ? 4 : (KT <= 32 ? 2 : 1);   // lanes per tail lag (wave-uniform; KT <= 64)
   float Gt = 0.f, Gtc = 0.f;   // tail lag T - KT + lane / lpl (lanes lpl j, j < KT)
@@ -297,7 +320,7 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
   };
   // next round's series, loaded while the current one is transformed (up to
   // n = 2048; above that the registers are needed by the split transforms)
-  constexpr bool PREFETCH = LOGT <= 11;
+  constexpr bool PREFETCH = LOGT <= DDM_PREFETCH_MAX_LOGT;
   float2 nx[PREFETCH ? PT : 1];
   if constexpr (PREFETCH) load_series(nx, wave);
   wave_sync();
@@ -396,9 +419,13 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
       wave_fft<LOGF, false, true, true, MAXLR>(v, xch, twl, lane);
       if (active) {
 #pragma unroll
-        for (int q = 0; q < PF; q++) acc[lane + 64 * q] += v[q].x * v[q].x + v[q].y * v[q].y;
+        for (int q = 0; q < PF; q++) {
+          const float a2 = v[q].x * v[q].x + v[q].y * v[q].y;
+          if constexpr (YREG) yacc[q] += a2;
+          else acc[lane + 64 * q] += a2;
+        }
 #pragma unroll
-        for (int q = 0; q < PT; q++) acc[n + lane + 64 * q] += y[q].x * y[q].x + y[q].y * y[q].y;
+        for (int q = 0; q < PT; q++) pacc[q] += y[q].x * y[q].x + y[q].y * y[q].y;
       }
     } else {
       // u_r(t1) = w_n^{r t1} sum_{t2 < m/2} y(t1 + t2 n/m) w_m^{r t2}: for
@@ -438,14 +465,24 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
         else wave_fft<LOGF, false, false, true, MAXLR>(v, xch, twl, lane);
         if (active) {
 #pragma unroll
-          for (int q = 0; q < PF; q++) acc[r * NF + lane + 64 * q] += v[q].x * v[q].x + v[q].y * v[q].y;
+          for (int q = 0; q < PF; q++) {
+            const float a2 = v[q].x * v[q].x + v[q].y * v[q].y;
+            if constexpr (YREG) yacc[r * PF + q] += a2;
+            else acc[r * NF + lane + 64 * q] += a2;
+          }
         }
       }
       if (active) {
 #pragma unroll
-        for (int q = 0; q < PT; q++) acc[n + lane + 64 * q] += y[q].x * y[q].x + y[q].y * y[q].y;
+        for (int q = 0; q < PT; q++) pacc[q] += y[q].x * y[q].x + y[q].y * y[q].y;
       }
     }
+  }
+#pragma unroll
+  for (int q = 0; q < PT; q++) acc[n + lane + 64 * q] = pacc[q];
+  if constexpr (YREG) {
+#pragma unroll
+    for (int i = 0; i < n / 64; i++) acc[(i / PF) * NF + lane + 64 * (i % PF)] = yacc[i];
   }
   __syncthreads();
   {
