@@ -26,7 +26,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
               "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
-KERNEL_SOURCES = ["ddm_kernels.hip", "ddm_spatial.hip", "ddm_fit.hip", "ddm_mlp.hip", "ddm_synth.hip"]
+KERNEL_SOURCES = ["ddm_kernels.hip", "ddm_temporal.hip", "ddm_spatial.hip", "ddm_fit.hip", "ddm_mlp.hip", "ddm_synth.hip"]
 HOST_SOURCES = ["ddm_engine.cpp", "ddm_fitter.cpp", "ddm_mlp_host.cpp", "ddm_synth_host.cpp", "ddm_batch.cpp",
                 "ddm_verdict.cpp", "ddm_clock.cpp"]
 

@@ -28,6 +28,7 @@ struct ddm_batch {
   // work-queue mode: one persistent temporal launch per transform length
   struct Queue {
     int logt = 0, maxlr = 3, nwork = 0, grid = 0;
+    int coop = 0;                         // 1: the cooperative kernel's queue form
     std::vector<int> vids;                // videos in the queue, in item order
     ddm::TemporalArgs* dargs = nullptr;   // device table, one entry per video
     const float2 *tw = nullptr, *tw_f = nullptr;
@@ -97,8 +98,11 @@ int enqueue_all(ddm_batch* b, bool capturing) {
   if (rc || b->queues.empty()) return rc;
   for (size_t q = 0; q < b->queues.size(); q++) {
     const ddm_batch::Queue& Q = b->queues[q];
-    hipError_t he = ddm::launch_temporal_wq(Q.logt, Q.maxlr, Q.dargs, (int)Q.vids.size(), Q.nwork, b->counters + q, Q.tw,
-                                            Q.tw_f, Q.grid, b->s0);
+    hipError_t he =
+        Q.coop ? ddm::launch_temporal_cq(Q.logt, Q.maxlr, Q.dargs, (int)Q.vids.size(), Q.nwork, b->counters + q, Q.tw,
+                                         Q.tw_f, Q.grid, b->s0)
+               : ddm::launch_temporal_wq(Q.logt, Q.maxlr, Q.dargs, (int)Q.vids.size(), Q.nwork, b->counters + q, Q.tw,
+                                         Q.tw_f, Q.grid, b->s0);
     if (he != hipSuccess) return bfail(b, DDM_EHIP, std::string("batch: work-queue launch: ") + hipGetErrorString(he));
   }
   hipEvent_t mid = new_event(b);
@@ -111,23 +115,25 @@ int enqueue_all(ddm_batch* b, bool capturing) {
 int build_queues(ddm_batch* b) {
   const int n = (int)b->eng.size();
   b->queued.assign(n, 0);
-  // one queue per (transform length, radix cap): slot 2 logt + (cap == 2)
-  std::vector<std::vector<ddm::TemporalArgs>> args(26);
-  std::vector<ddm_batch::Queue> qs(26);
+  // one queue per (kernel, transform length, radix cap): slot 32 coop + 2 logt + (cap == 2)
+  std::vector<std::vector<ddm::TemporalArgs>> args(64);
+  std::vector<ddm_batch::Queue> qs(64);
   for (int i = 0; i < n; i++) {
     ddm::TemporalArgs a{};
     int logt = 0;
     int rc = ddm::engine_temporal_plan(b->eng[i], b->order[i].data(), b->mask[i].data(), &a, &logt);
     if (rc) return bfail(b, rc, "batch: video " + std::to_string(i) + ": " + ddm_last_error(b->eng[i]));
     const int ni = ddm::engine_nitems(b->eng[i]);
-    if (logt < 8 || logt > 12 || ni < 1) continue;   // the video's own computeTemporal
+    if (logt < 8 || logt > 13 || ni < 1) continue;   // the video's own computeTemporal
     const int maxlr = ddm::engine_maxlr(b->eng[i]);
-    const int slot = 2 * logt + (maxlr == 2 ? 1 : 0);
+    const int coop = ddm::engine_tkern(b->eng[i]);
+    const int slot = 32 * coop + 2 * logt + (maxlr == 2 ? 1 : 0);
     ddm_batch::Queue& Q = qs[slot];
     a.item0 = Q.nwork;
     Q.nwork += ni;
     Q.logt = logt;
     Q.maxlr = maxlr;
+    Q.coop = coop;
     Q.vids.push_back(i);
     if (!Q.tw) {
       Q.tw = ddm::engine_tw_t(b->eng[i]);
@@ -136,10 +142,10 @@ int build_queues(ddm_batch* b) {
     args[slot].push_back(a);
     b->queued[i] = 1;
   }
-  for (int l = 16; l < 26; l++) {
+  for (int l = 0; l < 64; l++) {
     if (qs[l].vids.empty()) continue;
     ddm_batch::Queue& Q = qs[l];
-    const int g = ddm::temporal_wq_grid(Q.logt, Q.maxlr);
+    const int g = Q.coop ? ddm::temporal_cq_grid(Q.logt, Q.maxlr) : ddm::temporal_wq_grid(Q.logt, Q.maxlr);
     if (g < 1) return bfail(b, DDM_EHIP, "batch: work-queue occupancy query");
     Q.grid = std::min(g, Q.nwork);
     hipError_t he = hipMalloc((void**)&Q.dargs, args[l].size() * sizeof(ddm::TemporalArgs));

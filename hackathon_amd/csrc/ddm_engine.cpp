@@ -233,6 +233,7 @@ struct ddm_engine {
   int chunk = 0;
   int cols_f = 16;        // frames per column workgroup (wavefront kernels)
   int maxlr = 3;          // FFT radix cap (log2): 3 radix 8, 2 radix 4
+  int tkern = 0;          // temporal kernel: 0 one mode per wave (one per workgroup above TPAD 4096), 1 cooperative
   bool wave_spatial = false;
   int mean_frame = -1;    // rowinfo frame of the last frame ingested (-1: none yet)
   int mean_dtype = 0;
@@ -387,7 +388,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   if (tp) tn = *tp;
   if (tn.temporal_items < 0 || tn.chunk_mb < 0 || tn.wave_spatial < 0 || tn.wave_spatial > 2 ||
       (tn.cols_f != 0 && tn.cols_f != 8 && tn.cols_f != 16) ||
-      (tn.fft_radix != 0 && tn.fft_radix != 4 && tn.fft_radix != 8))
+      (tn.fft_radix != 0 && tn.fft_radix != 4 && tn.fft_radix != 8) || tn.temporal_kernel < 0 ||
+      tn.temporal_kernel > 2)
     return fail(nullptr, DDM_EINVAL, "ddm_create: tuning out of range");
   *out = nullptr;
   const ddm_geometry g = *gp;
@@ -432,7 +434,14 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   // Temporal work items: (bin, first mode, count); up to M modes of one bin.
   const long target_items =
       tn.temporal_items > 0 ? tn.temporal_items : env_long("DDM_TEMPORAL_ITEMS", 2048, 1, 1L << 22);
-  int M = (int)std::min<long>(64, std::max<long>(1, (binned + target_items - 1) / target_items));
+  // temporal kernel: the cooperative one (every wave of a workgroup on one
+  // mode) takes TPAD >= 2048; at TPAD 8192 its items hold at most 16 modes
+  // (its per-wave sums are one-level there)
+  // (0: the default, the cooperative kernel wherever it applies)
+  const long tk = tn.temporal_kernel ? tn.temporal_kernel : env_long("DDM_TEMPORAL_KERNEL", 0, 0, 2);
+  e->tkern = tk != 1 && e->logt >= 11 ? 1 : 0;
+  const int Mcap = e->tkern && e->logt >= 13 ? 16 : 64;
+  int M = (int)std::min<long>(Mcap, std::max<long>(1, (binned + target_items - 1) / target_items));
   std::vector<int> items_h, bin_modes_h(std::max<long>(binned, 1));
   e->bin_items_h.assign(g.NB + 1, 0);
   for (int b = 0; b < g.NB; b++) {
@@ -462,6 +471,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->tune.cols_f = e->cols_f;
   e->maxlr = (tn.fft_radix ? tn.fft_radix : env_long("DDM_FFT_RADIX", 8, 4, 8)) == 4 ? 2 : 3;
   e->tune.fft_radix = e->maxlr == 2 ? 4 : 8;
+  e->tune.temporal_kernel = e->tkern ? 2 : 1;
   int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
   chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
   e->chunk = chunk;
@@ -661,6 +671,10 @@ static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* m
       (he = timed(e, DDM_K_TEMPORAL, [&] {
          const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
          const int identity = e->last_identity ? 1 : 0;
+         if (e->tkern)        // cooperative kernel (512-point sub-transforms, r-major spectra)
+           return ddm::launch_temporal_c(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes,
+                                         e->order, e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part,
+                                         e->p_part, e->P_part, e->cur_KT, e->G_part, e->stream);
          if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
            return ddm::launch_temporal_w(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
                                          e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part, e->p_part,
@@ -966,7 +980,7 @@ int engine_temporal_end(ddm_engine* e) {
   hipError_t he = timed(e, DDM_K_FINALIZE, [&] {
     // the wavefront temporal kernel splits TPAD > 512 into 2^(logt-9)
     // sub-transforms and stores their spectra r-major
-    const int lm = e->logt <= 12 && e->logt > 9 ? e->logt - 9 : 0;
+    const int lm = (e->tkern || e->logt <= 12) && e->logt > 9 ? e->logt - 9 : 0;
     return ddm::launch_bin_finalize(e->logt, lm, e->S_part, e->p_part, e->P_part, e->G_part, e->cur_KT,
                                     e->bin_items, e->g.NB, e->nmodes, e->mask, e->cur_full ? 1 : 0, e->npairs,
                                     e->g.T, e->g.L, e->last_n0, e->tw_t64, e->Sbin, e->Gbin, e->pbin, e->Pbin,
@@ -987,9 +1001,11 @@ int engine_temporal_plan(ddm_engine* e, const uint32_t* order, const float* mask
   if (rc) return rc;
   *a = TemporalArgs{e->hist, e->items, e->bin_modes, e->order, e->mask, e->S_part, e->p_part, e->P_part,
                     e->G_part, e->g.T, identity ? 1 : 0, KT, 0, n0 > 0.5f ? 1.0f / n0 : 0.0f};
-  *logt = e->logt <= 12 ? e->logt : 0;
+  *logt = e->logt <= 12 || e->tkern ? e->logt : 0;
   return DDM_OK;
 }
+
+int engine_tkern(const ddm_engine* e) { return e->tkern; }
 
 int engine_nitems(const ddm_engine* e) { return e->nitems; }
 int engine_maxlr(const ddm_engine* e) { return e->maxlr; }

@@ -23,6 +23,12 @@ hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, con
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st);
+// the workgroup-cooperative form for logt 11..13 (items of at most 64 modes,
+// 16 at logt 13; S_part rows r-major with lm = logt - 9)
+hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
+                             const int* bin_modes, const uint32_t* order, const float* mask, int identity,
+                             float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
+                             float* P_part, int KT, float* G_part, hipStream_t st);
 // per-mode D, C, P of modes m0 .. m0 + nm - 1 (readTemporalFull)
 // (mode_list: entries m0 .. m0 + nm - 1 of a mode list instead; outD/outP may be null)
 hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
@@ -125,7 +131,7 @@ hipError_t launch_synth(int N, int T, int n, int n_sw, double sigma, double amp,
 // computeTemporal in parts, for the batch's work queue: begin (planning and
 // metadata uploads on the engine's stream), the temporal kernel (the caller's),
 // end (radial finalisation); plan gives the kernel arguments with no GPU work
-// (logt 0: the geometry takes the workgroup kernel)
+// (logt 0: the geometry takes the workgroup-per-mode kernel, TPAD 8192 without the cooperative one)
 int engine_temporal_begin(::ddm_engine* e, const uint32_t* order, const float* mask);
 int engine_temporal_end(::ddm_engine* e);
 int engine_temporal_plan(::ddm_engine* e, const uint32_t* order, const float* mask, TemporalArgs* a, int* logt);
@@ -138,6 +144,12 @@ const float2* engine_tw_f(const ::ddm_engine* e);
 hipError_t launch_temporal_wq(int logt, int maxlr, const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,
                               const float2* tw_f, int grid, hipStream_t st);
 int temporal_wq_grid(int logt, int maxlr);
+// the same over the cooperative kernel (logt 11..13)
+hipError_t launch_temporal_cq(int logt, int maxlr, const TemporalArgs* vids, int nvid, int nwork, int* counter,
+                              const float2* tw, const float2* tw_f, int grid, hipStream_t st);
+int temporal_cq_grid(int logt, int maxlr);
+// 1 when the engine's temporal stage runs the cooperative kernel
+int engine_tkern(const ::ddm_engine* e);
 int engine_maxlr(const ::ddm_engine* e);
 hipStream_t engine_stream(::ddm_engine* e);
 void engine_set_capturing(::ddm_engine* e, bool on);
