@@ -234,6 +234,9 @@ struct ddm_engine {
   int cols_f = 16;        // frames per column workgroup (wavefront kernels)
   int maxlr = 3;          // FFT radix cap (log2): 3 radix 8, 2 radix 4
   int tkern = 0;          // temporal kernel: 0 one mode per wave (one per workgroup above TPAD 4096), 1 cooperative
+  int twg_cu = 0;         // cooperative kernel: 0 one workgroup per item, k > 0 a persistent grid of k per CU
+  int cus = 0;            // compute units of the engine's device
+  int* tcounter = nullptr;  // the persistent grid's item counter
   bool wave_spatial = false;
   int mean_frame = -1;    // rowinfo frame of the last frame ingested (-1: none yet)
   int mean_dtype = 0;
@@ -359,7 +362,7 @@ void free_all(ddm_engine* e) {
   void* ptrs[] = {e->hist, e->scratch, e->rowinfo, e->stage, e->win, e->tw_n, e->tw_t, e->tw_t64, e->tw_f, e->wx,
                   e->items, e->bin_modes, e->bin_items, e->nmodes, e->order, e->mask, e->npairs, e->S_part,
                   e->p_part, e->P_part, e->radD, e->radP, e->Gbin, e->pbin, e->Pbin, e->Sbin, e->G_part,
-                  e->fullD, e->fullC, e->fullP, e->drift_acc, e->kxy, e->bin_off};
+                  e->fullD, e->fullC, e->fullP, e->drift_acc, e->kxy, e->bin_off, e->tcounter};
   for (void* p : ptrs)
     if (p) hipFree(p);
   void* hptrs[] = {e->meta_order, e->meta_mask, e->meta_np};
@@ -389,7 +392,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   if (tn.temporal_items < 0 || tn.chunk_mb < 0 || tn.wave_spatial < 0 || tn.wave_spatial > 2 ||
       (tn.cols_f != 0 && tn.cols_f != 8 && tn.cols_f != 16) ||
       (tn.fft_radix != 0 && tn.fft_radix != 4 && tn.fft_radix != 8) || tn.temporal_kernel < 0 ||
-      tn.temporal_kernel > 2)
+      tn.temporal_kernel > 2 || tn.temporal_wg_per_cu < 0 || tn.temporal_wg_per_cu > 16)
     return fail(nullptr, DDM_EINVAL, "ddm_create: tuning out of range");
   *out = nullptr;
   const ddm_geometry g = *gp;
@@ -427,6 +430,12 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   };
   hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (he != hipSuccess) return (hip_fail(e, he, "hipStreamCreate"), bail(DDM_EHIP));
+  {
+    int dev = 0;
+    if ((he = hipGetDevice(&dev)) != hipSuccess ||
+        (he = hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return (hip_fail(e, he, "ddm_create: device query"), bail(DDM_EHIP));
+  }
 
   e->bins = make_bins(g.N, g.NB, g.kmin, g.kmax, g.hypot_mode, g.knodes);
   const long binned = e->bins.offsets[g.NB];
@@ -440,6 +449,8 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   // (0: the default, the cooperative kernel wherever it applies)
   const long tk = tn.temporal_kernel ? tn.temporal_kernel : env_long("DDM_TEMPORAL_KERNEL", 0, 0, 2);
   e->tkern = tk != 1 && e->logt >= 11 ? 1 : 0;
+  e->twg_cu = e->tkern ? (int)(tn.temporal_wg_per_cu ? tn.temporal_wg_per_cu
+                                                     : env_long("DDM_TEMPORAL_WG_PER_CU", 0, 0, 16)) : 0;
   const int Mcap = e->tkern && e->logt >= 13 ? 16 : 64;
   int M = (int)std::min<long>(Mcap, std::max<long>(1, (binned + target_items - 1) / target_items));
   std::vector<int> items_h, bin_modes_h(std::max<long>(binned, 1));
@@ -472,6 +483,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->maxlr = (tn.fft_radix ? tn.fft_radix : env_long("DDM_FFT_RADIX", 8, 4, 8)) == 4 ? 2 : 3;
   e->tune.fft_radix = e->maxlr == 2 ? 4 : 8;
   e->tune.temporal_kernel = e->tkern ? 2 : 1;
+  e->tune.temporal_wg_per_cu = e->twg_cu;
   int chunk = (int)std::max<long>(F, (chunk_bytes / per_frame) / F * F);
   chunk = std::min(chunk, ((g.T + F - 1) / F) * F);
   e->chunk = chunk;
@@ -500,7 +512,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
       (he = dalloc(&e->Gbin, (size_t)g.NB * g.L)) != hipSuccess ||
       (he = dalloc(&e->Sbin, (size_t)g.NB * e->TPAD)) != hipSuccess ||
       (he = dalloc(&e->G_part, nitems * (size_t)e->KT_max)) != hipSuccess || (he = dalloc(&e->pbin, (size_t)g.NB * g.T)) != hipSuccess ||
-      (he = dalloc(&e->Pbin, g.NB)) != hipSuccess) {
+      (he = dalloc(&e->Pbin, g.NB)) != hipSuccess || (he = dalloc(&e->tcounter, 1)) != hipSuccess) {
     hip_fail(e, he, "ddm_create: device allocation");
     return bail(he == hipErrorOutOfMemory ? DDM_ENOMEM : DDM_EHIP);
   }
@@ -671,6 +683,12 @@ static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* m
       (he = timed(e, DDM_K_TEMPORAL, [&] {
          const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
          const int identity = e->last_identity ? 1 : 0;
+         if (e->tkern && e->twg_cu > 0) {   // cooperative kernel on a persistent grid
+           const ddm::TemporalArgs a{e->hist, e->items, e->bin_modes, e->order, e->mask, e->S_part, e->p_part,
+                                     e->P_part, e->G_part, e->g.T, identity, e->cur_KT, 0, inv_n0};
+           return ddm::launch_temporal_cp(e->logt, e->maxlr, a, e->nitems, e->tcounter, e->tw_t, e->tw_f,
+                                          std::min(e->nitems, e->twg_cu * e->cus), e->stream);
+         }
          if (e->tkern)        // cooperative kernel (512-point sub-transforms, r-major spectra)
            return ddm::launch_temporal_c(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes,
                                          e->order, e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part,

@@ -482,11 +482,47 @@ __device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {
   s = t;
 }
 
-// Wavefront sum of a float (all lanes get the result).
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// Cross-lane sums on the DPP lane network (no LDS traffic, no per-step
+// address arithmetic).  dpp<CTRL, ROWS>(v): v of the lane CTRL selects, in
+// the rows of 16 lanes that ROWS enables (0 elsewhere).  Controls used:
+// quad_perm [1,0,3,2] = 0xb1 and [2,3,0,1] = 0x4e, row_half_mirror = 0x141,
+// row_mirror = 0x140, row_bcast15 = 0x142, row_bcast31 = 0x143.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+// Sum over aligned segments of SEG lanes (SEG in 1, 2, 4, 8, 16); every lane
+// of a segment gets its sum: xor 1, xor 2 within quads, then the half-row and
+// row mirrors pair the quads and the half-rows.
+template <int SEG, typename T> __device__ __forceinline__ T seg_sum_dpp(T v) {
+  static_assert(SEG == 1 || SEG == 2 || SEG == 4 || SEG == 8 || SEG == 16, "segment");
+  if constexpr (SEG >= 2) v += dpp<0xb1>(v);
+  if constexpr (SEG >= 4) v += dpp<0x4e>(v);
+  if constexpr (SEG >= 8) v += dpp<0x141>(v);
+  if constexpr (SEG >= 16) v += dpp<0x140>(v);
   return v;
 }
+
+// Wavefront total, returned to every lane (a wave-uniform value): row sums,
+// then row 1 += row 0 and row 3 += row 2 (bcast15), rows 2, 3 += lane 31
+// (bcast31), so lane 63 holds the total.
+template <typename T> __device__ __forceinline__ T wave_total(T v) {
+  v = seg_sum_dpp<16>(v);
+  v += dpp<0x142, 0xa>(v);
+  v += dpp<0x143, 0xc>(v);
+  if constexpr (sizeof(T) == 4 && !__is_same(T, float)) {
+    return (T)__builtin_amdgcn_readlane((int)v, 63);
+  } else {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  }
+}
+
+// Wavefront sum of a float (all lanes get the result).
+__device__ __forceinline__ float wave_sum(float v) { return wave_total(v); }
 
 }  // namespace ddm

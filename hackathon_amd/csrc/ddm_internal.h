@@ -148,6 +148,9 @@ int temporal_wq_grid(int logt, int maxlr);
 hipError_t launch_temporal_cq(int logt, int maxlr, const TemporalArgs* vids, int nvid, int nwork, int* counter,
                               const float2* tw, const float2* tw_f, int grid, hipStream_t st);
 int temporal_cq_grid(int logt, int maxlr);
+// one video's items through a persistent grid of `grid` workgroups (counter: one device int, zeroed here)
+hipError_t launch_temporal_cp(int logt, int maxlr, const TemporalArgs& a, int nwork, int* counter, const float2* tw,
+                              const float2* tw_f, int grid, hipStream_t st);
 // 1 when the engine's temporal stage runs the cooperative kernel
 int engine_tkern(const ::ddm_engine* e);
 int engine_maxlr(const ::ddm_engine* e);

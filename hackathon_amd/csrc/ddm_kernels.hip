@@ -406,8 +406,7 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
           const float dx = b.x - a.x, dy = b.y - a.y;
           g += (xm[t] * xm[KT + t + j]) * (dx * dx + dy * dy);
         }
-      if (lpl >= 2) g += __shfl_xor(g, 1, 64);
-      if (lpl == 4) g += __shfl_xor(g, 2, 64);
+      g = lpl == 4 ? seg_sum_dpp<4>(g) : (lpl == 2 ? seg_sum_dpp<2>(g) : g);
       if (s4 == 0 && j < KT && active) two_sum_add(Gt, Gtc, g);
     }
     if constexpr (LM == 0) {

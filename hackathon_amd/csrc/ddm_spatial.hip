@@ -343,13 +343,8 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
         ia += (int)xa[q];
         ib += (int)xb[q];
       }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        ia += __shfl_xor(ia, o, 64);
-        ib += __shfl_xor(ib, o, 64);
-      }
-      Sa = ia;
-      Sb = ib;
+      Sa = wave_total(ia);
+      Sb = wave_total(ib);
     } else {
       // row mean as an unevaluated (hi, lo) pair: hi from the sum, lo from the residuals
       float sa = 0.f, sb = 0.f;

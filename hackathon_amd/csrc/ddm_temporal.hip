@@ -180,9 +180,10 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     float2 y[m];
 #pragma unroll
     for (int i = 0; i < m; i++) y[i] = nx[i];
-    // the mask bits and the pre-twiddle bases are re-read from opaque copies
-    // each mode: hoisted out of the loop, their per-sample selects and the
-    // 2 (m - 1) pre-twiddles would stay live in registers for the whole item
+    // the mask bits (and at m = 16 the pre-twiddle bases) are re-read from
+    // opaque copies each mode: hoisted out of the loop, their per-sample
+    // selects and the 2 (m - 1) pre-twiddles would stay live in registers for
+    // the whole item
     uint32_t mb = mbits;
     asm volatile("" : "+v"(mb));
     // masked sums: the mean, and sum |y|^2 for P
@@ -253,7 +254,7 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
 #pragma unroll
       for (int j = 0; j < 5; j++) {
         b[j] = twb[c][j];
-        asm volatile("" : "+v"(b[j].x), "+v"(b[j].y));
+        if constexpr (m == 16) asm volatile("" : "+v"(b[j].x), "+v"(b[j].y));
       }
       const int t1x = xpad(tid + 256 * c);
       U[t1x] = a[0];
@@ -270,9 +271,7 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
           const float dx = b.x - a.x, dy = b.y - a.y;
           g += (hm[t] * tm[t + tj]) * (dx * dx + dy * dy);
         }
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1)
-        if (o < lpl) g += __shfl_xor(g, o, 64);
+      g = lpl == 16 ? seg_sum_dpp<16>(g) : (lpl == 8 ? seg_sum_dpp<8>(g) : seg_sum_dpp<4>(g));
       if (ts == 0 && tj < KT) two_sum_add(Gt, Gtc, g);
     }
     // the wave's sub-transforms r = wave + 4 s
@@ -432,6 +431,55 @@ hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, con
 }
 
 __global__ void k_fill_counter(int* __restrict__ p) { *p = 0; }
+
+// One video's items through a persistent grid smaller than the item count
+// (the analysis's arguments by value; *counter zero before the launch): a
+// grid of a few workgroups per CU leaves room on every CU for another
+// analysis's memory-bound spatial kernels.
+template <int LOGT, int MAXLR, bool TWO_LEVEL>
+__global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cp(const TemporalArgs A, int nwork,
+                                                                       int* __restrict__ counter,
+                                                                       const float2* __restrict__ tw_g,
+                                                                       const float2* __restrict__ tw_f) {
+  extern __shared__ float2 lds[];
+  __shared__ int next;
+  float2 twb[2][5], wlast[8];
+  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast);
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = next;
+    if (w >= nwork) break;
+    temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, w, lds, twb, wlast);
+    __syncthreads();
+  }
+}
+
+template <int LOGT, int MAXLR, bool TWO_LEVEL>
+static hipError_t launch_temporal_cp_t(const TemporalArgs& a, int nwork, int* counter, const float2* tw,
+                                       const float2* tw_f, int grid, hipStream_t st) {
+  const size_t lds = sizeof(float2) * TcLayout<LOGT>::SIZE;
+  auto kern = k_temporal_cp<LOGT, MAXLR, TWO_LEVEL>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_counter, dim3(1), dim3(1), 0, st, counter);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, a, nwork, counter, tw, tw_f);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal_cp(int logt, int maxlr, const TemporalArgs& a, int nwork, int* counter, const float2* tw,
+                              const float2* tw_f, int grid, hipStream_t st) {
+  if (nwork < 1 || grid < 1 || a.KT < 1 || a.KT > 64 || a.KT > a.T || 2 * a.T > (1 << logt))
+    return hipErrorInvalidValue;
+  switch (logt * 4 + (maxlr == 2 ? 2 : 3)) {
+#define CASE(LG_, R_, TL_) \
+  case LG_ * 4 + R_: return launch_temporal_cp_t<LG_, R_, TL_>(a, nwork, counter, tw, tw_f, grid, st);
+    CASE(11, 2, true) CASE(12, 2, true) CASE(13, 2, false)
+    CASE(11, 3, true) CASE(12, 3, true) CASE(13, 3, false)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL>
 static hipError_t launch_temporal_cq_t(const TemporalArgs* vids, int nvid, int nwork, int* counter, const float2* tw,

@@ -55,7 +55,7 @@ class Tuning(C.Structure):
     """Launch-shape tuning (0 = default): see ddm_tuning in include/ddm.h."""
     _fields_ = [("temporal_items", C.c_int32), ("chunk_mb", C.c_int32), ("wave_spatial", C.c_int32),
                 ("cols_f", C.c_int32), ("fft_radix", C.c_int32),
-                ("temporal_kernel", C.c_int32)]
+                ("temporal_kernel", C.c_int32), ("temporal_wg_per_cu", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

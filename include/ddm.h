@@ -86,6 +86,8 @@ typedef struct ddm_tuning {
   int32_t fft_radix;       /* largest FFT radix of the spatial and temporal passes: 8 (default) or 4 */
   int32_t temporal_kernel; /* 1: one mode per wavefront (per workgroup above TPAD 4096), 2: one mode per
                               workgroup of four cooperating wavefronts (TPAD >= 2048) */
+  int32_t temporal_wg_per_cu; /* cooperative kernel: 0 one workgroup per work item; k in 1..16 a persistent
+                                 grid of k workgroups per CU taking the items from a counter */
 } ddm_tuning;
 
 /* DdmEngine.create (src/ddm-engine.ts:125-149): allocate device buffers for the
