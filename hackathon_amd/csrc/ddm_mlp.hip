@@ -5,14 +5,17 @@
 // GEMM: C[M x N] (f32) = A[M x K] (bf16) * B[K x N] (bf16), operands addressed
 // through (row, col) strides so the same kernel serves X W, dY W^T and X^T dY.
 // A workgroup of 4 waves computes a 64 x 64 tile (each wave 32 x 32 with
-// v_mfma_f32_32x32x8_bf16), K staged through LDS 32 at a time.
+// gfx950's v_mfma_f32_32x32x16_bf16), K staged through LDS 32 at a time from
+// 16-byte global loads.
+#include <cstdint>
 #include "ddm_internal.h"
 
 
 namespace ddm {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef unsigned short ushort8_t __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ unsigned short f2bf(float x) {
   // round to nearest even; NaN stays NaN
@@ -26,45 +29,89 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 namespace {
 constexpr int GT = 64;   // output tile
 constexpr int GK = 32;   // K chunk
+constexpr int KP = GK + 8;   // LDS row pitch (elements): 80 B rows keep the 16-byte k reads of a wave spread over the banks
 }
 
-// C = alpha_acc ? C + A B : A B.  A(i, k) = A[i * sai + k * sak], B(k, j) = B[k * sbk + j * sbj].
-__global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restrict__ A, long sai, long sak,
-                                                   const unsigned short* __restrict__ B, long sbk, long sbj,
-                                                   float* __restrict__ C, long ldc, int M, int N, int K, int acc) {
-  __shared__ unsigned short As[GT][GK + 4];
-  __shared__ unsigned short Bs[GK][GT + 4];
+// Operand staging of one K chunk: A(i, k) for the tile's 64 rows and B(k, j)
+// for its 64 columns, both stored k-contiguous in LDS (As[i][k], Bt[j][k]) so
+// that every lane reads its 8 consecutive k of an MFMA operand with one
+// 16-byte LDS load.  Each thread fetches one 8-element run along the operand's
+// contiguous global dimension (16-byte loads where the run is inside the
+// matrix and aligned, `vec`), else element by element with bounds checks.
+struct GemmOp {
+  const unsigned short* p;
+  long s_outer, s_k;   // A: (s_i, s_k); B: (s_j, s_k)
+  int n_outer;         // M for A, N for B
+  int kmajor;          // 1: k is the contiguous dimension (s_k == 1), 0: the outer one is (s_outer == 1)
+  int vec;             // 16-byte loads allowed
+};
+
+__device__ __forceinline__ void gemm_fetch(const GemmOp& o, int t, int o0, int k0, int K, ushort8_t& r) {
+  // run t of 256: kmajor -> outer = t / 4, k = 8 (t % 4); else k = t / 8, outer = 8 (t % 8)
+  const int oo = o.kmajor ? t / 4 : 8 * (t % 8);
+  const int kk = o.kmajor ? 8 * (t % 4) : t / 8;
+  const int go = o0 + oo, gk = k0 + kk;
+  if (o.kmajor) {
+    if (o.vec && go < o.n_outer && gk + 8 <= K) {
+      r = *reinterpret_cast<const ushort8_t*>(o.p + go * o.s_outer + gk);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) r[v] = (go < o.n_outer && gk + v < K) ? o.p[go * o.s_outer + gk + v] : (unsigned short)0;
+    }
+  } else {
+    if (o.vec && go + 8 <= o.n_outer && gk < K) {
+      r = *reinterpret_cast<const ushort8_t*>(o.p + go + gk * o.s_k);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) r[v] = (go + v < o.n_outer && gk < K) ? o.p[go + v + gk * o.s_k] : (unsigned short)0;
+    }
+  }
+}
+
+__device__ __forceinline__ void gemm_stash(const GemmOp& o, int t, const ushort8_t& r, unsigned short (*S)[KP]) {
+  if (o.kmajor) {
+    *reinterpret_cast<ushort8_t*>(&S[t / 4][8 * (t % 4)]) = r;
+  } else {
+#pragma unroll
+    for (int v = 0; v < 8; v++) S[8 * (t % 8) + v][t / 8] = r[v];
+  }
+}
+
+// C = acc ? C + A B : A B on v_mfma_f32_32x32x16_bf16 (gfx950): a workgroup of
+// 4 waves computes a 64 x 64 tile, each wave 32 x 32, K in chunks of 32 (two
+// MFMAs per wave per chunk); the next chunk's operands are fetched from global
+// memory into registers while the current chunk is multiplied.
+// MFMA operand layout: lane l holds A(i0 + l % 32, k0 + 8 (l / 32) + v) and
+// B(k0 + 8 (l / 32) + v, j0 + l % 32), v = 0..7; C register r holds row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmOp a, GemmOp b, float* __restrict__ C, long ldc, int M, int N,
+                                                   int K, int acc) {
+  __shared__ __attribute__((aligned(16))) unsigned short As[GT][KP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bt[GT][KP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
   const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
   float16_t c = {0};
+  ushort8_t ra, rb;
+  gemm_fetch(a, tid, i0, 0, K, ra);
+  gemm_fetch(b, tid, j0, 0, K, rb);
   for (int k0 = 0; k0 < K; k0 += GK) {
-    // stage 64 x 32 of A and 32 x 64 of B (8 elements per thread each)
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int e = tid + 256 * r;
-      const int ai = e / GK, ak = e % GK;
-      const int gi = i0 + ai, gk = k0 + ak;
-      As[ai][ak] = (gi < M && gk < K) ? A[gi * sai + gk * sak] : (unsigned short)0;
-      const int bk = e / GT, bj = e % GT;
-      const int gk2 = k0 + bk, gj = j0 + bj;
-      Bs[bk][bj] = (gk2 < K && gj < N) ? B[gk2 * sbk + gj * sbj] : (unsigned short)0;
-    }
+    __syncthreads();   // the previous chunk's LDS reads are done
+    gemm_stash(a, tid, ra, As);
+    gemm_stash(b, tid, rb, Bt);
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GK; kk += 8) {
-      short4_t a, b;
-      const int kb = kk + 4 * (lane >> 5);
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        a[v] = (short)As[wi + (lane & 31)][kb + v];
-        b[v] = (short)Bs[kb + v][wj + (lane & 31)];
-      }
-      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    if (k0 + GK < K) {
+      gemm_fetch(a, tid, i0, k0 + GK, K, ra);
+      gemm_fetch(b, tid, j0, k0 + GK, K, rb);
     }
-    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 16) {
+      const int kb = kk + 8 * (lane >> 5);
+      const bf16x8_t av = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const ushort8_t*>(&As[wi + (lane & 31)][kb]));
+      const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const ushort8_t*>(&Bt[wj + (lane & 31)][kb]));
+      c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+    }
   }
-  // C layout: register r -> row 8 (r/4) + 4 (lane/32) + r%4, column lane%32
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int gi = i0 + wi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -149,10 +196,29 @@ __global__ void k_to_bf16(const float* __restrict__ x, unsigned short* __restric
   if (i < n) y[i] = f2bf(x[i]);
 }
 
+// One operand's staging description: the contiguous global dimension (k, or
+// the outer one) and whether its 8-element runs may be read as aligned
+// 16-byte loads (base 16-byte aligned, every run start a multiple of 8).
+static GemmOp gemm_op(const unsigned short* p, long s_outer, long s_k, int n_outer) {
+  GemmOp o{p, s_outer, s_k, n_outer, 0, 0};
+  const bool base16 = ((uintptr_t)p & 15) == 0;
+  if (s_k == 1) {
+    o.kmajor = 1;
+    o.vec = base16 && s_outer % 8 == 0;
+  } else if (s_outer == 1) {
+    o.kmajor = 0;
+    o.vec = base16 && s_k % 8 == 0;
+  } else {
+    o.kmajor = 1;   // neither dimension contiguous: element loads
+  }
+  return o;
+}
+
 hipError_t launch_gemm_bf16(const unsigned short* A, long sai, long sak, const unsigned short* B, long sbk, long sbj,
                             float* C, long ldc, int M, int N, int K, int acc, hipStream_t st) {
   dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, A, sai, sak, B, sbk, sbj, C, ldc, M, N, K, acc);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, gemm_op(A, sai, sak, M), gemm_op(B, sbj, sbk, N), C, ldc,
+                     M, N, K, acc);
   return hipGetLastError();
 }
 
