@@ -304,33 +304,40 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
 
 template <typename T> __device__ __forceinline__ float px_to_float(T v) { return (float)v; }
 
-// FPW frames per workgroup (the same 16 rows of each): every wave transforms
-// two row pairs per frame, and the next pair's pixels are loaded while the
-// current pair is transformed (also across frames), so a wave keeps a load in
-// flight through its whole run and its set-up (twiddles, LDS addresses) is
-// shared by 2 FPW pairs.
+// FPW frames per workgroup (the same rows of each): every wave transforms
+// PPW row pairs per frame (two up to N = 1024: 16 rows per workgroup; one at
+// N = 2048: 8 rows), and the next pair's pixels are loaded while the current
+// pair is transformed (also across frames), so a wave keeps a load in flight
+// through its whole run and its set-up (twiddles, LDS addresses) is shared by
+// PPW FPW pairs.  At N = 2048 the kx-major tile shares its LDS with the
+// exchange buffers: a wave keeps its untangled spectra in registers until
+// every wave is done with its transform.
 template <int LOGN, typename Tin, int MAXLR, int FPW>
 __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, int nframes, float scale,
                                                const float* __restrict__ win, const float2* __restrict__ tw,
                                                float2* __restrict__ scratch, int2* __restrict__ rowinfo) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
-  constexpr int RW = 16;                       // rows per workgroup (4 per wave)
+  constexpr bool ALIAS = LOGN >= 11;           // tile and exchange buffers share the LDS
+  constexpr int PPW = ALIAS ? 1 : 2;           // row pairs per wave and frame
+  constexpr int RW = 8 * PPW;                  // rows per workgroup
   constexpr int XS = xch_size(N);
   constexpr int TS = RW + 1;                   // padded tile row
+  constexpr int KQ = P / 2 + 1;                // untangled outputs per lane: k = lane + 64 q <= N/2
   constexpr bool IS_INT = !std::is_same<Tin, float>::value;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * XS;
-  float2* tile = lds + 4 * XS;                 // NH x TS
+  float2* tile = ALIAS ? lds : lds + 4 * XS;   // NH x TS
   const int groups = N / RW;
   const int f0 = (blockIdx.x / groups) * FPW;
   const int y0 = (blockIdx.x % groups) * RW;
   if (f0 >= nframes) return;
   const int nf = min(FPW, nframes - f0);       // workgroup-uniform
-  // pair p of this wave: frame f0 + p / 2, rows ra = wave 4 + 2 (p & 1), ra + 1
+  const int npairs = PPW * nf;
+  // pair p of this wave: frame f0 + p / PPW, rows ra = wave 2 PPW + 2 (p % PPW), ra + 1
   auto load_pair = [&](int p, Tin (&a)[P], Tin (&b)[P]) {
-    const int ra = wave * 4 + 2 * (p & 1);
-    const Tin* rowa = frames + ((long)(f0 + (p >> 1)) * N + y0 + ra) * N;
+    const int ra = wave * 2 * PPW + 2 * (p % PPW);
+    const Tin* rowa = frames + ((long)(f0 + p / PPW) * N + y0 + ra) * N;
     const Tin* rowb = rowa + N;
 #pragma unroll
     for (int q = 0; q < P; q++) {
@@ -339,9 +346,10 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
       b[q] = __builtin_nontemporal_load(&rowb[lane + 64 * q]);
     }
   };
+  float2 ua[ALIAS ? KQ : 1], ub[ALIAS ? KQ : 1];   // ALIAS: the pair's untangled spectra until the tile is free
   auto transform_pair = [&](int p, const Tin (&pa)[P], const Tin (&pb)[P]) {
-    const int f = f0 + (p >> 1);
-    const int ra = wave * 4 + 2 * (p & 1);     // rows ra, ra+1 of the tile
+    const int f = f0 + p / PPW;
+    const int ra = wave * 2 * PPW + 2 * (p % PPW);   // rows ra, ra+1 of the tile
     float xa[P], xb[P];
 #pragma unroll
     for (int q = 0; q < P; q++) {
@@ -411,32 +419,57 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
     for (int q = 0; q < P; q++) xch[xpad(lane + 64 * q)] = v[q];
     wave_sync();
 #pragma unroll
-    for (int q = 0; q <= P / 2; q++) {
+    for (int q = 0; q < KQ; q++) {
       const int k = lane + 64 * q;
       if (k <= N / 2) {
         const float2 z = v[q];
         const float2 zc = cconj(xch[xpad((N - k) & (N - 1))]);
         const float2 u = csub(z, zc);
-        tile[k * TS + ra] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-        tile[k * TS + ra + 1] = make_float2(0.5f * u.y, -0.5f * u.x);
+        const float2 xa2 = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        const float2 xb2 = make_float2(0.5f * u.y, -0.5f * u.x);
+        if constexpr (ALIAS) {
+          ua[q] = xa2;
+          ub[q] = xb2;
+        } else {
+          tile[k * TS + ra] = xa2;
+          tile[k * TS + ra + 1] = xb2;
+        }
       }
     }
   };
-  Tin A0[P], A1[P], B0[P], B1[P];
-  load_pair(0, A0, A1);
-#pragma unroll 1
-  for (int fl = 0; fl < nf; fl++) {
-    load_pair(2 * fl + 1, B0, B1);
-    transform_pair(2 * fl, A0, A1);
-    if (fl + 1 < nf) load_pair(2 * fl + 2, A0, A1);
-    transform_pair(2 * fl + 1, B0, B1);
+  auto finish_frame = [&](int p) {   // after the frame's last pair p
+    const int f = f0 + p / PPW;
+    if constexpr (ALIAS) {
+      __syncthreads();   // every wave is done with its exchange buffer (the tile's LDS)
+      const int ra = wave * 2 * PPW + 2 * (p % PPW);
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        const int k = lane + 64 * q;
+        if (k <= N / 2) {
+          tile[k * TS + ra] = ua[q];
+          tile[k * TS + ra + 1] = ub[q];
+        }
+      }
+    }
     __syncthreads();
-    const int f = f0 + fl;
     for (int e = tid; e < NH * RW; e += NT) {
       const int r = e % RW, kx = e / RW;
       scratch[((long)f * NH + kx) * N + y0 + r] = tile[kx * TS + r];
     }
     __syncthreads();   // the tile is stored before the next frame's rows overwrite it
+  };
+  Tin A0[P], A1[P], B0[P], B1[P];
+  load_pair(0, A0, A1);
+#pragma unroll 1
+  for (int p = 0; p < npairs; p += 2) {
+    if (p + 1 < npairs) load_pair(p + 1, B0, B1);
+    transform_pair(p, A0, A1);
+    if (p % PPW == PPW - 1) finish_frame(p);
+    if (p + 2 < npairs) load_pair(p + 2, A0, A1);
+    if (p + 1 < npairs) {
+      transform_pair(p + 1, B0, B1);
+      if ((p + 1) % PPW == PPW - 1) finish_frame(p + 1);
+    }
   }
 }
 
@@ -645,11 +678,13 @@ template <typename Tin, int LOGN, int MAXLR, int FPW>
 static hipError_t launch_rows_w_f(const void* frames, int nframes, float scale, const float* win, const float2* tw,
                                   float2* scratch, int2* rowinfo, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
-  size_t lds = sizeof(float2) * (4 * xch_size(N) + NH * 17);
+  constexpr int RW = LOGN >= 11 ? 8 : 16;   // rows per workgroup (k_rows_w)
+  const size_t xch = sizeof(float2) * 4 * xch_size(N), tile = sizeof(float2) * NH * (RW + 1);
+  size_t lds = LOGN >= 11 ? (xch > tile ? xch : tile) : xch + tile;
   auto kern = k_rows_w<LOGN, Tin, MAXLR, FPW>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((nframes + FPW - 1) / FPW * (N / 16)), dim3(NT), lds, st, (const Tin*)frames, nframes,
+  hipLaunchKernelGGL(kern, dim3((nframes + FPW - 1) / FPW * (N / RW)), dim3(NT), lds, st, (const Tin*)frames, nframes,
                      scale, win, tw, scratch, rowinfo);
   return hipGetLastError();
 }
@@ -696,6 +731,9 @@ hipError_t launch_rows_w(int dtype, int logn, int maxlr, const void* frames, int
   if (logn == 10)
     return r4 ? launch_rows_w_l<10, 2>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st)
               : launch_rows_w_l<10, 3>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st);
+  if (logn == 11)
+    return r4 ? launch_rows_w_l<11, 2>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st)
+              : launch_rows_w_l<11, 3>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st);
   return hipErrorInvalidValue;
 }
 
@@ -706,8 +744,8 @@ hipError_t launch_cols_w(int logn, int F, int maxlr, const float2* scratch, cons
 #define CASE(LG_, F_, R_)                  \
   case (LG_ * 32 + F_) * 4 + R_:           \
     return launch_cols_w_t<LG_, F_, R_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
-    CASE(9, 16, 3) CASE(9, 8, 3) CASE(10, 8, 3) CASE(10, 16, 3)
-    CASE(9, 16, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(10, 16, 2)
+    CASE(9, 16, 3) CASE(9, 8, 3) CASE(10, 8, 3) CASE(10, 16, 3) CASE(11, 8, 3)
+    CASE(9, 16, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(10, 16, 2) CASE(11, 8, 2)
 #undef CASE
   }
   return hipErrorInvalidValue;
