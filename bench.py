@@ -38,9 +38,9 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 # what sets each kernel's time, from its counters (rocprofv3 --pmc summaries under profiles/): the
-# temporal kernel's HBM traffic is ~1.06x its algorithmic bytes, and its time is FP32 VALU issue
-LIMIT = {"temporal_wk": "VALU issue (FP32): counter traffic ~1.06x the algorithmic bytes, duration set by "
-                        "instruction issue, not by HBM",
+# temporal kernel's HBM traffic is ~1.03x its algorithmic bytes, and its time is FP32 VALU issue
+LIMIT = {"temporal_wk": "VALU issue (FP32): counter traffic ~1.03x the algorithmic bytes, duration set by "
+                        "instruction issue (roofline.valu), not by HBM",
          "spatial_rows": "HBM", "spatial_cols_cornerturn": "HBM", "radial_finalize": "latency"}
 METRIC = "frame-pixels/s to D(q,τ) (%HBM roofline); fits/s seeded vs plain LM @equal χ²"
 
@@ -297,10 +297,14 @@ def main():
     raw_bytes = T * N * N * 2
     per_step_algo = {"spatial_rows": raw_bytes, "spatial_cols_cornerturn": spec_bytes, "temporal_wk": binned_bytes,
                      "radial_finalize": 0}
-    cols_f = engs[0].tuning()["cols_f"]
-    kernel_symbol = {"spatial_rows": "ddm::k_rows_w<9, unsigned short>",
-                     "spatial_cols_cornerturn": f"ddm::k_cols_w<9, {cols_f}>",
-                     "temporal_wk": "ddm::k_temporal_w<11, 2>", "radial_finalize": "ddm::k_bin_sum<11>"}
+    tun = engs[0].tuning()
+    lr = 3 if tun["fft_radix"] == 8 else 2
+    # the kernels' names as the profiler reports them (the traffic table below is keyed by them)
+    kernel_symbol = {"spatial_rows": f"ddm::k_rows_w<9, unsigned short, {lr}, 2>",
+                     "spatial_cols_cornerturn": f"ddm::k_cols_w<9, {tun['cols_f']}, {lr}>",
+                     "temporal_wk": (f"ddm::k_temporal_c<11, {lr}, true>" if tun["temporal_kernel"] == 2
+                                     else f"ddm::k_temporal_w<11, 2, {lr}>"),
+                     "radial_finalize": "ddm::k_bin_sum<11>"}
     per_kernel = {}
     for name, (ms, cnt) in times.items():
         per_kernel[name] = {"ms_per_step": ms / prof_steps, "launches_per_step": cnt / prof_steps,
@@ -315,15 +319,29 @@ def main():
     algo_launch = per_step_algo[dom] / launches
     achieved = algo_launch / (per_kernel[dom]["avg_launch_ms"] * 1e-3) / 1e9
     traffic = None
-    # calibrated FETCH_SIZE + WRITE_SIZE per launch of this workload (rocprofv3 passes summarised in
-    # profiles/c2_1024x512_hbm_traffic.md)
+    valu = None
+    # calibrated FETCH_SIZE + WRITE_SIZE per launch of this workload and the SQ_INSTS_VALU count of the same
+    # kernels (rocprofv3 passes summarised in profiles/c2_1024x512_hbm_traffic.md and
+    # profiles/c2_1024x512_sq_counters.md)
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hackathon_amd", "data",
                         "hbm_traffic_1024x512_u16.json")
     if (T, N, NB) == (1024, 512, 256) and os.path.exists(prof):
         k = json.load(open(prof))["kernels"].get(kernel_symbol[dom])
         if k:
             traffic = (k["fetch_MiB"] + k["write_MiB"]) * 2 ** 20
-    kern = {"temporal_wk": "k_temporal_w (masked WK: gather, mean removal, TPAD FFT, |Y|^2, per-bin partials)",
+            if "valu_insts" in k:
+                # FP32 VALU issue roofline: a CU issues one wave64 VALU instruction per cycle (4 SIMDs, 4
+                # cycles each); peak at the CU count and the 2.4 GHz peak engine clock
+                props = torch.cuda.get_device_properties(0)
+                peak = props.multi_processor_count * 2.4e9
+                rate = k["valu_insts"] / (per_kernel[dom]["avg_launch_ms"] * 1e-3)
+                valu = {"insts_per_launch": k["valu_insts"], "achieved_Ginst_s": rate / 1e9,
+                        "peak_Ginst_s": peak / 1e9, "frac": rate / peak,
+                        "note": "the kernel's binding limit: wave64 VALU instructions issued per second"}
+    kern = {"temporal_wk": ("k_temporal_c (four waves per mode: gather, masked mean removal, 4 x 512-point "
+                            "DIF-split TPAD FFT, |Y|^2 and p sums, direct tail lags, per-item partials)"
+                            if tun["temporal_kernel"] == 2 else
+                            "k_temporal_w (masked WK: gather, mean removal, TPAD FFT, |Y|^2, per-bin partials)"),
             "spatial_cols_cornerturn": "k_cols_w (column FFT + corner turn into the history)",
             "spatial_rows": "k_rows_w (decode, row-mean removal, window, R2C row FFT)",
             "radial_finalize": "k_bin_sum + k_bin_ifft + k_bin_lags(_full)"}[dom]
@@ -349,7 +367,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": algo_launch,
                      "avg_launch_ms": per_kernel[dom]["avg_launch_ms"],
-                     "limit": LIMIT.get(dom)},
+                     "limit": LIMIT.get(dom), "valu": valu},
         "pipeline": {"algorithmic_bytes": pipe_bytes, "GBs": pipeline_gbs,
                      "frac_of_hbm_peak": pipeline_gbs / HBM_PEAK_GBS, "inflight": a.inflight,
                      "serial_ms_per_step": serial_ms,

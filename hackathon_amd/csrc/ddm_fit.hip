@@ -285,18 +285,24 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
 // objective is resolved far below the 1e-6 relative level at which fits are
 // compared, so LM accept/reject and termination decisions are not driven by
 // FP32 rounding of the model.
+// The lane's weighted residuals at theta go to res (they also form the LM's
+// gradient, so that the gradient is the one of the FP64 objective: with an
+// FP32 model value in the residual the LM would settle on the FP32 model's
+// optimum, measurably above the FP64 one in slow valleys).
 template <int GEO>
 __device__ __noinline__ double chi2_at(const double* th, const float* tv, const float* dv, int nt, float w, const float* q,
-                                       int K, float V, const Isf2dDev& T, bool swim, int lane) {
+                                       int K, float V, const Isf2dDev& T, bool swim, int lane,
+                                       double (&res)[MAXT / 64]) {
   double s = 0;
   const double alpha = swim ? th[2] : 0.0, vbar = exp(th[3]), Z = exp(th[4]), Dd = exp(th[5]);
 #pragma unroll
   for (int r = 0; r < MAXT / 64; r++) {
     const int j = lane + 64 * r;
+    res[r] = 0;
     if (j < nt) {
       const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim);
-      const double res = (double)w * ((double)dv[r] - (th[0] * c + th[1]));
-      s += res * res;
+      res[r] = (double)w * ((double)dv[r] - (th[0] * c + th[1]));
+      s += res[r] * res[r];
     }
   }
   return wave_sum_d(s);
@@ -381,7 +387,8 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
   constexpr int p = SWIM ? 6 : 3;
   constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
   for (int k = 0; k < 6; k++) th[k] = clampd(th[k], lo[k], hi[k]);
-  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane);
+  double rcur[MAXT / 64], rtry[MAXT / 64];   // FP64 residuals at theta and at the trial point
+  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane, rcur);
   double lam = 1e-3, nu = 2;
   int it = 0;
   bool done = false;
@@ -408,7 +415,7 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
         F[3] = (double)w * th[0] * dlv;
         F[4] = (double)w * th[0] * dlz;
         F[5] = (double)w * th[0] * dld;
-        const double res = (double)w * ((double)dv[r] - ((double)th[0] * c + th[1]));
+        const double res = rcur[r];
         int h = 0;
 #pragma unroll
         for (int a = 0; a < 6; a++) {
@@ -509,11 +516,13 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
           const double dk = nth[k] - th[k];
           pred += dk * (lam * diag[a] * dk + g[a]);
         }
-        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane);
+        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane, rtry);
         const double rho = pred > 0 ? (chi - nchi) / pred : -1;
         if (nchi < chi && rho > 0) {
           const double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
           for (int k = 0; k < 6; k++) th[k] = nth[k];
+#pragma unroll
+          for (int r = 0; r < MAXT / 64; r++) rcur[r] = rtry[r];
           chi = nchi;
           const double f = 1 - (2 * rho - 1) * (2 * rho - 1) * (2 * rho - 1);
           lam *= f > 1.0 / 3 ? f : 1.0 / 3;

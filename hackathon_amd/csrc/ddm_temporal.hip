@@ -71,7 +71,8 @@ template <int LOGT> struct TcLayout {
   static constexpr int TW = m * RS;                 // 512-point twiddle table at twpad() slots
   static constexpr int TAIL = TW + twpad(512);      // 2 x 64 samples (head, tail) + 2 x 64 masks (as floats)
   static constexpr int RED = TAIL + 128 + 64;       // 2 parities x 4 waves x (sx, sy); then P
-  static constexpr int SIZE = RED + 8 + 4;
+  static constexpr int ORD = RED + 8 + 4;           // m <= 8: the frame order (n/2 entries) of a gathered series
+  static constexpr int SIZE = ORD + (m <= 8 ? (512 * m) / 4 : 0);
 };
 
 // Modes summed plainly per wave before the sums are folded into a second
@@ -121,6 +122,14 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   float* tm = hm + 64;
   float* red = reinterpret_cast<float*>(lds + L::RED);
 
+  // a gathered series reads its slots through the order: staged in LDS where
+  // it fits (m <= 8), else re-read from the cache with each mode
+  const bool identity = A.identity != 0;
+  uint32_t* ordl = reinterpret_cast<uint32_t*>(lds + L::ORD);
+  if constexpr (m <= 8) {
+    if (!identity)
+      for (int i = tid; i < 256 * m; i += NT) ordl[i] = A.order[i];
+  }
   // sample i of this thread: t = tid + 256 i = t1 + 512 t2 with t1 = tid + 256 (i & 1), t2 = i >> 1
   uint32_t mbits = 0;
 #pragma unroll
@@ -128,8 +137,8 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     const int t = tid + 256 * i;
     if (t < T && A.mask[t] != 0.f) mbits |= 1u << i;
   }
+  // (also the barrier behind the order's staging)
   const bool dense = __syncthreads_and(mbits == (m == 32 ? 0xffffffffu : ((1u << m) - 1u)));
-  const bool identity = A.identity != 0;
 
   float yacc[SW][8], yacc2[TWO_LEVEL ? SW : 1][8];
   float pacc[m], pacc2[TWO_LEVEL ? m : 1];
@@ -159,15 +168,17 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   // non-identity order gathers sample t from slot order[t] (order's entries
   // past T are 0)
   // (one 8-byte non-temporal load per sample, a wave-uniform base plus a
-  // 32-bit per-sample offset; the order is re-read, from the cache, with each
-  // mode: m more registers held across the loop would spill at n = 8192)
+  // 32-bit per-sample offset; the order is re-read with each mode, from LDS
+  // or the cache: m more registers held across the loop would spill at
+  // n = 8192)
   auto load_series = [&](float2 (&dst)[m], int slot) {
     const char* x = reinterpret_cast<const char*>(hist + (long)A.bin_modes[start + slot] * T);
     const uint32_t* op = A.order;
     asm volatile("" : "+s"(op));
 #pragma unroll
     for (int i = 0; i < m; i++) {
-      const uint32_t off = 8u * (identity ? (uint32_t)(tid + 256 * i) : op[tid + 256 * i]);
+      const uint32_t off =
+          8u * (identity ? (uint32_t)(tid + 256 * i) : (m <= 8 ? ordl[tid + 256 * i] : op[tid + 256 * i]));
       const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(x + off));
       dst[i] = make_float2(v.x, v.y);
     }
