@@ -471,7 +471,7 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   // wave_spatial: 0 default (environment, else on where built), 1 off, 2 on
   // (N = 2048: 8-frame column tiles; 2.6x faster per frame than the workgroup-level kernels, measured)
   const long ws = tn.wave_spatial ? tn.wave_spatial - 1 : env_long("DDM_WAVE_SPATIAL", 1, 0, 1);
-  e->wave_spatial = is_pow2(g.N) && (e->logn == 9 || e->logn == 10 || e->logn == 11) && ws;
+  e->wave_spatial = is_pow2(g.N) && e->logn >= 8 && e->logn <= 11 && ws;
   e->cols_f = e->logn == 11 ? 8 : (tn.cols_f ? tn.cols_f : env_long("DDM_COLS_F", 16, 8, 16)) >= 16 ? 16 : 8;
   const int F = 16;   // chunks are a multiple of every column-kernel frame tile
   long per_frame = (long)e->NH * g.N * 8;

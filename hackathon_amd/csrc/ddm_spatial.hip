@@ -720,11 +720,14 @@ static hipError_t launch_rows_w_l(int dtype, const void* frames, int nframes, fl
                       : launch_rows_w_t<float, LOGN, MAXLR>(frames, nframes, scale, win, tw, scratch, rowinfo, st);
 }
 
-// Wavefront spatial kernels: N = 512 and 1024 (registers hold N/64 points
+// Wavefront spatial kernels: N = 256 .. 2048 (registers hold N/64 points
 // per lane); F = frames per column workgroup (8 or 16); maxlr: radix cap.
 hipError_t launch_rows_w(int dtype, int logn, int maxlr, const void* frames, int nframes, float scale,
                          const float* win, const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st) {
   const bool r4 = maxlr == 2;
+  if (logn == 8)
+    return r4 ? launch_rows_w_l<8, 2>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st)
+              : launch_rows_w_l<8, 3>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st);
   if (logn == 9)
     return r4 ? launch_rows_w_l<9, 2>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st)
               : launch_rows_w_l<9, 3>(dtype, frames, nframes, scale, win, tw, scratch, rowinfo, st);
@@ -744,8 +747,8 @@ hipError_t launch_cols_w(int logn, int F, int maxlr, const float2* scratch, cons
 #define CASE(LG_, F_, R_)                  \
   case (LG_ * 32 + F_) * 4 + R_:           \
     return launch_cols_w_t<LG_, F_, R_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
-    CASE(9, 16, 3) CASE(9, 8, 3) CASE(10, 8, 3) CASE(10, 16, 3) CASE(11, 8, 3)
-    CASE(9, 16, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(10, 16, 2) CASE(11, 8, 2)
+    CASE(8, 16, 3) CASE(8, 8, 3) CASE(9, 16, 3) CASE(9, 8, 3) CASE(10, 8, 3) CASE(10, 16, 3) CASE(11, 8, 3)
+    CASE(8, 16, 2) CASE(8, 8, 2) CASE(9, 16, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(10, 16, 2) CASE(11, 8, 2)
 #undef CASE
   }
   return hipErrorInvalidValue;
