@@ -404,9 +404,10 @@ __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, flo
   }
 }
 
-template <int LOGN, int NSLOG, bool INV, bool HALF, bool TWP = false, bool WL = false, int MAXLR = 3>
+template <int LOGN, int NSLOG, bool INV, bool HALF, bool TWP = false, bool WL = false, int MAXLR = 3,
+          bool W2 = false>
 __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
-                                          const float2* wl = nullptr) {
+                                          const float2* wl = nullptr, const float2* w2 = nullptr) {
   constexpr int n = 1 << LOGN;
   constexpr int LR = wave_lr(LOGN, NSLOG, MAXLR);
   constexpr int R = 1 << LR;
@@ -430,6 +431,9 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
       if constexpr (WL && LAST && R == 8 && NBL == 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) w[r] = wl[r];
+      } else if constexpr (W2 && NSLOG == 3 && R == 8 && NBL == 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) w[r] = w2[r];
       } else {
         butterfly_twiddles<R, INV, TWP>(tw, k << (LOGN - NSLOG - LR), w);
       }
@@ -454,7 +458,7 @@ __device__ __forceinline__ void wave_pass(float2 (&v)[(1 << LOGN) / 64], float2*
     wave_sync();
 #pragma unroll
     for (int q = 0; q < n / 64; q++) v[q] = xch[xpad(lane + 64 * q)];
-    wave_pass<LOGN, NSLOG + LR, INV, false, TWP, WL, MAXLR>(v, xch, tw, lane, wl);
+    wave_pass<LOGN, NSLOG + LR, INV, false, TWP, WL, MAXLR, W2>(v, xch, tw, lane, wl, w2);
   }
 }
 
@@ -472,6 +476,13 @@ template <int LOGN, bool INV, bool HALF, bool TWP>
 __device__ __forceinline__ void wave_fft_wl(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
                                             const float2* wl) {
   wave_pass<LOGN, 0, INV, HALF, TWP, true>(v, xch, tw, lane, wl);
+}
+// The same with the second pass's twiddles supplied too (w2[1..7]; a 512-point
+// transform's middle radix-8 pass, k = lane mod 8).
+template <int LOGN, bool INV, bool HALF, bool TWP>
+__device__ __forceinline__ void wave_fft_wl2(float2 (&v)[(1 << LOGN) / 64], float2* xch, const float2* tw, int lane,
+                                             const float2* w2, const float2* wl) {
+  wave_pass<LOGN, 0, INV, HALF, TWP, true, 3, true>(v, xch, tw, lane, wl, w2);
 }
 
 // 2Sum accumulate (branch-free): (s, c) += x, value s + c.

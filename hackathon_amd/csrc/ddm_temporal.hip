@@ -69,8 +69,8 @@ template <int LOGT> struct TcLayout {
   static constexpr int RS = xch_size(512);          // one folded row, xpad-indexed (the wave's exchange buffer)
   static constexpr int U = 0;                       // m rows
   static constexpr int TW = m * RS;                 // 512-point twiddle table at twpad() slots
-  static constexpr int TAIL = TW + twpad(512);      // 2 x 64 samples (head, tail) + 2 x 64 masks (as floats)
-  static constexpr int RED = TAIL + 128 + 64;       // 2 parities x 4 waves x (sx, sy); then P
+  static constexpr int TAIL = TW + twpad(512);      // 2 x 64 (y, mask) as float4: head, tail
+  static constexpr int RED = TAIL + 256;            // 2 parities x 4 waves x (sx, sy); then P
   static constexpr int ORD = RED + 8 + 4;           // m <= 8: the frame order (n/2 entries) of a gathered series
   static constexpr int SIZE = ORD + (m <= 8 ? (512 * m) / 4 : 0);
 };
@@ -102,7 +102,8 @@ __device__ __forceinline__ void store_folded(float2* U, int rs, int t1x, const f
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL>
 __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int item, float2* lds,
-                                                const float2 (&twb)[2][5], const float2 (&wlast)[8]) {
+                                                const float2 (&twb)[2][5], const float2 (&wlast)[8],
+                                                const float2 (&wmid)[8]) {
   using L = TcLayout<LOGT>;
   constexpr int m = L::m;          // sub-transforms (and samples per thread)
   constexpr int MH = m / 2;        // nonzero rows of the fold
@@ -116,10 +117,8 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   const int start = it.y, count = it.z;
   float2* U = lds + L::U;
   const float2* twl = lds + L::TW;
-  float2* head = lds + L::TAIL;
-  float2* tailv = head + 64;
-  float* hm = reinterpret_cast<float*>(head + 128);
-  float* tm = hm + 64;
+  float4* head = reinterpret_cast<float4*>(lds + L::TAIL);   // (y, mask, 0) of t < KT
+  float4* tailv = head + 64;                                  // of T - KT <= t < T
   float* red = reinterpret_cast<float*>(lds + L::RED);
 
   // a gathered series reads its slots through the order: staged in LDS where
@@ -157,11 +156,17 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     for (int i = 0; i < m; i++) pacc2[i] = 0.f;
   }
   float aP = 0.f, cP = 0.f;
-  // tail lags T - KT + j: lpl threads per lag (KT <= 64), thread lpl j + s
-  // sums the pairs t = s, s + lpl, ...
-  const int lpl = KT <= 16 ? 16 : (KT <= 32 ? 8 : 4);
-  const int tj = tid / lpl, ts = tid % lpl;
-  float Gt = 0.f, Gtc = 0.f;
+  // tail lags T - KT + j (KT <= 64): a group of G threads per lag for
+  // KT <= 32; above, lag j and lag KT - 1 - j share a group (KT + 1 pairs
+  // between them, so the groups are balanced and the pass takes ~2/3 of the
+  // one-lag groups' iterations; at KT <= 32 the one-lag groups do less work
+  // in all, measured); thread G gi + s sums the pairs t = s, s + G, ...
+  const bool paired = KT > 32;
+  const int ngrp = paired ? (KT + 1) / 2 : KT;
+  const int G = (paired || KT > 16) ? 8 : 16;
+  const int gi = tid / G, gs = tid % G;
+  const int ja = gi, jb = paired ? KT - 1 - gi : gi;   // jb == ja: one lag (the middle one of an odd KT)
+  float Gt = 0.f, Gtc = 0.f, Gt2 = 0.f, Gtc2 = 0.f;
 
   // series loads without guards: t < n/2 stays inside the history, which
   // carries n/2 - T spare samples behind its last mode (masked off); a
@@ -236,20 +241,14 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     // addresses are formed here, for the one or two i taken, not hoisted)
     int tv = tid;
     asm volatile("" : "+v"(tv));
-    if (tv < KT) {
-      head[tv] = y[0];
-      hm[tv] = (mb & 1u) ? 1.f : 0.f;
-    }
+    if (tv < KT) head[tv] = make_float4(y[0].x, y[0].y, (mb & 1u) ? 1.f : 0.f, 0.f);
     {
       const int i0 = (T - KT) >> 8, i1 = (T - 1) >> 8;
 #pragma unroll
       for (int i = 0; i < m; i++) {
         if (i >= i0 && i <= i1) {
           const int t = tv + 256 * i;
-          if (t >= T - KT && t < T) {
-            tailv[t - (T - KT)] = y[i];
-            tm[t - (T - KT)] = ((mb >> i) & 1u) ? 1.f : 0.f;
-          }
+          if (t >= T - KT && t < T) tailv[t - (T - KT)] = make_float4(y[i].x, y[i].y, ((mb >> i) & 1u) ? 1.f : 0.f, 0.f);
         }
       }
     }
@@ -275,15 +274,31 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     if (mi + 1 < count) load_series(nx, mi + 1);
     // direct tail sums, difference form sum m(t) m(t + tau) |y(t + tau) - y(t)|^2
     {
-      float g = 0.f;
-      if (tj < KT)
-        for (int t = ts; t < KT - tj; t += lpl) {
-          const float2 a = head[t], b = tailv[t + tj];
+      float g = 0.f, g2 = 0.f;
+      if (gi < ngrp) {
+        for (int t = gs; t < KT - ja; t += G) {
+          const float4 a = head[t], b = tailv[t + ja];
           const float dx = b.x - a.x, dy = b.y - a.y;
-          g += (hm[t] * tm[t + tj]) * (dx * dx + dy * dy);
+          g += (a.z * b.z) * (dx * dx + dy * dy);
         }
-      g = lpl == 16 ? seg_sum_dpp<16>(g) : (lpl == 8 ? seg_sum_dpp<8>(g) : seg_sum_dpp<4>(g));
-      if (ts == 0 && tj < KT) two_sum_add(Gt, Gtc, g);
+        if (jb > ja)
+          for (int t = gs; t < KT - jb; t += G) {
+            const float4 a = head[t], b = tailv[t + jb];
+            const float dx = b.x - a.x, dy = b.y - a.y;
+            g2 += (a.z * b.z) * (dx * dx + dy * dy);
+          }
+      }
+      if (G == 16) {
+        g = seg_sum_dpp<16>(g);
+        g2 = seg_sum_dpp<16>(g2);
+      } else {
+        g = seg_sum_dpp<8>(g);
+        g2 = seg_sum_dpp<8>(g2);
+      }
+      if (gs == 0 && gi < ngrp) {
+        two_sum_add(Gt, Gtc, g);
+        two_sum_add(Gt2, Gtc2, g2);
+      }
     }
     // the wave's sub-transforms r = wave + 4 s
 #pragma unroll
@@ -293,7 +308,10 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
       float2 v[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) v[q] = row[xpad(lane + 64 * q)];
-      if constexpr (MAXLR == 3) wave_fft_wl<9, false, false, true>(v, row, twl, lane, wlast);
+      // (at m = 16 the middle pass's twiddles are formed per mode: 14 more
+      // registers held across the loop would spill there)
+      if constexpr (MAXLR == 3 && m <= 8) wave_fft_wl2<9, false, false, true>(v, row, twl, lane, wmid, wlast);
+      else if constexpr (MAXLR == 3) wave_fft_wl<9, false, false, true>(v, row, twl, lane, wlast);
       else wave_fft<9, false, false, true, MAXLR>(v, row, twl, lane);
 #pragma unroll
       for (int q = 0; q < 8; q++) yacc[s][q] += v[q].x * v[q].x + v[q].y * v[q].y;
@@ -333,7 +351,10 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   if (lane == 0) red[16 + wave] = pw;
   __syncthreads();
   if (tid == 0) A.P_part[item] = (red[16] + red[17]) + (red[18] + red[19]);
-  if (ts == 0 && tj < KT) A.G_part[(long)item * KT + tj] = Gt + Gtc;
+  if (gs == 0 && gi < ngrp) {
+    A.G_part[(long)item * KT + ja] = Gt + Gtc;
+    if (jb > ja) A.G_part[(long)item * KT + jb] = Gt2 + Gtc2;
+  }
 }
 
 // Per-workgroup set-up: the 512-point twiddle table in LDS, the last pass's
@@ -342,7 +363,7 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
 template <int LOGT, int MAXLR>
 __device__ __forceinline__ void temporal_c_setup(float2* lds, const float2* __restrict__ tw_g,
                                                  const float2* __restrict__ tw_f, float2 (&twb)[2][5],
-                                                 float2 (&wlast)[8]) {
+                                                 float2 (&wlast)[8], float2 (&wmid)[8]) {
   using L = TcLayout<LOGT>;
   constexpr int n = 1 << LOGT;
   const int tid = threadIdx.x;
@@ -358,7 +379,12 @@ __device__ __forceinline__ void temporal_c_setup(float2* lds, const float2* __re
     twb[c][4] = tw_g[(8 * t1) & (n - 1)];
   }
   __syncthreads();
-  if constexpr (MAXLR == 3) butterfly_twiddles<8, false, true>(twl, tid & 63, wlast);
+  // the 512-point radix-8 passes' twiddles that are the same for every mode:
+  // the last pass's (k = lane) and the middle pass's (k = lane mod 8)
+  if constexpr (MAXLR == 3) {
+    butterfly_twiddles<8, false, true>(twl, tid & 63, wlast);
+    if constexpr (LOGT <= 12) butterfly_twiddles<8, false, true>(twl, (tid & 7) << 3, wmid);
+  }
 }
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL>
@@ -368,10 +394,10 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_c(
     const float2* __restrict__ tw_g, const float2* __restrict__ tw_f, float* __restrict__ S_part,
     float* __restrict__ p_part, float* __restrict__ P_part, int KT, float* __restrict__ G_part) {
   extern __shared__ float2 lds[];
-  float2 twb[2][5], wlast[8];
-  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast);
+  float2 twb[2][5], wlast[8], wmid[8];
+  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast, wmid);
   const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, identity, KT, 0, inv_n0};
-  temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, blockIdx.x, lds, twb, wlast);
+  temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, blockIdx.x, lds, twb, wlast, wmid);
 }
 
 // The batch form: a persistent grid over the work items of many videos of
@@ -388,8 +414,8 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cq(const Te
                                                                        const float2* __restrict__ tw_f) {
   extern __shared__ float2 lds[];
   __shared__ int next;
-  float2 twb[2][5], wlast[8];
-  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast);
+  float2 twb[2][5], wlast[8], wmid[8];
+  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast, wmid);
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1);
     __syncthreads();
@@ -402,7 +428,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cq(const Te
       else hi = mid - 1;
     }
     const TemporalArgs A = vids[lo];
-    temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, w - A.item0, lds, twb, wlast);
+    temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, w - A.item0, lds, twb, wlast, wmid);
     __syncthreads();   // the item's last reads of `next` and of the LDS rows before the next item
   }
 }
@@ -454,14 +480,14 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cp(const Te
                                                                        const float2* __restrict__ tw_f) {
   extern __shared__ float2 lds[];
   __shared__ int next;
-  float2 twb[2][5], wlast[8];
-  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast);
+  float2 twb[2][5], wlast[8], wmid[8];
+  temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast, wmid);
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1);
     __syncthreads();
     const int w = next;
     if (w >= nwork) break;
-    temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, w, lds, twb, wlast);
+    temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, w, lds, twb, wlast, wmid);
     __syncthreads();
   }
 }
