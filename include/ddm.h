@@ -336,8 +336,9 @@ typedef struct ddm_batch_video {
 enum { DDM_BATCH_EAGER = 0, DDM_BATCH_GRAPH = 1, DDM_BATCH_QUEUE = 2 };
 
 /* Creates the engines (and records the graph).  `chains` (>= 1) splits the
- * videos round-robin into that many independent chains that may run
- * concurrently; videos within a chain run one after the other. */
+ * videos into that many independent chains that may run concurrently, balanced
+ * by estimated work (each video, longest first, onto the least loaded chain);
+ * videos within a chain run one after the other. */
 int ddm_batch_create(const ddm_batch_video *v, int32_t n, int32_t mode, int32_t chains, ddm_batch **out);
 /* One pass over every video; blocks until done; *gpu_ms (may be NULL) is the
  * pass's duration from HIP events. */
