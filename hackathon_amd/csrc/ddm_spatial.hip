@@ -304,6 +304,19 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
 
 template <typename T> __device__ __forceinline__ float px_to_float(T v) { return (float)v; }
 
+// XCD-aware block order for a kernel whose workgroups each store half of
+// every 128-byte line they touch, the other half coming from a partner
+// workgroup (logical blocks 2j and 2j + 1).  The dispatcher hands workgroup b
+// to XCD b mod 8, so the partners go to physical blocks b and b + 8: the same
+// XCD in consecutive rounds, where the two halves meet in that XCD's L2 and
+// leave it as one full line instead of two partial-line writes.  Blocks at or
+// past the last multiple of 16 keep their order.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_pair_block(int b, int G) {
+  if (b >= (G & ~15)) return b;
+  const int x = b & 7, r = b >> 3;
+  return 2 * (x + 8 * (r >> 1)) + (r & 1);
+}
+
 // FPW frames per workgroup (the same rows of each): every wave transforms
 // PPW row pairs per frame (two up to N = 1024: 16 rows per workgroup; one at
 // N = 2048: 8 rows), and the next pair's pixels are loaded while the current
@@ -329,8 +342,11 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
   float2* xch = lds + wave * XS;
   float2* tile = ALIAS ? lds : lds + 4 * XS;   // NH x TS
   const int groups = N / RW;
-  const int f0 = (blockIdx.x / groups) * FPW;
-  const int y0 = (blockIdx.x % groups) * RW;
+  // 8-row workgroups (N = 2048) store 64-byte runs of a 128-byte line: row
+  // groups 2g and 2g + 1 of a frame run on one XCD
+  const int blk = RW < 16 ? xcd_pair_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int f0 = (blk / groups) * FPW;
+  const int y0 = (blk % groups) * RW;
   if (f0 >= nframes) return;
   const int nf = min(FPW, nframes - f0);       // workgroup-uniform
   const int npairs = PPW * nf;
@@ -473,28 +489,50 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
   }
 }
 
+// W waves per workgroup: four, or eight at N = 2048 (one frame per wave: 170
+// rather than 336 VGPRs, two waves per SIMD instead of one)
+template <int LOGN> constexpr int cols_waves() { return LOGN >= 11 ? 8 : 4; }
+
 template <int LOGN, int F, int MAXLR>
-__global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratch, const int2* __restrict__ rowinfo,
-                                               int nframes, int is_int, float scale, const float* __restrict__ win,
-                                               const float2* __restrict__ tw, const float2* __restrict__ wx,
-                                               float2* __restrict__ hist, int T, int first_slot) {
+__global__ __launch_bounds__(64 * cols_waves<LOGN>()) void k_cols_w(
+    const float2* __restrict__ scratch, const int2* __restrict__ rowinfo, int nframes, int is_int, float scale,
+    const float* __restrict__ win, const float2* __restrict__ tw, const float2* __restrict__ wx,
+    float2* __restrict__ hist, int T, int first_slot) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
-  constexpr int FW = F / 4;                    // frames per wave
+  constexpr int W = cols_waves<LOGN>(), NTC = 64 * W;
+  static_assert(F % W == 0, "whole frames per wave");
+  constexpr int FW = F / W;                    // frames per wave
   constexpr int XS = xch_size(N);
   constexpr int TS = F + 1;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * XS;
   float2* tile = lds;                          // (N/2) x TS, reused after every wave's FFTs
-  const int kx = blockIdx.x;
-  const int f0 = blockIdx.y * F;
+  int kx, ft;   // column kx, frame tile ft
+  if constexpr (F < 16) {
+    // F-frame (64-byte) runs of the history: frame tiles 2i and 2i + 1 of a
+    // column run on one XCD (a 1D grid of NH x tiles; an odd last tile alone)
+    const int ntiles = gridDim.x / NH, Gp = NH * (ntiles & ~1);
+    const int L = blockIdx.x < Gp ? xcd_pair_block(blockIdx.x, Gp) : blockIdx.x;
+    if (L < Gp) {
+      kx = (L >> 1) % NH;
+      ft = 2 * ((L >> 1) / NH) + (L & 1);
+    } else {
+      kx = L - Gp;
+      ft = ntiles - 1;
+    }
+  } else {
+    kx = blockIdx.x;
+    ft = blockIdx.y;
+  }
+  const int f0 = ft * F;
   // N <= 512: the column FFT's twiddle table in LDS (twpad slots, behind the
   // tile/exchange area), read there instead of waiting on global loads; at
   // N = 1024 the extra 8.5 KB would cost a resident workgroup
   constexpr bool TWL = LOGN <= 9;
-  float2* twl = lds + ((N / 2) * TS > 4 * XS ? (N / 2) * TS : 4 * XS);
+  float2* twl = lds + ((N / 2) * TS > W * XS ? (N / 2) * TS : W * XS);
   if constexpr (TWL) {
-    for (int i = tid; i < N; i += NT) twl[twpad(i)] = tw[i];
+    for (int i = tid; i < N; i += NTC) twl[twpad(i)] = tw[i];
     __syncthreads();
   }
   float2 res[FW][P];
@@ -569,7 +607,7 @@ __global__ __launch_bounds__(NT) void k_cols_w(const float2* __restrict__ scratc
       }
     }
     __syncthreads();
-    for (int e = tid; e < (N / 2) * F; e += NT) {
+    for (int e = tid; e < (N / 2) * F; e += NTC) {
       const int fl = e % F, ky = h * (N / 2) + e / F;
       if (f0 + fl >= nframes) continue;
       const int slot = (first_slot + f0 + fl) % T;
@@ -702,13 +740,15 @@ static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, in
                                   const float* win, const float2* tw, const float2* wx, float2* hist, int T,
                                   int first_slot, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
-  size_t tile = sizeof(float2) * (N / 2) * (F + 1), xch = sizeof(float2) * 4 * xch_size(N);
+  size_t tile = sizeof(float2) * (N / 2) * (F + 1), xch = sizeof(float2) * cols_waves<LOGN>() * xch_size(N);
   size_t lds = (tile > xch ? tile : xch) + (LOGN <= 9 ? sizeof(float2) * twpad(N) : 0);
   auto kern = k_cols_w<LOGN, F, MAXLR>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(NH, (nframes + F - 1) / F), dim3(NT), lds, st, scratch, rowinfo, nframes, is_int, scale,
-                     win, tw, wx, hist, T, first_slot);
+  const int ntiles = (nframes + F - 1) / F;
+  const dim3 grid = F < 16 ? dim3(NH * ntiles) : dim3(NH, ntiles);
+  hipLaunchKernelGGL(kern, grid, dim3(64 * cols_waves<LOGN>()), lds, st, scratch, rowinfo, nframes, is_int, scale, win,
+                     tw, wx, hist, T, first_slot);
   return hipGetLastError();
 }
 

@@ -65,6 +65,21 @@ def test_spectrum_parity_large(gpu, N, tuning):
     assert err < 1e-5, err
 
 
+@pytest.mark.parametrize("radix", [8, 4])
+def test_spectrum_parity_2048_tiles(gpu, radix):
+    """N = 2048 over 24 frames: three 8-frame column tiles (tiles 0 and 1 paired on one XCD, tile 2 alone)
+    and 8-row row groups paired the same way; every slot of the history against the FP64 oracle."""
+    N, T = 2048, 24
+    fr = _frames(T, N, np.uint16, seed=99)
+    print(f"spectrum N={N} T={T} radix={radix}", flush=True)
+    with DdmEngine.create(N=N, T=T, L=2, NB=4, kmin=0, kmax=N // 2, tuning=dict(fft_radix=radix)) as e:
+        assert e.ingestFrames(fr, 0)
+        h = e.readHistory()
+    ref = O.history(fr, T)
+    err = _spec_err(h, ref)
+    assert err < 1e-5, err
+
+
 def test_spectrum_config1(gpu):
     fr = synth.config1()
     with DdmEngine.create(N=128, T=64, L=64, NB=64, kmin=0, kmax=64) as e:
