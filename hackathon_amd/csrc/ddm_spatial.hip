@@ -489,9 +489,12 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
   }
 }
 
-// W waves per workgroup: four, or eight at N = 2048 (one frame per wave: 170
-// rather than 336 VGPRs, two waves per SIMD instead of one)
-template <int LOGN> constexpr int cols_waves() { return LOGN >= 11 ? 8 : 4; }
+// Eight waves per column workgroup (F/8 frames per wave): at N = 2048 one
+// frame per wave takes 216 VGPRs where two took 336 (two waves per SIMD
+// instead of one), and at every N the extra waves hide the column loads
+// (per frame, against four waves: 2048 10.7 -> 8.8 us, 1024 2.79 -> 2.63,
+// 512 0.52 -> 0.47, 256 0.121 -> 0.110).
+template <int LOGN> constexpr int cols_waves() { return 8; }
 
 template <int LOGN, int F, int MAXLR>
 __global__ __launch_bounds__(64 * cols_waves<LOGN>()) void k_cols_w(
