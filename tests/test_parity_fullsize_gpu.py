@@ -77,8 +77,8 @@ def test_c2_full_size_against_fp64(gpu):
         assert rmax < 1e-5, (name, tag, rmax)
 
 
-def test_long_series_block_kernel_against_fp64(gpu):
-    """T = 2000 (TPAD 4096: the wavefront kernel's eight 512-point sub-transforms), all lags."""
+def test_long_series_against_fp64(gpu):
+    """T = 2000 (TPAD 4096: the cooperative temporal kernel's eight 512-point sub-transforms), all lags."""
     T, N, NB = 2000, 256, 128
     g = GpuSynth()
     fr = g.video(g.params(T, N, 200, 1.5, 100.0, 20.0, D=0.5, alpha=0.5, vbar_px_frame=0.15, Z=4.0, dtype="uint8",
