@@ -692,8 +692,9 @@ static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* m
          }
          if (e->tkern)        // cooperative kernel (512-point sub-transforms, r-major spectra)
            return ddm::launch_temporal_c(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes,
-                                         e->order, e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part,
-                                         e->p_part, e->P_part, e->cur_KT, e->G_part, e->stream);
+                                         e->order, e->mask, identity, e->cur_full && 2 * e->g.T == e->TPAD, inv_n0,
+                                         e->tw_t, e->tw_f, e->S_part, e->p_part, e->P_part, e->cur_KT, e->G_part,
+                                         e->stream);
          if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
            return ddm::launch_temporal_w(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
                                          e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part, e->p_part,

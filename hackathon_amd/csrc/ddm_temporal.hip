@@ -100,7 +100,9 @@ __device__ __forceinline__ void store_folded(float2* U, int rs, int t1x, const f
   }
 }
 
-template <int LOGT, int MAXLR, bool TWO_LEVEL>
+// DENSE: every sample t < n/2 is a valid frame (T = n/2, a full mask), known
+// at the launch: the per-sample mask tests and selects drop out.
+template <int LOGT, int MAXLR, bool TWO_LEVEL, bool DENSE = false>
 __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int item, float2* lds,
                                                 const float2 (&twb)[2][5], const float2 (&wlast)[8],
                                                 const float2 (&wmid)[8]) {
@@ -130,14 +132,20 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
       for (int i = tid; i < 256 * m; i += NT) ordl[i] = A.order[i];
   }
   // sample i of this thread: t = tid + 256 i = t1 + 512 t2 with t1 = tid + 256 (i & 1), t2 = i >> 1
-  uint32_t mbits = 0;
+  uint32_t mbits = (1u << m) - 1u;
+  bool dense = true;
+  if constexpr (DENSE) {
+    __syncthreads();   // the order's staging
+  } else {
+    mbits = 0;
 #pragma unroll
-  for (int i = 0; i < m; i++) {
-    const int t = tid + 256 * i;
-    if (t < T && A.mask[t] != 0.f) mbits |= 1u << i;
+    for (int i = 0; i < m; i++) {
+      const int t = tid + 256 * i;
+      if (t < T && A.mask[t] != 0.f) mbits |= 1u << i;
+    }
+    // (also the barrier behind the order's staging)
+    dense = __syncthreads_and(mbits == ((1u << m) - 1u));
   }
-  // (also the barrier behind the order's staging)
-  const bool dense = __syncthreads_and(mbits == (m == 32 ? 0xffffffffu : ((1u << m) - 1u)));
 
   float yacc[SW][8], yacc2[TWO_LEVEL ? SW : 1][8];
   float pacc[m], pacc2[TWO_LEVEL ? m : 1];
@@ -201,7 +209,7 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
     // selects and the 2 (m - 1) pre-twiddles would stay live in registers for
     // the whole item
     uint32_t mb = mbits;
-    asm volatile("" : "+v"(mb));
+    if constexpr (!DENSE) asm volatile("" : "+v"(mb));
     // masked sums: the mean, and sum |y|^2 for P
     float sx = 0.f, sy = 0.f, ps = 0.f;
     if (!dense) {
@@ -387,7 +395,7 @@ __device__ __forceinline__ void temporal_c_setup(float2* lds, const float2* __re
   }
 }
 
-template <int LOGT, int MAXLR, bool TWO_LEVEL>
+template <int LOGT, int MAXLR, bool TWO_LEVEL, bool DENSE>
 __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_c(
     const float2* __restrict__ hist, int T, const int4* __restrict__ items, const int* __restrict__ bin_modes,
     const uint32_t* __restrict__ order, const float* __restrict__ mask, int identity, float inv_n0,
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_c(
   float2 twb[2][5], wlast[8], wmid[8];
   temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast, wmid);
   const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, identity, KT, 0, inv_n0};
-  temporal_c_item<LOGT, MAXLR, TWO_LEVEL>(A, blockIdx.x, lds, twb, wlast, wmid);
+  temporal_c_item<LOGT, MAXLR, TWO_LEVEL, DENSE>(A, blockIdx.x, lds, twb, wlast, wmid);
 }
 
 // The batch form: a persistent grid over the work items of many videos of
@@ -435,11 +443,11 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cq(const Te
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL>
 static hipError_t launch_temporal_c_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
-                                      const uint32_t* order, const float* mask, int identity, float inv_n0,
-                                      const float2* tw, const float2* tw_f, float* S_part, float* p_part,
-                                      float* P_part, int KT, float* G_part, hipStream_t st) {
+                                      const uint32_t* order, const float* mask, int identity, int dense,
+                                      float inv_n0, const float2* tw, const float2* tw_f, float* S_part,
+                                      float* p_part, float* P_part, int KT, float* G_part, hipStream_t st) {
   const size_t lds = sizeof(float2) * TcLayout<LOGT>::SIZE;
-  auto kern = k_temporal_c<LOGT, MAXLR, TWO_LEVEL>;
+  auto kern = dense ? k_temporal_c<LOGT, MAXLR, TWO_LEVEL, true> : k_temporal_c<LOGT, MAXLR, TWO_LEVEL, false>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
@@ -451,15 +459,16 @@ static hipError_t launch_temporal_c_t(const float2* hist, int T, const int4* ite
 // logt = 13, whose sums are one-level); S_part rows r-major with
 // lm = logt - 9.
 hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
-                             const int* bin_modes, const uint32_t* order, const float* mask, int identity,
+                             const int* bin_modes, const uint32_t* order, const float* mask, int identity, int dense,
                              float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                              float* P_part, int KT, float* G_part, hipStream_t st) {
-  if (KT < 1 || KT > 64 || KT > T || 2 * T > (1 << logt)) return hipErrorInvalidValue;
+  if (KT < 1 || KT > 64 || KT > T || 2 * T > (1 << logt) || (dense && 2 * T != (1 << logt)))
+    return hipErrorInvalidValue;
   switch (logt * 4 + (maxlr == 2 ? 2 : 3)) {
 #define CASE(LG_, R_, TL_)                                                                                          \
   case LG_ * 4 + R_:                                                                                               \
-    return launch_temporal_c_t<LG_, R_, TL_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, \
-                                             tw_f, S_part, p_part, P_part, KT, G_part, st);
+    return launch_temporal_c_t<LG_, R_, TL_>(hist, T, items, nitems, bin_modes, order, mask, identity, dense,     \
+                                             inv_n0, tw, tw_f, S_part, p_part, P_part, KT, G_part, st);
     CASE(11, 2, true) CASE(12, 2, true) CASE(13, 2, false)
     CASE(11, 3, true) CASE(12, 3, true) CASE(13, 3, false)
 #undef CASE

@@ -1,7 +1,8 @@
 """The workgroup-cooperative temporal kernel (k_temporal_c: four waves per mode, transform lengths 2048 ..
 8192) against the FP64 oracle on the engine's own history, and against the one-mode-per-wave kernels.
 
-Lengths T = 600 .. 3000 run TPAD 2048 (m = 4 sub-transforms of 512 points), 4096 (m = 8) and 8192 (m = 16).
+Lengths T = 600 .. 4096 run TPAD 2048 (m = 4 sub-transforms of 512 points), 4096 (m = 8) and 8192 (m = 16);
+T = TPAD/2 with every frame valid (1024, 2048, 4096 "full") runs the kernel built without the mask tests.
 Work items of up to 64 modes (temporal_items = 16 over ~1600 binned modes) exercise the kernel's two-level
 sums; at TPAD 8192 items hold at most 16 modes.  Identity order with every frame valid, the live ring (a
 rotated order, drops and a guard) and a shuffled order with drops; the per-element bar as in
@@ -16,7 +17,7 @@ from tests.test_parity_gpu import _gpu_video, _radial_errors, _ring_meta, _run_r
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("T", [600, 1000, 1500, 2000, 2100, 3000])
+@pytest.mark.parametrize("T", [600, 1000, 1024, 1500, 2000, 2048, 2100, 3000, 4096])
 @pytest.mark.parametrize("case", ["full", "ring", "shuffled"])
 def test_coop_temporal_parity(gpu, T, case):
     N, NB = 64, 32
