@@ -302,7 +302,7 @@ def main():
     # the kernels' names as the profiler reports them (the traffic table below is keyed by them)
     kernel_symbol = {"spatial_rows": f"ddm::k_rows_w<9, unsigned short, {lr}, 2>",
                      "spatial_cols_cornerturn": f"ddm::k_cols_w<9, {tun['cols_f']}, {lr}>",
-                     "temporal_wk": (f"ddm::k_temporal_c<11, {lr}, true>" if tun["temporal_kernel"] == 2
+                     "temporal_wk": (f"ddm::k_temporal_c<11, {lr}, true, true>" if tun["temporal_kernel"] == 2
                                      else f"ddm::k_temporal_w<11, 2, {lr}>"),
                      "radial_finalize": "ddm::k_bin_sum<11>"}
     per_kernel = {}
