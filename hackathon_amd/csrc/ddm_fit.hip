@@ -258,7 +258,62 @@ __device__ __noinline__ double one_minus_isf3d_d(double x, double Z) {
   return 1 - sin(Z * phi) * exp(-0.5 * Z * log1p(u * u)) / (Z * u);
 }
 
-// Column value at one lag in FP64 (the objective; the Jacobian stays FP32).
+// 1 - g(x; Z) in FP64 with its derivatives with respect to ln x and ln Z
+// (x held fixed), the series below u = 1e-4 as above.
+__device__ __noinline__ double one_minus_isf3d_dd(double x, double Z, double& dlx, double& dlz) {
+  const double u = x / (Z + 1);
+  if (u < 1e-4) {
+    dlx = 2 * x * x * (Z + 2) / (6 * (Z + 1));
+    dlz = -Z * u * u / 6;
+    return x * x * (Z + 2) / (6 * (Z + 1));
+  }
+  const double phi = atan(u), s = sin(Z * phi), cz = cos(Z * phi);
+  const double pre = exp(-0.5 * Z * log1p(u * u));
+  const double udg = pre / Z * (Z * cz / (1 + u * u) - s * (Z * u / (1 + u * u) + 1 / u));
+  const double zdgz = pre / u * (phi * cz - s * (0.5 * log1p(u * u) + 1 / Z));
+  dlx = -udg;
+  dlz = -(zdgz - Z / (Z + 1) * udg);
+  return 1 - s * pre / (Z * u);
+}
+
+// The per-q LM's Jacobian column at one lag in FP64: the value c and its
+// derivatives with respect to alpha, ln vbar, ln Z and ln D.  (With an FP32
+// Jacobian the LM stalled in slow valleys, 2-4 % of the benchmark's bins
+// stopping 1-15 % above the FP64 LM's chi^2 from the same start: its steps
+// lost the valley direction below the FP32 derivatives' rounding.)
+template <int GEO>
+__device__ __forceinline__ void column_jd(double t, const float* q, int K, double alpha, double vbar, double Z,
+                                          double Dd, double V, const Isf2dDev& T, bool swim, double& c, double& da,
+                                          double& dlv, double& dlz, double& dld) {
+  c = da = dlv = dlz = dld = 0;
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) {
+    if (i >= K) break;
+    const double qi = q[i];
+    const double s = Dd * qi * qi * t;
+    double E = exp(-s), omE = -expm1(-s);
+    if (V > 0) {
+      const double J = bessel_j0(qi * V * t);
+      omE += E * (1 - J);
+      E *= J;
+    }
+    double ov = 0, ox = 0, oz = 0;
+    if (swim) ov = GEO == 1 ? one_minus_isf2d(qi * vbar * t, Z, T, ox, oz) : one_minus_isf3d_dd(qi * vbar * t, Z, ox, oz);
+    c += omE + E * alpha * ov;
+    da += E * ov;
+    dlv += E * alpha * ox;
+    dlz += E * alpha * oz;
+    dld += s * E * (1 - alpha * ov);
+  }
+  const double inv = 1.0 / K;
+  c *= inv;
+  da *= inv;
+  dlv *= inv;
+  dlz *= inv;
+  dld *= inv;
+}
+
+// Column value at one lag in FP64 (the objective).
 template <int GEO>
 __device__ double column_d(double t, const float* q, int K, double alpha, double vbar, double Z, double Dd, double V,
                            const Isf2dDev& T, bool swim) {
@@ -398,16 +453,14 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
     for (int k = 0; k < 21; k++) H[k] = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) g[k] = 0;
-    // the Jacobian in FP32 at the FP32-rounded parameters (theta itself stays FP64, so steps below
-    // an FP32 ulp of a parameter are kept and the FP64 objective sees them)
-    const float alpha = swim ? (float)th[2] : 0.f, vbar = expf((float)th[3]), Z = expf((float)th[4]),
-                Dd = expf((float)th[5]);
+    // the Jacobian in FP64 at theta
+    const double alpha = swim ? th[2] : 0.0, vbar = exp(th[3]), Z = exp(th[4]), Dd = exp(th[5]);
 #pragma unroll
     for (int r = 0; r < MAXT / 64; r++) {
       const int j = lane + 64 * r;
       if (j < nt) {
-        float c, da, dlv, dlz, dld;
-        column<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, true, c, da, dlv, dlz, dld);
+        double c, da, dlv, dlz, dld;
+        column_jd<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, c, da, dlv, dlz, dld);
         double F[6];
         F[0] = (double)w * c;
         F[1] = w;

@@ -199,7 +199,7 @@ class Fitter:
         return out
 
     def fit(self, tau, data, weight, qnodes, bnds, model=SWIMMER_3D, seeds=None, nstarts=3,
-            max_iter=100, seed_ab=True, vmag=None) -> FitResult:
+            max_iter=400, seed_ab=True, vmag=None) -> FitResult:
         """tau/data: lists (or 2D arrays) of per-fit lag times and D values; weight (nfits,);
         qnodes (nfits, K); bnds (nfits, 4); seeds (nfits, 6) for seeded LM, else grid seeding.
         seed_ab=False takes only the nonlinear parameters from the seeds and solves A, B by NNLS.
@@ -322,7 +322,7 @@ def problems_from(pr: Prepared):
 
 
 def analyzeMotilityLM(radial: Radial, dxUm: float, fps: float, model=SWIMMER_3D, fitter: Fitter | None = None,
-                      seeds=None, nstarts=3, max_iter=100, vmag=None):
+                      seeds=None, nstarts=3, max_iter=400, vmag=None):
     """Per-q LM fits of every usable bin of one Radial record -> (Prepared, FitResult).  vmag: a uniform
     drift speed (um/s, e.g. estimate_drift's mag) held fixed in every bin's model."""
     pr = prepare(radial, dxUm, fps)

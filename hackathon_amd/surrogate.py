@@ -250,7 +250,7 @@ class Surrogate:
 
 
 def compare(fitter: Fitter, probs: Problems, seeds: np.ndarray, truth4=None, mlp_ms=0.0, seed_ab=False,
-            max_iter=100, nstarts=3):
+            max_iter=400, nstarts=3):
     """Plain grid-seeded LM vs surrogate-seeded LM on the same problems."""
     args = probs.arrays()
     plain = fitter.fit(*args, model=SWIMMER_3D, nstarts=nstarts, max_iter=max_iter)

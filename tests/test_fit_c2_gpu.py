@@ -3,10 +3,11 @@ against the FP64 oracle LM on the same prepared data.
 
 D(q, tau) comes from the engine; prepare (src/motility_fit.js:383-401) picks the bins and the lag grid; both
 LMs start from the reference's start grid (the 3 best of src/motility_fit.js:577-611's starts) within the same
-bounds.  The per-q landscape is multimodal and slow valleys meet the 100-step cap, so agreement is held bin by
-bin as rates:
+bounds.  The per-q landscape is multimodal with slow valleys, so agreement is held bin by bin as rates:
+  - both LMs run up to 400 steps per start, the library's default (at a 100-step cap a few slow valleys left
+    both mid-descent, where the chi^2 reached depends on the trajectory);
   - from the oracle's own three starts, the GPU LM reaches the oracle LM's chi2 (1e-3) in all but 1 % of bins
-    (the LM arithmetic: FP32 Jacobian, FP64 parameters, objective and normal equations);
+    (the LM arithmetic: FP64 parameters, Jacobian, objective and normal equations);
   - from its own start grid, the GPU reaches the oracle's chi2 within 1e-4 in 90 % of bins and is worse than
     1e-3 in at most 2 % (near-equal grid chi2 can rank a different third start from FP32 inputs);
   - where both reach the same chi2, 80 % agree in every parameter to 1e-3 (the rest sit on flat directions:
@@ -48,13 +49,13 @@ def test_c2_per_bin_schulz_lm_against_fp64_oracle(gpu):
     from_starts = np.min([f.fit(*probs, model=F.SWIMMER_3D, seeds=starts[s]).chi2 for s in range(3)], axis=0)
     same_start_worse = 0
     for i, bn in enumerate(opr["bins"]):
-        oc = min(O.lm_fit(opr["tau"], bn["d"], bn["w"], bn["qn"], b4, starts[s, i], maxit=100)[1] for s in range(3))
+        oc = min(O.lm_fit(opr["tau"], bn["d"], bn["w"], bn["qn"], b4, starts[s, i], maxit=400)[1] for s in range(3))
         same_start_worse += from_starts[i] > oc * (1 + 1e-3)
     print(f"from the oracle's starts, GPU worse than 1e-3: {same_start_worse}/{nb}", flush=True)
     assert same_start_worse <= 0.01 * nb
     worse, same, agree, rows = [], 0, 0, []
     for i, bn in enumerate(opr["bins"]):
-        th, chi, _ = O.lm_plain(opr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, "3d")
+        th, chi, _ = O.lm_plain(opr["tau"], bn["d"], bn["w"], bn["qn"], b4, True, "3d", maxit=400)
         rel = (res.chi2[i] - chi) / max(chi, 1e-300)
         p_gpu = np.array([res.alpha[i], res.vbar[i], res.Z[i], res.D[i]])
         p_orc = np.array([th[2], np.exp(th[3]), np.exp(th[4]), np.exp(th[5])])
