@@ -83,7 +83,7 @@ export function analyzeMotilityLM(radial, opt = {}) {
   const batch = {
     nfits: n, ntauMax, knodes: K, model: pick(opt.model, MODEL.SWIMMER_3D),
     seedMode: opt.seeds ? (opt.nnlsAmplitudes ? SEED.NONLINEAR : SEED.GIVEN) : SEED.GRID,
-    nstarts: pick(opt.nstarts, 3), maxIter: pick(opt.maxIter, 100),
+    nstarts: pick(opt.nstarts, 3), maxIter: pick(opt.maxIter, 400),
     ntau: new Int32Array(n).fill(ntauMax), tau: new Float64Array(n * ntauMax), data: new Float64Array(n * ntauMax),
     weight: new Float64Array(n), qnodes: new Float64Array(n * K), bounds: new Float64Array(4 * n),
     seeds: opt.seeds || null,

@@ -595,7 +595,7 @@ static napi_value js_fit(napi_env env, napi_callback_info info) {
   b.model = (int32_t)get_prop_num(env, argv[1], "model", DDM_MODEL_SWIMMER_3D);
   b.seed_mode = (int32_t)get_prop_num(env, argv[1], "seedMode", DDM_SEED_GRID);
   b.nstarts = (int32_t)get_prop_num(env, argv[1], "nstarts", 3);
-  b.max_iter = (int32_t)get_prop_num(env, argv[1], "maxIter", 100);
+  b.max_iter = (int32_t)get_prop_num(env, argv[1], "maxIter", 400);
   void *ntau, *tau, *data, *w, *qn, *bd, *sd, *vm;
   size_t n0, n1, n2, n3, n4, n5, n6, n7;
   if (!get_typed(env, argv[1], "ntau", napi_int32_array, &ntau, &n0, 1) ||
