@@ -47,7 +47,7 @@ def _device_fit(L, fitter, tau, data, w, qn, bd, model, ntau=None):
     th = torch.full((n, 6), -7.0, dtype=torch.float64, device="cuda")
     chi = torch.full((n,), -7.0, dtype=torch.float64, device="cuda")
     it = torch.full((n,), -7, dtype=torch.int32, device="cuda")
-    b = F.FitBatch(n, m, qn.shape[1], model, F.SEED_GRID, 3, 100, nt.data_ptr(), *[a.data_ptr() for a in arrs],
+    b = F.FitBatch(n, m, qn.shape[1], model, F.SEED_GRID, 3, 400, nt.data_ptr(), *[a.data_ptr() for a in arrs],
                    None, None)
     rc = L.ddm_fit_device_noalloc(fitter._h, C.byref(b), th.data_ptr(), chi.data_ptr(), it.data_ptr(), 1)
     assert rc == 0, L.ddm_fitter_last_error(fitter._h).decode()
