@@ -304,6 +304,13 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
 
 template <typename T> __device__ __forceinline__ float px_to_float(T v) { return (float)v; }
 
+// Row pairs per wave and frame in k_rows_w.  One at N = 512: 70 VGPRs and
+// 36 KB of LDS per workgroup against 133 and 52 KB with two, so the other
+// analysis's kernels find room beside it (two in flight: 1.129 -> 1.112 ms
+// per 1024 x 512^2 step, the serial row pass unchanged); one at N = 2048,
+// where the row-spectra tile shares the exchange buffers' LDS.
+__host__ __device__ constexpr int rows_pairs_w(int logn) { return (logn == 9 || logn >= 11) ? 1 : 2; }
+
 // XCD-aware block order for a kernel whose workgroups each store half of
 // every 128-byte line they touch, the other half coming from a partner
 // workgroup (logical blocks 2j and 2j + 1).  The dispatcher hands workgroup b
@@ -318,11 +325,11 @@ __device__ __forceinline__ int xcd_pair_block(int b, int G) {
 }
 
 // FPW frames per workgroup (the same rows of each): every wave transforms
-// PPW row pairs per frame (two up to N = 1024: 16 rows per workgroup; one at
-// N = 2048: 8 rows), and the next pair's pixels are loaded while the current
-// pair is transformed (also across frames), so a wave keeps a load in flight
-// through its whole run and its set-up (twiddles, LDS addresses) is shared by
-// PPW FPW pairs.  At N = 2048 the kx-major tile shares its LDS with the
+// PPW row pairs per frame (rows_pairs_w: two at N = 256 and 1024, 16 rows per
+// workgroup; one at N = 512 and 2048, 8 rows), and the next pair's pixels are
+// loaded while the current pair is transformed (also across frames), so a
+// wave keeps a load in flight through its whole run and its set-up
+// (twiddles, LDS addresses) is shared by PPW FPW pairs.  At N = 2048 the kx-major tile shares its LDS with the
 // exchange buffers: a wave keeps its untangled spectra in registers until
 // every wave is done with its transform.
 template <int LOGN, typename Tin, int MAXLR, int FPW>
@@ -331,7 +338,7 @@ __global__ __launch_bounds__(NT) void k_rows_w(const Tin* __restrict__ frames, i
                                                float2* __restrict__ scratch, int2* __restrict__ rowinfo) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
   constexpr bool ALIAS = LOGN >= 11;           // tile and exchange buffers share the LDS
-  constexpr int PPW = ALIAS ? 1 : 2;           // row pairs per wave and frame
+  constexpr int PPW = rows_pairs_w(LOGN);      // row pairs per wave and frame
   constexpr int RW = 8 * PPW;                  // rows per workgroup
   constexpr int XS = xch_size(N);
   constexpr int TS = RW + 1;                   // padded tile row
@@ -719,7 +726,7 @@ template <typename Tin, int LOGN, int MAXLR, int FPW>
 static hipError_t launch_rows_w_f(const void* frames, int nframes, float scale, const float* win, const float2* tw,
                                   float2* scratch, int2* rowinfo, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
-  constexpr int RW = LOGN >= 11 ? 8 : 16;   // rows per workgroup (k_rows_w)
+  constexpr int RW = 8 * rows_pairs_w(LOGN);   // rows per workgroup (k_rows_w)
   const size_t xch = sizeof(float2) * 4 * xch_size(N), tile = sizeof(float2) * NH * (RW + 1);
   size_t lds = LOGN >= 11 ? (xch > tile ? xch : tile) : xch + tile;
   auto kern = k_rows_w<LOGN, Tin, MAXLR, FPW>;
