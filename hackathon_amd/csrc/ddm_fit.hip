@@ -344,10 +344,15 @@ __device__ double column_d(double t, const float* q, int K, double alpha, double
 // gradient, so that the gradient is the one of the FP64 objective: with an
 // FP32 model value in the residual the LM would settle on the FP32 model's
 // optimum, measurably above the FP64 one in slow valleys).
+// The Jacobian columns at theta come from the same pass (the model's
+// transcendentals are shared): jac[(5 r + k) * 64 + lane] holds the
+// lane's lag slot r, k = 0..4 for c, dc/dalpha, dc/dln vbar, dc/dln Z,
+// dc/dln D (a per-wave LDS buffer, so a trial that is accepted hands the LM
+// its next Jacobian without a second model evaluation).
 template <int GEO>
-__device__ __noinline__ double chi2_at(const double* th, const float* tv, const float* dv, int nt, float w, const float* q,
-                                       int K, float V, const Isf2dDev& T, bool swim, int lane,
-                                       double (&res)[MAXT / 64]) {
+__device__ __noinline__ double chi2_jac_at(const double* th, const float* tv, const float* dv, int nt, float w,
+                                           const float* q, int K, float V, const Isf2dDev& T, bool swim, int lane,
+                                           double (&res)[MAXT / 64], double* __restrict__ jac) {
   double s = 0;
   const double alpha = swim ? th[2] : 0.0, vbar = exp(th[3]), Z = exp(th[4]), Dd = exp(th[5]);
 #pragma unroll
@@ -355,9 +360,16 @@ __device__ __noinline__ double chi2_at(const double* th, const float* tv, const 
     const int j = lane + 64 * r;
     res[r] = 0;
     if (j < nt) {
-      const double c = column_d<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim);
+      double c, da, dlv, dlz, dld;
+      column_jd<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, c, da, dlv, dlz, dld);
       res[r] = (double)w * ((double)dv[r] - (th[0] * c + th[1]));
       s += res[r] * res[r];
+      double* jr = jac + 5 * 64 * r + lane;
+      jr[0] = c;
+      jr[64] = da;
+      jr[128] = dlv;
+      jr[192] = dlz;
+      jr[256] = dld;
     }
   }
   return wave_sum_d(s);
@@ -437,13 +449,17 @@ __device__ __forceinline__ double clampd(double x, float lo, float hi) {
 // LM from theta (in/out); returns chi^2, adds the steps taken to *steps.
 template <bool SWIM, int GEO>
 __device__ double lm_run(double* th, const float* lo, const float* hi, const float* tv, const float* dv, int nt, float w,
-                         const float* q, int K, float V, const Isf2dDev& T, int maxit, int lane, int* steps) {
+                         const float* q, int K, float V, const Isf2dDev& T, int maxit, int lane, int* steps,
+                         double* jbuf) {
   constexpr bool swim = SWIM;
   constexpr int p = SWIM ? 6 : 3;
   constexpr int idx[6] = {0, 1, SWIM ? 2 : 5, 3, 4, 5};
   for (int k = 0; k < 6; k++) th[k] = clampd(th[k], lo[k], hi[k]);
   double rcur[MAXT / 64], rtry[MAXT / 64];   // FP64 residuals at theta and at the trial point
-  double chi = chi2_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane, rcur);
+  // jcur: the Jacobian at theta, jtry: at the trial point (two halves of the wave's LDS buffer)
+  double* jcur = jbuf;
+  double* jtry = jbuf + 5 * MAXT;
+  double chi = chi2_jac_at<GEO>(th, tv, dv, nt, w, q, K, V, T, swim, lane, rcur, jcur);
   double lam = 1e-3, nu = 2;
   int it = 0;
   bool done = false;
@@ -453,14 +469,13 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
     for (int k = 0; k < 21; k++) H[k] = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) g[k] = 0;
-    // the Jacobian in FP64 at theta
-    const double alpha = swim ? th[2] : 0.0, vbar = exp(th[3]), Z = exp(th[4]), Dd = exp(th[5]);
+    // the Jacobian in FP64 at theta, from the evaluation that accepted it
 #pragma unroll
     for (int r = 0; r < MAXT / 64; r++) {
       const int j = lane + 64 * r;
       if (j < nt) {
-        double c, da, dlv, dlz, dld;
-        column_jd<GEO>(tv[r], q, K, alpha, vbar, Z, Dd, V, T, swim, c, da, dlv, dlz, dld);
+        const double* jr = jcur + 5 * 64 * r + lane;
+        const double c = jr[0], da = jr[64], dlv = jr[128], dlz = jr[192], dld = jr[256];
         double F[6];
         F[0] = (double)w * c;
         F[1] = w;
@@ -569,13 +584,16 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
           const double dk = nth[k] - th[k];
           pred += dk * (lam * diag[a] * dk + g[a]);
         }
-        const double nchi = chi2_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane, rtry);
+        const double nchi = chi2_jac_at<GEO>(nth, tv, dv, nt, w, q, K, V, T, swim, lane, rtry, jtry);
         const double rho = pred > 0 ? (chi - nchi) / pred : -1;
         if (nchi < chi && rho > 0) {
           const double rel = (chi - nchi) / (chi > 1e-300 ? chi : 1e-300);
           for (int k = 0; k < 6; k++) th[k] = nth[k];
 #pragma unroll
           for (int r = 0; r < MAXT / 64; r++) rcur[r] = rtry[r];
+          double* jt = jcur;
+          jcur = jtry;
+          jtry = jt;
           chi = nchi;
           const double f = 1 - (2 * rho - 1) * (2 * rho - 1) * (2 * rho - 1);
           lam *= f > 1.0 / 3 ? f : 1.0 / 3;
@@ -611,6 +629,8 @@ template <bool SWIM, int GEO>
 __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
                                                 float* __restrict__ theta_out, double* __restrict__ chi_out,
                                                 int* __restrict__ steps_out) {
+  // each wave's Jacobian columns at theta and at the trial point (FP64, lane-indexed)
+  __shared__ double lm_jac[FNT / 64][2 * 5 * MAXT];
   const int lane = threadIdx.x & 63;
   const int f = blockIdx.x * (FNT / 64) + (threadIdx.x >> 6);
   if (f >= nfits) return;
@@ -715,7 +735,8 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
     double th[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
-    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps);
+    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps,
+                                         lm_jac[threadIdx.x >> 6]);
     if (chi < bestchi) {
       bestchi = chi;
 #pragma unroll
