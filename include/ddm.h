@@ -81,7 +81,7 @@ typedef struct ddm_sizes {
 typedef struct ddm_tuning {
   int32_t temporal_items;  /* target count of temporal work items (default 2048; <= 64 modes each) */
   int32_t chunk_mb;        /* spatial chunk budget in MiB of row spectra (default 128) */
-  int32_t wave_spatial;    /* 1: workgroup-level spatial FFTs, 2: wavefront-level (N = 512, 1024, 2048 only) */
+  int32_t wave_spatial;    /* 1: workgroup-level spatial FFTs, 2: wavefront-level (N = 256 .. 2048 only) */
   int32_t cols_f;          /* frames per column workgroup of the wavefront kernel: 8 or 16 */
   int32_t fft_radix;       /* largest FFT radix of the spatial and temporal passes: 8 (default) or 4 */
   int32_t temporal_kernel; /* 1: one mode per wavefront (per workgroup above TPAD 4096), 2: one mode per
