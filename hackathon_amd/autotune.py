@@ -3,7 +3,7 @@
 Candidates cover the engine's tunables (include/ddm.h `ddm_tuning`): the
 temporal work split (target item count: modes per workgroup), the spatial
 chunk (row-spectrum scratch budget, i.e. how long a chunk stays in the
-last-level cache), workgroup- vs wavefront-level spatial FFTs (N = 512, 1024),
+last-level cache), workgroup- vs wavefront-level spatial FFTs (N = 256 .. 2048),
 frames per column workgroup and the FFT radix (radix-8 passes, or radix 4 only,
 in the spatial and temporal transforms).  Every candidate first has to reproduce the
 default configuration's D(q, tau) and P(q) on the caller's frames (relative
@@ -34,11 +34,12 @@ PARITY_RTOL = 1e-5
 def candidates(N: int, quick: bool = False):
     items = (2048, 8192) if quick else (1024, 2048, 4096)
     chunks = (128,) if quick else (64, 96, 128, 192, 256, 512, 1024)
-    waves = (1, 2) if N in (512, 1024) else (1,)
+    waves = (1, 2) if N in (256, 512, 1024, 2048) else (1,)
     radices = (8, 4)
     out = []
     for it, ch, wv, rx in itertools.product(items, chunks, waves, radices):
-        for cf in ((8, 16) if wv == 2 else (16,)):
+        # the wavefront column pass takes 8- or 16-frame tiles (8 only at N = 2048)
+        for cf in (((8,) if N == 2048 else (8, 16)) if wv == 2 else (16,)):
             out.append(dict(temporal_items=it, chunk_mb=ch, wave_spatial=wv, cols_f=cf, fft_radix=rx))
     return out
 

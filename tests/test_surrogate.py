@@ -125,7 +125,9 @@ def test_autotune_candidates_and_cache(tmp_path):
     c = AT.candidates(512)
     assert len(c) == len({tuple(sorted(x.items())) for x in c})          # no duplicates
     assert all(x["cols_f"] in (8, 16) and x["wave_spatial"] in (1, 2) for x in c)
-    assert all(x["wave_spatial"] == 1 for x in AT.candidates(256))       # wavefront spatial only at 512/1024
+    assert {x["wave_spatial"] for x in AT.candidates(256)} == {1, 2}      # wavefront spatial at N = 256 .. 2048
+    assert all(x["cols_f"] == 8 for x in AT.candidates(2048) if x["wave_spatial"] == 2)
+    assert all(x["wave_spatial"] == 1 for x in AT.candidates(384))       # sides with a factor 3: workgroup-level
     path = str(tmp_path / "cache.json")
     AT.save_cache({AT.key(512, 1024, 1024, 256, "uint16"): c[0]}, path)
     AT.save_cache({AT.key(256, 64, 64, 32, "uint8"): c[1]}, path)
