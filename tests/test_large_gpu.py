@@ -113,7 +113,7 @@ def test_large_pipeline_4096x2048(gpu):
     assert len(pr_.bins) > 900 and np.all(np.isfinite(res.chi2))
 
     # the global Schulz-model fit of the whole record (alpha, vbar, Z, D shared by its bins), seeded by the
-    # surrogate with the plain grid-seeded fit as fallback and judge.  The network was trained on 256^2 x 512
+    # surrogate with the plain grid-seeded fit as fallback and reference.  The network was trained on 256^2 x 512
     # videos with 32 bins over |k| in [4, 100) of 256 (3 units wide) and lags to 255 frames; here the unit
     # bins covering the same |k|/N ranges (24 per training bin, weighted by mode count) and the first 256
     # lags form its input.

@@ -73,7 +73,7 @@ def test_c2_full_size_against_fp64(gpu):
     for (name, tag), (hz, pr, rmax) in worst.items():
         assert hz < 1e-5, (name, tag, hz)
         assert pr < 1e-5, (name, tag, pr)
-        # every element of D within 1e-5 relative of FP64
+        # the parity bar: every element of D within 1e-5 relative of FP64
         assert rmax < 1e-5, (name, tag, rmax)
 
 
