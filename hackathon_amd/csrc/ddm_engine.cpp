@@ -229,6 +229,7 @@ Bins make_bins(int N, int NB, double kmin, double kmax, int hypot_mode, int K) {
 struct ddm_engine {
   ddm_geometry g{};
   int NH = 0, TPAD = 0, logn = 0, logt = 0;
+  int HS = 0;             // history row stride: T rounded up to 16 slots (128 B)
   long modes = 0;
   int chunk = 0;
   int cols_f = 16;        // frames per column workgroup (wavefront kernels)
@@ -413,6 +414,12 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->modes = (long)g.N * e->NH;
   e->logn = ilog2(g.N);
   e->TPAD = std::max(256, 1 << ilog2(2L * g.T));
+  // Each mode's series starts on a 128-byte line, so the column pass's
+  // 16-frame corner-turn runs from slot 16j fill whole lines: with a stride
+  // of T they straddle two lines for T % 16 != 0, and the pass ran 1.5-2.4x
+  // slower (512^2 frames, T = 1025 .. 1100).  TPAD/2 >= 128 is a power of two
+  // >= T, so HS <= TPAD/2.
+  e->HS = (g.T + 15) & ~15;
   // Lags with few valid pairs take Re G from direct sums: the FFT route's
   // error is a few ulp x sum_t |y|^2 whatever the lag, i.e. ~ulp x T / N(tau)
   // relative to D(tau) there.  KT (the default, for readTemporalFull) covers
@@ -491,11 +498,12 @@ int ddm_create_tuned(const ddm_geometry* gp, const ddm_tuning* tp, ddm_engine** 
   e->stage_bytes = (size_t)chunk * g.N * g.N * 4;
 
   const size_t nitems = std::max(e->nitems, 1);
-  // TPAD/2 - T spare samples behind the last mode: the temporal kernel reads
-  // every series (and the frame order) as TPAD/2 entries without a guard; the
-  // entries past T belong to the next mode or to the pad and are masked off,
-  // the order's pad stays 0
-  const size_t hist_len = (size_t)e->modes * g.T + (size_t)(e->TPAD / 2 - g.T);
+  // Rows of HS slots (the HS - T behind each series stay zero), then TPAD/2 -
+  // HS spare samples behind the last row: the temporal kernel reads every
+  // series (and the frame order) as TPAD/2 entries without a guard; the
+  // entries past T belong to the row's padding, the next mode or the spare
+  // samples and are masked off, the order's pad stays 0
+  const size_t hist_len = (size_t)e->modes * e->HS + (size_t)(e->TPAD / 2 - e->HS);
   if ((he = dalloc(&e->hist, hist_len)) != hipSuccess ||
       (he = dalloc(&e->scratch, (size_t)chunk * e->NH * g.N)) != hipSuccess ||
       (he = dalloc(&e->rowinfo, (size_t)chunk * g.N)) != hipSuccess ||
@@ -599,6 +607,7 @@ int ddm_get_sizes(const ddm_engine* e, ddm_sizes* s) {
   s->hist_bytes = (int64_t)e->modes * e->g.T * 8;
   s->work_items = e->nitems;
   s->chunk_frames = e->chunk;
+  s->hist_stride = e->HS;
   return DDM_OK;
 }
 
@@ -636,9 +645,9 @@ static int run_spatial(ddm_engine* e, const void* dframes, int dtype, int nframe
   he = timed(e, DDM_K_COLS, [&] {
     if (e->wave_spatial)
       return ddm::launch_cols_w(e->logn, e->cols_f, e->maxlr, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win,
-                                e->tw_n, e->wx, e->hist, e->g.T, slot % e->g.T, e->stream);
+                                e->tw_n, e->wx, e->hist, e->g.T, e->HS, slot % e->g.T, e->stream);
     return ddm::launch_cols(e->g.N, e->maxlr, e->scratch, e->rowinfo, nframes, dtype != DDM_F32, scale, e->win, e->tw_n, e->wx,
-                            e->hist, e->g.T, slot % e->g.T, e->stream);
+                            e->hist, e->g.T, e->HS, slot % e->g.T, e->stream);
   });
   if (he != hipSuccess) return hip_fail(e, he, "spatial cols launch");
   e->mean_frame = nframes - 1;   // its row sums stay in rowinfo until the next ingest
@@ -686,20 +695,20 @@ static int compute_temporal(ddm_engine* e, const uint32_t* order, const float* m
          const int identity = e->last_identity ? 1 : 0;
          if (e->tkern && e->twg_cu > 0) {   // cooperative kernel on a persistent grid
            const ddm::TemporalArgs a{e->hist, e->items, e->bin_modes, e->order, e->mask, e->S_part, e->p_part,
-                                     e->P_part, e->G_part, e->g.T, identity, e->cur_KT, 0, inv_n0};
+                                     e->P_part, e->G_part, e->g.T, e->HS, identity, e->cur_KT, 0, inv_n0};
            return ddm::launch_temporal_cp(e->logt, e->maxlr, a, e->nitems, e->tcounter, e->tw_t, e->tw_f,
                                           std::min(e->nitems, e->twg_cu * e->cus), e->stream);
          }
          if (e->tkern)        // cooperative kernel (512-point sub-transforms, r-major spectra)
-           return ddm::launch_temporal_c(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes,
+           return ddm::launch_temporal_c(e->logt, e->maxlr, e->hist, e->g.T, e->HS, e->items, e->nitems, e->bin_modes,
                                          e->order, e->mask, identity, e->cur_full && 2 * e->g.T == e->TPAD, inv_n0,
                                          e->tw_t, e->tw_f, e->S_part, e->p_part, e->P_part, e->cur_KT, e->G_part,
                                          e->stream);
          if (e->logt <= 12)   // wavefront-per-mode kernel (sub-transforms of at most 512 points)
-           return ddm::launch_temporal_w(e->logt, e->maxlr, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order,
+           return ddm::launch_temporal_w(e->logt, e->maxlr, e->hist, e->g.T, e->HS, e->items, e->nitems, e->bin_modes, e->order,
                                          e->mask, identity, inv_n0, e->tw_t, e->tw_f, e->S_part, e->p_part,
                                          e->P_part, e->cur_KT, e->G_part, e->stream);
-         return ddm::launch_temporal(e->logt, e->hist, e->g.T, e->items, e->nitems, e->bin_modes, e->order, e->mask,
+         return ddm::launch_temporal(e->logt, e->hist, e->g.T, e->HS, e->items, e->nitems, e->bin_modes, e->order, e->mask,
                                      identity, inv_n0, e->tw_t, e->S_part, e->p_part, e->P_part, e->cur_KT,
                                      e->G_part, e->stream);
        })) != hipSuccess)
@@ -750,7 +759,7 @@ static int compute_drift(ddm_engine* e, float* drift) {
     // bins whose mode ranges meet [c0, c0 + nc)
     const int b0 = (int)(std::upper_bound(off.begin(), off.end(), (uint32_t)c0) - off.begin()) - 1;
     const int b1 = (int)(std::upper_bound(off.begin(), off.end(), (uint32_t)(c0 + nc - 1)) - off.begin()) - 1;
-    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, L, c0, nc, e->order, e->mask, e->last_identity ? 1 : 0,
+    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, e->HS, L, c0, nc, e->order, e->mask, e->last_identity ? 1 : 0,
                                         inv_n0, e->npairs, e->tw_t, e->KT, e->bin_modes, nullptr, e->fullC, nullptr,
                                         e->stream),
            "drift: per-mode C");
@@ -807,7 +816,19 @@ int ddm_read_history(ddm_engine* e, float* out) {
   if (!e) return DDM_EINVAL;
   if (!out) return fail(e, DDM_EINVAL, "readHistory: null output");
   HIPCHK(e, hipStreamSynchronize(e->stream), "readHistory: stream");
-  HIPCHK(e, hipMemcpy(out, e->hist, (size_t)e->modes * e->g.T * 8, hipMemcpyDeviceToHost), "readHistory: copy");
+  const size_t row = (size_t)e->g.T * 8, pitch = (size_t)e->HS * 8;
+  if (row == pitch) {
+    HIPCHK(e, hipMemcpy(out, e->hist, (size_t)e->modes * row, hipMemcpyDeviceToHost), "readHistory: copy");
+    return DDM_OK;
+  }
+  // rows of HS slots on the device, of T in the output (strided copies of at
+  // most 16384 rows each)
+  for (long m0 = 0; m0 < e->modes; m0 += 16384) {
+    const size_t rows = (size_t)std::min<long>(16384, e->modes - m0);
+    HIPCHK(e, hipMemcpy2D(reinterpret_cast<char*>(out) + m0 * row, row, e->hist + m0 * e->HS, pitch, row, rows,
+                          hipMemcpyDeviceToHost),
+           "readHistory: copy");
+  }
   return DDM_OK;
 }
 
@@ -824,7 +845,7 @@ int ddm_read_temporal_full(ddm_engine* e, float* D, float* C, float* P) {
   const float inv_n0 = e->last_n0 > 0.5f ? 1.0f / e->last_n0 : 0.0f;
   for (long m0 = 0; m0 < e->modes; m0 += MC) {
     const int nm = (int)std::min<long>(MC, e->modes - m0);
-    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, L, m0, nm, e->order, e->mask, e->last_identity ? 1 : 0,
+    HIPCHK(e, ddm::launch_temporal_full(e->logt, e->hist, T, e->HS, L, m0, nm, e->order, e->mask, e->last_identity ? 1 : 0,
                                         inv_n0, e->npairs, e->tw_t, e->KT, nullptr, e->fullD, e->fullC, e->fullP,
                                         e->stream),
            "readTemporalFull: launch");
@@ -1020,7 +1041,7 @@ int engine_temporal_plan(ddm_engine* e, const uint32_t* order, const float* mask
   int rc = plan_temporal(e, order, mask, nullptr, nullptr, nullptr, &identity, &full, &n0, &KT);
   if (rc) return rc;
   *a = TemporalArgs{e->hist, e->items, e->bin_modes, e->order, e->mask, e->S_part, e->p_part, e->P_part,
-                    e->G_part, e->g.T, identity ? 1 : 0, KT, 0, n0 > 0.5f ? 1.0f / n0 : 0.0f};
+                    e->G_part, e->g.T, e->HS, identity ? 1 : 0, KT, 0, n0 > 0.5f ? 1.0f / n0 : 0.0f};
   *logt = e->logt <= 12 || e->tkern ? e->logt : 0;
   return DDM_OK;
 }

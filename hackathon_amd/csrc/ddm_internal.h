@@ -14,25 +14,25 @@ hipError_t launch_rows(int dtype, int n, int maxlr, const void* frames, int nfra
                        const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);
 int cols_frames_per_wg(int n);
 hipError_t launch_cols(int n, int maxlr, const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
-                       const float* win, const float2* tw, const float2* wx, float2* hist, int T, int first_slot,
-                       hipStream_t st);
-hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+                       const float* win, const float2* tw, const float2* wx, float2* hist, int T, int hs,
+                       int first_slot, hipStream_t st);
+hipError_t launch_temporal(int logt, const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                            float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st);
-hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                              const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                              const float2* tw_f, float* S_part, float* p_part, float* P_part, int KT, float* G_part,
                              hipStream_t st);
 // the workgroup-cooperative form for logt 11..13 (items of at most 64 modes,
 // 16 at logt 13; S_part rows r-major with lm = logt - 9)
 // dense: T = 2^logt / 2 and every frame valid (the mask tests compiled out)
-hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
+hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, int hs, const int4* items, int nitems,
                              const int* bin_modes, const uint32_t* order, const float* mask, int identity, int dense,
                              float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                              float* P_part, int KT, float* G_part, hipStream_t st);
 // per-mode D, C, P of modes m0 .. m0 + nm - 1 (readTemporalFull)
 // (mode_list: entries m0 .. m0 + nm - 1 of a mode list instead; outD/outP may be null)
-hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+hipError_t launch_temporal_full(int logt, const float2* hist, int T, int hs, int L, long m0, int nm, const uint32_t* order,
                                 const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
                                 int KT, const int* mode_list, float* outD, float2* outC, float* outP, hipStream_t st);
 // phase-drift sums of bins b0 .. b0 + nb - 1 over binned modes c0 .. c0 + nc - 1 (C rows of those modes)
@@ -54,7 +54,7 @@ hipError_t launch_rows_w(int dtype, int logn, int maxlr, const void* frames, int
                          const float2* tw, float2* scratch, int2* rowinfo, hipStream_t st);
 hipError_t launch_cols_w(int logn, int F, int maxlr, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
                          float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
-                         int first_slot, hipStream_t st);
+                         int hs, int first_slot, hipStream_t st);
 
 // Device copy of the in-plane ISF table (ddm_isf2d.h): row i's values at
 // v[off[i] .. off[i] + n[i] + 1], sampled at ln Lambda = u0[i] + k du[i].
@@ -70,7 +70,7 @@ struct TemporalArgs {
   float* p_part;
   float* P_part;
   float* G_part;
-  int T, identity, KT, item0;
+  int T, hs, identity, KT, item0;   // hs: history row stride (complex elements)
   float inv_n0;
 };
 

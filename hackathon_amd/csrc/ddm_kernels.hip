@@ -47,7 +47,7 @@ __device__ __forceinline__ float2 block_sum2(float2 v, float* red, int tid) {
 }
 
 template <int LOGT>
-__global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist, int T, const int4* __restrict__ items,
+__global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist, int T, int hs, const int4* __restrict__ items,
                                                  const int* __restrict__ bin_modes, const uint32_t* __restrict__ order,
                                                  const float* __restrict__ mask, int identity, float inv_n0,
                                                  const float2* __restrict__ tw_g, float* __restrict__ S_part,
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(NT) void k_temporal(const float2* __restrict__ hist
 
   for (int mi = 0; mi < count; mi++) {
     const long mode = bin_modes[start + mi];
-    const float2* x = hist + mode * T;
+    const float2* x = hist + mode * hs;
     float2 y[TJ];
 #pragma unroll
     for (int i = 0; i < TJ; i++) {
@@ -247,7 +247,7 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
   float* __restrict__ p_part = A.p_part;
   float* __restrict__ P_part = A.P_part;
   float* __restrict__ G_part = A.G_part;
-  const int T = A.T, identity = A.identity, KT = A.KT;
+  const int T = A.T, hs = A.hs, identity = A.identity, KT = A.KT;
   const float inv_n0 = A.inv_n0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   float2* xch = lds + wave * WS;
@@ -291,9 +291,10 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
   const int rounds = (count + 3) / 4;
   // Series loads without guards: a wave past the item's last mode re-reads
   // that mode (its contributions are dropped through `active`); samples
-  // t in [T, n/2) read the next mode's head or the n/2 - T spare samples the
-  // history buffer carries behind its last mode (they exist only when the
-  // series is not dense, and the mask multiply zeroes them).  One base
+  // t in [T, n/2) read the row's hs - T padding slots, the next mode's head
+  // or the n/2 - hs spare samples the history buffer carries behind its last
+  // row (they exist only when the series is not dense, and the mask multiply
+  // zeroes them).  One base
   // address with immediate offsets; guarded or clamped loads would cost a
   // branch region or an address pair per sample.
   // A non-identity order is applied by the loads themselves: sample t is
@@ -303,7 +304,7 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
 #pragma unroll
   for (int q = 0; q < PT; q++) ord[q] = identity ? (uint32_t)(lane + 64 * q) : order[lane + 64 * q];
   auto load_series = [&](float2 (&dst)[PT], int slot) {
-    const float2* x = hist + (long)bin_modes[start + (slot < count ? slot : count - 1)] * T;
+    const float2* x = hist + (long)bin_modes[start + (slot < count ? slot : count - 1)] * hs;
     if (identity) {
 #pragma unroll
       for (int q = 0; q < PT; q++) {   // each series is read once: non-temporal loads
@@ -517,7 +518,7 @@ __device__ __forceinline__ void temporal_w_item(const TemporalArgs& A, const int
 template <int LOGT, int LM, int MAXLR>
 // TPAD = 4096: one workgroup per CU by LDS (the per-wave accumulators), so
 // up to 512 registers per lane instead of spilling at 256
-__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const float2* __restrict__ hist, int T,
+__global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const float2* __restrict__ hist, int T, int hs,
                                                       const int4* __restrict__ items,
                                                       const int* __restrict__ bin_modes,
                                                       const uint32_t* __restrict__ order,
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_w(const flo
   extern __shared__ float2 lds[];
   float2 tbase[(1 << LM) > 1 ? (1 << LM) : 1], wlast[8];
   temporal_w_setup<LOGT, LM, MAXLR>(lds, tw_g, tw_f, tbase, wlast);
-  const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, identity, KT, 0, inv_n0};
+  const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, hs, identity, KT, 0, inv_n0};
   temporal_w_item<LOGT, LM, MAXLR>(A, blockIdx.x, lds, tbase, wlast);
 }
 
@@ -593,7 +594,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 1 : 2) void k_temporal_wq(const Te
 // G' and A' + B' from direct sums (the transforms' error there is ~ulp x T /
 // N(tau) of the value).
 template <int LOGT>
-__global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__ hist, int T, int L, long m0,
+__global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__ hist, int T, int hs, int L, long m0,
                                                       const uint32_t* __restrict__ order,
                                                       const float* __restrict__ mask, int identity, float inv_n0,
                                                       const float* __restrict__ npairs, const float2* __restrict__ tw_g,
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(NT) void k_temporal_full(const float2* __restrict__
   // modes m0 + blockIdx.x, or entries m0 + blockIdx.x of mode_list
   const long mode = mode_list ? (long)mode_list[m0 + blockIdx.x] : m0 + blockIdx.x;
   for (int i = tid; i < n; i += NT) tw[i] = tw_g[i];
-  const float2* x = hist + mode * T;
+  const float2* x = hist + mode * hs;
   float mk[TJ];
   float2 y[TJ];
 #pragma unroll
@@ -1046,7 +1047,7 @@ __global__ __launch_bounds__(NT) void k_bin_lags(const double* __restrict__ Gbin
 // ----------------------------------------------------------------- launchers
 
 template <int LOGT>
-static hipError_t launch_temporal_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+static hipError_t launch_temporal_t(const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                                     const uint32_t* order, const float* mask, int identity, float inv_n0,
                                     const float2* tw, float* S_part, float* p_part, float* P_part, int KT,
                                     float* G_part, hipStream_t st) {
@@ -1055,19 +1056,19 @@ static hipError_t launch_temporal_t(const float2* hist, int T, const int4* items
   auto kern = k_temporal<LOGT>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
+  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, hs, items, bin_modes, order, mask, identity, inv_n0,
                      tw, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
 }
 
-hipError_t launch_temporal(int logt, const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+hipError_t launch_temporal(int logt, const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                            const uint32_t* order, const float* mask, int identity, float inv_n0, const float2* tw,
                            float* S_part, float* p_part, float* P_part, int KT, float* G_part, hipStream_t st) {
-  if (KT < 1 || KT > NT / 4 || KT > T) return hipErrorInvalidValue;
+  if (KT < 1 || KT > NT / 4 || KT > T || hs < T) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                    \
   case LG_:                                                                                                        \
-    return launch_temporal_t<LG_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
+    return launch_temporal_t<LG_>(hist, T, hs, items, nitems, bin_modes, order, mask, identity, inv_n0, tw, S_part, p_part, \
                                 P_part, KT, G_part, st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE
@@ -1082,7 +1083,7 @@ static size_t temporal_w_lds() {
 }
 
 template <int LOGT, int LM, int MAXLR>
-static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+static hipError_t launch_temporal_w_t(const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                                       const uint32_t* order, const float* mask, int identity, float inv_n0,
                                       const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                                       float* P_part, int KT, float* G_part, hipStream_t st) {
@@ -1090,7 +1091,7 @@ static hipError_t launch_temporal_w_t(const float2* hist, int T, const int4* ite
   auto kern = k_temporal_w<LOGT, LM, MAXLR>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
+  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, hs, items, bin_modes, order, mask, identity, inv_n0,
                      tw, tw_f, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
 }
@@ -1153,16 +1154,17 @@ hipError_t launch_temporal_wq(int logt, int maxlr, const TemporalArgs* vids, int
 // tw: exp(-2 pi i j/n) for the TPAD length n; tw_f: the same for the
 // sub-transform length n/m (m = 1 for n <= 512, n/512 above); maxlr: the
 // FFT radix cap (3: radix 8, 2: radix 4).
-hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
+hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, int hs, const int4* items, int nitems,
                              const int* bin_modes, const uint32_t* order, const float* mask, int identity,
                              float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                              float* P_part, int KT, float* G_part, hipStream_t st) {
   // KT tail lags: four lanes each, and their samples must fit the exchange buffer
-  if (KT < 1 || KT > 64 || KT > T || 2 * KT > xch_size(std::min(1 << logt, 512))) return hipErrorInvalidValue;
+  if (KT < 1 || KT > 64 || KT > T || hs < T || 2 * KT > xch_size(std::min(1 << logt, 512)))
+    return hipErrorInvalidValue;
   switch (logt * 4 + (maxlr == 2 ? 2 : 3)) {
 #define CASE(LG_, LM_, R_)                                                                                           \
   case LG_ * 4 + R_:                                                                                                \
-    return launch_temporal_w_t<LG_, LM_, R_>(hist, T, items, nitems, bin_modes, order, mask, identity, inv_n0, tw,  \
+    return launch_temporal_w_t<LG_, LM_, R_>(hist, T, hs, items, nitems, bin_modes, order, mask, identity, inv_n0, tw,  \
                                              tw_f, S_part, p_part, P_part, KT, G_part, st);
     CASE(8, 0, 2) CASE(9, 0, 2) CASE(10, 1, 2) CASE(11, 2, 2) CASE(12, 3, 2)
     CASE(8, 0, 3) CASE(9, 0, 3) CASE(10, 1, 3) CASE(11, 2, 3) CASE(12, 3, 3)
@@ -1172,7 +1174,7 @@ hipError_t launch_temporal_w(int logt, int maxlr, const float2* hist, int T, con
 }
 
 template <int LOGT>
-static hipError_t launch_temporal_full_t(const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+static hipError_t launch_temporal_full_t(const float2* hist, int T, int hs, int L, long m0, int nm, const uint32_t* order,
                                          const float* mask, int identity, float inv_n0, const float* npairs,
                                          const float2* tw, int KT, const int* mode_list, float* outD, float2* outC,
                                          float* outP, hipStream_t st) {
@@ -1181,19 +1183,19 @@ static hipError_t launch_temporal_full_t(const float2* hist, int T, int L, long 
   auto kern = k_temporal_full<LOGT>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(nm), dim3(NT), lds, st, hist, T, L, m0, order, mask, identity, inv_n0, npairs, tw, KT,
+  hipLaunchKernelGGL(kern, dim3(nm), dim3(NT), lds, st, hist, T, hs, L, m0, order, mask, identity, inv_n0, npairs, tw, KT,
                      mode_list, outD, outC, outP);
   return hipGetLastError();
 }
 
-hipError_t launch_temporal_full(int logt, const float2* hist, int T, int L, long m0, int nm, const uint32_t* order,
+hipError_t launch_temporal_full(int logt, const float2* hist, int T, int hs, int L, long m0, int nm, const uint32_t* order,
                                 const float* mask, int identity, float inv_n0, const float* npairs, const float2* tw,
                                 int KT, const int* mode_list, float* outD, float2* outC, float* outP, hipStream_t st) {
-  if (KT < 1 || KT > 64 || KT > T || nm < 1) return hipErrorInvalidValue;
+  if (KT < 1 || KT > 64 || KT > T || hs < T || nm < 1) return hipErrorInvalidValue;
   switch (logt) {
 #define CASE(LG_)                                                                                                   \
   case LG_:                                                                                                       \
-    return launch_temporal_full_t<LG_>(hist, T, L, m0, nm, order, mask, identity, inv_n0, npairs, tw, KT,         \
+    return launch_temporal_full_t<LG_>(hist, T, hs, L, m0, nm, order, mask, identity, inv_n0, npairs, tw, KT,         \
                                        mode_list, outD, outC, outP, st);
     CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
 #undef CASE

@@ -225,7 +225,7 @@ template <int N, int F, int MAXLR>
 __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch, const int2* __restrict__ rowinfo,
                                              int nframes, int is_int, float scale, const float* __restrict__ win,
                                              const float2* __restrict__ tw_g, const float2* __restrict__ wx,
-                                             float2* __restrict__ hist, int T, int first_slot) {
+                                             float2* __restrict__ hist, int T, int hs, int first_slot) {
   constexpr int NH = N / 2 + 1;
   extern __shared__ float2 lds[];
   float2* buf = lds;                  // F * N
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(NT) void k_cols(const float2* __restrict__ scratch,
     if (f0 + f >= nframes) continue;
     int slot = (first_slot + f0 + f) % T;
     float2 v = buf[f * N + ky];
-    hist[((long)ky * NH + kx) * T + slot] = make_float2(v.x * s, v.y * s);
+    hist[((long)ky * NH + kx) * hs + slot] = make_float2(v.x * s, v.y * s);
   }
 }
 
@@ -507,7 +507,7 @@ template <int LOGN, int F, int MAXLR>
 __global__ __launch_bounds__(64 * cols_waves<LOGN>()) void k_cols_w(
     const float2* __restrict__ scratch, const int2* __restrict__ rowinfo, int nframes, int is_int, float scale,
     const float* __restrict__ win, const float2* __restrict__ tw, const float2* __restrict__ wx,
-    float2* __restrict__ hist, int T, int first_slot) {
+    float2* __restrict__ hist, int T, int hs, int first_slot) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1, P = N / 64;
   constexpr int W = cols_waves<LOGN>(), NTC = 64 * W;
   static_assert(F % W == 0, "whole frames per wave");
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(64 * cols_waves<LOGN>()) void k_cols_w(
       // non-temporal: the history is read back only by the temporal pass, after
       // the whole window; keeping it out of the caches leaves them to the row spectra
       const float2 z = tile[(e / F) * TS + fl];
-      float2* dst = &hist[((long)ky * NH + kx) * T + slot];
+      float2* dst = &hist[((long)ky * NH + kx) * hs + slot];
       __builtin_nontemporal_store(z.x, &dst->x);
       __builtin_nontemporal_store(z.y, &dst->y);
     }
@@ -687,7 +687,7 @@ int cols_frames_per_wg(int n) { return n >= 1024 ? 8 : 16; }
 template <int N, int MAXLR>
 static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                                 const float* win, const float2* tw, const float2* wx, float2* hist, int T,
-                                int first_slot, hipStream_t st) {
+                                int hs, int first_slot, hipStream_t st) {
   constexpr int NH = N / 2 + 1;
   constexpr int F = N >= 1024 ? 8 : 16;
   size_t lds = sizeof(float2) * (F * N + N) + sizeof(float) * 4 * F + sizeof(long long) * F;
@@ -695,7 +695,7 @@ static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int 
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid(NH, (nframes + F - 1) / F);
-  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T,
+  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, hs,
                      first_slot);
   return hipGetLastError();
 }
@@ -703,11 +703,11 @@ static hipError_t launch_cols_t(const float2* scratch, const int2* rowinfo, int 
 template <int MAXLR>
 static hipError_t launch_cols_r(int n, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
                                 float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
-                                int first_slot, hipStream_t st) {
+                                int hs, int first_slot, hipStream_t st) {
   switch (n) {
 #define CASE(N_) \
   case N_:       \
-    return launch_cols_t<N_, MAXLR>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+    return launch_cols_t<N_, MAXLR>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, hs, first_slot, st);
     CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
     CASE(48) CASE(96) CASE(192) CASE(384) CASE(768) CASE(1536)
 #undef CASE
@@ -717,9 +717,9 @@ static hipError_t launch_cols_r(int n, const float2* scratch, const int2* rowinf
 
 hipError_t launch_cols(int n, int maxlr, const float2* scratch, const int2* rowinfo, int nframes, int is_int,
                        float scale, const float* win, const float2* tw, const float2* wx, float2* hist, int T,
-                       int first_slot, hipStream_t st) {
-  return maxlr == 2 ? launch_cols_r<2>(n, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st)
-                    : launch_cols_r<3>(n, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+                       int hs, int first_slot, hipStream_t st) {
+  return maxlr == 2 ? launch_cols_r<2>(n, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, hs, first_slot, st)
+                    : launch_cols_r<3>(n, scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, hs, first_slot, st);
 }
 
 template <typename Tin, int LOGN, int MAXLR, int FPW>
@@ -748,7 +748,7 @@ static hipError_t launch_rows_w_t(const void* frames, int nframes, float scale, 
 template <int LOGN, int F, int MAXLR>
 static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, int nframes, int is_int, float scale,
                                   const float* win, const float2* tw, const float2* wx, float2* hist, int T,
-                                  int first_slot, hipStream_t st) {
+                                  int hs, int first_slot, hipStream_t st) {
   constexpr int N = 1 << LOGN, NH = N / 2 + 1;
   size_t tile = sizeof(float2) * (N / 2) * (F + 1), xch = sizeof(float2) * cols_waves<LOGN>() * xch_size(N);
   size_t lds = (tile > xch ? tile : xch) + (LOGN <= 9 ? sizeof(float2) * twpad(N) : 0);
@@ -758,7 +758,7 @@ static hipError_t launch_cols_w_t(const float2* scratch, const int2* rowinfo, in
   const int ntiles = (nframes + F - 1) / F;
   const dim3 grid = F < 16 ? dim3(NH * ntiles) : dim3(NH, ntiles);
   hipLaunchKernelGGL(kern, grid, dim3(64 * cols_waves<LOGN>()), lds, st, scratch, rowinfo, nframes, is_int, scale, win,
-                     tw, wx, hist, T, first_slot);
+                     tw, wx, hist, T, hs, first_slot);
   return hipGetLastError();
 }
 
@@ -792,11 +792,11 @@ hipError_t launch_rows_w(int dtype, int logn, int maxlr, const void* frames, int
 
 hipError_t launch_cols_w(int logn, int F, int maxlr, const float2* scratch, const int2* rowinfo, int nframes,
                          int is_int, float scale, const float* win, const float2* tw, const float2* wx, float2* hist,
-                         int T, int first_slot, hipStream_t st) {
+                         int T, int hs, int first_slot, hipStream_t st) {
   switch ((logn * 32 + F) * 4 + (maxlr == 2 ? 2 : 3)) {
 #define CASE(LG_, F_, R_)                  \
   case (LG_ * 32 + F_) * 4 + R_:           \
-    return launch_cols_w_t<LG_, F_, R_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, first_slot, st);
+    return launch_cols_w_t<LG_, F_, R_>(scratch, rowinfo, nframes, is_int, scale, win, tw, wx, hist, T, hs, first_slot, st);
     CASE(8, 16, 3) CASE(8, 8, 3) CASE(9, 16, 3) CASE(9, 8, 3) CASE(10, 8, 3) CASE(10, 16, 3) CASE(11, 8, 3)
     CASE(8, 16, 2) CASE(8, 8, 2) CASE(9, 16, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(10, 16, 2) CASE(11, 8, 2)
 #undef CASE

@@ -113,7 +113,7 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   static_assert(m >= 4 && m <= 16, "n = 2048 .. 8192");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const float2* __restrict__ hist = A.hist;
-  const int T = A.T, KT = A.KT;
+  const int T = A.T, hs = A.hs, KT = A.KT;
   const float inv_n0 = A.inv_n0;
   const int4 it = A.items[item];
   const int start = it.y, count = it.z;
@@ -176,16 +176,16 @@ __device__ __forceinline__ void temporal_c_item(const TemporalArgs& A, const int
   const int ja = gi, jb = paired ? KT - 1 - gi : gi;   // jb == ja: one lag (the middle one of an odd KT)
   float Gt = 0.f, Gtc = 0.f, Gt2 = 0.f, Gtc2 = 0.f;
 
-  // series loads without guards: t < n/2 stays inside the history, which
-  // carries n/2 - T spare samples behind its last mode (masked off); a
-  // non-identity order gathers sample t from slot order[t] (order's entries
-  // past T are 0)
+  // series loads without guards: t < n/2 stays inside the history (rows of
+  // hs >= T slots, the last one followed by n/2 - hs spare samples; samples
+  // t >= T are masked off); a non-identity order gathers sample t from slot
+  // order[t] (order's entries past T are 0)
   // (one 8-byte non-temporal load per sample, a wave-uniform base plus a
   // 32-bit per-sample offset; the order is re-read with each mode, from LDS
   // or the cache: m more registers held across the loop would spill at
   // n = 8192)
   auto load_series = [&](float2 (&dst)[m], int slot) {
-    const char* x = reinterpret_cast<const char*>(hist + (long)A.bin_modes[start + slot] * T);
+    const char* x = reinterpret_cast<const char*>(hist + (long)A.bin_modes[start + slot] * hs);
     const uint32_t* op = A.order;
     asm volatile("" : "+s"(op));
 #pragma unroll
@@ -397,14 +397,14 @@ __device__ __forceinline__ void temporal_c_setup(float2* lds, const float2* __re
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL, bool DENSE>
 __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_c(
-    const float2* __restrict__ hist, int T, const int4* __restrict__ items, const int* __restrict__ bin_modes,
+    const float2* __restrict__ hist, int T, int hs, const int4* __restrict__ items, const int* __restrict__ bin_modes,
     const uint32_t* __restrict__ order, const float* __restrict__ mask, int identity, float inv_n0,
     const float2* __restrict__ tw_g, const float2* __restrict__ tw_f, float* __restrict__ S_part,
     float* __restrict__ p_part, float* __restrict__ P_part, int KT, float* __restrict__ G_part) {
   extern __shared__ float2 lds[];
   float2 twb[2][5], wlast[8], wmid[8];
   temporal_c_setup<LOGT, MAXLR>(lds, tw_g, tw_f, twb, wlast, wmid);
-  const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, identity, KT, 0, inv_n0};
+  const TemporalArgs A{hist, items, bin_modes, order, mask, S_part, p_part, P_part, G_part, T, hs, identity, KT, 0, inv_n0};
   temporal_c_item<LOGT, MAXLR, TWO_LEVEL, DENSE>(A, blockIdx.x, lds, twb, wlast, wmid);
 }
 
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(NT, LOGT >= 12 ? 2 : 3) void k_temporal_cq(const Te
 }
 
 template <int LOGT, int MAXLR, bool TWO_LEVEL>
-static hipError_t launch_temporal_c_t(const float2* hist, int T, const int4* items, int nitems, const int* bin_modes,
+static hipError_t launch_temporal_c_t(const float2* hist, int T, int hs, const int4* items, int nitems, const int* bin_modes,
                                       const uint32_t* order, const float* mask, int identity, int dense,
                                       float inv_n0, const float2* tw, const float2* tw_f, float* S_part,
                                       float* p_part, float* P_part, int KT, float* G_part, hipStream_t st) {
@@ -450,7 +450,7 @@ static hipError_t launch_temporal_c_t(const float2* hist, int T, const int4* ite
   auto kern = dense ? k_temporal_c<LOGT, MAXLR, TWO_LEVEL, true> : k_temporal_c<LOGT, MAXLR, TWO_LEVEL, false>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, items, bin_modes, order, mask, identity, inv_n0,
+  hipLaunchKernelGGL(kern, dim3(nitems), dim3(NT), lds, st, hist, T, hs, items, bin_modes, order, mask, identity, inv_n0,
                      tw, tw_f, S_part, p_part, P_part, KT, G_part);
   return hipGetLastError();
 }
@@ -458,16 +458,16 @@ static hipError_t launch_temporal_c_t(const float2* hist, int T, const int4* ite
 // n = 2^logt in [2048, 8192]; items of at most 64 modes (at most 16 at
 // logt = 13, whose sums are one-level); S_part rows r-major with
 // lm = logt - 9.
-hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, const int4* items, int nitems,
+hipError_t launch_temporal_c(int logt, int maxlr, const float2* hist, int T, int hs, const int4* items, int nitems,
                              const int* bin_modes, const uint32_t* order, const float* mask, int identity, int dense,
                              float inv_n0, const float2* tw, const float2* tw_f, float* S_part, float* p_part,
                              float* P_part, int KT, float* G_part, hipStream_t st) {
-  if (KT < 1 || KT > 64 || KT > T || 2 * T > (1 << logt) || (dense && 2 * T != (1 << logt)))
+  if (KT < 1 || KT > 64 || KT > T || hs < T || 2 * T > (1 << logt) || (dense && 2 * T != (1 << logt)))
     return hipErrorInvalidValue;
   switch (logt * 4 + (maxlr == 2 ? 2 : 3)) {
 #define CASE(LG_, R_, TL_)                                                                                          \
   case LG_ * 4 + R_:                                                                                               \
-    return launch_temporal_c_t<LG_, R_, TL_>(hist, T, items, nitems, bin_modes, order, mask, identity, dense,     \
+    return launch_temporal_c_t<LG_, R_, TL_>(hist, T, hs, items, nitems, bin_modes, order, mask, identity, dense, \
                                              inv_n0, tw, tw_f, S_part, p_part, P_part, KT, G_part, st);
     CASE(11, 2, true) CASE(12, 2, true) CASE(13, 2, false)
     CASE(11, 3, true) CASE(12, 3, true) CASE(13, 3, false)
@@ -515,7 +515,7 @@ static hipError_t launch_temporal_cp_t(const TemporalArgs& a, int nwork, int* co
 
 hipError_t launch_temporal_cp(int logt, int maxlr, const TemporalArgs& a, int nwork, int* counter, const float2* tw,
                               const float2* tw_f, int grid, hipStream_t st) {
-  if (nwork < 1 || grid < 1 || a.KT < 1 || a.KT > 64 || a.KT > a.T || 2 * a.T > (1 << logt))
+  if (nwork < 1 || grid < 1 || a.KT < 1 || a.KT > 64 || a.KT > a.T || a.hs < a.T || 2 * a.T > (1 << logt))
     return hipErrorInvalidValue;
   switch (logt * 4 + (maxlr == 2 ? 2 : 3)) {
 #define CASE(LG_, R_, TL_) \

@@ -64,7 +64,8 @@ class Tuning(C.Structure):
 class Sizes(C.Structure):
     _fields_ = [("N", C.c_int32), ("NH", C.c_int32), ("T", C.c_int32), ("TPAD", C.c_int32), ("L", C.c_int32),
                 ("NB", C.c_int32), ("knodes", C.c_int32), ("modes", C.c_int64), ("binned_modes", C.c_int64),
-                ("hist_bytes", C.c_int64), ("work_items", C.c_int32), ("chunk_frames", C.c_int32)]
+                ("hist_bytes", C.c_int64), ("work_items", C.c_int32), ("chunk_frames", C.c_int32),
+                ("hist_stride", C.c_int32)]
 
 
 _lib = None
@@ -278,8 +279,9 @@ class DdmEngine:
                       self.bins.nModes.copy(), D.astype(np.float64), P.astype(np.float64), drift.astype(np.float64))
 
     def historyDevice(self) -> int:
-        """Device address of the spectral history (modes x T complex64, mode-major), for callers that keep
-        the data on the GPU (ddm_history_device)."""
+        """Device address of the spectral history (modes rows of sizes.hist_stride complex64, mode-major;
+        slots 0..T-1 of a row hold the mode's series), for callers that keep the data on the GPU
+        (ddm_history_device)."""
         self._alive()
         p = C.c_void_p()
         self._check(self._lib.ddm_history_device(self._h, C.byref(p)))

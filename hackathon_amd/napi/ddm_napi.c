@@ -156,7 +156,8 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* sizes(h) -> {N, NH, T, TPAD, L, NB, knodes, modes, binnedModes, histBytes, workItems, chunkFrames} */
+/* sizes(h) -> {N, NH, T, TPAD, L, NB, knodes, modes, binnedModes, histBytes, workItems, chunkFrames,
+ * histStride} */
 static napi_value js_sizes(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
@@ -179,6 +180,7 @@ static napi_value js_sizes(napi_env env, napi_callback_info info) {
   set_prop(env, out, "histBytes", num(env, (double)s.hist_bytes));
   set_prop(env, out, "workItems", num(env, s.work_items));
   set_prop(env, out, "chunkFrames", num(env, s.chunk_frames));
+  set_prop(env, out, "histStride", num(env, s.hist_stride));
   return out;
 }
 

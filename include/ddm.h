@@ -68,9 +68,10 @@ typedef struct ddm_sizes {
   int32_t N, NH, T, TPAD, L, NB, knodes;
   int64_t modes;          /* N * NH */
   int64_t binned_modes;   /* modes that fall in a bin */
-  int64_t hist_bytes;     /* modes * T * 8 */
+  int64_t hist_bytes;     /* modes * T * 8 (ddm_read_history's output) */
   int32_t work_items;     /* temporal work items (bin, mode range) */
   int32_t chunk_frames;   /* frames per spatial chunk */
+  int32_t hist_stride;    /* slots per mode row of the device history: T rounded up to 16 */
 } ddm_sizes;
 
 /* Launch-shape tuning of an engine (0 in any field = the default, which the
@@ -162,8 +163,9 @@ int ddm_read_temporal_full(ddm_engine *e, float *D, float *C, float *P);
 /* idle (src/ddm-engine.ts:417-419): wait for all work on the engine's stream. */
 int ddm_idle(ddm_engine *e);
 
-/* Device pointer of the spectral history (modes*T complex f32), for callers
- * that keep data on the GPU. */
+/* Device pointer of the spectral history, for callers that keep data on the
+ * GPU: modes rows of hist_stride complex f32 (ddm_sizes), mode-major; slots
+ * 0 .. T-1 of a row hold the mode's series, the rest stay zero. */
 int ddm_history_device(ddm_engine *e, void **dptr);
 
 /* Per-kernel timing with HIP events on the engine's stream.  enable != 0 starts

@@ -55,18 +55,19 @@ def test_large_pipeline_4096x2048(gpu):
         kt = e.kernelTimes()
         r = e.readRadial()
         base = e.historyDevice()
-        # frame 0's spectrum: history slot 0 of every mode (stride T)
+        hs = e.sizes.hist_stride   # slots per mode row of the device history
+        # frame 0's spectrum: history slot 0 of every mode
         slot0 = np.zeros(modes, np.complex64)
         for m0 in range(0, modes, 16384):   # strided 2D copies of at most 16384 rows
             rows = min(16384, modes - m0)
-            assert hip.hipMemcpy2D(slot0[m0:].ctypes.data, 8, base + m0 * T * 8, T * 8, 8, rows, D2H) == 0
+            assert hip.hipMemcpy2D(slot0[m0:].ctypes.data, 8, base + m0 * hs * 8, hs * 8, 8, rows, D2H) == 0
         bins = e.bins
         series = {}
         for b in CHECK_BINS:
             ms = [int(m) for m in bins.binModes[bins.binOffsets[b]:bins.binOffsets[b + 1], 0]]
             h = np.zeros((len(ms), T), np.complex64)
             for i, m in enumerate(ms):
-                assert hip.hipMemcpy(h[i].ctypes.data, base + m * T * 8, T * 8, D2H) == 0
+                assert hip.hipMemcpy(h[i].ctypes.data, base + m * hs * 8, T * 8, D2H) == 0
             series[b] = h
         frame0 = fr[0].cpu().numpy().view(np.uint16)   # torch holds u16 frames as int16
     del fr
