@@ -29,7 +29,11 @@ namespace ddm {
 
 namespace {
 
-constexpr int FNT = 256;          // 4 fits per workgroup
+// One fit (one wave) per workgroup: a fit's LM steps vary from a few to
+// thousands, and a workgroup's slot frees only when its last wave ends; with
+// four fits per workgroup three slots idled behind the slowest (131,072
+// per-q Schulz fits from their start grids: 697 -> 504 ms).
+constexpr int FNT = 64;
 constexpr int MAXT = 128;         // lag points per fit
 constexpr int MAXK = 8;           // q nodes per bin
 
@@ -629,7 +633,7 @@ template <bool SWIM, int GEO>
 __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
                                                 float* __restrict__ theta_out, double* __restrict__ chi_out,
                                                 int* __restrict__ steps_out) {
-  // each wave's Jacobian columns at theta and at the trial point (FP64, lane-indexed)
+  // the wave's Jacobian columns at theta and at the trial point (FP64, lane-indexed)
   __shared__ double lm_jac[FNT / 64][2 * 5 * MAXT];
   const int lane = threadIdx.x & 63;
   const int f = blockIdx.x * (FNT / 64) + (threadIdx.x >> 6);
