@@ -623,21 +623,50 @@ __device__ double lm_run(double* th, const float* lo, const float* hi, const flo
 
 }  // namespace
 
+// A fit's intermediate results: its starts (at most LM_NS, ranked) and each
+// start's LM result.  A grid-seeded fit runs its starts on separate waves: LM
+// from one start takes up to maxit steps, and with the starts run one after
+// another a batch of fewer fits than the device holds waited on its slowest
+// fit's three runs in a row.
+constexpr int LM_NS = 3;
+struct LmWork {
+  double* chi;     // [nfits][LM_NS]
+  float* starts;   // [nfits][LM_NS][6]
+  float* theta;    // [nfits][LM_NS][6]
+  int* nrun;       // [nfits]
+  int* steps;      // [nfits][LM_NS]
+};
+LmWork lm_work(void* ws, int nfits) {
+  const size_t n = (size_t)nfits;
+  char* p = static_cast<char*>(ws);
+  LmWork w;
+  w.chi = reinterpret_cast<double*>(p);
+  w.starts = reinterpret_cast<float*>(p + n * LM_NS * 8);
+  w.theta = w.starts + n * LM_NS * 6;
+  w.nrun = reinterpret_cast<int*>(w.theta + n * LM_NS * 6);
+  w.steps = w.nrun + n;
+  return w;
+}
+
 // mode 0: LM from the seeds; 1: plain grid-seeded LM from the best `nstarts`
 // starts; 2: nonlinear parameters from the seeds, A and B by NNLS there.
-template <bool SWIM, int GEO>
+// Three launches: RUN = false sets each fit's starts (ranks the start grid in
+// mode 1; one wave per fit), RUN = true runs LM from one start (one wave per
+// fit and start), k_lm_pick keeps each fit's best start.  Without the start
+// grid's code beside it the LM spills 35-53 registers instead of 264-285
+// (131,072 plain per-q fits 502 -> 343 ms).
+template <bool SWIM, int GEO, bool RUN>
 // Three waves per SIMD: the FP64 normal equations and Cholesky would keep
 // ~256 VGPRs live, and the kernel is latency-bound; capped at 168 VGPRs (the
 // rest spills) it is fastest (10,240 Schulz fits: 52 ms at one wave per SIMD,
 // 32 ms at two, 29 ms at three, 30 ms at four).
 __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode, int nstarts, int maxit,
-                                                float* __restrict__ theta_out, double* __restrict__ chi_out,
-                                                int* __restrict__ steps_out) {
-  // the wave's Jacobian columns at theta and at the trial point (FP64, lane-indexed)
-  __shared__ double lm_jac[FNT / 64][2 * 5 * MAXT];
+                                                LmWork ws) {
   const int lane = threadIdx.x & 63;
-  const int f = blockIdx.x * (FNT / 64) + (threadIdx.x >> 6);
+  const int f = RUN ? (int)blockIdx.x / nstarts : (int)blockIdx.x;
   if (f >= nfits) return;
+  const int s_own = RUN ? (int)blockIdx.x % nstarts : 0;
+  if (RUN && s_own >= ws.nrun[f]) return;
   const int nt = min(in.ntau[f], MAXT);
   const int K = min(in.K, MAXK);
   float tv[MAXT / 64], dv[MAXT / 64], q[MAXK];
@@ -656,6 +685,21 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
   const float lo[6] = {0.f, 0.f, 0.f, logf(vmin), logf(0.5f), logf(Dmin)};
   const float hi[6] = {3.4e38f, 3.4e38f, 1.f, logf(vmax), logf(100.f), logf(Dmax)};
   int steps = 0;
+  if constexpr (RUN) {
+    // the wave's Jacobian columns at theta and at the trial point (FP64, lane-indexed)
+    __shared__ double lm_jac[2 * 5 * MAXT];
+    double th[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) th[k] = ws.starts[((long)f * LM_NS + s_own) * 6 + k];
+    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps, lm_jac);
+    if (lane == 0) {
+      const long o = (long)f * LM_NS + s_own;
+      for (int k = 0; k < 6; k++) ws.theta[o * 6 + k] = (float)th[k];
+      ws.chi[o] = chi;
+      ws.steps[o] = steps;
+    }
+    return;
+  }
   float st[3][6];
   int nrun = 1;
   if (mode != 1) {
@@ -733,25 +777,36 @@ __global__ __launch_bounds__(FNT, 3) void k_lm_fit(FitIn in, int nfits, int mode
       nrun = min(nstarts, 3);
     }
   }
-  float best[6];
-  double bestchi = 1e300;
-  for (int s = 0; s < nrun; s++) {
-    double th[6];
+  if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 6; k++) th[k] = s == 0 ? st[0][k] : (s == 1 ? st[1][k] : st[2][k]);
-    const double chi = lm_run<SWIM, GEO>(th, lo, hi, tv, dv, nt, w, q, K, V, in.t2d, maxit, lane, &steps,
-                                         lm_jac[threadIdx.x >> 6]);
-    if (chi < bestchi) {
-      bestchi = chi;
+    for (int s = 0; s < 3; s++)
+      if (s < nrun)
 #pragma unroll
-      for (int k = 0; k < 6; k++) best[k] = (float)th[k];
+        for (int k = 0; k < 6; k++) ws.starts[((long)f * LM_NS + s) * 6 + k] = st[s][k];
+    ws.nrun[f] = nrun;
+  }
+}
+
+// Each fit's best start (the first of equal chi^2 in rank order) and its LM
+// steps over all its starts.
+__global__ __launch_bounds__(256) void k_lm_pick(LmWork ws, int nfits, float* __restrict__ theta_out,
+                                                 double* __restrict__ chi_out, int* __restrict__ steps_out) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= nfits) return;
+  const int nr = ws.nrun[f];
+  double best = 1e300;
+  int bs = 0, steps = 0;
+  for (int s = 0; s < nr; s++) {
+    const double c = ws.chi[(long)f * LM_NS + s];
+    steps += ws.steps[(long)f * LM_NS + s];
+    if (c < best) {
+      best = c;
+      bs = s;
     }
   }
-  if (lane == 0) {
-    for (int k = 0; k < 6; k++) theta_out[(long)f * 6 + k] = best[k];
-    chi_out[f] = bestchi;
-    steps_out[f] = steps;
-  }
+  for (int k = 0; k < 6; k++) theta_out[(long)f * 6 + k] = ws.theta[((long)f * LM_NS + bs) * 6 + k];
+  chi_out[f] = best;
+  steps_out[f] = steps;
 }
 
 // ----------------------------------------------------------------- in-plane ISF, FP64 quadrature
@@ -1336,25 +1391,38 @@ hipError_t launch_lm_global(const float* tau, const float* d, const float* w, co
   return hipGetLastError();
 }
 
+size_t lm_fit_workspace_bytes(int nfits) {
+  return (size_t)nfits * (LM_NS * 8 + 2 * LM_NS * 6 * 4 + 4 + LM_NS * 4);
+}
+
+template <bool SWIM, int GEO>
+static hipError_t launch_lm_fit_t(const FitIn& in, int nfits, int mode, int nstarts, int maxit, float* theta,
+                                  double* chi, int* steps, void* ws, hipStream_t st) {
+  static_assert(FNT == 64, "one wave per workgroup: the LM launch numbers its workgroups by (fit, start)");
+  if (!ws) return hipErrorInvalidValue;
+  const LmWork w = lm_work(ws, nfits);
+  const int ns = SWIM && mode == 1 ? nstarts : 1;   // seeded fits and the Brownian grid: one start
+  hipLaunchKernelGGL((k_lm_fit<SWIM, GEO, false>), dim3(nfits), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, w);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_lm_fit<SWIM, GEO, true>), dim3(nfits * ns), dim3(FNT), 0, st, in, nfits, mode, ns, maxit, w);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lm_pick, dim3((nfits + 255) / 256), dim3(256), 0, st, w, nfits, theta, chi, steps);
+  return hipGetLastError();
+}
+
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
                          const float* bounds, const float* seeds, const float* vmag, int nfits, int model,
                          const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
-                         int* steps, hipStream_t st) {
+                         int* steps, void* ws, hipStream_t st) {
   // seed_mode: 0 grid, 1 given, 2 nonlinear given (the ABI's DDM_SEED_*) -> kernel mode
   const int mode = seed_mode == 0 ? 1 : (seed_mode == 2 ? 2 : 0);
-  if (K > MAXK || model < 0 || model > 2 || (model == 2 && !t2d)) return hipErrorInvalidValue;
+  if (K > MAXK || model < 0 || model > 2 || (model == 2 && !t2d) || nstarts < 1 || nstarts > LM_NS || nfits < 1)
+    return hipErrorInvalidValue;
   FitIn in{tau, d, ntau, w, qn, bounds, seeds, vmag, K, model == 2 ? *t2d : Isf2dDev{}};
-  const int blocks = (nfits + FNT / 64 - 1) / (FNT / 64);
-  if (model == 2)
-    hipLaunchKernelGGL((k_lm_fit<true, 1>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
-                       chi, steps);
-  else if (model == 1)
-    hipLaunchKernelGGL((k_lm_fit<true, 0>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
-                       chi, steps);
-  else
-    hipLaunchKernelGGL((k_lm_fit<false, 0>), dim3(blocks), dim3(FNT), 0, st, in, nfits, mode, nstarts, maxit, theta,
-                       chi, steps);
-  return hipGetLastError();
+  if (model == 2) return launch_lm_fit_t<true, 1>(in, nfits, mode, nstarts, maxit, theta, chi, steps, ws, st);
+  if (model == 1) return launch_lm_fit_t<true, 0>(in, nfits, mode, nstarts, maxit, theta, chi, steps, ws, st);
+  return launch_lm_fit_t<false, 0>(in, nfits, mode, nstarts, maxit, theta, chi, steps, ws, st);
 }
 
 }  // namespace ddm
@@ -1391,7 +1459,8 @@ constexpr size_t kNaValid = kNaNtau + kNaFits * 4;
 constexpr size_t kNaTheta = kNaValid + kNaFits * 4;
 constexpr size_t kNaChi = kNaTheta + kNaFits * 6 * 4;
 constexpr size_t kNaSteps = kNaChi + kNaFits * 8;
-constexpr size_t kNaBytes = kNaSteps + kNaFits * 4;
+constexpr size_t kNaWork = (kNaSteps + kNaFits * 4 + 255) / 256 * 256;   // the LM's workspace
+constexpr size_t kNaBytes = kNaWork + kNaFits * (LM_NS * 8 + 2 * LM_NS * 6 * 4 + 4 + LM_NS * 4);
 static_assert(kNaChi % 8 == 0, "FP64 chi^2 slots");
 static_assert(kNaTabZ + Isf2dTable::nz * (3 * 8 + 2 * 4) <= kNaTab, "table region");
 
@@ -1506,7 +1575,7 @@ hipError_t launch_fit_noalloc(int dev, const ddm_fit_batch* b, double* theta, do
                     b->vmag ? reinterpret_cast<const float*>(s + kNaVmag) : nullptr, b->nfits, b->model, &t2d,
                     b->seed_mode, b->nstarts < 1 ? 1 : (b->nstarts > 3 ? 3 : b->nstarts),
                     b->max_iter < 0 ? 0 : b->max_iter, reinterpret_cast<float*>(s + kNaTheta),
-                    reinterpret_cast<double*>(s + kNaChi), reinterpret_cast<int*>(s + kNaSteps), st);
+                    reinterpret_cast<double*>(s + kNaChi), reinterpret_cast<int*>(s + kNaSteps), s + kNaWork, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fit_unpack, dim3((b->nfits + 255) / 256), dim3(256), 0, st, s, b->nfits, theta, chi2, iters);
   return hipGetLastError();

@@ -25,6 +25,7 @@ struct ddm_fitter {
   float* theta = nullptr;
   double* chi = nullptr;
   int* steps = nullptr;
+  void* lmws = nullptr;   // LM workspace (ranked starts and per-start results)
   // global-parameter fits: device arrays (grown on demand)
   size_t gcap_vid = 0, gcap_bins = 0;
   float *gtau = nullptr, *gd = nullptr, *gw = nullptr, *gqn = nullptr, *gbounds = nullptr, *gseeds = nullptr,
@@ -42,12 +43,14 @@ constexpr int kMaxT = 128;
 constexpr int kMaxK = 8;
 
 void release(ddm_fitter* f) {
-  void* ptrs[] = {f->tau, f->d, f->w, f->qn, f->bounds, f->seeds, f->vmag, f->ntau, f->theta, f->chi, f->steps};
+  void* ptrs[] = {f->tau, f->d, f->w, f->qn, f->bounds, f->seeds, f->vmag, f->ntau, f->theta, f->chi, f->steps,
+                  f->lmws};
   for (void* p : ptrs)
     if (p) hipFree(p);
   f->tau = f->d = f->w = f->qn = f->bounds = f->seeds = f->vmag = f->theta = nullptr;
   f->ntau = f->steps = nullptr;
   f->chi = nullptr;
+  f->lmws = nullptr;
   f->cap = 0;
 }
 
@@ -136,6 +139,7 @@ hipError_t grow(ddm_fitter* f, size_t n) {
   if ((e = hipMalloc((void**)&f->theta, n * 24)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->chi, n * 8)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&f->steps, n * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc(&f->lmws, ddm::lm_fit_workspace_bytes((int)n))) != hipSuccess) return e;
   f->cap = n;
   return hipSuccess;
 }
@@ -226,7 +230,7 @@ int ddm_fit(ddm_fitter* f, const ddm_fit_batch* b, double* theta, double* chi2, 
   e = ddm::launch_lm_fit(f->tau, f->d, f->ntau, f->w, f->qn, b->knodes, f->bounds, f->seeds,
                          b->vmag ? f->vmag : nullptr, b->nfits, b->model, &f->t2d, b->seed_mode,
                          std::max(1, std::min(3, b->nstarts)), std::max(0, b->max_iter), f->theta, f->chi, f->steps,
-                         f->stream);
+                         f->lmws, f->stream);
   if (e != hipSuccess) return ffail(f, DDM_EHIP, std::string("fit: launch: ") + hipGetErrorString(e));
   hipEventRecord(f->ev1, f->stream);
   std::vector<float> th(n * 6);

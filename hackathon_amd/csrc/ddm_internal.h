@@ -90,11 +90,14 @@ hipError_t launch_isf2d_eval(const double* x, const double* Z, int n, int exact,
                              hipStream_t st);
 
 // model: 0 Brownian, 1 3D swimmer, 2 in-plane swimmer (t2d required); vmag:
-// per-fit uniform drift speed or null
+// per-fit uniform drift speed or null; nstarts in [1, 3]; ws: device
+// workspace of lm_fit_workspace_bytes(nfits) bytes (each fit's starts and
+// their results)
+size_t lm_fit_workspace_bytes(int nfits);
 hipError_t launch_lm_fit(const float* tau, const float* d, const int* ntau, const float* w, const float* qn, int K,
                          const float* bounds, const float* seeds, const float* vmag, int nfits, int model,
                          const Isf2dDev* t2d, int seed_mode, int nstarts, int maxit, float* theta, double* chi,
-                         int* steps, hipStream_t st);
+                         int* steps, void* ws, hipStream_t st);
 
 // global-parameter fits (one workgroup per video); model 0 Brownian, 1 3D
 // swimmer, 2 in-plane swimmer (t2d required); mode 0 from seeds, 1 plain;

@@ -278,7 +278,7 @@ int ddm_isf2d_eval(ddm_fitter *f, const double *x, const double *Z, int32_t n, i
  * capture).  Results are bit-identical to ddm_fit's.  A fit whose ntau,
  * bounds or vmag fails ddm_fit's checks gets chi2 = NaN, iters = -1 and NaN
  * theta instead of an error status.  Buffers beyond the inputs and outputs
- * live in one fixed device array of about 20 MiB per GPU (set up by
+ * live in one fixed device array of about 23 MiB per GPU (set up by
  * ddm_fitter_create), so nfits <= DDM_FIT_NOALLOC_MAX; every call on a GPU
  * shares that array, so calls that overlap on one GPU (several fitters'
  * streams at once) race. */
