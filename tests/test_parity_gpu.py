@@ -95,6 +95,7 @@ def test_history_slots_wrap_and_chunks(gpu, monkeypatch):
     monkeypatch.setenv("DDM_CHUNK_MB", "8")   # 16-frame chunks
     with DdmEngine.create(N=N, T=T, L=4, NB=4, kmin=0, kmax=256) as e:
         assert e.sizes.chunk_frames < T
+        assert e.sizes.hist_stride == 48   # device rows of T rounded up to 16 slots; readHistory returns T per row
         e.ingestFrames(fr[:30], 25)    # slots 25..39, 0..14
         e.ingestFrames(fr[30:], 15)    # slots 15..24
         h = e.readHistory()
