@@ -7,8 +7,9 @@
 //              the column pass reads contiguous columns.
 //  k_cols      row spectra -> history.  Mean correction on the kx <= 3 columns
 //              (turns the per-row means into the frame mean, see below), FFT
-//              along y, x 1/N, corner turn into hist[(m*T + slot)] with F frames
-//              per workgroup so every store run is F*8 contiguous bytes.
+//              along y, x 1/N, corner turn into hist[m*hs + slot] (rows of hs =
+//              T rounded up to 16 slots) with F frames per workgroup so every
+//              store run is F*8 contiguous bytes, line-aligned from slot 16j.
 //  k_rows_w / k_cols_w  the same passes with one wavefront per transform
 //              (N = 512, 1024).
 // Frame sides: powers of two, and 3 * 2^m through a radix-3 first pass.
